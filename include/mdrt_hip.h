@@ -1,0 +1,254 @@
+/* mdrt_hip.h — C ABI of libmdrt_hip.so, the MI355X (gfx950) runtime and
+ * per-step force path.
+ *
+ * Two groups of entry points:
+ *
+ *  1. The runtime ABI of the reference's runtime/mdrt_cuda.c, same names and
+ *     signatures, on HIP (modules are gfx950 code objects, streams are HIP
+ *     streams, the FFT entry points run this library's own kernels).
+ *
+ *  2. The force path as explicit entry points (the reference emits these
+ *     kernels from lib/Conversion/MDExecToGPU; a compiled program cannot be
+ *     produced without its MLIR toolchain, so the path is exposed here with
+ *     plain pointers and sizes): a system handle, forces and energies,
+ *     velocity-Verlet steps with SETTLE waters, and the neighbor pair set.
+ *
+ * Units are the reference's: nm, ps, amu, kJ/mol, e.  All pointers in
+ * group 2 are host pointers unless the name says device.
+ */
+#ifndef MDRT_HIP_H
+#define MDRT_HIP_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------
+ * 1. Runtime ABI (runtime/mdrt_cuda.c)
+ * ------------------------------------------------------------------- */
+
+typedef void *mgpuModule;   /* hipModule_t   */
+typedef void *mgpuFunction; /* hipFunction_t */
+typedef void *mgpuStream;   /* hipStream_t   */
+typedef void *mgpuEvent;    /* hipEvent_t    */
+
+/* mdrt_cuda.c:420 */
+void mgpuSetDefaultDevice(int32_t device);
+/* mdrt_cuda.c:476 — data is a gfx950 code object (ELF) */
+mgpuModule mgpuModuleLoad(void *data, size_t size);
+/* mdrt_cuda.c:502 — no JIT on this path: a code object is loaded as is */
+mgpuModule mgpuModuleLoadJIT(void *data, int optLevel, size_t size);
+/* mdrt_cuda.c:526 */
+void mgpuModuleUnload(mgpuModule module);
+/* mdrt_cuda.c:553 */
+mgpuFunction mgpuModuleGetFunction(mgpuModule module, const char *name);
+/* mdrt_cuda.c:669 */
+void mgpuLaunchKernel(mgpuFunction function, intptr_t gridX, intptr_t gridY,
+                      intptr_t gridZ, intptr_t blockX, intptr_t blockY,
+                      intptr_t blockZ, int32_t smem, mgpuStream stream,
+                      void **params, void **extra, size_t paramsCount);
+/* mdrt_cuda.c:679 — CDNA has no thread-block clusters: cluster extents
+   other than 1 are refused. */
+void mgpuLaunchKernelCooperative(mgpuFunction function, intptr_t gridX,
+                                 intptr_t gridY, intptr_t gridZ,
+                                 intptr_t clusterX, intptr_t clusterY,
+                                 intptr_t clusterZ, intptr_t blockX,
+                                 intptr_t blockY, intptr_t blockZ,
+                                 int32_t smem, mgpuStream stream,
+                                 void **params, void **extra);
+/* mdrt_cuda.c:401 */
+intptr_t mdrtCooperativeGridSize(intptr_t requested);
+/* mdrt_cuda.c:695, :775, :1747, :1757 */
+mgpuStream mgpuStreamCreate(void);
+void mgpuStreamDestroy(mgpuStream stream);
+void mgpuStreamSynchronize(mgpuStream stream);
+void mgpuStreamWaitEvent(mgpuStream stream, mgpuEvent event);
+/* mdrt_cuda.c:1762-1782 */
+mgpuEvent mgpuEventCreate(void);
+void mgpuEventDestroy(mgpuEvent event);
+void mgpuEventSynchronize(mgpuEvent event);
+void mgpuEventRecord(mgpuEvent event, mgpuStream stream);
+/* mdrt_cuda.c:2147, :2238 */
+void *mgpuMemAlloc(uint64_t size, mgpuStream stream, bool isHostShared);
+void mgpuMemFree(void *pointer, mgpuStream stream);
+/* mdrt_cuda.c:2431 */
+void mgpuMemcpy(void *destination, void *source, size_t size,
+                mgpuStream stream);
+/* mdrt_cuda.c:2494, :2510 */
+void mgpuMemset32(void *destination, unsigned int value, size_t count,
+                  mgpuStream stream);
+void mgpuMemset16(void *destination, unsigned short value, size_t count,
+                  mgpuStream stream);
+/* mdrt_cuda.c:729-742 — side streams joined by events */
+void mdrtSideBegin(void);
+void mdrtSideEnd(void);
+void mdrtSide2Begin(void);
+void mdrtSide2End(void);
+void mdrtSideJoin(void);
+/* mdrt_cuda.c:762, :770 */
+void *mdrtUseStream(void *stream);
+void mdrtRestoreStream(void *previous);
+
+/* The descriptor of a memref of one dimension (mdrt_cuda.c:1530-1536). */
+typedef struct {
+  void *allocated;
+  void *aligned;
+  int64_t offset;
+  int64_t size;
+  int64_t stride;
+} mdrtBuffer1;
+
+/* mdrt_cuda.c:1615, :1642, :1691, :1710 — flags in mapped host memory */
+void _mlir_ciface_mdrtFlagMemory(mdrtBuffer1 *result, int64_t place);
+int64_t mdrtFlagStart(void *device);
+int32_t mdrtFlagFinish(int64_t slot);
+int32_t mdrtFlagTake(void *device);
+/* mdrt_cuda.c:1548 */
+void _mlir_ciface_mdrtHostResults(mdrtBuffer1 *result, int64_t place,
+                                  int64_t count);
+/* mdrt_cuda.c:154 */
+void mdrtCountHostRead(size_t bytes, int waits, int flags);
+/* mdrt_cuda.c:2566 */
+void mdrtCudaSetStopHandler(void (*handler)(const char *));
+/* mdrt_cuda.c:2824, :2836 */
+uint64_t mdrtDeviceTrim(void);
+void mdrtDeviceMemoryCounts(int64_t counts[8]);
+
+/* mdrt_cuda.c:2901-2991 — the 3D transforms of the PME grid, with this
+   library's FFT kernels: real k1 x k2 x k3 <-> half spectrum
+   k1 x k2 x (k3/2+1) interleaved complex, forward exp(-2 pi i),
+   backward exp(+2 pi i), unnormalised.  Buffers are device memory. */
+void _mlir_ciface_mdrtCudaFFTForward3D(mdrtBuffer1 *real, mdrtBuffer1 *complex,
+                                       int64_t k1, int64_t k2, int64_t k3);
+void _mlir_ciface_mdrtCudaFFTBackward3D(mdrtBuffer1 *complex,
+                                        mdrtBuffer1 *real, int64_t k1,
+                                        int64_t k2, int64_t k3);
+void _mlir_ciface_mdrtCudaFFTForward3DF32(mdrtBuffer1 *real,
+                                          mdrtBuffer1 *complex, int64_t k1,
+                                          int64_t k2, int64_t k3);
+void _mlir_ciface_mdrtCudaFFTBackward3DF32(mdrtBuffer1 *complex,
+                                           mdrtBuffer1 *real, int64_t k1,
+                                           int64_t k2, int64_t k3);
+/* mdrt_cuda.c:2986 — backward with the separable Gaussian influence applied
+   as the transform loads the spectrum (pme_callback.cu:5-21): tables holds
+   rows t_1 | t_2 | t_3 (t_1 includes f/(pi V)) then m_1 | m_2 | m_3, each
+   row max(k1,k2,k3) long. */
+void _mlir_ciface_mdrtCudaFFTGaussianBackwardF32(mdrtBuffer1 *complex,
+                                                 mdrtBuffer1 *real,
+                                                 mdrtBuffer1 *tables,
+                                                 int64_t k1, int64_t k2,
+                                                 int64_t k3);
+/* mdrt_cuda.c:2765, :2777 */
+int mdrtRetainFFTPlan(int64_t k1, int64_t k2, int64_t k3, int forward,
+                      int single, const float *tables);
+void mdrtReleaseFFTPlan(int handle);
+
+/* ---------------------------------------------------------------------
+ * 2. The force path
+ * ------------------------------------------------------------------- */
+
+enum {
+  MDRT_LJ_NONE = 0,            /* TermLennardJones.cpp:66-81            */
+  MDRT_LJ_POTENTIAL_SHIFT = 1, /* TermLennardJones.cpp:108-111          */
+  MDRT_LJ_FORCE_SWITCH = 2,    /* TermLennardJones.cpp:112-135          */
+  MDRT_LJ_POTENTIAL_SWITCH = 3 /* ExpandTruncation.cpp:19-45            */
+};
+
+/* Energy terms, kJ/mol. */
+enum {
+  MDRT_E_LJ = 0,
+  MDRT_E_COUL_DIRECT = 1,
+  MDRT_E_COUL_EXCL = 2,
+  MDRT_E_COUL_RECIP = 3,
+  MDRT_E_COUL_SELF = 4,
+  MDRT_E_DISP_CORR = 5,
+  MDRT_E_LJ14 = 6,
+  MDRT_E_COUL14 = 7,
+  MDRT_E_KINETIC = 8,
+  MDRT_NUM_ENERGIES = 9
+};
+
+/* A system: the topology and settings of lib/Driver (Topology, Control).
+   Arrays are copied at creation. */
+typedef struct {
+  int64_t n;                  /* particles                              */
+  const double *positions;    /* n x 3                                  */
+  const double *velocities;   /* n x 3 or NULL (zero)                   */
+  const double *masses;       /* n                                      */
+  const double *charges;      /* n                                      */
+  const int32_t *types;       /* n, in [0, ntypes)                      */
+  int32_t ntypes;             /* <= 32                                  */
+  const double *sigma;        /* ntypes x ntypes                        */
+  const double *epsilon;      /* ntypes x ntypes                        */
+  double box[3];              /* orthorhombic, each edge > 2 (rc+skin)  */
+  double cutoff;              /* rc                                     */
+  double switch_from;         /* rs for the switches                    */
+  int32_t lj_modifier;        /* MDRT_LJ_*                              */
+  int32_t coulomb_shift;      /* POTENTIAL_SHIFT of the direct Coulomb  */
+  int32_t dispersion;         /* dispersion correction on/off           */
+  double beta;                /* Ewald beta (nm^-1); 0: no Coulomb      */
+  int32_t pme_grid[3];        /* K1 K2 K3 (factors 2,3,5,7)             */
+  int32_t pme_order;          /* 4, 6 or 8                              */
+  int64_t nexclusions;
+  const int32_t *exclusions;  /* nexclusions x 2                        */
+  int64_t npairs14;
+  const int32_t *pairs14;     /* npairs14 x 2                           */
+  const double *pairs14_sigma;
+  const double *pairs14_eps;  /* eps * s_LJ                             */
+  const double *pairs14_qq;   /* f q_i q_j s_C                          */
+  int64_t nwaters;
+  const int32_t *water_first; /* first atom (oxygen) of each water      */
+  double d_oh, d_hh;          /* SETTLE geometry                        */
+  double skin;                /* list buffer beyond rc (nm)             */
+} mdrtHipSystemDesc;
+
+/* Creates a system on the current device; NULL on error (see
+   mdrtHipLastError). */
+void *mdrtHipSystemCreate(const mdrtHipSystemDesc *desc);
+void mdrtHipSystemDestroy(void *system);
+const char *mdrtHipLastError(void);
+
+int mdrtHipSetPositions(void *system, const double *x);
+int mdrtHipSetVelocities(void *system, const double *v);
+int mdrtHipGetPositions(void *system, double *x);
+int mdrtHipGetVelocities(void *system, double *v);
+
+/* Forces (n x 3, may be NULL) and energies (MDRT_NUM_ENERGIES, may be NULL)
+   at the current positions; builds the list. */
+int mdrtHipComputeForces(void *system, double *forces, double *energies);
+
+/* `steps` velocity-Verlet steps of dt (ps) with SETTLE waters.  graph != 0
+   replays the step sequence as a captured HIP graph of `graph` steps.
+   kinetic (may be NULL) receives the kinetic energy after the last step. */
+int mdrtHipRun(void *system, int64_t steps, double dt, int32_t graph,
+               double *kinetic);
+
+/* Steepest descent of `steps` accepted-or-rejected moves, rigid waters. */
+int mdrtHipMinimize(void *system, int64_t steps, double max_step,
+                    double *energy);
+
+/* The neighbor pair set of the reference's matrix build
+   (NeighborsMatrix.mlir:193-460) at `reach`: unordered pairs (i < j) whose
+   f32 minimum-image distance is within reach + 3e-6 (Lx+Ly+Lz), read from
+   this library's cluster list.  With `excluded` != 0 the excluded pairs are
+   dropped as the exclusion filter of MDExecKernels.cpp:1748-1823 drops them.
+   Writes at most `capacity` pairs; *count receives the number found
+   (duplicates removed). */
+int mdrtHipNeighborPairs(void *system, double reach, int32_t excluded,
+                         int64_t capacity, int32_t *pairs, int64_t *count);
+
+/* Counters: builds done, and list statistics (clusters, entries). */
+int mdrtHipStats(void *system, int64_t *stats /* 8 */);
+
+/* Kernel timing of one force evaluation of the step (device events):
+   ms per launch of pairs, spread, fft, gather, integrate, build. */
+int mdrtHipProfileStep(void *system, int32_t repeats, double *ms /* 8 */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1196 @@
+// engine.hip — the per-step force path on one MI355X: buffers, launch order,
+// step graphs and the C entry points of include/mdrt_hip.h (group 2).
+//
+// Step order (the reference's, 08-gpu.md:112-121 and SURVEY call stack):
+//   integrate_first (half kick, drift, SETTLE, f32 positions, validity)
+//   nb_bin / nb_sort / nb_search   (no-ops unless the validity test failed)
+//   nb_pairs                       (LJ + erfc Ewald, 64-bit force words)
+//   nb_special                     (excluded-pair correction, 1-4 pairs)
+//   pme_spread, fft z/y, fft x + influence + inverse x, fft y/z, pme_gather
+//   integrate_second               (half kick, velocity constraint)
+// All kernels are launched on one stream, so a run of steps captures into
+// one HIP graph and replays without host decisions.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mdrt_hip.h"
+#include "integrate.hip"
+#include "nbnxm.hip"
+#include "pairs.hip"
+#include "pme.hip"
+
+namespace mdhip {
+
+static thread_local std::string g_last_error;
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define HIPCHK(expr)                                                               \
+  do {                                                                             \
+    hipError_t _e = (expr);                                                        \
+    if (_e != hipSuccess)                                                          \
+      throw ::mdhip::HipError(std::string(#expr) + ": " + hipGetErrorString(_e));  \
+  } while (0)
+
+template <typename T>
+static T *dalloc(size_t count) {
+  void *p = nullptr;
+  if (count == 0) count = 1;
+  HIPCHK(hipMalloc(&p, count * sizeof(T)));
+  HIPCHK(hipMemset(p, 0, count * sizeof(T)));
+  return static_cast<T *>(p);
+}
+
+template <typename T>
+static void dfree(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+template <typename T>
+static T *dupload(const std::vector<T> &v) {
+  T *p = dalloc<T>(v.size());
+  if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+
+static std::vector<int> radices(int K) {
+  std::vector<int> r;
+  int rest = K;
+  for (int f : {8, 4, 2, 3, 5, 7})
+    while (rest % f == 0) {
+      r.push_back(f);
+      rest /= f;
+    }
+  if (rest != 1 || (int)r.size() > kMaxRadices)
+    throw HipError("FFT length " + std::to_string(K) + " has a factor other than 2, 3, 5, 7");
+  return r;
+}
+
+// Twiddles exp(-2 pi i m / K), m < K, rounded once from f64.
+template <typename T>
+struct FftTables {
+  int K = 0;
+  cpx<T> *tw = nullptr;
+  FftAxis axis{};
+  void make(int k) {
+    K = k;
+    std::vector<cpx<T>> h(K);
+    for (int m = 0; m < K; ++m) {
+      const double a = -2.0 * M_PI * (double)m / (double)K;
+      h[m] = {(T)cos(a), (T)sin(a)};
+    }
+    tw = dupload(h);
+    std::vector<int> r = radices(K);
+    axis.K = K;
+    axis.nrad = (int)r.size();
+    for (int i = 0; i < axis.nrad; ++i) axis.rad[i] = r[i];
+    axis.tw = tw;
+  }
+  void release() { dfree(tw); }
+};
+
+// The 3D real transform of a grid: passes z (rows), y (columns), x.
+template <typename T>
+struct Fft3D {
+  int K[3] = {0, 0, 0};
+  FftTables<T> ax[3];
+  void make(const int *k) {
+    for (int a = 0; a < 3; ++a) {
+      K[a] = k[a];
+      ax[a].make(k[a]);
+    }
+  }
+  void release() {
+    for (auto &a : ax) a.release();
+  }
+  static constexpr int TL = tile_lines<T>();
+  size_t lds(int k) const { return (size_t)2 * TL * k * sizeof(cpx<T>); }
+  int H() const { return K[2] / 2 + 1; }
+  // real (or 64-bit words) -> half spectrum
+  void forward(const void *src, bool fixed, cpx<T> *cplx, hipStream_t s) {
+    const int nrows = K[0] * K[1];
+    const int blocks = (nrows + 2 * TL - 1) / (2 * TL);
+    if constexpr (sizeof(T) == 4) {
+      if (fixed)
+        fft_rows_r2c<T, true><<<blocks, 256, lds(K[2]), s>>>(src, cplx, nrows, ax[2].axis);
+      else
+        fft_rows_r2c<T, false><<<blocks, 256, lds(K[2]), s>>>(src, cplx, nrows, ax[2].axis);
+    } else {
+      fft_rows_r2c<T, false><<<blocks, 256, lds(K[2]), s>>>(src, cplx, nrows, ax[2].axis);
+    }
+    const int tiles = (H() + TL - 1) / TL;
+    fft_cols_y<T><<<K[0] * tiles, 256, lds(K[1]), s>>>(cplx, K[1], H(), ax[1].axis, -1);
+  }
+  void x_pass(cpx<T> *cplx, int mode, const ConvTables &ct, double *epart, hipStream_t s) {
+    const int tiles = (H() + TL - 1) / TL;
+    fft_x_conv<T><<<K[1] * tiles, 256, lds(K[0]), s>>>(cplx, K[0], K[1], H(), ax[0].axis,
+                                                         ct, mode, epart);
+  }
+  int x_blocks() const { return K[1] * ((H() + TL - 1) / TL); }
+  void backward(cpx<T> *cplx, T *dst, hipStream_t s) {
+    const int tiles = (H() + TL - 1) / TL;
+    fft_cols_y<T><<<K[0] * tiles, 256, lds(K[1]), s>>>(cplx, K[1], H(), ax[1].axis, +1);
+    const int nrows = K[0] * K[1];
+    const int blocks = (nrows + 2 * TL - 1) / (2 * TL);
+    fft_rows_c2r<T><<<blocks, 256, lds(K[2]), s>>>(cplx, dst, nrows, ax[2].axis);
+  }
+};
+
+static inline int blocks_for(size_t n, int b = 256) { return (int)((n + b - 1) / b); }
+
+struct Engine {
+  // ---- topology ----
+  int n = 0, ntypes = 0, nunits = 0;
+  double box[3] = {0, 0, 0};
+  double cutoff = 0, switch_from = 0, skin = 0.1, beta = 0;
+  int lj_modifier = 0, coulomb_shift = 0, dispersion = 0;
+  int pme_grid[3] = {0, 0, 0}, pme_order = 4;
+  std::vector<double> h_q, h_mass, h_sigma, h_eps;
+  std::vector<int> h_type;
+  std::vector<int> h_excl_pairs;  // flat i<j
+  double e_self = 0, e_disp = 0;
+
+  hipStream_t stream = nullptr;
+
+  // ---- state (atom order) ----
+  double *d_x = nullptr, *d_v = nullptr, *d_xref = nullptr, *d_shift = nullptr;
+  double *d_q = nullptr, *d_inv_mass = nullptr;
+  int *d_type = nullptr, *d_place_of = nullptr;
+  double *d_save_x = nullptr, *d_save_v = nullptr;
+
+  // ---- places, clusters, list ----
+  GridParams g{};
+  int max_places = 0;
+  int *d_atom_at = nullptr, *d_type_at = nullptr;
+  float4 *d_xq = nullptr;
+  long long *d_F = nullptr;
+  int *d_col_count = nullptr, *d_bucket = nullptr;
+  float4 *d_bb_lo = nullptr, *d_bb_hi = nullptr;
+  int *d_cluster_col = nullptr;
+  uint32_t *d_real_mask = nullptr;
+  int *d_col_cl_start = nullptr, *d_col_cl_count = nullptr;
+  int *d_counts = nullptr;
+  PairEntry *d_rows = nullptr;
+  int *d_row_count = nullptr;
+  int *d_excl_start = nullptr, *d_excl_atoms = nullptr;
+  int *d_flags = nullptr;
+  double list_reach = 0;  // rlist of the current list configuration
+
+  // ---- nonbonded ----
+  NbParams nbp{};
+  float2 *d_lj = nullptr;
+  double *d_epair = nullptr;
+
+  // ---- special pairs ----
+  int nspecial = 0;
+  int2 *d_special = nullptr;
+  int *d_is14 = nullptr, *d_isexcl = nullptr;
+  double *d_p14_sigma = nullptr, *d_p14_eps = nullptr, *d_p14_qq = nullptr;
+  SpecialParams spp{};
+  double *d_espec = nullptr;
+
+  // ---- PME ----
+  bool use_pme = false;
+  PmeParams pmp{};
+  Fft3D<float> fft;
+  unsigned long long *d_grid = nullptr;
+  cpx<float> *d_cplx = nullptr;
+  float *d_phi = nullptr;
+  float *d_tab = nullptr;
+  ConvTables ct{};
+  double *d_erecip = nullptr;
+
+  // ---- integration ----
+  int *d_unit_first = nullptr, *d_unit_size = nullptr;
+  IntParams ip{};
+  double *d_kin = nullptr;
+  double dt_graph = -1;
+
+  bool forces_valid = false;
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_steps = 0;
+  long long builds_seen = 0;
+
+  ~Engine() { release(); }
+
+  void release() {
+    if (graph_exec) hipGraphExecDestroy(graph_exec);
+    graph_exec = nullptr;
+    dfree(d_x); dfree(d_v); dfree(d_xref); dfree(d_shift); dfree(d_q);
+    dfree(d_inv_mass); dfree(d_type); dfree(d_place_of); dfree(d_save_x); dfree(d_save_v);
+    release_list();
+    dfree(d_excl_start); dfree(d_excl_atoms); dfree(d_flags); dfree(d_lj);
+    dfree(d_special); dfree(d_is14); dfree(d_isexcl); dfree(d_p14_sigma);
+    dfree(d_p14_eps); dfree(d_p14_qq); dfree(d_espec);
+    fft.release();
+    dfree(d_grid); dfree(d_cplx); dfree(d_phi); dfree(d_tab); dfree(d_erecip);
+    dfree(d_unit_first); dfree(d_unit_size); dfree(d_kin);
+    if (stream) hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+
+  void release_list() {
+    dfree(d_atom_at); dfree(d_type_at); dfree(d_xq); dfree(d_F); dfree(d_col_count);
+    dfree(d_bucket); dfree(d_bb_lo); dfree(d_bb_hi); dfree(d_cluster_col);
+    dfree(d_real_mask); dfree(d_col_cl_start); dfree(d_col_cl_count); dfree(d_counts);
+    dfree(d_rows); dfree(d_row_count); dfree(d_epair);
+  }
+
+  // Column grid, capacities and the list buffers for a search reach.
+  void configure_list(double reach, int col_cap_min = 0, int row_cap_min = 0) {
+    const double volume = box[0] * box[1] * box[2];
+    const double rho = n / volume;
+    const double target = std::cbrt((double)kCluster / rho);
+    GridParams ng{};
+    for (int k = 0; k < 3; ++k) {
+      ng.box[k] = box[k];
+      ng.inv_box[k] = 1.0 / box[k];
+      ng.half_box[k] = 0.5 * box[k];
+    }
+    ng.ncx = std::max(1, (int)floor(box[0] / target));
+    ng.ncy = std::max(1, (int)floor(box[1] / target));
+    ng.ncol = ng.ncx * ng.ncy;
+    ng.wx = box[0] / ng.ncx;
+    ng.wy = box[1] / ng.ncy;
+    const double per_col = (double)n / ng.ncol;
+    int cap = (int)(2.0 * per_col + 128.0);
+    cap = std::max(cap, col_cap_min);
+    cap = (cap + 63) & ~63;
+    if (cap > kSortCapMax) cap = kSortCapMax;
+    ng.col_cap = cap;
+    const double rb = reach + 1e-4;
+    ng.rlist2 = (float)(rb * rb);
+    ng.rx = (int)(rb / ng.wx) + 1;
+    ng.ry = (int)(rb / ng.wy) + 1;
+    const int places = n + (kCluster - 1) * ng.ncol + kCluster;
+    ng.max_clusters = places / kCluster + 1;
+    // entries of a row: clusters whose boxes are within the reach
+    const double h = (double)kCluster / (rho * ng.wx * ng.wy);
+    double est = (ng.wx + 2 * rb) * (ng.wy + 2 * rb) * (std::min(h, box[2]) + 2 * rb) /
+                 (ng.wx * ng.wy * h);
+    est = std::min(est, (double)ng.max_clusters * 3.0);
+    int rc_ = (int)(1.5 * est + 64);
+    rc_ = std::max(rc_, row_cap_min);
+    ng.row_cap = (rc_ + 31) & ~31;
+    ng.max_excl = 16;
+    const bool same = d_rows && ng.ncol == g.ncol && ng.col_cap == g.col_cap &&
+                      ng.row_cap == g.row_cap && ng.max_clusters == g.max_clusters;
+    g = ng;
+    list_reach = reach;
+    if (same) return;
+    release_list();
+    max_places = g.max_clusters * kCluster;
+    d_atom_at = dalloc<int>(max_places);
+    HIPCHK(hipMemset(d_atom_at, 0xFF, max_places * sizeof(int)));  // -1: no atom
+    d_type_at = dalloc<int>(max_places);
+    d_xq = dalloc<float4>(max_places);
+    d_F = dalloc<long long>(3 * (size_t)max_places);
+    d_col_count = dalloc<int>(g.ncol);
+    d_bucket = dalloc<int>((size_t)g.ncol * g.col_cap);
+    d_bb_lo = dalloc<float4>(g.max_clusters);
+    d_bb_hi = dalloc<float4>(g.max_clusters);
+    d_cluster_col = dalloc<int>(g.max_clusters);
+    d_real_mask = dalloc<uint32_t>(g.max_clusters);
+    d_col_cl_start = dalloc<int>(g.ncol);
+    d_col_cl_count = dalloc<int>(g.ncol);
+    d_counts = dalloc<int>(4);
+    d_rows = dalloc<PairEntry>((size_t)g.max_clusters * g.row_cap);
+    d_row_count = dalloc<int>(g.max_clusters);
+    d_epair = dalloc<double>(2 * (size_t)g.max_clusters);
+    forces_valid = false;
+    if (graph_exec) {
+      hipGraphExecDestroy(graph_exec);
+      graph_exec = nullptr;
+    }
+  }
+
+  void create(const mdrtHipSystemDesc *d) {
+    n = (int)d->n;
+    if (n <= 0) throw HipError("a system needs particles");
+    ntypes = d->ntypes;
+    if (ntypes < 1 || ntypes > kMaxTypes)
+      throw HipError("ntypes must be in [1, " + std::to_string(kMaxTypes) + "]");
+    for (int k = 0; k < 3; ++k) box[k] = d->box[k];
+    cutoff = d->cutoff;
+    switch_from = d->switch_from;
+    skin = d->skin > 0 ? d->skin : 0.1;
+    beta = d->beta;
+    lj_modifier = d->lj_modifier;
+    coulomb_shift = d->coulomb_shift;
+    dispersion = d->dispersion;
+    pme_order = d->pme_order > 0 ? d->pme_order : 4;
+    for (int k = 0; k < 3; ++k) pme_grid[k] = d->pme_grid[k];
+    const double rl = cutoff + skin;
+    for (int k = 0; k < 3; ++k)
+      if (box[k] <= 2.0 * rl)
+        throw HipError("every box edge must exceed 2 (cutoff + skin)");
+    if (lj_modifier < 0 || lj_modifier > 3) throw HipError("unknown Lennard-Jones modifier");
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+
+    h_q.assign(d->charges, d->charges + n);
+    h_mass.assign(d->masses, d->masses + n);
+    h_type.assign(d->types, d->types + n);
+    h_sigma.assign(d->sigma, d->sigma + ntypes * ntypes);
+    h_eps.assign(d->epsilon, d->epsilon + ntypes * ntypes);
+    for (int t : h_type)
+      if (t < 0 || t >= ntypes) throw HipError("type index out of range");
+
+    d_x = dalloc<double>(3 * (size_t)n);
+    d_v = dalloc<double>(3 * (size_t)n);
+    d_xref = dalloc<double>(3 * (size_t)n);
+    d_shift = dalloc<double>(3 * (size_t)n);
+    d_save_x = dalloc<double>(3 * (size_t)n);
+    d_save_v = dalloc<double>(3 * (size_t)n);
+    HIPCHK(hipMemcpy(d_x, d->positions, 3 * n * sizeof(double), hipMemcpyHostToDevice));
+    if (d->velocities)
+      HIPCHK(hipMemcpy(d_v, d->velocities, 3 * n * sizeof(double), hipMemcpyHostToDevice));
+    d_q = dupload(h_q);
+    std::vector<double> im(n);
+    for (int i = 0; i < n; ++i) im[i] = h_mass[i] > 0 ? 1.0 / h_mass[i] : 0.0;
+    d_inv_mass = dupload(im);
+    d_type = dupload(h_type);
+    d_place_of = dalloc<int>(n);
+    d_flags = dalloc<int>(kNumFlags);
+
+    // exclusions: symmetric CSR and the i<j list
+    std::vector<std::vector<int>> rows(n);
+    for (int64_t e = 0; e < d->nexclusions; ++e) {
+      int i = d->exclusions[2 * e], j = d->exclusions[2 * e + 1];
+      if (i < 0 || j < 0 || i >= n || j >= n || i == j) throw HipError("bad exclusion");
+      if (i > j) std::swap(i, j);
+      rows[i].push_back(j);
+      rows[j].push_back(i);
+      h_excl_pairs.push_back(i);
+      h_excl_pairs.push_back(j);
+    }
+    std::vector<int> es(n + 1, 0), ea;
+    for (int i = 0; i < n; ++i) {
+      std::sort(rows[i].begin(), rows[i].end());
+      if ((int)rows[i].size() > 16) throw HipError("more than 16 exclusions of one particle");
+      es[i + 1] = es[i] + (int)rows[i].size();
+      for (int j : rows[i]) ea.push_back(j);
+    }
+    d_excl_start = dupload(es);
+    d_excl_atoms = dupload(ea);
+
+    // Lennard-Jones table (c6 = 4 eps sigma^6, c12 = 4 eps sigma^12)
+    std::vector<float2> lj(ntypes * ntypes);
+    for (int k = 0; k < ntypes * ntypes; ++k) {
+      const double s6 = pow(h_sigma[k], 6.0);
+      lj[k] = make_float2((float)(4.0 * h_eps[k] * s6), (float)(4.0 * h_eps[k] * s6 * s6));
+    }
+    d_lj = dupload(lj);
+
+    // nonbonded constants
+    for (int k = 0; k < 3; ++k) {
+      nbp.box[k] = (float)box[k];
+      nbp.half_box[k] = (float)(0.5 * box[k]);
+    }
+    nbp.rc = (float)cutoff;
+    nbp.rc2 = (float)(cutoff * cutoff);
+    nbp.rs = (float)switch_from;
+    nbp.beta = (float)beta;
+    nbp.coulomb = (float)kCoulomb;
+    nbp.shift_c = (coulomb_shift && beta > 0) ? (float)(erfc(beta * cutoff) / cutoff) : 0.0f;
+    nbp.ntypes = ntypes;
+    nbp.lj_modifier = lj_modifier;
+    nbp.shift12 = (float)pow(cutoff, -12.0);
+    nbp.shift6 = (float)pow(cutoff, -6.0);
+    if (lj_modifier == kLJForceSwitch) {
+      const double rc = cutoff, rs = switch_from, w = rc - rs;
+      if (!(w > 0)) throw HipError("force switch needs switch_from < cutoff");
+      double A[2], Bc[2], Cc[2];
+      const double ns[2] = {12.0, 6.0};
+      for (int k = 0; k < 2; ++k) {
+        const double nn = ns[k];
+        A[k] = -nn * ((nn + 4.0) * rc - (nn + 1.0) * rs) / (pow(rc, nn + 2.0) * w * w);
+        Bc[k] = nn * ((nn + 3.0) * rc - (nn + 1.0) * rs) / (pow(rc, nn + 2.0) * w * w * w);
+        Cc[k] = pow(rc, -nn) - A[k] / 3.0 * pow(w, 3.0) - Bc[k] / 4.0 * pow(w, 4.0);
+      }
+      nbp.sw_a12 = (float)A[0]; nbp.sw_b12 = (float)Bc[0]; nbp.sw_c12 = (float)Cc[0];
+      nbp.sw_a6 = (float)A[1]; nbp.sw_b6 = (float)Bc[1]; nbp.sw_c6 = (float)Cc[1];
+    }
+    if (lj_modifier == kLJPotentialSwitch && !(switch_from < cutoff))
+      throw HipError("potential switch needs switch_from < cutoff");
+
+    // special pairs: exclusions (Ewald correction) and 1-4 pairs
+    {
+      std::vector<int2> sp;
+      std::vector<int> is14, isex;
+      for (size_t e = 0; e < h_excl_pairs.size() / 2; ++e) {
+        sp.push_back(make_int2(h_excl_pairs[2 * e], h_excl_pairs[2 * e + 1]));
+        is14.push_back(-1);
+        isex.push_back(1);
+      }
+      for (int64_t p = 0; p < d->npairs14; ++p) {
+        int i = d->pairs14[2 * p], j = d->pairs14[2 * p + 1];
+        if (i > j) std::swap(i, j);
+        bool found = false;
+        for (size_t e = 0; e < sp.size(); ++e)
+          if (sp[e].x == i && sp[e].y == j && isex[e]) {
+            is14[e] = (int)p;
+            found = true;
+            break;
+          }
+        if (!found)
+          throw HipError("a 1-4 pair must also be an excluded pair");
+      }
+      nspecial = (int)sp.size();
+      d_special = dupload(sp);
+      d_is14 = dupload(is14);
+      d_isexcl = dupload(isex);
+      std::vector<double> s14(d->npairs14), e14(d->npairs14), q14(d->npairs14);
+      for (int64_t p = 0; p < d->npairs14; ++p) {
+        s14[p] = d->pairs14_sigma[p];
+        e14[p] = d->pairs14_eps[p];
+        q14[p] = d->pairs14_qq[p];
+      }
+      d_p14_sigma = dupload(s14);
+      d_p14_eps = dupload(e14);
+      d_p14_qq = dupload(q14);
+      for (int k = 0; k < 3; ++k) spp.box[k] = box[k];
+      spp.beta = beta;
+      spp.coulomb = kCoulomb;
+      spp.rc = cutoff;
+      spp.rs = switch_from;
+      spp.lj_modifier = lj_modifier;
+      d_espec = dalloc<double>(3 * (size_t)std::max(1, blocks_for(nspecial)));
+    }
+
+    // host constants: self + background, dispersion correction
+    {
+      double sq = 0, net = 0;
+      for (double qq : h_q) {
+        sq += qq * qq;
+        net += qq;
+      }
+      const double volume = box[0] * box[1] * box[2];
+      e_self = beta > 0 ? -kCoulomb * beta / sqrt(M_PI) * sq -
+                              kCoulomb * M_PI * net * net / (2.0 * volume * beta * beta)
+                        : 0.0;
+      e_disp = 0;
+      if (dispersion) {
+        std::vector<double> num(ntypes, 0.0);
+        for (int t : h_type) num[t] += 1.0;
+        double sum = 0;
+        for (int a = 0; a < ntypes; ++a)
+          for (int b = 0; b < ntypes; ++b)
+            sum += num[a] * (num[b] - (a == b ? 1.0 : 0.0)) * 4.0 * h_eps[a * ntypes + b] *
+                   pow(h_sigma[a * ntypes + b], 6.0);
+        const size_t ne = h_excl_pairs.size() / 2;
+        for (size_t e = 0; e < ne; ++e) {
+          const int a = h_type[h_excl_pairs[2 * e]], b = h_type[h_excl_pairs[2 * e + 1]];
+          sum -= 2.0 * 4.0 * h_eps[a * ntypes + b] * pow(h_sigma[a * ntypes + b], 6.0);
+        }
+        const double nn = n, pairs = nn * (nn - 1.0) - 2.0 * (double)ne;
+        const double mean = pairs > 0 ? sum / pairs : 0.0;
+        e_disp = -2.0 * M_PI / (3.0 * volume) * nn * nn * mean / (cutoff * cutoff * cutoff);
+      }
+    }
+
+    // PME
+    use_pme = beta > 0;
+    if (use_pme) {
+      for (int k = 0; k < 3; ++k) {
+        if (pme_grid[k] < 2 * pme_order) throw HipError("PME grid smaller than twice the order");
+        if (pme_grid[k] > 1024) throw HipError("PME grid edge above 1024");
+        pmp.K[k] = pme_grid[k];
+        pmp.box[k] = box[k];
+        pmp.inv_box[k] = 1.0 / box[k];
+      }
+      if (pme_order > 8 || pme_order < 3) throw HipError("PME order must be in [3, 8]");
+      pmp.order = pme_order;
+      fft.make(pme_grid);
+      const size_t np = (size_t)pme_grid[0] * pme_grid[1] * pme_grid[2];
+      const size_t nh = (size_t)pme_grid[0] * pme_grid[1] * (pme_grid[2] / 2 + 1);
+      d_grid = dalloc<unsigned long long>(np);
+      d_cplx = dalloc<cpx<float>>(nh);
+      d_phi = dalloc<float>(np);
+      make_influence_tables();
+      d_erecip = dalloc<double>(fft.x_blocks());
+    }
+
+    // integration units: waters, then the other particles
+    {
+      std::vector<int> uf, us;
+      std::vector<char> in_water(n, 0);
+      for (int64_t w = 0; w < d->nwaters; ++w) {
+        const int o = d->water_first[w];
+        if (o < 0 || o + 2 >= n) throw HipError("bad water index");
+        uf.push_back(o);
+        us.push_back(3);
+        in_water[o] = in_water[o + 1] = in_water[o + 2] = 1;
+      }
+      for (int i = 0; i < n; ++i)
+        if (!in_water[i]) {
+          uf.push_back(i);
+          us.push_back(1);
+        }
+      nunits = (int)uf.size();
+      d_unit_first = dupload(uf);
+      d_unit_size = dupload(us);
+      d_kin = dalloc<double>(blocks_for(nunits));
+      for (int k = 0; k < 3; ++k) ip.half_box[k] = 0.5 * box[k];
+      ip.skin_half2 = 0.25 * skin * skin;
+      if (d->nwaters > 0) {
+        const int o = d->water_first[0];
+        ip.m_o = h_mass[o];
+        ip.m_h = h_mass[o + 1];
+        ip.d_oh = d->d_oh;
+        ip.d_hh = d->d_hh;
+        const double total = ip.m_o + 2 * ip.m_h;
+        ip.rc = 0.5 * ip.d_hh;
+        const double height = sqrt(ip.d_oh * ip.d_oh - ip.rc * ip.rc);
+        ip.ra = 2.0 * ip.m_h * height / total;
+        ip.rb = height - ip.ra;
+      }
+    }
+    configure_list(cutoff + skin);
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+
+  void make_influence_tables() {
+    const int Kmax = std::max({pme_grid[0], pme_grid[1], pme_grid[2]});
+    std::vector<float> tab(6 * (size_t)Kmax, 0.0f);
+    const double volume = box[0] * box[1] * box[2];
+    for (int a = 0; a < 3; ++a) {
+      const int K = pme_grid[a];
+      // |b(k)|^2 moduli (Influence.cpp:168-191)
+      std::vector<double> m(pme_order + 1, 0.0), next(pme_order + 1, 0.0);
+      m[1] = 1.0;
+      for (int nn = 3; nn <= pme_order; ++nn) {
+        std::fill(next.begin(), next.end(), 0.0);
+        for (int xx = 1; xx < nn; ++xx) next[xx] = (xx * m[xx] + (nn - xx) * m[xx - 1]) / (nn - 1);
+        m.swap(next);
+      }
+      for (int k = 0; k < K; ++k) {
+        double re = 0, im = 0;
+        for (int j = 0; j <= pme_order - 2; ++j) {
+          const double ang = 2.0 * M_PI * (double)k * j / (double)K;
+          re += m[j + 1] * cos(ang);
+          im += m[j + 1] * sin(ang);
+        }
+        const double modulus = 1.0 / (re * re + im * im);
+        const int ks = k <= K / 2 ? k : k - K;
+        const double mk = (double)ks / box[a];
+        double t = exp(-M_PI * M_PI * mk * mk / (beta * beta)) * modulus;
+        if (a == 0) t *= kCoulomb / (M_PI * volume);
+        tab[a * Kmax + k] = (float)t;
+        tab[(3 + a) * Kmax + k] = (float)mk;
+      }
+    }
+    d_tab = dupload(tab);
+    for (int a = 0; a < 3; ++a) {
+      ct.t[a] = d_tab + a * Kmax;
+      ct.m[a] = d_tab + (3 + a) * Kmax;
+    }
+    ct.K3 = pme_grid[2];
+  }
+
+  // ---- launches ----
+
+  void request_build() { nb_request_build<<<1, 64, 0, stream>>>(d_flags); }
+
+  void build() {
+    nb_bin<<<blocks_for(n), 256, 0, stream>>>(n, d_x, d_shift, d_xref, d_col_count,
+                                               d_bucket, d_flags, g);
+    nb_sort<<<g.ncol, 256, 0, stream>>>(d_x, d_shift, d_q, d_type, d_col_count, d_bucket,
+                                         d_atom_at, d_place_of, d_xq, d_type_at, d_bb_lo,
+                                         d_bb_hi, d_cluster_col, d_real_mask, d_col_cl_start,
+                                         d_col_cl_count, d_counts, d_flags, g);
+    nb_search<<<blocks_for(g.max_clusters, 4), 256, 0, stream>>>(
+        d_bb_lo, d_bb_hi, d_cluster_col, d_real_mask, d_col_cl_start, d_col_cl_count,
+        d_atom_at, d_place_of, d_excl_start, d_excl_atoms, d_counts, d_col_count, d_rows,
+        d_row_count, d_flags, g);
+  }
+
+  template <int MOD>
+  void launch_pairs(bool energy) {
+    const int blocks = blocks_for(g.max_clusters, 4);
+    const bool coul = beta > 0;
+    if (coul && energy)
+      nb_pairs<MOD, true, true><<<blocks, 256, 0, stream>>>(d_xq, d_type_at, d_rows, d_row_count, g.row_cap, d_counts, d_lj, nbp, d_F, d_epair, d_flags);
+    else if (coul)
+      nb_pairs<MOD, true, false><<<blocks, 256, 0, stream>>>(d_xq, d_type_at, d_rows, d_row_count, g.row_cap, d_counts, d_lj, nbp, d_F, d_epair, d_flags);
+    else if (energy)
+      nb_pairs<MOD, false, true><<<blocks, 256, 0, stream>>>(d_xq, d_type_at, d_rows, d_row_count, g.row_cap, d_counts, d_lj, nbp, d_F, d_epair, d_flags);
+    else
+      nb_pairs<MOD, false, false><<<blocks, 256, 0, stream>>>(d_xq, d_type_at, d_rows, d_row_count, g.row_cap, d_counts, d_lj, nbp, d_F, d_epair, d_flags);
+  }
+
+  void pairs(bool energy) {
+    switch (lj_modifier) {
+      case kLJNone: launch_pairs<kLJNone>(energy); break;
+      case kLJPotentialShift: launch_pairs<kLJPotentialShift>(energy); break;
+      case kLJForceSwitch: launch_pairs<kLJForceSwitch>(energy); break;
+      default: launch_pairs<kLJPotentialSwitch>(energy); break;
+    }
+  }
+
+  void special(bool energy) {
+    if (nspecial == 0) return;
+    nb_special<<<blocks_for(nspecial), 256, 0, stream>>>(
+        nspecial, d_special, d_is14, d_p14_sigma, d_p14_eps, d_p14_qq, d_x, d_q,
+        d_place_of, spp, d_F, energy ? d_espec : nullptr);
+  }
+
+  void pme_spread_launch() {
+    const int nn = pme_order * pme_order;
+    pme_spread<<<blocks_for((size_t)max_places * nn), 256, 0, stream>>>(
+        d_counts, d_atom_at, d_x, d_q, pmp, d_grid);
+  }
+
+  void pme(bool energy) {
+    if (!use_pme) return;
+    pme_spread_launch();
+    fft.forward(d_grid, true, d_cplx, stream);
+    fft.x_pass(d_cplx, 0, ct, energy ? d_erecip : nullptr, stream);
+    fft.backward(d_cplx, d_phi, stream);
+    pme_gather<<<blocks_for(max_places), 256, 0, stream>>>(d_counts, d_atom_at, d_x, d_q,
+                                                           pmp, d_phi, d_F);
+  }
+
+  void forces(bool energy) {
+    pairs(energy);
+    special(energy);
+    pme(energy);
+  }
+
+  void clear_forces() {
+    clear_words<<<256, 256, 0, stream>>>(d_F, 3 * (size_t)max_places);
+  }
+
+  void step_first(double dt) {
+    ip.dt = dt;
+    integrate_first<<<blocks_for(nunits), 256, 0, stream>>>(
+        nunits, d_unit_first, d_unit_size, d_x, d_v, d_inv_mass, d_F, d_place_of, d_shift,
+        d_xref, d_q, d_xq, ip, d_flags);
+  }
+
+  void step_second(double dt, bool kinetic) {
+    ip.dt = dt;
+    integrate_second<<<blocks_for(nunits), 256, 0, stream>>>(
+        nunits, d_unit_first, d_unit_size, d_x, d_v, d_inv_mass, d_F, d_place_of, ip,
+        kinetic ? d_kin : nullptr);
+  }
+
+  void one_step(double dt) {
+    step_first(dt);
+    build();
+    forces(false);
+    step_second(dt, false);
+  }
+
+  std::vector<int> read_flags() {
+    std::vector<int> f(kNumFlags);
+    HIPCHK(hipMemcpyAsync(f.data(), d_flags, kNumFlags * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return f;
+  }
+
+  void clear_error_flags() {
+    std::vector<int> z(kNumFlags, 0);
+    HIPCHK(hipMemcpyAsync(d_flags, z.data(), kNumFlags * sizeof(int), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+
+  // Overflow of a column bucket or a row: grow and report true.
+  bool grow_if_overflowed(const std::vector<int> &f) {
+    if (f[kFlagOverflowRow] & 2) throw HipError("more than 128 exclusions within one cluster");
+    if (!f[kFlagOverflowColumn] && !f[kFlagOverflowRow]) return false;
+    int col_min = f[kFlagOverflowColumn] ? g.col_cap * 2 : g.col_cap;
+    int row_min = f[kFlagOverflowRow] ? g.row_cap * 2 : g.row_cap;
+    if (col_min > kSortCapMax && f[kFlagOverflowColumn])
+      throw HipError("a column holds more than 1024 particles");
+    configure_list(list_reach, col_min, row_min);
+    clear_error_flags();
+    return true;
+  }
+
+  void evaluate(bool energy) {
+    for (int attempt = 0; attempt < 8; ++attempt) {
+      clear_forces();
+      request_build();
+      build();
+      forces(energy);
+      const std::vector<int> f = read_flags();
+      if (!grow_if_overflowed(f)) {
+        forces_valid = true;
+        return;
+      }
+    }
+    throw HipError("the neighbor list kept overflowing");
+  }
+
+  void compute(double *F_out, double *E_out) {
+    if (list_reach != cutoff + skin) configure_list(cutoff + skin);
+    evaluate(true);
+    if (F_out) {
+      double *d_out = dalloc<double>(3 * (size_t)n);
+      forces_to_atoms<<<blocks_for(n), 256, 0, stream>>>(n, d_place_of, d_F, d_out);
+      HIPCHK(hipMemcpyAsync(F_out, d_out, 3 * n * sizeof(double), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      dfree(d_out);
+    }
+    if (E_out) {
+      for (int k = 0; k < MDRT_NUM_ENERGIES; ++k) E_out[k] = 0.0;
+      std::vector<int> counts(4);
+      HIPCHK(hipMemcpy(counts.data(), d_counts, 4 * sizeof(int), hipMemcpyDeviceToHost));
+      const int ncl = counts[1];
+      std::vector<double> ep(2 * (size_t)ncl);
+      if (ncl) HIPCHK(hipMemcpy(ep.data(), d_epair, ep.size() * sizeof(double), hipMemcpyDeviceToHost));
+      for (int c = 0; c < ncl; ++c) {
+        E_out[MDRT_E_LJ] += ep[2 * c];
+        E_out[MDRT_E_COUL_DIRECT] += ep[2 * c + 1];
+      }
+      if (nspecial) {
+        const int nb = blocks_for(nspecial);
+        std::vector<double> es(3 * (size_t)nb);
+        HIPCHK(hipMemcpy(es.data(), d_espec, es.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int b = 0; b < nb; ++b) {
+          E_out[MDRT_E_COUL_EXCL] += es[3 * b];
+          E_out[MDRT_E_LJ14] += es[3 * b + 1];
+          E_out[MDRT_E_COUL14] += es[3 * b + 2];
+        }
+      }
+      if (use_pme) {
+        std::vector<double> er(fft.x_blocks());
+        HIPCHK(hipMemcpy(er.data(), d_erecip, er.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (double e : er) E_out[MDRT_E_COUL_RECIP] += e;
+      }
+      E_out[MDRT_E_COUL_SELF] = e_self;
+      E_out[MDRT_E_DISP_CORR] = e_disp;
+      E_out[MDRT_E_KINETIC] = kinetic_energy();
+    }
+  }
+
+  double kinetic_energy() {
+    std::vector<double> v(3 * (size_t)n);
+    HIPCHK(hipMemcpy(v.data(), d_v, v.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double ke = 0;
+    for (int i = 0; i < n; ++i)
+      ke += 0.5 * h_mass[i] * (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
+    return ke;
+  }
+
+  void capture(int steps, double dt) {
+    if (graph_exec) {
+      hipGraphExecDestroy(graph_exec);
+      graph_exec = nullptr;
+    }
+    hipGraph_t graph;
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    for (int s = 0; s < steps; ++s) one_step(dt);
+    HIPCHK(hipStreamEndCapture(stream, &graph));
+    HIPCHK(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(graph));
+    graph_steps = steps;
+    dt_graph = dt;
+  }
+
+  void run(int64_t steps, double dt, int graph, double *kinetic) {
+    if (list_reach != cutoff + skin) configure_list(cutoff + skin);
+    if (!forces_valid) evaluate(false);
+    for (int attempt = 0; attempt < 8; ++attempt) {
+      HIPCHK(hipMemcpyAsync(d_save_x, d_x, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+      HIPCHK(hipMemcpyAsync(d_save_v, d_v, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+      int64_t done = 0;
+      if (graph > 0) {
+        if (!graph_exec || graph_steps != graph || dt_graph != dt) capture(graph, dt);
+        while (done + graph <= steps) {
+          HIPCHK(hipGraphLaunch(graph_exec, stream));
+          done += graph;
+        }
+      }
+      for (; done < steps; ++done) one_step(dt);
+      const std::vector<int> f = read_flags();
+      if (!grow_if_overflowed(f)) break;
+      // restore and redo from the saved state with the larger list
+      HIPCHK(hipMemcpyAsync(d_x, d_save_x, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+      HIPCHK(hipMemcpyAsync(d_v, d_save_v, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+      evaluate(false);
+      if (attempt == 7) throw HipError("the neighbor list kept overflowing");
+    }
+    HIPCHK(hipStreamSynchronize(stream));
+    if (kinetic) *kinetic = kinetic_energy();
+  }
+
+  void minimize(int64_t steps, double max_step, double *energy_out) {
+    double E[MDRT_NUM_ENERGIES];
+    std::vector<double> F(3 * (size_t)n);
+    compute(F.data(), E);
+    auto potential = [](const double *e) {
+      double s = 0;
+      for (int k = 0; k < MDRT_E_KINETIC; ++k) s += e[k];
+      return s;
+    };
+    double e0 = potential(E);
+    double h = max_step;
+    double *d_x0 = dalloc<double>(3 * (size_t)n);
+    for (int64_t s = 0; s < steps; ++s) {
+      double fmax = 0;
+      for (double f : F) fmax = std::max(fmax, fabs(f));
+      if (fmax == 0) break;
+      HIPCHK(hipMemcpyAsync(d_x0, d_x, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+      // the force words of the current positions are still in d_F
+      descend<<<blocks_for(nunits), 256, 0, stream>>>(nunits, d_unit_first, d_unit_size, d_x0,
+                                                      d_x, d_F, d_place_of, h / fmax, ip);
+      std::vector<double> F1(3 * (size_t)n);
+      double E1[MDRT_NUM_ENERGIES];
+      compute(F1.data(), E1);
+      const double e1 = potential(E1);
+      if (e1 < e0) {
+        e0 = e1;
+        F.swap(F1);
+        h *= 1.2;
+      } else {
+        HIPCHK(hipMemcpyAsync(d_x, d_x0, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+        compute(F.data(), E);
+        h *= 0.2;
+      }
+    }
+    dfree(d_x0);
+    forces_valid = false;
+    if (energy_out) *energy_out = e0;
+  }
+
+  int64_t neighbor_pairs(double reach, int excluded, int64_t capacity, int32_t *out) {
+    const double margin = 3.0e-6 * ((box[0] + box[1]) + box[2]);
+    for (int k = 0; k < 3; ++k)
+      if (box[k] <= 2.0 * (reach + margin + 1e-4))
+        throw HipError("the reach must be below half of every box edge");
+    // a list without exclusions when they are kept
+    int *keep_start = d_excl_start;
+    int *empty = nullptr;
+    if (!excluded) {
+      empty = dalloc<int>(n + 1);
+      d_excl_start = empty;
+    }
+    configure_list(reach + margin);
+    std::vector<int> f;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+      request_build();
+      build();
+      clear_forces();
+      pairs(false);  // clears the request
+      f = read_flags();
+      if (!grow_if_overflowed(f)) break;
+    }
+    d_excl_start = keep_start;
+    dfree(empty);
+    const double far = reach + margin;
+    const float limit2 = (float)(far * far);
+    const unsigned long long cap2 = (unsigned long long)std::max<int64_t>(capacity, 1) * 4 + 1024;
+    int2 *d_pairs = dalloc<int2>(cap2);
+    unsigned long long *d_count = dalloc<unsigned long long>(1);
+    nb_export_pairs<<<blocks_for(g.max_clusters, 4), 256, 0, stream>>>(
+        d_x, d_atom_at, d_rows, d_row_count, d_counts, g.row_cap, (float)box[0],
+        (float)box[1], (float)box[2], box[0], box[1], box[2], limit2, d_pairs, d_count, cap2);
+    unsigned long long cnt = 0;
+    HIPCHK(hipMemcpyAsync(&cnt, d_count, sizeof(cnt), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    if (cnt > cap2) cnt = cap2;
+    std::vector<int2> h(cnt);
+    if (cnt) HIPCHK(hipMemcpy(h.data(), d_pairs, cnt * sizeof(int2), hipMemcpyDeviceToHost));
+    dfree(d_pairs);
+    dfree(d_count);
+    std::vector<long long> keys(cnt);
+    for (size_t i = 0; i < cnt; ++i) keys[i] = ((long long)h[i].x << 32) | (unsigned)h[i].y;
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    const int64_t total = (int64_t)keys.size();
+    for (int64_t i = 0; i < std::min(total, capacity); ++i) {
+      out[2 * i] = (int32_t)(keys[i] >> 32);
+      out[2 * i + 1] = (int32_t)(keys[i] & 0xffffffff);
+    }
+    configure_list(cutoff + skin);
+    forces_valid = false;
+    return total;
+  }
+
+  void stats(int64_t *s) {
+    std::vector<int> counts(4);
+    HIPCHK(hipMemcpy(counts.data(), d_counts, 4 * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<int> rc(g.max_clusters);
+    HIPCHK(hipMemcpy(rc.data(), d_row_count, rc.size() * sizeof(int), hipMemcpyDeviceToHost));
+    int64_t entries = 0, largest = 0;
+    for (int c = 0; c < counts[1]; ++c) {
+      entries += rc[c];
+      largest = std::max<int64_t>(largest, rc[c]);
+    }
+    s[0] = counts[0];
+    s[1] = counts[1];
+    s[2] = entries;
+    s[3] = largest;
+    s[4] = g.row_cap;
+    s[5] = g.ncol;
+    s[6] = g.col_cap;
+    s[7] = max_places;
+  }
+
+  // Device-event timing of the parts of one step.
+  void profile(int repeats, double *ms) {
+    if (!forces_valid) evaluate(false);
+    hipEvent_t ev[8];
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int r = 0; r < repeats; ++r) {
+      HIPCHK(hipEventRecord(ev[0], stream));
+      step_first(0.002);
+      HIPCHK(hipEventRecord(ev[1], stream));
+      build();
+      HIPCHK(hipEventRecord(ev[2], stream));
+      pairs(false);
+      special(false);
+      HIPCHK(hipEventRecord(ev[3], stream));
+      if (use_pme) pme_spread_launch();
+      HIPCHK(hipEventRecord(ev[4], stream));
+      if (use_pme) {
+        fft.forward(d_grid, true, d_cplx, stream);
+        fft.x_pass(d_cplx, 0, ct, nullptr, stream);
+        fft.backward(d_cplx, d_phi, stream);
+      }
+      HIPCHK(hipEventRecord(ev[5], stream));
+      if (use_pme)
+        pme_gather<<<blocks_for(max_places), 256, 0, stream>>>(d_counts, d_atom_at, d_x, d_q,
+                                                               pmp, d_phi, d_F);
+      HIPCHK(hipEventRecord(ev[6], stream));
+      step_second(0.002, false);
+      HIPCHK(hipEventRecord(ev[7], stream));
+      HIPCHK(hipEventSynchronize(ev[7]));
+      for (int k = 0; k < 7; ++k) {
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+        acc[k] += t;
+      }
+    }
+    // integrate (both halves), build, pairs+special, spread, fft, gather
+    ms[0] = (acc[0] + acc[6]) / repeats;
+    ms[1] = acc[1] / repeats;
+    ms[2] = acc[2] / repeats;
+    ms[3] = acc[3] / repeats;
+    ms[4] = acc[4] / repeats;
+    ms[5] = acc[5] / repeats;
+    ms[6] = 0;
+    ms[7] = 0;
+    for (auto &e : ev) hipEventDestroy(e);
+  }
+};
+
+}  // namespace mdhip
+
+using mdhip::Engine;
+using mdhip::g_last_error;
+
+template <typename F>
+static int guarded(F &&f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return -1;
+  }
+}
+
+// ---- FFT plans of the runtime ABI (mdrt_hip.c) -------------------------
+
+namespace {
+struct AbiPlan {
+  int64_t k[3];
+  int single;
+  int users;
+  mdhip::Fft3D<float> f32;
+  mdhip::Fft3D<double> f64;
+};
+std::vector<AbiPlan *> g_plans;
+
+AbiPlan *find_plan(int64_t k1, int64_t k2, int64_t k3, int single) {
+  for (AbiPlan *p : g_plans)
+    if (p->k[0] == k1 && p->k[1] == k2 && p->k[2] == k3 && p->single == single) return p;
+  AbiPlan *p = new AbiPlan();
+  p->k[0] = k1;
+  p->k[1] = k2;
+  p->k[2] = k3;
+  p->single = single;
+  p->users = 0;
+  const int K[3] = {(int)k1, (int)k2, (int)k3};
+  try {
+    if (single)
+      p->f32.make(K);
+    else
+      p->f64.make(K);
+  } catch (...) {
+    delete p;
+    throw;
+  }
+  g_plans.push_back(p);
+  return p;
+}
+}  // namespace
+
+extern "C" int mdhip_fft_execute(int direction, int single, void *in, void *out, int64_t k1,
+                                 int64_t k2, int64_t k3, const float *tables, void *stream) {
+  return guarded([&] {
+    AbiPlan *p = find_plan(k1, k2, k3, single);
+    hipStream_t s = (hipStream_t)stream;
+    mdhip::ConvTables ct{};
+    const int kmax = (int)std::max({k1, k2, k3});
+    if (tables)
+      for (int a = 0; a < 3; ++a) {
+        ct.t[a] = tables + a * kmax;
+        ct.m[a] = tables + (3 + a) * kmax;
+      }
+    ct.K3 = (int)k3;
+    if (single) {
+      auto &f = p->f32;
+      if (direction < 0) {
+        f.forward(in, false, (mdhip::cpx<float> *)out, s);
+        f.x_pass((mdhip::cpx<float> *)out, -1, ct, nullptr, s);
+      } else {
+        f.x_pass((mdhip::cpx<float> *)in, tables ? 2 : 1, ct, nullptr, s);
+        f.backward((mdhip::cpx<float> *)in, (float *)out, s);
+      }
+    } else {
+      if (tables) throw mdhip::HipError("the Gaussian backward transform is f32 only");
+      auto &f = p->f64;
+      if (direction < 0) {
+        f.forward(in, false, (mdhip::cpx<double> *)out, s);
+        f.x_pass((mdhip::cpx<double> *)out, -1, ct, nullptr, s);
+      } else {
+        f.x_pass((mdhip::cpx<double> *)in, 1, ct, nullptr, s);
+        f.backward((mdhip::cpx<double> *)in, (double *)out, s);
+      }
+    }
+    HIPCHK(hipGetLastError());
+  });
+}
+
+extern "C" int mdhip_fft_retain(int64_t k1, int64_t k2, int64_t k3, int single) {
+  int handle = -1;
+  if (guarded([&] {
+        AbiPlan *p = find_plan(k1, k2, k3, single);
+        ++p->users;
+        for (size_t i = 0; i < g_plans.size(); ++i)
+          if (g_plans[i] == p) handle = (int)i;
+      }) != 0)
+    return -1;
+  return handle;
+}
+
+extern "C" void mdhip_fft_release(int handle) {
+  if (handle >= 0 && handle < (int)g_plans.size() && g_plans[handle]->users > 0)
+    --g_plans[handle]->users;
+}
+
+extern "C" int64_t mdhip_fft_plan_count(void) { return (int64_t)g_plans.size(); }
+
+extern "C" int64_t mdhip_fft_plan_bytes(void) {
+  int64_t b = 0;
+  for (AbiPlan *p : g_plans)
+    for (int a = 0; a < 3; ++a) b += p->k[a] * (p->single ? 8 : 16);
+  return b;
+}
+
+extern "C" void mdhip_fft_trim(void) {
+  std::vector<AbiPlan *> keep;
+  for (AbiPlan *p : g_plans) {
+    if (p->users) {
+      keep.push_back(p);
+      continue;
+    }
+    p->f32.release();
+    p->f64.release();
+    delete p;
+  }
+  g_plans.swap(keep);
+}
+
+
+extern "C" {
+
+const char *mdrtHipLastError(void) { return g_last_error.c_str(); }
+
+void *mdrtHipSystemCreate(const mdrtHipSystemDesc *desc) {
+  Engine *e = new Engine();
+  if (guarded([&] { e->create(desc); }) != 0) {
+    delete e;
+    return nullptr;
+  }
+  return e;
+}
+
+void mdrtHipSystemDestroy(void *system) { delete static_cast<Engine *>(system); }
+
+int mdrtHipSetPositions(void *system, const double *x) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] {
+    HIPCHK(hipMemcpy(e->d_x, x, 3 * e->n * sizeof(double), hipMemcpyHostToDevice));
+    e->forces_valid = false;
+  });
+}
+
+int mdrtHipSetVelocities(void *system, const double *v) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] {
+    HIPCHK(hipMemcpy(e->d_v, v, 3 * e->n * sizeof(double), hipMemcpyHostToDevice));
+  });
+}
+
+int mdrtHipGetPositions(void *system, double *x) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] {
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpy(x, e->d_x, 3 * e->n * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+int mdrtHipGetVelocities(void *system, double *v) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] {
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipMemcpy(v, e->d_v, 3 * e->n * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+int mdrtHipComputeForces(void *system, double *forces, double *energies) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] { e->compute(forces, energies); });
+}
+
+int mdrtHipRun(void *system, int64_t steps, double dt, int32_t graph, double *kinetic) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] { e->run(steps, dt, graph, kinetic); });
+}
+
+int mdrtHipMinimize(void *system, int64_t steps, double max_step, double *energy) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] { e->minimize(steps, max_step, energy); });
+}
+
+int mdrtHipNeighborPairs(void *system, double reach, int32_t excluded, int64_t capacity,
+                         int32_t *pairs, int64_t *count) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] { *count = e->neighbor_pairs(reach, excluded, capacity, pairs); });
+}
+
+int mdrtHipStats(void *system, int64_t *stats) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] { e->stats(stats); });
+}
+
+int mdrtHipProfileStep(void *system, int32_t repeats, double *ms) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] { e->profile(repeats, ms); });
+}
+
+}  // extern "C"

@@ -1,0 +1,140 @@
+// fft.h — LDS-resident mixed-radix Stockham FFT for the PME grids.
+//
+// Conventions of the reference's transforms (runtime/mdrt_fft.c:110-256,
+// cuFFT calls of runtime/mdrt_cuda.c:2901-2991): forward exp(-2 pi i k m/K),
+// backward exp(+2 pi i k m/K), unnormalised; the real grid K1 x K2 x K3 is
+// row-major (z fastest) and its half spectrum K1 x K2 x (K3/2+1) complex
+// numbers, interleaved.
+//
+// A workgroup holds a tile of TL lines of one axis in LDS (element e of line
+// t at [e * TL + t], so a row of the tile is TL consecutive complex numbers
+// of the grid and the global loads of a tile are coalesced) and runs every
+// radix pass of the line there; only the first load and the last store touch
+// HBM.  Radices 8, 4, 2 (then 3, 5, 7) in registers, twiddles from a table of
+// exp(-2 pi i m / K) computed in f64 on the host.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace mdhip {
+
+template <typename T>
+struct cpx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
+// a * (s i), s = +-1
+template <typename T>
+__device__ __forceinline__ cpx<T> cmuli(cpx<T> a, int s) {
+  return s > 0 ? cpx<T>{-a.y, a.x} : cpx<T>{a.y, -a.x};
+}
+
+constexpr int kMaxRadices = 8;
+
+struct FftAxis {
+  int K;
+  int nrad;
+  int rad[kMaxRadices];
+  const void *tw;  // cpx<T>[K]: exp(-2 pi i m / K)
+};
+
+// y[k] = sum_r v[r] exp(s 2 pi i r k / R)
+template <typename T>
+__device__ __forceinline__ void butterfly(cpx<T> *v, int R, int s) {
+  if (R == 2) {
+    cpx<T> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if (R == 4) {
+    cpx<T> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+    cpx<T> b0 = cadd(v[1], v[3]), b1 = cmuli(csub(v[1], v[3]), s);
+    v[0] = cadd(a0, b0);
+    v[2] = csub(a0, b0);
+    v[1] = cadd(a1, b1);
+    v[3] = csub(a1, b1);
+  } else if (R == 8) {
+    const T h = (T)0.70710678118654752440;
+    // radix-2 split: even / odd, each a radix-4
+    cpx<T> e[4] = {v[0], v[2], v[4], v[6]};
+    cpx<T> o[4] = {v[1], v[3], v[5], v[7]};
+    butterfly<T>(e, 4, s);
+    butterfly<T>(o, 4, s);
+    // twiddles w8^k, k = 0..3 with sign s
+    cpx<T> t1 = {h * (o[1].x - s * o[1].y), h * (o[1].y + s * o[1].x)};
+    cpx<T> t2 = cmuli(o[2], s);
+    cpx<T> t3 = {h * (-o[3].x - s * o[3].y), h * (-o[3].y + s * o[3].x)};
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], t1);
+    v[5] = csub(e[1], t1);
+    v[2] = cadd(e[2], t2);
+    v[6] = csub(e[2], t2);
+    v[3] = cadd(e[3], t3);
+    v[7] = csub(e[3], t3);
+  } else {
+    // radices 3, 5, 7: direct sum with exact constants
+    cpx<T> in[7];
+    for (int r = 0; r < R; ++r) in[r] = v[r];
+    for (int k = 0; k < R; ++k) {
+      cpx<T> acc = in[0];
+      for (int r = 1; r < R; ++r) {
+        const int m = (r * k) % R;
+        const double ang = 6.283185307179586476925286766559 * m / R;
+        const T c = (T)cos(ang), sn = (T)(s * sin(ang));
+        acc.x += in[r].x * c - in[r].y * sn;
+        acc.y += in[r].x * sn + in[r].y * c;
+      }
+      v[k] = acc;
+    }
+  }
+}
+
+// Runs all passes of TL lines of length ax.K held in `a` (scratch `b`),
+// sign -1 forward / +1 backward.  Returns the buffer holding the result.
+// Stockham autosort (Govindaraju et al., SC'08): pass with radix R after
+// sub-transforms of length Ns reads v[r] = in[j + r K/R], twiddles it by
+// w^{(j mod Ns) r K/(Ns R)} and writes out[(j/Ns) Ns R + j mod Ns + r Ns].
+template <typename T, int TL>
+__device__ cpx<T> *lds_fft(cpx<T> *a, cpx<T> *b, const FftAxis &ax, int sign) {
+  const int K = ax.K;
+  const cpx<T> *tw = static_cast<const cpx<T> *>(ax.tw);
+  int Ns = 1;
+  cpx<T> *in = a, *out = b;
+  for (int s = 0; s < ax.nrad; ++s) {
+    const int R = ax.rad[s];
+    const int Kr = K / R;
+    const int tstep = K / (Ns * R);
+    for (int w = threadIdx.x; w < TL * Kr; w += blockDim.x) {
+      const int line = w % TL, j = w / TL;
+      cpx<T> v[8];
+      for (int r = 0; r < R; ++r) v[r] = in[(j + r * Kr) * TL + line];
+      const int k = j % Ns;
+      if (Ns > 1) {
+        for (int r = 1; r < R; ++r) {
+          cpx<T> t = tw[(k * r * tstep) % K];
+          if (sign > 0) t.y = -t.y;
+          v[r] = cmul(v[r], t);
+        }
+      }
+      butterfly<T>(v, R, sign);
+      const int base = (j / Ns) * Ns * R + k;
+      for (int r = 0; r < R; ++r) out[(base + r * Ns) * TL + line] = v[r];
+    }
+    __syncthreads();
+    cpx<T> *t = in;
+    in = out;
+    out = t;
+    Ns *= R;
+  }
+  return in;
+}
+
+}  // namespace mdhip

@@ -1,0 +1,275 @@
+// integrate.hip — velocity Verlet with SETTLE waters, in f64.
+//
+// Reference semantics: dyn.kick / dyn.drift (lib/Driver/DynamicsIR.cpp:102-112)
+// fused as the integration run of MDExecKernels.cpp:1121 (08-gpu.md:112-121):
+//   half kick, drift, SETTLE positions (ConstraintIR.cpp:473-593, the method
+//   of Miyamoto & Kollman, J. Comput. Chem. 13, 952 (1992)), v += dx_c/dt,
+//   the validity test |x - x_ref|^2 > (skin/2)^2 (ExposeValidity.cpp:23);
+//   then after the forces: half kick and the velocity constraint
+//   (ConstraintIR.cpp:392-471).
+// A unit is a SETTLE water (3 consecutive atoms) or a free atom; one thread
+// owns a unit, reads its force words and clears them for the next step.
+#include "md_common.h"
+
+namespace mdhip {
+
+struct IntParams {
+  double dt;
+  double skin_half2;
+  double half_box[3];
+  double m_o, m_h;
+  double ra, rb, rc;   // SETTLE canonical triangle
+  double d_oh, d_hh;
+};
+
+__device__ __forceinline__ void cross(const double *a, const double *b, double *c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ __forceinline__ double dot(const double *a, const double *b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+__device__ __forceinline__ void unit(double *a) {
+  const double s = rsqrt(dot(a, a));
+  a[0] *= s;
+  a[1] *= s;
+  a[2] *= s;
+}
+
+// SETTLE of one water: old positions A0 B0 C0 (satisfying the constraints),
+// new unconstrained A1 B1 C1 -> constrained, in place.
+__device__ void settle_positions(const double *A0, const double *B0, const double *C0,
+                                 double *A1, double *B1, double *C1,
+                                 const IntParams &p) {
+  const double total = p.m_o + 2.0 * p.m_h;
+  double b0[3], c0[3], b1r[3], c1r[3];
+  for (int k = 0; k < 3; ++k) {
+    b0[k] = B0[k] - A0[k];
+    c0[k] = C0[k] - A0[k];
+    b1r[k] = B1[k] - A1[k];
+    c1r[k] = C1[k] - A1[k];
+  }
+  double com[3], a1[3], b1[3], c1[3];
+  for (int k = 0; k < 3; ++k) {
+    com[k] = p.m_h / total * (b1r[k] + c1r[k]);
+    a1[k] = -com[k];
+    b1[k] = b1r[k] - com[k];
+    c1[k] = c1r[k] - com[k];
+  }
+  double ez[3], ex[3], ey[3];
+  cross(b0, c0, ez);
+  unit(ez);
+  cross(a1, ez, ex);
+  unit(ex);
+  cross(ez, ex, ey);
+  const double xb0 = dot(b0, ex), yb0 = dot(b0, ey);
+  const double xc0 = dot(c0, ex), yc0 = dot(c0, ey);
+  const double za1 = dot(a1, ez);
+  const double xb1 = dot(b1, ex), yb1 = dot(b1, ey), zb1 = dot(b1, ez);
+  const double xc1 = dot(c1, ex), yc1 = dot(c1, ey), zc1 = dot(c1, ez);
+  const double sinphi = za1 / p.ra;
+  const double cosphi = sqrt(1.0 - sinphi * sinphi);
+  const double sinpsi = (zb1 - zc1) / (2.0 * p.rc * cosphi);
+  const double cospsi = sqrt(1.0 - sinpsi * sinpsi);
+  const double ya2 = p.ra * cosphi;
+  const double xb2 = -p.rc * cospsi;
+  const double t1 = -p.rb * cosphi, t2 = p.rc * sinpsi * sinphi;
+  const double yb2 = t1 - t2, yc2 = t1 + t2;
+  const double alpha = xb2 * (xb0 - xc0) + yb0 * yb2 + yc0 * yc2;
+  const double beta = xb2 * (yc0 - yb0) + xb0 * yb2 + xc0 * yc2;
+  const double gamma = (xb0 * yb1 - xb1 * yb0) + (xc0 * yc1 - xc1 * yc0);
+  const double both = alpha * alpha + beta * beta;
+  const double sintheta = (alpha * gamma - beta * sqrt(both - gamma * gamma)) / both;
+  const double costheta = sqrt(1.0 - sintheta * sintheta);
+  const double xa3 = -ya2 * sintheta, ya3 = ya2 * costheta;
+  const double xb3 = xb2 * costheta - yb2 * sintheta;
+  const double yb3 = xb2 * sintheta + yb2 * costheta;
+  const double xc3 = -xb2 * costheta - yc2 * sintheta;
+  const double yc3 = -xb2 * sintheta + yc2 * costheta;
+  double base[3] = {A1[0], A1[1], A1[2]};
+  for (int k = 0; k < 3; ++k) {
+    A1[k] = base[k] + xa3 * ex[k] + ya3 * ey[k] + za1 * ez[k] + com[k];
+    B1[k] = base[k] + xb3 * ex[k] + yb3 * ey[k] + zb1 * ez[k] + com[k];
+    C1[k] = base[k] + xc3 * ex[k] + yc3 * ey[k] + zc1 * ez[k] + com[k];
+  }
+}
+
+__device__ __forceinline__ double det3(double m00, double m01, double m02, double m10,
+                                       double m11, double m12, double m20, double m21,
+                                       double m22) {
+  return m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) +
+         m02 * (m10 * m21 - m11 * m20);
+}
+
+// Impulses along the three bonds that zero the bond-relative velocities.
+__device__ void settle_velocities(const double *xa, const double *xb, const double *xc,
+                                  double *va, double *vb, double *vc,
+                                  const IntParams &p) {
+  const double io = 1.0 / p.m_o, ih = 1.0 / p.m_h;
+  double e0[3], e1[3], e2[3], dvb[3], dvc[3], dvcb[3];
+  for (int k = 0; k < 3; ++k) {
+    e0[k] = xb[k] - xa[k];
+    e1[k] = xc[k] - xa[k];
+    e2[k] = xc[k] - xb[k];
+    dvb[k] = vb[k] - va[k];
+    dvc[k] = vc[k] - va[k];
+    dvcb[k] = vc[k] - vb[k];
+  }
+  unit(e0);
+  unit(e1);
+  unit(e2);
+  const double r0 = -dot(e0, dvb), r1 = -dot(e1, dvc), r2 = -dot(e2, dvcb);
+  const double a00 = io + ih, a11 = io + ih, a22 = 2.0 * ih;
+  const double a01 = dot(e0, e1) * io;
+  const double a02 = -dot(e0, e2) * ih;
+  const double a12 = dot(e1, e2) * ih;
+  const double det = det3(a00, a01, a02, a01, a11, a12, a02, a12, a22);
+  const double s0 = det3(r0, a01, a02, r1, a11, a12, r2, a12, a22) / det;
+  const double s1 = det3(a00, r0, a02, a01, r1, a12, a02, r2, a22) / det;
+  const double s2 = det3(a00, a01, r0, a01, a11, r1, a02, a12, r2) / det;
+  for (int k = 0; k < 3; ++k) {
+    const double i0 = s0 * e0[k], i1 = s1 * e1[k], i2 = s2 * e2[k];
+    va[k] += -io * (i0 + i1);
+    vb[k] += ih * (i0 - i2);
+    vc[k] += ih * (i1 + i2);
+  }
+}
+
+// Unit u: first atom unit_first[u], size 3 (water) or 1.
+// Half kick with the force words (cleared), drift, SETTLE, v += dx_c/dt,
+// f32 positions of the pair loop, validity flag.
+__global__ void __launch_bounds__(256) integrate_first(
+    int nunits, const int *__restrict__ unit_first, const int *__restrict__ unit_size,
+    double *__restrict__ x, double *__restrict__ v, const double *__restrict__ inv_mass,
+    long long *__restrict__ F, const int *__restrict__ place_of,
+    const double *__restrict__ shift, const double *__restrict__ xref,
+    const double *__restrict__ q, float4 *__restrict__ xq, IntParams p,
+    int *__restrict__ flags) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= nunits) return;
+  const int first = unit_first[u], size = unit_size[u];
+  double xo[3][3], xn[3][3], vv[3][3];
+  for (int m = 0; m < size; ++m) {
+    const int at = first + m;
+    const int pl = place_of[at];
+    const double im = inv_mass[at];
+    for (int k = 0; k < 3; ++k) {
+      const double f = (double)F[3 * pl + k] * kForceUnscale;
+      F[3 * pl + k] = 0;
+      const double vk = v[3 * at + k] + 0.5 * p.dt * f * im;
+      vv[m][k] = vk;
+      xo[m][k] = x[3 * at + k];
+      xn[m][k] = xo[m][k] + p.dt * vk;
+    }
+  }
+  if (size == 3) {
+    double xu[3][3];
+    for (int m = 0; m < 3; ++m)
+      for (int k = 0; k < 3; ++k) xu[m][k] = xn[m][k];
+    settle_positions(xo[0], xo[1], xo[2], xn[0], xn[1], xn[2], p);
+    const double idt = 1.0 / p.dt;
+    for (int m = 0; m < 3; ++m)
+      for (int k = 0; k < 3; ++k) vv[m][k] += (xn[m][k] - xu[m][k]) * idt;
+  }
+  bool moved = false;
+  for (int m = 0; m < size; ++m) {
+    const int at = first + m;
+    double d2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      x[3 * at + k] = xn[m][k];
+      v[3 * at + k] = vv[m][k];
+      const double d = xn[m][k] - xref[3 * at + k];
+      d2 += d * d;
+    }
+    moved |= d2 > p.skin_half2;
+    const int pl = place_of[at];
+    xq[pl] = make_float4((float)(xn[m][0] - shift[3 * at + 0] - p.half_box[0]),
+                         (float)(xn[m][1] - shift[3 * at + 1] - p.half_box[1]),
+                         (float)(xn[m][2] - shift[3 * at + 2] - p.half_box[2]),
+                         (float)q[at]);
+  }
+  if (moved) atomicOr(&flags[kFlagRebuild], 1);
+}
+
+// Half kick with the new forces (not cleared: the next step's first half
+// kick reads them) and the velocity constraint of the waters.  With
+// kinetic != null, each block adds its kinetic energy (f64) to a partial.
+__global__ void __launch_bounds__(256) integrate_second(
+    int nunits, const int *__restrict__ unit_first, const int *__restrict__ unit_size,
+    const double *__restrict__ x, double *__restrict__ v,
+    const double *__restrict__ inv_mass, const long long *__restrict__ F,
+    const int *__restrict__ place_of, IntParams p, double *__restrict__ kinetic) {
+  __shared__ double s_k[256];
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  double ke = 0.0;
+  if (u < nunits) {
+    const int first = unit_first[u], size = unit_size[u];
+    double vv[3][3];
+    for (int m = 0; m < size; ++m) {
+      const int at = first + m;
+      const int pl = place_of[at];
+      for (int k = 0; k < 3; ++k)
+        vv[m][k] = v[3 * at + k] +
+                   0.5 * p.dt * ((double)F[3 * pl + k] * kForceUnscale) * inv_mass[at];
+    }
+    if (size == 3)
+      settle_velocities(x + 3 * first, x + 3 * first + 3, x + 3 * first + 6, vv[0],
+                        vv[1], vv[2], p);
+    for (int m = 0; m < size; ++m) {
+      const int at = first + m;
+      for (int k = 0; k < 3; ++k) v[3 * at + k] = vv[m][k];
+      ke += 0.5 / inv_mass[at] * (vv[m][0] * vv[m][0] + vv[m][1] * vv[m][1] + vv[m][2] * vv[m][2]);
+    }
+  }
+  if (kinetic) {
+    s_k[threadIdx.x] = ke;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (threadIdx.x < s) s_k[threadIdx.x] += s_k[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) kinetic[blockIdx.x] = s_k[0];
+  }
+}
+
+// Force words (place order) -> f64 forces (atom order).
+__global__ void forces_to_atoms(int n, const int *__restrict__ place_of,
+                                const long long *__restrict__ F, double *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int pl = place_of[i];
+  for (int k = 0; k < 3; ++k) out[3 * i + k] = (double)F[3 * pl + k] * kForceUnscale;
+}
+
+__global__ void clear_words(long long *__restrict__ w, size_t count) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < count; i += stride) w[i] = 0;
+}
+
+// Steepest descent of rigid waters (setup only): x_new = x + h F / |F|max,
+// constrained from the old positions; the step is kept by the host when the
+// energy falls.
+__global__ void descend(int nunits, const int *__restrict__ unit_first,
+                        const int *__restrict__ unit_size, const double *__restrict__ x0,
+                        double *__restrict__ x, const long long *__restrict__ F,
+                        const int *__restrict__ place_of, double step, IntParams p) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= nunits) return;
+  const int first = unit_first[u], size = unit_size[u];
+  double xo[3][3], xn[3][3];
+  for (int m = 0; m < size; ++m) {
+    const int at = first + m;
+    const int pl = place_of[at];
+    for (int k = 0; k < 3; ++k) {
+      xo[m][k] = x0[3 * at + k];
+      xn[m][k] = xo[m][k] + step * (double)F[3 * pl + k] * kForceUnscale;
+    }
+  }
+  if (size == 3) settle_positions(xo[0], xo[1], xo[2], xn[0], xn[1], xn[2], p);
+  for (int m = 0; m < size; ++m)
+    for (int k = 0; k < 3; ++k) x[3 * (first + m) + k] = xn[m][k];
+}
+
+}  // namespace mdhip

@@ -1,0 +1,73 @@
+// md_common.h — shared definitions of the gfx950 force-path kernels.
+//
+// Layout in HBM (one system):
+//   atom order   x, v, x_ref, wrap shift: double[3n]; mass, inverse mass,
+//                charge: double[n]; place_of: int[n]
+//   place order  xq: float4 (x, y, z, q) relative to the box centre, type_at,
+//                atom_at (-1 for padding), force words: int64[3 np]
+//   clusters     8 consecutive places; bbox lo/hi float4; column of cluster
+//   pair list    rows of capacity row_cap entries per i-cluster: {cj, shift,
+//                64-bit interaction mask}
+// Forces are accumulated as 64-bit fixed point, 2^32 words per kJ/mol/nm,
+// by integer atomics: the sum does not depend on the order of the adds.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mdhip {
+
+// Coulomb constant f = 138.93545755023302 kJ nm mol^-1 e^-2 as the exact
+// binary64 value of include/mdir/Support/PhysicalConstants.h:18-21.
+constexpr double kCoulomb = 0x1.15def44ac2182p+7;
+
+constexpr int kCluster = 8;          // atoms per cluster (8 x 8 tile = wave64)
+constexpr double kForceScale = 4294967296.0;         // 2^32 words per kJ/mol/nm
+constexpr double kForceUnscale = 1.0 / 4294967296.0;
+constexpr double kGridScale = 1099511627776.0;       // 2^40 (PME charge grid)
+constexpr double kGridUnscale = 1.0 / 1099511627776.0;
+
+// Lennard-Jones modifiers (lib/Driver/TermLennardJones.cpp:26-168,
+// ExpandTruncation.cpp:19-45).
+enum LJModifier { kLJNone = 0, kLJPotentialShift = 1, kLJForceSwitch = 2,
+                  kLJPotentialSwitch = 3 };
+
+// Device flags word layout (int32 each).
+enum Flag { kFlagRebuild = 0, kFlagOverflowColumn = 1, kFlagOverflowRow = 2,
+            kFlagBuilds = 3, kFlagSettleFail = 4, kNumFlags = 8 };
+
+// One entry of a cluster-pair row.
+struct __attribute__((aligned(16))) PairEntry {
+  int32_t cj;        // j cluster
+  uint32_t shift;    // (sx+1) | (sy+1) << 2 | (sz+1) << 4
+  uint64_t mask;     // bit (a + 8 b): i atom a, j atom b interact
+};
+
+// Per-system constants of the nonbonded and PME kernels (passed by value).
+struct NbParams {
+  float box[3];
+  float half_box[3];
+  float rc2;
+  float rc;
+  float rs;            // switch start
+  float beta;          // Ewald splitting, 0 = no Coulomb
+  float coulomb;       // f
+  float shift_c;       // erfc(beta rc)/rc under POTENTIAL_SHIFT, else 0
+  float sw_a12, sw_b12, sw_c12, sw_a6, sw_b6, sw_c6;   // force switch
+  float shift12, shift6;                               // potential shift
+  int ntypes;
+  int lj_modifier;
+};
+
+__device__ __forceinline__ long long to_fixed(float f) {
+  return __double2ll_rn((double)f * kForceScale);
+}
+
+__device__ __forceinline__ void shift_vector(uint32_t code, const float *box,
+                                             float &sx, float &sy, float &sz) {
+  sx = (float)((int)(code & 3u) - 1) * box[0];
+  sy = (float)((int)((code >> 2) & 3u) - 1) * box[1];
+  sz = (float)((int)((code >> 4) & 3u) - 1) * box[2];
+}
+
+}  // namespace mdhip

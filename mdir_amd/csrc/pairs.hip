@@ -1,0 +1,303 @@
+// pairs.hip — the nonbonded pair loop and the excluded / 1-4 pair tuples.
+//
+// Reference semantics: the primary sum LJ + f qq [erfc(beta r)/r - s] over
+// the non-excluded pairs within rc (lib/Driver/TermNonbonded.cpp:1423-1556,
+// :456-467; LJ modifiers TermLennardJones.cpp:26-168 and the potential
+// switch of ExpandTruncation.cpp:19-45), the excluded-pair correction
+// -f qq erf(beta r)/r (TermNonbonded.cpp:1615-1671) and scaled 1-4 pairs
+// (TermPairs14.cpp:11-17).
+//
+// Pair kernel: one wave per i-cluster row.  Lane l holds i atom a = l & 7 and
+// j atom b = l >> 3 of each 8 x 8 tile, so a tile is one instruction stream
+// with one pair per lane.  The force on j is summed over the 8 lanes of a
+// column by three DPP row operations and added as 64-bit fixed point; the
+// force on i stays in registers across the row and is summed once at the end.
+#include "md_common.h"
+
+namespace mdhip {
+
+constexpr int kMaxTypes = 32;
+
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over the 8 consecutive lanes of a half row (result in all 8 lanes):
+// quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror.
+__device__ __forceinline__ float sum8(float v) {
+  v += dpp<0xB1>(v);
+  v += dpp<0x4E>(v);
+  v += dpp<0x141>(v);
+  return v;
+}
+
+__device__ __forceinline__ void add_fixed(long long *F, int place, float fx,
+                                          float fy, float fz) {
+  unsigned long long *u = reinterpret_cast<unsigned long long *>(F + 3 * place);
+  long long wx = to_fixed(fx), wy = to_fixed(fy), wz = to_fixed(fz);
+  if (wx) atomicAdd(u + 0, (unsigned long long)wx);
+  if (wy) atomicAdd(u + 1, (unsigned long long)wy);
+  if (wz) atomicAdd(u + 2, (unsigned long long)wz);
+}
+
+// Scalar force -dU/dr / r and energy of one pair in f32.
+template <int MOD, bool COUL, bool ENERGY>
+__device__ __forceinline__ float pair_force(float r2, float qq, float2 lj,
+                                            const NbParams &p, float &e_lj,
+                                            float &e_c) {
+  const float rinv = rsqrtf(r2);
+  const float rinv2 = rinv * rinv;
+  float fscal = 0.0f;
+  if (COUL) {
+    const float r = r2 * rinv;
+    const float br = p.beta * r;
+    const float expt = __expf(-br * br);
+    const float erfcv = erfcf(br);
+    const float tbs = 1.1283791670955126f * p.beta;  // 2 beta / sqrt(pi)
+    fscal = qq * (erfcv * rinv + tbs * expt) * rinv2;
+    if (ENERGY) e_c += qq * (erfcv * rinv - p.shift_c);
+  }
+  const float c6 = lj.x, c12 = lj.y;
+  if (c6 != 0.0f || c12 != 0.0f) {
+    const float rinv6 = rinv2 * rinv2 * rinv2;
+    float flj = (12.0f * c12 * rinv6 - 6.0f * c6) * rinv6 * rinv2;
+    float ulj = (c12 * rinv6 - c6) * rinv6;
+    if (MOD == kLJPotentialShift) {
+      ulj -= c12 * p.shift12 - c6 * p.shift6;
+    } else if (MOD == kLJForceSwitch) {
+      const float r = r2 * rinv;
+      const float u = fmaxf(r - p.rs, 0.0f);
+      const float u2 = u * u, u3 = u2 * u;
+      const float dp12 = p.sw_a12 * u2 + p.sw_b12 * u3;
+      const float dp6 = p.sw_a6 * u2 + p.sw_b6 * u3;
+      flj += (c12 * dp12 - c6 * dp6) * rinv;
+      if (ENERGY) {
+        const float p12 = p.sw_c12 + u3 * (p.sw_a12 * (1.0f / 3.0f) + p.sw_b12 * 0.25f * u);
+        const float p6 = p.sw_c6 + u3 * (p.sw_a6 * (1.0f / 3.0f) + p.sw_b6 * 0.25f * u);
+        ulj -= c12 * p12 - c6 * p6;
+      }
+    } else if (MOD == kLJPotentialSwitch) {
+      const float r = r2 * rinv;
+      if (r > p.rs) {
+        const float width = p.rc - p.rs;
+        const float t = (r - p.rs) / width;
+        const float t2 = t * t, t3 = t2 * t;
+        const float S = 1.0f + t3 * (-10.0f + t * (15.0f - 6.0f * t));
+        const float dS = (-30.0f * t2 + 60.0f * t3 - 30.0f * t3 * t) / width;
+        flj = flj * S - ulj * dS * rinv;
+        ulj *= S;
+      }
+    }
+    fscal += flj;
+    if (ENERGY) e_lj += ulj;
+  }
+  return fscal;
+}
+
+template <int MOD, bool COUL, bool ENERGY>
+__global__ void __launch_bounds__(256) nb_pairs(
+    const float4 *__restrict__ xq, const int *__restrict__ type_at,
+    const PairEntry *__restrict__ rows, const int *__restrict__ row_count,
+    int row_cap, const int *__restrict__ counts_in,
+    const float2 *__restrict__ ljtab, NbParams p, long long *__restrict__ F,
+    double *__restrict__ energy_partial, int *__restrict__ flags) {
+  __shared__ float2 s_lj[kMaxTypes * kMaxTypes];
+  const int nt2 = p.ntypes * p.ntypes;
+  for (int k = threadIdx.x; k < nt2; k += blockDim.x) s_lj[k] = ljtab[k];
+  // The build kernels of this step have finished: clear the request.
+  if (blockIdx.x == 0 && threadIdx.x == 0) flags[kFlagRebuild] = 0;
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ci = blockIdx.x * 4 + wave;
+  if (ci >= counts_in[1]) return;
+  const int a = lane & 7, b = lane >> 3;
+  const float4 xi = xq[ci * kCluster + a];
+  const int ti = type_at[ci * kCluster + a];
+  const float qi = xi.w * p.coulomb;
+  float fix = 0.0f, fiy = 0.0f, fiz = 0.0f;
+  double e_lj = 0.0, e_c = 0.0;  // energy steps only: f64 partial sums
+  const int n = row_count[ci];
+  const PairEntry *row = rows + (size_t)ci * row_cap;
+  for (int e = 0; e < n; ++e) {
+    const PairEntry E = row[e];
+    const int pj = E.cj * kCluster + b;
+    const float4 xj = xq[pj];
+    const int tj = type_at[pj];
+    float sx, sy, sz;
+    shift_vector(E.shift, p.box, sx, sy, sz);
+    const float dx = xi.x - (xj.x + sx);
+    const float dy = xi.y - (xj.y + sy);
+    const float dz = xi.z - (xj.z + sz);
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const bool on = ((E.mask >> lane) & 1ull) && (r2 < p.rc2);
+    float fscal = 0.0f;
+    if (on) {
+      float el = 0.0f, ec = 0.0f;
+      fscal = pair_force<MOD, COUL, ENERGY>(r2, qi * xj.w, s_lj[ti * p.ntypes + tj], p, el, ec);
+      if (ENERGY) {
+        e_lj += (double)el;
+        e_c += (double)ec;
+      }
+    }
+    const float fx = fscal * dx, fy = fscal * dy, fz = fscal * dz;
+    fix += fx;
+    fiy += fy;
+    fiz += fz;
+    const float gx = sum8(fx), gy = sum8(fy), gz = sum8(fz);
+    const bool jreal = ((E.mask >> (kCluster * b)) & 0xFFull) != 0ull;
+    if (a == 0 && jreal) add_fixed(F, pj, -gx, -gy, -gz);
+  }
+  // sum the i force over the 8 columns of the wave
+#pragma unroll
+  for (int m = 8; m < 64; m <<= 1) {
+    fix += __shfl_xor(fix, m);
+    fiy += __shfl_xor(fiy, m);
+    fiz += __shfl_xor(fiz, m);
+  }
+  if (b == 0 && n > 0) add_fixed(F, ci * kCluster + a, fix, fiy, fiz);
+  if (ENERGY) {
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+      e_lj += __shfl_xor(e_lj, m);
+      e_c += __shfl_xor(e_c, m);
+    }
+    if (lane == 0) {
+      energy_partial[2 * ci] = e_lj;
+      energy_partial[2 * ci + 1] = e_c;
+    }
+  }
+}
+
+// Excluded pairs (every excluded pair, no cutoff) and scaled 1-4 pairs, in
+// f64: thread per pair.  pairs14 params may be null when there are none.
+struct SpecialParams {
+  double box[3];
+  double beta;
+  double coulomb;
+  double rc, rs;
+  int lj_modifier;
+};
+
+__device__ void lj_pair_f64(double sigma, double eps, double r,
+                            const SpecialParams &p, double &u, double &dudr) {
+  u = 0.0;
+  dudr = 0.0;
+  if (eps == 0.0 || sigma == 0.0) return;
+  const double e4 = 4.0 * eps, sr = sigma / r;
+  const double s3 = sr * sr * sr, s6 = s3 * s3, s12 = s6 * s6;
+  double raw = e4 * (s12 - s6), draw = e4 * (-12.0 * s12 + 6.0 * s6) / r;
+  if (p.lj_modifier == kLJPotentialShift) {
+    const double g3 = sigma * sigma * sigma, g6 = g3 * g3;
+    raw -= e4 * (g6 * g6 * pow(p.rc, -12.0) - g6 * pow(p.rc, -6.0));
+  } else if (p.lj_modifier == kLJPotentialSwitch && r > p.rs) {
+    const double width = p.rc - p.rs, t = (r - p.rs) / width;
+    const double S = 1.0 + t * t * t * (-10.0 + t * (15.0 - 6.0 * t));
+    const double dS = (-30.0 * t * t + 60.0 * t * t * t - 30.0 * t * t * t * t) / width;
+    draw = draw * S + raw * dS;
+    raw *= S;
+  } else if (p.lj_modifier == kLJForceSwitch) {
+    const double rc = p.rc, rs = p.rs, width = rc - rs;
+    const double uu = r > rs ? r - rs : 0.0;
+    const double g3 = sigma * sigma * sigma, g6 = g3 * g3, g12 = g6 * g6;
+    double P[2], dP[2];
+    const double ns[2] = {12.0, 6.0};
+    for (int k = 0; k < 2; ++k) {
+      const double nn = ns[k];
+      const double an = -nn * ((nn + 4.0) * rc - (nn + 1.0) * rs) / (pow(rc, nn + 2.0) * width * width);
+      const double bn = nn * ((nn + 3.0) * rc - (nn + 1.0) * rs) / (pow(rc, nn + 2.0) * width * width * width);
+      const double cn = pow(rc, -nn) - an / 3.0 * pow(width, 3.0) - bn / 4.0 * pow(width, 4.0);
+      P[k] = cn + uu * uu * uu * (an / 3.0 + bn / 4.0 * uu);
+      dP[k] = an * uu * uu + bn * uu * uu * uu;
+    }
+    const double rinv = 1.0 / r, r6 = pow(rinv, 6.0), r12 = r6 * r6;
+    raw = e4 * (g12 * (r12 - P[0]) - g6 * (r6 - P[1]));
+    draw = e4 * (g12 * (-12.0 * r12 * rinv - dP[0]) - g6 * (-6.0 * r6 * rinv - dP[1]));
+  }
+  u = raw;
+  dudr = draw;
+}
+
+__global__ void __launch_bounds__(256) nb_special(
+    int npairs, const int2 *__restrict__ pairs, const int *__restrict__ is14,
+    const double *__restrict__ p14_sigma, const double *__restrict__ p14_eps,
+    const double *__restrict__ p14_qq, const double *__restrict__ x,
+    const double *__restrict__ q, const int *__restrict__ place_of,
+    SpecialParams p, long long *__restrict__ F, double *__restrict__ energy_partial) {
+  __shared__ double s_e[3][256];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  double e_ex = 0.0, e_lj14 = 0.0, e_c14 = 0.0;
+  if (t < npairs) {
+    const int2 ij = pairs[t];
+    double d[3], r2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      double dd = x[3 * ij.x + k] - x[3 * ij.y + k];
+      dd -= p.box[k] * rint(dd / p.box[k]);
+      d[k] = dd;
+      r2 += dd * dd;
+    }
+    const double r = sqrt(r2);
+    double dudr = 0.0;
+    if (p.beta > 0.0) {
+      const double qq = p.coulomb * q[ij.x] * q[ij.y];
+      const double ef = erf(p.beta * r);
+      e_ex = -qq * ef / r;
+      dudr += -qq * (1.1283791670955126 * p.beta * exp(-p.beta * p.beta * r2) / r - ef / r2);
+    }
+    if (is14 && is14[t] >= 0) {
+      const int k14 = is14[t];
+      double u, du;
+      lj_pair_f64(p14_sigma[k14], p14_eps[k14], r, p, u, du);
+      const double uc = p14_qq[k14] / r;
+      e_lj14 = u;
+      e_c14 = uc;
+      dudr += du - uc / r;
+    }
+    const double fs = -dudr / r;
+    const long long wx = __double2ll_rn(fs * d[0] * kForceScale);
+    const long long wy = __double2ll_rn(fs * d[1] * kForceScale);
+    const long long wz = __double2ll_rn(fs * d[2] * kForceScale);
+    unsigned long long *fi = reinterpret_cast<unsigned long long *>(F + 3 * place_of[ij.x]);
+    unsigned long long *fj = reinterpret_cast<unsigned long long *>(F + 3 * place_of[ij.y]);
+    atomicAdd(fi + 0, (unsigned long long)wx);
+    atomicAdd(fi + 1, (unsigned long long)wy);
+    atomicAdd(fi + 2, (unsigned long long)wz);
+    atomicAdd(fj + 0, (unsigned long long)(-wx));
+    atomicAdd(fj + 1, (unsigned long long)(-wy));
+    atomicAdd(fj + 2, (unsigned long long)(-wz));
+  }
+  if (energy_partial) {
+    s_e[0][threadIdx.x] = e_ex;
+    s_e[1][threadIdx.x] = e_lj14;
+    s_e[2][threadIdx.x] = e_c14;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (threadIdx.x < s)
+        for (int k = 0; k < 3; ++k) s_e[k][threadIdx.x] += s_e[k][threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0)
+      for (int k = 0; k < 3; ++k) energy_partial[3 * blockIdx.x + k] = s_e[k][0];
+  }
+}
+
+// Instantiations used by the engine.
+#define MDHIP_PAIRS(MOD)                                                                 \
+  template __global__ void nb_pairs<MOD, true, false>(const float4 *, const int *,      \
+      const PairEntry *, const int *, int, const int *, const float2 *, NbParams,       \
+      long long *, double *, int *);                                                    \
+  template __global__ void nb_pairs<MOD, true, true>(const float4 *, const int *,       \
+      const PairEntry *, const int *, int, const int *, const float2 *, NbParams,       \
+      long long *, double *, int *);                                                    \
+  template __global__ void nb_pairs<MOD, false, false>(const float4 *, const int *,     \
+      const PairEntry *, const int *, int, const int *, const float2 *, NbParams,       \
+      long long *, double *, int *);                                                    \
+  template __global__ void nb_pairs<MOD, false, true>(const float4 *, const int *,      \
+      const PairEntry *, const int *, int, const int *, const float2 *, NbParams,       \
+      long long *, double *, int *);
+MDHIP_PAIRS(kLJNone)
+MDHIP_PAIRS(kLJPotentialShift)
+MDHIP_PAIRS(kLJForceSwitch)
+MDHIP_PAIRS(kLJPotentialSwitch)
+
+}  // namespace mdhip

@@ -1,0 +1,147 @@
+"""ctypes binding of libmdrt_hip.so (include/mdrt_hip.h).
+
+The library is built in-tree by mdir_amd.build; importing this module does
+not build it.  Every entry point fails loudly when the library is missing:
+there is no CPU fallback of the force path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmdrt_hip.so")
+
+E_LJ, E_COUL_DIRECT, E_COUL_EXCL, E_COUL_RECIP, E_COUL_SELF, E_DISP_CORR, \
+    E_LJ14, E_COUL14, E_KINETIC = range(9)
+NUM_ENERGIES = 9
+ENERGY_NAMES = ["lj", "coul_direct", "coul_excl", "coul_recip", "coul_self",
+                "disp_corr", "lj14", "coul14", "kinetic"]
+
+D, I32, I64 = C.c_double, C.c_int32, C.c_int64
+PD, PI32, PI64 = C.POINTER(D), C.POINTER(I32), C.POINTER(I64)
+
+
+class SystemDesc(C.Structure):
+    """mdrtHipSystemDesc."""
+    _fields_ = [("n", I64), ("positions", PD), ("velocities", PD),
+                ("masses", PD), ("charges", PD), ("types", PI32),
+                ("ntypes", I32), ("sigma", PD), ("epsilon", PD),
+                ("box", D * 3), ("cutoff", D), ("switch_from", D),
+                ("lj_modifier", I32), ("coulomb_shift", I32),
+                ("dispersion", I32), ("beta", D), ("pme_grid", I32 * 3),
+                ("pme_order", I32), ("nexclusions", I64), ("exclusions", PI32),
+                ("npairs14", I64), ("pairs14", PI32), ("pairs14_sigma", PD),
+                ("pairs14_eps", PD), ("pairs14_qq", PD), ("nwaters", I64),
+                ("water_first", PI32), ("d_oh", D), ("d_hh", D), ("skin", D)]
+
+
+class Buffer1(C.Structure):
+    """mdrtBuffer1: the one-dimensional memref descriptor."""
+    _fields_ = [("allocated", C.c_void_p), ("aligned", C.c_void_p),
+                ("offset", I64), ("size", I64), ("stride", I64)]
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+_lib = None
+
+# (name, restype, argtypes) of every entry point of include/mdrt_hip.h
+SIGNATURES = [
+    ("mgpuSetDefaultDevice", None, [I32]),
+    ("mgpuModuleLoad", C.c_void_p, [C.c_void_p, C.c_size_t]),
+    ("mgpuModuleLoadJIT", C.c_void_p, [C.c_void_p, C.c_int, C.c_size_t]),
+    ("mgpuModuleUnload", None, [C.c_void_p]),
+    ("mgpuModuleGetFunction", C.c_void_p, [C.c_void_p, C.c_char_p]),
+    ("mgpuLaunchKernel", None, [C.c_void_p, C.c_ssize_t, C.c_ssize_t, C.c_ssize_t,
+                                C.c_ssize_t, C.c_ssize_t, C.c_ssize_t, I32,
+                                C.c_void_p, C.POINTER(C.c_void_p),
+                                C.POINTER(C.c_void_p), C.c_size_t]),
+    ("mgpuLaunchKernelCooperative", None, [C.c_void_p] + [C.c_ssize_t] * 9 +
+     [I32, C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    ("mdrtCooperativeGridSize", C.c_ssize_t, [C.c_ssize_t]),
+    ("mgpuStreamCreate", C.c_void_p, []),
+    ("mgpuStreamDestroy", None, [C.c_void_p]),
+    ("mgpuStreamSynchronize", None, [C.c_void_p]),
+    ("mgpuStreamWaitEvent", None, [C.c_void_p, C.c_void_p]),
+    ("mgpuEventCreate", C.c_void_p, []),
+    ("mgpuEventDestroy", None, [C.c_void_p]),
+    ("mgpuEventSynchronize", None, [C.c_void_p]),
+    ("mgpuEventRecord", None, [C.c_void_p, C.c_void_p]),
+    ("mgpuMemAlloc", C.c_void_p, [C.c_uint64, C.c_void_p, C.c_bool]),
+    ("mgpuMemFree", None, [C.c_void_p, C.c_void_p]),
+    ("mgpuMemcpy", None, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    ("mgpuMemset32", None, [C.c_void_p, C.c_uint, C.c_size_t, C.c_void_p]),
+    ("mgpuMemset16", None, [C.c_void_p, C.c_ushort, C.c_size_t, C.c_void_p]),
+    ("mdrtSideBegin", None, []),
+    ("mdrtSideEnd", None, []),
+    ("mdrtSide2Begin", None, []),
+    ("mdrtSide2End", None, []),
+    ("mdrtSideJoin", None, []),
+    ("mdrtUseStream", C.c_void_p, [C.c_void_p]),
+    ("mdrtRestoreStream", None, [C.c_void_p]),
+    ("_mlir_ciface_mdrtFlagMemory", None, [C.POINTER(Buffer1), I64]),
+    ("mdrtFlagStart", I64, [C.c_void_p]),
+    ("mdrtFlagFinish", I32, [I64]),
+    ("mdrtFlagTake", I32, [C.c_void_p]),
+    ("_mlir_ciface_mdrtHostResults", None, [C.POINTER(Buffer1), I64, I64]),
+    ("mdrtCountHostRead", None, [C.c_size_t, C.c_int, C.c_int]),
+    ("mdrtCudaSetStopHandler", None, [C.c_void_p]),
+    ("mdrtDeviceTrim", C.c_uint64, []),
+    ("mdrtDeviceMemoryCounts", None, [PI64]),
+    ("_mlir_ciface_mdrtCudaFFTForward3D", None,
+     [C.POINTER(Buffer1), C.POINTER(Buffer1), I64, I64, I64]),
+    ("_mlir_ciface_mdrtCudaFFTBackward3D", None,
+     [C.POINTER(Buffer1), C.POINTER(Buffer1), I64, I64, I64]),
+    ("_mlir_ciface_mdrtCudaFFTForward3DF32", None,
+     [C.POINTER(Buffer1), C.POINTER(Buffer1), I64, I64, I64]),
+    ("_mlir_ciface_mdrtCudaFFTBackward3DF32", None,
+     [C.POINTER(Buffer1), C.POINTER(Buffer1), I64, I64, I64]),
+    ("_mlir_ciface_mdrtCudaFFTGaussianBackwardF32", None,
+     [C.POINTER(Buffer1), C.POINTER(Buffer1), C.POINTER(Buffer1), I64, I64, I64]),
+    ("mdrtRetainFFTPlan", C.c_int, [I64, I64, I64, C.c_int, C.c_int, C.c_void_p]),
+    ("mdrtReleaseFFTPlan", None, [C.c_int]),
+    ("mdrtHipSystemCreate", C.c_void_p, [C.POINTER(SystemDesc)]),
+    ("mdrtHipSystemDestroy", None, [C.c_void_p]),
+    ("mdrtHipLastError", C.c_char_p, []),
+    ("mdrtHipSetPositions", C.c_int, [C.c_void_p, PD]),
+    ("mdrtHipSetVelocities", C.c_int, [C.c_void_p, PD]),
+    ("mdrtHipGetPositions", C.c_int, [C.c_void_p, PD]),
+    ("mdrtHipGetVelocities", C.c_int, [C.c_void_p, PD]),
+    ("mdrtHipComputeForces", C.c_int, [C.c_void_p, PD, PD]),
+    ("mdrtHipRun", C.c_int, [C.c_void_p, I64, D, I32, PD]),
+    ("mdrtHipMinimize", C.c_int, [C.c_void_p, I64, D, PD]),
+    ("mdrtHipNeighborPairs", C.c_int, [C.c_void_p, D, I32, I64, PI32, PI64]),
+    ("mdrtHipStats", C.c_int, [C.c_void_p, PI64]),
+    ("mdrtHipProfileStep", C.c_int, [C.c_void_p, I32, PD]),
+]
+
+
+def lib():
+    """The loaded library; raises LibraryMissing when it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LibraryMissing(
+                f"{LIB_PATH} is missing: build it with `python -m mdir_amd.build` "
+                "(the force path has no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != 0:
+        raise RuntimeError("mdrt_hip: " + lib().mdrtHipLastError().decode())
+
+
+def ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))

@@ -1,0 +1,163 @@
+"""Simulation: a system on the GPU, its forces and energies, and NVE steps.
+
+Mirrors the part of the reference's Python API that drives the per-step
+force path (python/Bindings.cpp:1625 Simulation, :2230 run; Simulation.cpp
+:1679 run, :1537 runPart): construct from a system, run a number of steps,
+read the state and the energies.  The arithmetic runs in libmdrt_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import runtime as rt
+from .systems import System
+
+
+class Simulation:
+    def __init__(self, system: System, skin: float = 0.1):
+        self.system = system
+        s = system
+        L = rt.lib()
+        self._arrays = []
+
+        def keep(a):
+            self._arrays.append(a)
+            return a
+
+        x = keep(np.ascontiguousarray(s.positions, np.float64))
+        v = keep(np.ascontiguousarray(
+            s.velocities if s.velocities is not None else np.zeros_like(x), np.float64))
+        d = rt.SystemDesc()
+        d.n = s.n
+        d.positions = rt.ptr(x, C.c_double)
+        d.velocities = rt.ptr(v, C.c_double)
+        d.masses = rt.ptr(keep(np.ascontiguousarray(s.masses, np.float64)), C.c_double)
+        d.charges = rt.ptr(keep(np.ascontiguousarray(s.charges, np.float64)), C.c_double)
+        d.types = rt.ptr(keep(np.ascontiguousarray(s.types, np.int32)), C.c_int32)
+        d.ntypes = s.ntypes
+        d.sigma = rt.ptr(keep(np.ascontiguousarray(s.sigma, np.float64)), C.c_double)
+        d.epsilon = rt.ptr(keep(np.ascontiguousarray(s.epsilon, np.float64)), C.c_double)
+        for k in range(3):
+            d.box[k] = float(s.box[k])
+            d.pme_grid[k] = int(s.pme_grid[k])
+        d.cutoff = s.cutoff
+        d.switch_from = s.switch_from
+        d.lj_modifier = s.lj_modifier
+        d.coulomb_shift = int(s.coulomb_shift)
+        d.dispersion = int(s.dispersion)
+        d.beta = s.beta
+        d.pme_order = s.pme_order
+        ex = keep(np.ascontiguousarray(s.exclusions.reshape(-1, 2), np.int32))
+        d.nexclusions = ex.shape[0]
+        d.exclusions = rt.ptr(ex, C.c_int32)
+        p14 = keep(np.ascontiguousarray(s.pairs14.reshape(-1, 2), np.int32))
+        d.npairs14 = p14.shape[0]
+        d.pairs14 = rt.ptr(p14, C.c_int32)
+        d.pairs14_sigma = rt.ptr(keep(np.ascontiguousarray(s.pairs14_sigma, np.float64)), C.c_double)
+        d.pairs14_eps = rt.ptr(keep(np.ascontiguousarray(s.pairs14_eps, np.float64)), C.c_double)
+        d.pairs14_qq = rt.ptr(keep(np.ascontiguousarray(s.pairs14_qq, np.float64)), C.c_double)
+        w = keep(np.ascontiguousarray(s.waters, np.int32))
+        d.nwaters = w.shape[0]
+        d.water_first = rt.ptr(w, C.c_int32)
+        d.d_oh, d.d_hh = s.water_geometry
+        d.skin = skin
+        self._desc = d
+        h = L.mdrtHipSystemCreate(C.byref(d))
+        if not h:
+            raise RuntimeError("mdrt_hip: " + L.mdrtHipLastError().decode())
+        self._h = h
+        self._arrays = []
+        self.n = s.n
+
+    def close(self):
+        if getattr(self, "_h", None):
+            rt.lib().mdrtHipSystemDestroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- state ----
+    @property
+    def positions(self) -> np.ndarray:
+        x = np.zeros((self.n, 3))
+        rt.check(rt.lib().mdrtHipGetPositions(self._h, rt.ptr(x, C.c_double)))
+        return x
+
+    @positions.setter
+    def positions(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        rt.check(rt.lib().mdrtHipSetPositions(self._h, rt.ptr(x, C.c_double)))
+
+    @property
+    def velocities(self) -> np.ndarray:
+        v = np.zeros((self.n, 3))
+        rt.check(rt.lib().mdrtHipGetVelocities(self._h, rt.ptr(v, C.c_double)))
+        return v
+
+    @velocities.setter
+    def velocities(self, v):
+        v = np.ascontiguousarray(v, np.float64)
+        rt.check(rt.lib().mdrtHipSetVelocities(self._h, rt.ptr(v, C.c_double)))
+
+    # ---- evaluation ----
+    def forces(self):
+        """Forces (n, 3) and the energy terms (rt.NUM_ENERGIES,)."""
+        F = np.zeros((self.n, 3))
+        e = np.zeros(rt.NUM_ENERGIES)
+        rt.check(rt.lib().mdrtHipComputeForces(self._h, rt.ptr(F, C.c_double),
+                                               rt.ptr(e, C.c_double)))
+        return F, e
+
+    def energies(self) -> dict:
+        _, e = self.forces()
+        return dict(zip(rt.ENERGY_NAMES, e))
+
+    def run(self, steps: int, dt: float = 0.002, graph: int = 0) -> float:
+        """`steps` NVE steps; returns the kinetic energy at the end."""
+        ke = C.c_double(0.0)
+        rt.check(rt.lib().mdrtHipRun(self._h, int(steps), float(dt), int(graph),
+                                     C.byref(ke)))
+        return ke.value
+
+    def minimize(self, steps: int = 200, max_step: float = 0.01) -> float:
+        e = C.c_double(0.0)
+        rt.check(rt.lib().mdrtHipMinimize(self._h, int(steps), float(max_step),
+                                          C.byref(e)))
+        return e.value
+
+    def neighbor_pairs(self, reach: float, excluded: bool = False,
+                       capacity: int | None = None) -> np.ndarray:
+        """Unordered pairs (i < j) of the reference's matrix test at `reach`."""
+        if capacity is None:
+            import math
+            s = self.system
+            rho = s.n / float(np.prod(s.box))
+            capacity = int(s.n * rho * 4.19 * (reach + 0.05) ** 3 * 0.75) + 4096
+        out = np.zeros((capacity, 2), np.int32)
+        count = C.c_int64(0)
+        rt.check(rt.lib().mdrtHipNeighborPairs(self._h, float(reach), int(excluded),
+                                               capacity, rt.ptr(out, C.c_int32),
+                                               C.byref(count)))
+        if count.value > capacity:
+            return self.neighbor_pairs(reach, excluded, count.value + 16)
+        return out[:count.value].copy()
+
+    def stats(self) -> dict:
+        s = np.zeros(8, np.int64)
+        rt.check(rt.lib().mdrtHipStats(self._h, rt.ptr(s, C.c_int64)))
+        keys = ["places", "clusters", "entries", "largest_row", "row_cap",
+                "columns", "column_cap", "max_places"]
+        return dict(zip(keys, (int(v) for v in s)))
+
+    def profile_step(self, repeats: int = 20) -> dict:
+        ms = np.zeros(8)
+        rt.check(rt.lib().mdrtHipProfileStep(self._h, int(repeats),
+                                             rt.ptr(ms, C.c_double)))
+        keys = ["integrate", "build", "pairs", "spread", "fft", "gather"]
+        return dict(zip(keys, ms[:6]))

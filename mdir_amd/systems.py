@@ -1,0 +1,233 @@
+"""Systems of the force path: topology, parameters and synthetic generators.
+
+A :class:`System` holds what the per-step force path reads: positions,
+masses, charges, Lennard-Jones types with per type-pair sigma/epsilon
+tables, excluded pairs, scaled 1-4 pairs, rigid (SETTLE) waters, the
+orthorhombic box and the nonbonded/PME settings.  Field meanings follow the
+reference's Topology and Control (lib/Driver/TermNonbonded.cpp,
+lib/Driver/TermPairs14.cpp:11-17, lib/Driver/ConstraintIR.cpp:473-593).
+
+Units are the reference's: nm, ps, amu, kJ/mol, e.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Optional
+
+import numpy as np
+
+# include/mdir/Support/PhysicalConstants.h:18-21 (SI defining constants and
+# CODATA 2022 permittivity), as exact binary64 values.
+COULOMB = float.fromhex("0x1.15def44ac2182p+7")   # kJ nm mol^-1 e^-2
+BOLTZMANN = float.fromhex("0x1.1072c483af26dp-7")  # kJ mol^-1 K^-1
+KJ_PER_KCAL = 4.184
+
+LJ_NONE, LJ_POTENTIAL_SHIFT, LJ_FORCE_SWITCH, LJ_POTENTIAL_SWITCH = 0, 1, 2, 3
+
+# TIP3P (Jorgensen 1983) as config 2 states it.
+TIP3P = dict(q_o=-0.834, q_h=0.417, sigma_o=0.31506, eps_o=0.6364,
+             m_o=15.9994, m_h=1.008, d_oh=0.09572, d_hh=0.15139)
+
+
+def ewald_beta(cutoff: float, tolerance: float = 1e-5) -> float:
+    """beta with erfc(beta rc) = tolerance by 100 halvings
+    (lib/Driver/TermNonbonded.cpp:778-794)."""
+    low, high = 0.0, 1.0
+    while math.erfc(high * cutoff) > tolerance:
+        high *= 2.0
+    for _ in range(100):
+        middle = 0.5 * (low + high)
+        if math.erfc(middle * cutoff) > tolerance:
+            low = middle
+        else:
+            high = middle
+    return 0.5 * (low + high)
+
+
+def smooth_size(least: int) -> int:
+    """Smallest even size >= least with factors 2,3,5,7
+    (lib/Driver/TermNonbonded.cpp:756-767)."""
+    size = least
+    while True:
+        if size % 2 == 0:
+            rest = size
+            for f in (2, 3, 5, 7):
+                while rest % f == 0:
+                    rest //= f
+            if rest == 1:
+                return size
+        size += 1
+
+
+def pme_grid(box, spacing: float = 0.12):
+    """Grid no wider than `spacing` (TermNonbonded.cpp:1108-1127)."""
+    return tuple(smooth_size(int(math.ceil(L / spacing - 1e-9))) for L in box)
+
+
+@dataclasses.dataclass
+class System:
+    positions: np.ndarray            # (n, 3) f64
+    masses: np.ndarray               # (n,) f64
+    charges: np.ndarray              # (n,) f64
+    types: np.ndarray                # (n,) i32
+    sigma: np.ndarray                # (ntypes, ntypes) f64
+    epsilon: np.ndarray              # (ntypes, ntypes) f64
+    box: np.ndarray                  # (3,) f64
+    cutoff: float = 0.9
+    switch_from: float = 0.0
+    lj_modifier: int = LJ_NONE
+    coulomb_shift: bool = False
+    dispersion: bool = False
+    beta: float = 0.0                # 0: no electrostatics
+    pme_grid: tuple = (0, 0, 0)
+    pme_order: int = 4
+    exclusions: np.ndarray = dataclasses.field(
+        default_factory=lambda: np.zeros((0, 2), np.int32))   # (m, 2) i<j
+    pairs14: np.ndarray = dataclasses.field(
+        default_factory=lambda: np.zeros((0, 2), np.int32))
+    pairs14_sigma: np.ndarray = dataclasses.field(
+        default_factory=lambda: np.zeros(0))
+    pairs14_eps: np.ndarray = dataclasses.field(          # eps * s_LJ
+        default_factory=lambda: np.zeros(0))
+    pairs14_qq: np.ndarray = dataclasses.field(           # f q_i q_j s_C
+        default_factory=lambda: np.zeros(0))
+    waters: np.ndarray = dataclasses.field(               # first atom of each
+        default_factory=lambda: np.zeros(0, np.int32))
+    water_geometry: tuple = (TIP3P["d_oh"], TIP3P["d_hh"])
+    velocities: Optional[np.ndarray] = None
+
+    @property
+    def n(self) -> int:
+        return int(self.positions.shape[0])
+
+    @property
+    def ntypes(self) -> int:
+        return int(self.sigma.shape[0])
+
+    def degrees_of_freedom(self) -> int:
+        """3N - 3 - 3 per rigid water (System.cpp getDegreesOfFreedom)."""
+        return 3 * self.n - 3 - 3 * len(self.waters)
+
+    def exclusion_csr(self):
+        """Symmetric CSR of excluded partners per atom."""
+        n = self.n
+        rows = [[] for _ in range(n)]
+        for i, j in self.exclusions:
+            rows[int(i)].append(int(j))
+            rows[int(j)].append(int(i))
+        start = np.zeros(n + 1, np.int32)
+        for i in range(n):
+            start[i + 1] = start[i] + len(rows[i])
+        atoms = np.array([j for r in rows for j in sorted(r)], np.int32)
+        return start, atoms
+
+    def kinetic_energy(self) -> float:
+        v = self.velocities
+        return 0.5 * float(np.sum(self.masses[:, None] * v * v))
+
+    def copy(self) -> "System":
+        return dataclasses.replace(
+            self, positions=self.positions.copy(),
+            velocities=None if self.velocities is None else self.velocities.copy())
+
+
+def argon_fcc(cells: int = 6, lattice: float = 0.57964333, sigma: float = 0.34,
+              epsilon: float = 0.99773551, mass: float = 39.95,
+              cutoff: float = 0.85, switch_from: float = 0.75,
+              lj_modifier: int = LJ_POTENTIAL_SWITCH,
+              dispersion: bool = False) -> System:
+    """Argon on a face-centred cubic lattice, atoms ordered as
+    examples/argon/argon.mlir:102-134 places them (corner, then the face
+    centres normal to z, y, x), offset a quarter cell."""
+    n = 4 * cells ** 3
+    x = np.zeros((n, 3))
+    for i in range(n):
+        cell, basis = divmod(i, 4)
+        rest, cx = divmod(cell, cells)
+        cz, cy = divmod(rest, cells)
+        c = (cx, cy, cz)
+        normal = 3 - basis
+        for k in range(3):
+            offset = 0.0 if (basis == 0 or k == normal) else 0.5
+            x[i, k] = (c[k] + offset + 0.25) * lattice
+    box = np.full(3, cells * lattice)
+    return System(positions=x, masses=np.full(n, mass), charges=np.zeros(n),
+                  types=np.zeros(n, np.int32), sigma=np.array([[sigma]]),
+                  epsilon=np.array([[epsilon]]), box=box, cutoff=cutoff,
+                  switch_from=switch_from, lj_modifier=lj_modifier,
+                  dispersion=dispersion)
+
+
+def argon_config1() -> System:
+    """Liquid argon, 864 atoms on an FCC lattice (6x6x6 cells of 4), cubic
+    box 3.47 nm, sigma 0.3405 nm, epsilon 0.996 kJ/mol, cutoff 1.0 nm with
+    the dispersion correction."""
+    return argon_fcc(cells=6, lattice=3.47 / 6, sigma=0.3405, epsilon=0.996,
+                     cutoff=1.0, switch_from=0.0, lj_modifier=LJ_NONE,
+                     dispersion=True)
+
+
+def _random_rotations(rng: np.random.Generator, count: int) -> np.ndarray:
+    """Uniform rotations from unit quaternions (Shoemake 1992)."""
+    u = rng.random((count, 3))
+    q = np.stack([np.sqrt(1 - u[:, 0]) * np.sin(2 * np.pi * u[:, 1]),
+                  np.sqrt(1 - u[:, 0]) * np.cos(2 * np.pi * u[:, 1]),
+                  np.sqrt(u[:, 0]) * np.sin(2 * np.pi * u[:, 2]),
+                  np.sqrt(u[:, 0]) * np.cos(2 * np.pi * u[:, 2])], axis=1)
+    a, b, c, d = q.T
+    return np.stack([
+        np.stack([1 - 2 * (c * c + d * d), 2 * (b * c - a * d), 2 * (b * d + a * c)], 1),
+        np.stack([2 * (b * c + a * d), 1 - 2 * (b * b + d * d), 2 * (c * d - a * b)], 1),
+        np.stack([2 * (b * d - a * c), 2 * (c * d + a * b), 1 - 2 * (b * b + c * c)], 1),
+    ], axis=1)
+
+
+def tip3p_box(nwaters: int, edge: float, seed: int = 2024, cutoff: float = 0.9,
+              tolerance: float = 1e-5, grid=None, order: int = 4,
+              jitter: float = 0.08) -> System:
+    """A TIP3P water box: waters on a seeded jittered cubic lattice of
+    ceil(n^(1/3))^3 sites (a seeded choice of n of them) with uniformly
+    random orientations; PME with beta from `tolerance`, grid of spacing
+    <= 0.12 nm unless given, SETTLE-rigid waters, O-H/H-H excluded."""
+    rng = np.random.default_rng(seed)
+    m = int(math.ceil(nwaters ** (1.0 / 3.0) - 1e-9))
+    spacing = edge / m
+    sites = np.stack(np.meshgrid(np.arange(m), np.arange(m), np.arange(m),
+                                 indexing="ij"), -1).reshape(-1, 3)
+    keep = np.sort(rng.choice(len(sites), size=nwaters, replace=False))
+    centres = (sites[keep] + 0.5) * spacing
+    centres += (rng.random((nwaters, 3)) - 0.5) * 2 * jitter * spacing
+    p = TIP3P
+    half = 0.5 * p["d_hh"]
+    h = math.sqrt(p["d_oh"] ** 2 - half ** 2)
+    total = p["m_o"] + 2 * p["m_h"]
+    ra = 2 * p["m_h"] * h / total
+    body = np.array([[0.0, ra, 0.0], [-half, ra - h, 0.0], [half, ra - h, 0.0]])
+    rot = _random_rotations(rng, nwaters)
+    x = (centres[:, None, :] + np.einsum("wij,aj->wai", rot, body)).reshape(-1, 3)
+    n = 3 * nwaters
+    q = np.tile([p["q_o"], p["q_h"], p["q_h"]], nwaters)
+    masses = np.tile([p["m_o"], p["m_h"], p["m_h"]], nwaters)
+    types = np.tile(np.array([0, 1, 1], np.int32), nwaters)
+    sigma = np.array([[p["sigma_o"], 0.0], [0.0, 0.0]])
+    eps = np.array([[p["eps_o"], 0.0], [0.0, 0.0]])
+    first = np.arange(0, n, 3, dtype=np.int32)
+    excl = np.concatenate([np.stack([first, first + 1], 1),
+                           np.stack([first, first + 2], 1),
+                           np.stack([first + 1, first + 2], 1)]).astype(np.int32)
+    excl = excl[np.lexsort((excl[:, 1], excl[:, 0]))]
+    box = np.full(3, float(edge))
+    if grid is None:
+        grid = pme_grid(box)
+    return System(positions=x, masses=masses, charges=q, types=types,
+                  sigma=sigma, epsilon=eps, box=box, cutoff=cutoff,
+                  beta=ewald_beta(cutoff, tolerance), pme_grid=tuple(grid),
+                  pme_order=order, exclusions=excl, waters=first,
+                  water_geometry=(p["d_oh"], p["d_hh"]))
+
+
+def tip3p_config2(seed: int = 2024) -> System:
+    """7,900 TIP3P waters (23,700 atoms) in a 6.2 nm cube, PME rc 0.9 nm,
+    tolerance 1e-5, 64^3 grid, order 4."""
+    return tip3p_box(7900, 6.2, seed=seed, cutoff=0.9, grid=(64, 64, 64))

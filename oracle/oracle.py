@@ -1,0 +1,374 @@
+"""ctypes front of the FP64 oracle (oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package mdir_amd.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+REF_FFT = os.path.join(HERE, "_ref", "libmdrt_fft_ref.so")
+
+E_LJ, E_COUL_DIRECT, E_COUL_EXCL, E_COUL_RECIP, E_COUL_SELF, E_DISP_CORR, \
+    E_LJ14, E_COUL14 = range(8)
+NTERMS = 8
+TERM_NAMES = ["lj", "coul_direct", "coul_excl", "coul_recip", "coul_self",
+              "disp_corr", "lj14", "coul14"]
+
+_lib = None
+
+
+def build(ref: bool = True) -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    if ref:
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build(ref=False)
+        _lib = C.CDLL(LIB)
+        _declare(_lib)
+    return _lib
+
+
+def _p(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+D, I32, I64 = C.c_double, C.c_int32, C.c_int64
+PD, PI32, PI64 = C.POINTER(D), C.POINTER(I32), C.POINTER(I64)
+
+
+class OrcSystem(C.Structure):
+    _fields_ = [("n", I64), ("q", PD), ("type", PI32), ("ntypes", I32),
+                ("sigma", PD), ("epsilon", PD), ("mass", PD), ("box", D * 3),
+                ("cutoff", D), ("switch_from", D), ("lj_modifier", I32),
+                ("coulomb_shift", I32), ("dispersion", I32), ("beta", D),
+                ("coulomb", D), ("excl_start", PI32), ("excl_atoms", PI32),
+                ("nexcl_pairs", I64), ("excl_pairs", PI32), ("K", I64 * 3),
+                ("order", I32), ("nwaters", I64), ("water_first", PI32),
+                ("m_o", D), ("m_h", D), ("d_oh", D), ("d_hh", D)]
+
+
+def _declare(L):
+    L.orc_cell_count.restype = I64
+    L.orc_cell_count.argtypes = [D, D]
+    L.orc_cell_range.restype = I64
+    L.orc_cell_range.argtypes = [D, I64, D]
+    L.orc_cell_width.restype = D
+    L.orc_cell_width.argtypes = [I64, PD, D, D]
+    L.orc_neighbors_matrix.restype = I64
+    L.orc_neighbors_matrix.argtypes = [I64, PD, PD, D, D, PI32, PI32, I64]
+    L.orc_exclusion_filter.argtypes = [I64, PI32, PI32, I64, PI32, PI32]
+    L.orc_nonbonded.restype = C.c_int
+    L.orc_nonbonded.argtypes = [I64, PD, PD, PI32, I32, PD, PD, PD, D, D, I32,
+                                I32, D, D, PI32, PI32, I32, PD, PD, PD]
+    L.orc_excluded.argtypes = [I64, PI32, PD, PD, PD, D, D, PD, PD]
+    L.orc_pairs14.argtypes = [I64, PI32, PD, PD, PD, PD, PD, D, D, I32, PD,
+                              PD, PD]
+    L.orc_ewald_beta.restype = D
+    L.orc_ewald_beta.argtypes = [D, D]
+    L.orc_smooth_size.restype = I64
+    L.orc_smooth_size.argtypes = [I64]
+    L.orc_self_energy.restype = D
+    L.orc_self_energy.argtypes = [I64, PD, D, D, PD]
+    L.orc_dispersion_energy.restype = D
+    L.orc_dispersion_energy.argtypes = [I64, PI32, I32, PD, PD, I64, PI32, PD, D]
+    L.orc_bspline.argtypes = [D, C.c_int, PD, PD]
+    L.orc_spline_moduli.argtypes = [I64, C.c_int, PD]
+    L.orc_pme_spread.argtypes = [I64, PD, PD, PD, PI64, C.c_int, PD]
+    L.orc_pme_influence.argtypes = [PD, PI64, C.c_int, D, D, PD]
+    L.orc_pme_gather.argtypes = [I64, PD, PD, PD, PI64, C.c_int, PD, PD]
+    L.orc_pme.restype = D
+    L.orc_pme.argtypes = [I64, PD, PD, PD, PI64, C.c_int, D, D, PD, PD]
+    L.orc_fft_line.argtypes = [I64, PD, C.c_int]
+    L.orc_dft_line.argtypes = [I64, PD, PD, C.c_int]
+    L.orc_fft3d_r2c.argtypes = [PI64, PD, PD]
+    L.orc_fft3d_c2r.argtypes = [PI64, PD, PD]
+    L.orc_ewald_reciprocal.restype = D
+    L.orc_ewald_reciprocal.argtypes = [I64, PD, PD, PD, D, D, PD]
+    L.orc_settle_positions.argtypes = [I64, PI32, D, D, D, D, PD, PD]
+    L.orc_settle_velocities.argtypes = [I64, PI32, D, D, PD, PD]
+    L.orc_total.restype = C.c_int
+    L.orc_total.argtypes = [C.POINTER(OrcSystem), PD, PD, PD, I32]
+    L.orc_nve.restype = C.c_int
+    L.orc_nve.argtypes = [C.POINTER(OrcSystem), PD, PD, D, I64, PD]
+    L.orc_splitmix64.restype = C.c_uint64
+    L.orc_splitmix64.argtypes = [C.POINTER(C.c_uint64)]
+    L.orc_initial_velocities.argtypes = [C.c_uint64, I64, PD, D, PD, I64, PI32,
+                                         I64, PD]
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def neighbors_matrix(x, box, reach, cell_width=None, row_width=None):
+    """Rows of the reference's matrix build; returns (counts, index, largest)."""
+    L = lib()
+    x, box = f64(x), f64(box)
+    n = x.shape[0]
+    if cell_width is None:
+        cell_width = L.orc_cell_width(n, _p(box, D), reach, 0.0)
+    width = row_width or 64
+    while True:
+        counts = np.zeros(n, np.int32)
+        index = np.full((n, width), -1, np.int32)
+        largest = L.orc_neighbors_matrix(n, _p(x, D), _p(box, D), reach,
+                                         cell_width, _p(counts, I32),
+                                         _p(index, I32), width)
+        if row_width or largest <= width:
+            return counts, index, largest
+        width = int(largest * 1.25) + 1
+
+
+def exclusion_filter(counts, index, excl_start, excl_atoms):
+    L = lib()
+    counts, index = i32(counts).copy(), i32(index).copy()
+    L.orc_exclusion_filter(counts.shape[0], _p(counts, I32), _p(index, I32),
+                           index.shape[1], _p(i32(excl_start), I32),
+                           _p(i32(excl_atoms), I32))
+    return counts, index
+
+
+class Oracle:
+    """The oracle bound to one system (mdir_amd.systems.System)."""
+
+    def __init__(self, system):
+        self.s = system
+        self._keep = []
+        s = system
+        self.q = f64(s.charges)
+        self.types = i32(s.types)
+        self.sig = f64(s.sigma)
+        self.eps = f64(s.epsilon)
+        self.mass = f64(s.masses)
+        self.excl_start, self.excl_atoms = s.exclusion_csr()
+        self.excl_pairs = i32(s.exclusions.reshape(-1, 2))
+        self.waters = i32(s.waters)
+        st = OrcSystem()
+        st.n = s.n
+        st.q = _p(self.q, D)
+        st.type = _p(self.types, I32)
+        st.ntypes = s.ntypes
+        st.sigma = _p(self.sig, D)
+        st.epsilon = _p(self.eps, D)
+        st.mass = _p(self.mass, D)
+        for k in range(3):
+            st.box[k] = float(s.box[k])
+            st.K[k] = int(s.pme_grid[k])
+        st.cutoff = s.cutoff
+        st.switch_from = s.switch_from
+        st.lj_modifier = s.lj_modifier
+        st.coulomb_shift = int(s.coulomb_shift)
+        st.dispersion = int(s.dispersion)
+        st.beta = s.beta
+        from mdir_amd.systems import COULOMB, TIP3P
+        st.coulomb = COULOMB
+        st.excl_start = _p(self.excl_start, I32)
+        st.excl_atoms = _p(self.excl_atoms, I32)
+        st.nexcl_pairs = self.excl_pairs.shape[0]
+        st.excl_pairs = _p(self.excl_pairs, I32)
+        st.order = s.pme_order
+        st.nwaters = len(s.waters)
+        st.water_first = _p(self.waters, I32)
+        if len(s.waters):
+            st.m_o = float(s.masses[s.waters[0]])
+            st.m_h = float(s.masses[s.waters[0] + 1])
+        else:
+            st.m_o, st.m_h = TIP3P["m_o"], TIP3P["m_h"]
+        st.d_oh, st.d_hh = s.water_geometry
+        self.st = st
+
+    def total(self, x=None, mixed=False):
+        """Forces (n,3) and the energy terms (ORC_NTERMS,)."""
+        x = f64(self.s.positions if x is None else x)
+        F = np.zeros_like(x)
+        e = np.zeros(NTERMS)
+        lib().orc_total(C.byref(self.st), _p(x, D), _p(F, D), _p(e, D),
+                        int(mixed))
+        if self.s.pairs14.shape[0]:
+            el, ec = D(0.0), D(0.0)
+            lib().orc_pairs14(self.s.pairs14.shape[0], _p(i32(self.s.pairs14), I32),
+                              _p(f64(self.s.pairs14_sigma), D),
+                              _p(f64(self.s.pairs14_eps), D),
+                              _p(f64(self.s.pairs14_qq), D), _p(x, D),
+                              _p(f64(self.s.box), D), self.s.cutoff,
+                              self.s.switch_from, self.s.lj_modifier,
+                              _p(F, D), C.byref(el), C.byref(ec))
+            e[E_LJ14] += el.value
+            e[E_COUL14] += ec.value
+        return F, e
+
+    def nonbonded(self, x=None, virial=False):
+        s = self.s
+        x = f64(s.positions if x is None else x)
+        F = np.zeros_like(x)
+        e = np.zeros(NTERMS)
+        w = np.zeros(9)
+        from mdir_amd.systems import COULOMB
+        lib().orc_nonbonded(s.n, _p(x, D), _p(self.q, D), _p(self.types, I32),
+                            s.ntypes, _p(self.sig, D), _p(self.eps, D),
+                            _p(f64(s.box), D), s.cutoff, s.switch_from,
+                            s.lj_modifier, int(s.coulomb_shift), s.beta,
+                            COULOMB, _p(self.excl_start, I32),
+                            _p(self.excl_atoms, I32), 0, _p(F, D), _p(e, D),
+                            _p(w, D) if virial else None)
+        return (F, e, w.reshape(3, 3)) if virial else (F, e)
+
+    def pme(self, x=None, virial=False):
+        s = self.s
+        x = f64(s.positions if x is None else x)
+        F = np.zeros_like(x)
+        w = np.zeros(6)
+        K = np.array(s.pme_grid, np.int64)
+        from mdir_amd.systems import COULOMB
+        e = lib().orc_pme(s.n, _p(x, D), _p(self.q, D), _p(f64(s.box), D),
+                          _p(K, I64), s.pme_order, s.beta, COULOMB, _p(F, D),
+                          _p(w, D))
+        return (F, e, w) if virial else (F, e)
+
+    def ewald_reciprocal(self, x=None, forces=True):
+        s = self.s
+        x = f64(s.positions if x is None else x)
+        F = np.zeros_like(x) if forces else None
+        from mdir_amd.systems import COULOMB
+        e = lib().orc_ewald_reciprocal(s.n, _p(x, D), _p(self.q, D),
+                                       _p(f64(s.box), D), s.beta, COULOMB,
+                                       _p(F, D) if forces else None)
+        return (F, e) if forces else e
+
+    def settle_positions(self, x_old, x_new):
+        s = self.s
+        out = f64(x_new).copy()
+        lib().orc_settle_positions(len(s.waters), _p(self.waters, I32),
+                                   self.st.m_o, self.st.m_h, self.st.d_oh,
+                                   self.st.d_hh, _p(f64(x_old), D), _p(out, D))
+        return out
+
+    def settle_velocities(self, x, v):
+        s = self.s
+        out = f64(v).copy()
+        lib().orc_settle_velocities(len(s.waters), _p(self.waters, I32),
+                                    self.st.m_o, self.st.m_h, _p(f64(x), D),
+                                    _p(out, D))
+        return out
+
+    def nve(self, x, v, dt, steps):
+        x = f64(x).copy()
+        v = f64(v).copy()
+        trace = np.zeros((steps, 2))
+        lib().orc_nve(C.byref(self.st), _p(x, D), _p(v, D), dt, steps,
+                      _p(trace, D))
+        return x, v, trace
+
+    def initial_velocities(self, seed, temperature):
+        s = self.s
+        v = np.zeros((s.n, 3))
+        lib().orc_initial_velocities(seed, s.n, _p(self.mass, D), temperature,
+                                     _p(f64(s.positions), D), len(s.waters),
+                                     _p(self.waters, I32),
+                                     s.degrees_of_freedom(), _p(v, D))
+        return v
+
+
+# ---- FFT helpers ----
+
+def fft3d_r2c(real):
+    real = f64(real)
+    K = np.array(real.shape, np.int64)
+    out = np.zeros((K[0], K[1], K[2] // 2 + 1, 2))
+    lib().orc_fft3d_r2c(_p(K, I64), _p(real, D), _p(out, D))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def fft3d_c2r(cplx, k3):
+    cplx = np.ascontiguousarray(cplx)
+    K = np.array([cplx.shape[0], cplx.shape[1], k3], np.int64)
+    inter = f64(np.stack([cplx.real, cplx.imag], -1))
+    out = np.zeros(tuple(K))
+    lib().orc_fft3d_c2r(_p(K, I64), _p(inter, D), _p(out, D))
+    return out
+
+
+def pme_spread(system, x=None):
+    s = system
+    x = f64(s.positions if x is None else x)
+    K = np.array(s.pme_grid, np.int64)
+    grid = np.zeros(tuple(K))
+    lib().orc_pme_spread(s.n, _p(x, D), _p(f64(s.charges), D),
+                         _p(f64(s.box), D), _p(K, I64), s.pme_order,
+                         _p(grid, D))
+    return grid
+
+
+def pme_influence(system):
+    from mdir_amd.systems import COULOMB
+    s = system
+    K = np.array(s.pme_grid, np.int64)
+    bc = np.zeros((K[0], K[1], K[2] // 2 + 1))
+    lib().orc_pme_influence(_p(f64(s.box), D), _p(K, I64), s.pme_order, s.beta,
+                            COULOMB, _p(bc, D))
+    return bc
+
+
+def bspline(w, order):
+    M = np.zeros(order)
+    dM = np.zeros(order)
+    lib().orc_bspline(w, order, _p(M, D), _p(dM, D))
+    return M, dM
+
+
+# ---- the reference's own host FFT (oracle/_ref) ----
+
+class _Buffer(C.Structure):
+    _fields_ = [("allocated", C.c_void_p), ("aligned", C.c_void_p),
+                ("offset", I64), ("size", I64), ("stride", I64)]
+
+
+def ref_fft_available():
+    return os.path.exists(REF_FFT)
+
+
+def ref_fft3d_r2c(real):
+    """runtime/mdrt_fft.c _mlir_ciface_mdrtFFT64Forward3D (built into _ref)."""
+    L = C.CDLL(REF_FFT)
+    real = f64(real).copy()
+    K = real.shape
+    out = np.zeros((K[0], K[1], K[2] // 2 + 1, 2))
+    rb = _Buffer(real.ctypes.data, real.ctypes.data, 0, real.size, 1)
+    cb = _Buffer(out.ctypes.data, out.ctypes.data, 0, out.size, 1)
+    L._mlir_ciface_mdrtFFT64Forward3D(C.byref(rb), C.byref(cb), I64(K[0]),
+                                      I64(K[1]), I64(K[2]))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def ref_fft3d_c2r(cplx, k3):
+    L = C.CDLL(REF_FFT)
+    inter = f64(np.stack([cplx.real, cplx.imag], -1)).copy()
+    K = (cplx.shape[0], cplx.shape[1], k3)
+    out = np.zeros(K)
+    cb = _Buffer(inter.ctypes.data, inter.ctypes.data, 0, inter.size, 1)
+    rb = _Buffer(out.ctypes.data, out.ctypes.data, 0, out.size, 1)
+    L._mlir_ciface_mdrtFFT64Backward3D(C.byref(cb), C.byref(rb), I64(K[0]),
+                                       I64(K[1]), I64(K[2]))
+    return out
+
+
+def rel_rms(a, b):
+    """sqrt(sum|a-b|^2 / sum|b|^2) — test/Driver/Inputs/compare_forces_rms.py:20-22."""
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b * b)))
