@@ -1,0 +1,201 @@
+"""ns/day of the ~24k-atom PME water box on one MI355X.
+
+Workload: 7,900 TIP3P waters (23,700 atoms) in a 6.2 nm cube, seeded
+jittered lattice relaxed by 200 steepest-descent steps, PME with rc 0.9 nm,
+Ewald tolerance 1e-5, 64^3 grid, order 4, SETTLE waters, NVE velocity
+Verlet at 2 fs, mixed precision (f32 pair/PME arithmetic, 64-bit fixed-point
+force words, f64 integration and constraints).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+A step is one velocity-Verlet step of the whole box (neighbor validity
+test and conditional rebuild, pair loop, excluded pairs, PME, integration),
+replayed from captured HIP graphs.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Algorithmic work per unit, fixed before measuring.
+FLOP_PER_COULOMB_PAIR = 50.0     # force-only erfc Ewald pair
+FLOP_PER_LJ_COULOMB_PAIR = 60.0  # LJ + erfc Ewald pair
+PEAK_FP32_TFLOPS = 157.3         # MI355X FP32 (vector and matrix)
+PEAK_HBM_GBS = 8000.0            # MI355X HBM3E
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5000)
+    p.add_argument("--warmup", type=int, default=500)
+    p.add_argument("--graph", type=int, default=50, help="steps per captured graph")
+    p.add_argument("--skin", type=float, default=0.1)
+    p.add_argument("--seed", type=int, default=2024)
+    p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--profile-repeats", type=int, default=50)
+    return p.parse_args()
+
+
+def total_energy(sim):
+    _, e = sim.forces()
+    return float(e.sum()), e
+
+
+def pme_bytes(n_atoms, K):
+    """Algorithmic HBM bytes per launch of the PME kernels (64^3 f32 grid):
+    spread reads N x 28 B (x f64 x3 + q f32) and writes K^3 x 8 B words;
+    each FFT pass reads and writes the half spectrum (8 B per point) or the
+    real grid; gather reads K^3 x 4 B + N x 28 B and writes N x 24 B."""
+    k3 = K[0] * K[1] * K[2]
+    half = K[0] * K[1] * (K[2] // 2 + 1) * 8
+    return {
+        "pme_spread": n_atoms * 28 + k3 * 8,
+        "fft_rows_r2c": k3 * 8 + half,
+        "fft_cols_y_fwd": 2 * half,
+        "fft_x_conv": 2 * half,
+        "fft_cols_y_bwd": 2 * half,
+        "fft_rows_c2r": half + k3 * 4,
+        "pme_gather": k3 * 4 + n_atoms * 28 + n_atoms * 24,
+    }
+
+
+def cpu_baseline(system, x, v, dt, seconds):
+    """The oracle's C/OpenMP restatement of the reference CPU path (f32 pair
+    arithmetic as its mixed mode, f64 PME and SETTLE), timed on this host."""
+    threads = len(os.sched_getaffinity(0))
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    from oracle import oracle as O
+    o = O.Oracle(system)
+    used = O.lib().orc_threads()
+    t0 = time.perf_counter()
+    o.nve(x, v, dt, 1, mixed=True)
+    one = time.perf_counter() - t0
+    steps = max(1, int(seconds / max(one, 1e-3)))
+    t0 = time.perf_counter()
+    o.nve(x, v, dt, steps, mixed=True)
+    elapsed = time.perf_counter() - t0
+    per_step = elapsed / steps
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": round(dt * 1e-3 * 86400.0 / per_step, 4), "unit": "ns/day",
+            "cores": used, "kind": "port",
+            "ms_per_step": round(per_step * 1e3, 3),
+            "sample": f"{steps} velocity-Verlet steps of the same 23,700-atom box "
+                      f"(oracle: f32 pair rows, f64 PME, SETTLE) on {used} threads"
+                      + (f" of {model}" if model else "")}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
+              "(every box here has one MI355X; the path shards as described in "
+              "DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    from mdir_amd import systems as S
+    from mdir_amd.simulation import Simulation
+
+    dt = 0.002
+    system = S.tip3p_config2(seed=args.seed)
+    sim = Simulation(system, skin=args.skin)
+    sim.minimize(200, 0.01)
+    x = sim.positions
+    relaxed = system.copy()
+    relaxed.positions = x
+    v = S.initial_velocities(relaxed, 42, 300.0)
+    sim.velocities = v
+    print(f"setup: {system.n} atoms minimised, velocities at 300 K", file=sys.stderr,
+          flush=True)
+
+    sim.run(args.warmup, dt, graph=args.graph)
+    e0, _ = total_energy(sim)
+    t0 = time.perf_counter()
+    sim.run(args.steps, dt, graph=args.graph)   # returns after a device sync
+    t1 = time.perf_counter()
+    e1, terms = total_energy(sim)
+    elapsed = t1 - t0
+    ms_per_step = elapsed / args.steps * 1e3
+    ns_day = args.steps * dt * 1e-3 / (elapsed / 86400.0)
+    drift = (e1 - e0) / (args.steps * dt) / system.n  # kJ/mol/ps per atom
+    print(f"timed: {args.steps} steps in {elapsed:.3f} s, {ns_day:.1f} ns/day",
+          file=sys.stderr, flush=True)
+
+    # per-kernel device time (HIP events on the kernels' stream)
+    prof = sim.profile_step(args.profile_repeats)
+    # pairs within rc for the algorithmic FLOP count (f32 test of the
+    # reference matrix, exclusions removed)
+    pairs = sim.neighbor_pairs(system.cutoff, excluded=True)
+    types = system.types
+    both_o = (types[pairs[:, 0]] == 0) & (types[pairs[:, 1]] == 0)
+    n_lj = int(both_o.sum())
+    n_pairs = int(len(pairs))
+    flops = FLOP_PER_LJ_COULOMB_PAIR * n_lj + FLOP_PER_COULOMB_PAIR * (n_pairs - n_lj)
+    t_pairs = prof["nb_pairs"] * 1e-3
+    pair_tflops = flops / t_pairs / 1e12
+    kernels_ms = {k: round(v_, 5) for k, v_ in prof.items()}
+    dominant = max(prof, key=prof.get)
+    pmeb = pme_bytes(system.n, system.pme_grid)
+    hbm = {k: round(b / (prof[k] * 1e-3) / 1e9, 1) for k, b in pmeb.items()}
+
+    roofline = {"kernel": "nb_pairs", "bound": "mfma", "pipe": "fp32 vector ALU",
+                "achieved": round(pair_tflops, 3), "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(pair_tflops / PEAK_FP32_TFLOPS, 4),
+                "traffic": None,
+                "work": {"pairs_within_rc": n_pairs, "lj_pairs": n_lj,
+                         "flop_per_pair": [FLOP_PER_COULOMB_PAIR, FLOP_PER_LJ_COULOMB_PAIR],
+                         "flop_per_launch": flops}}
+    if dominant != "nb_pairs" and dominant in pmeb:
+        roofline = {"kernel": dominant, "bound": "hbm",
+                    "achieved": hbm[dominant], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": round(hbm[dominant] / PEAK_HBM_GBS, 4), "traffic": None,
+                    "work": {"bytes_per_launch": pmeb[dominant]}}
+
+    out = {
+        "metric": "ns/day on ~24k-atom PME water box (1 GPU)",
+        "value": round(ns_day, 2),
+        "unit": "ns/day",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "tip3p_7900_waters_pme64", "atoms": system.n,
+                   "box_nm": 6.2, "cutoff_nm": system.cutoff, "pme_grid": list(system.pme_grid),
+                   "pme_order": system.pme_order, "ewald_beta": system.beta,
+                   "dt_fs": 2.0, "ensemble": "NVE", "constraints": "SETTLE",
+                   "list_skin_nm": args.skin, "graph_steps": args.graph},
+        "roofline": roofline,
+        "kernels_ms": kernels_ms,
+        "pme_hbm_gbs": hbm,
+        "pair_tflops": round(pair_tflops, 3),
+        "nve_drift_kj_mol_ps_atom": drift,
+        "energy_terms_kj_mol": [round(float(t), 4) for t in terms],
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(system, sim.positions, sim.velocities, dt,
+                                           args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -40,12 +40,16 @@ struct HipError : std::runtime_error {
       throw ::mdhip::HipError(std::string(#expr) + ": " + hipGetErrorString(_e));  \
   } while (0)
 
+// Setup allocations are cleared and uploaded through the null stream and
+// then waited for on the whole device: the engine's own stream is
+// non-blocking and would not otherwise be ordered after them.
 template <typename T>
 static T *dalloc(size_t count) {
   void *p = nullptr;
   if (count == 0) count = 1;
   HIPCHK(hipMalloc(&p, count * sizeof(T)));
   HIPCHK(hipMemset(p, 0, count * sizeof(T)));
+  HIPCHK(hipDeviceSynchronize());
   return static_cast<T *>(p);
 }
 
@@ -59,6 +63,7 @@ template <typename T>
 static T *dupload(const std::vector<T> &v) {
   T *p = dalloc<T>(v.size());
   if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  HIPCHK(hipDeviceSynchronize());
   return p;
 }
 
@@ -222,6 +227,19 @@ struct Engine {
 
   ~Engine() { release(); }
 
+  // Transfers ordered on the engine's stream and complete on return.
+  void to_host(void *dst, const void *src, size_t bytes) {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+  void to_device(void *dst, const void *src, size_t bytes) {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+  void on_device(void *dst, const void *src, size_t bytes) {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream));
+  }
+
   void release() {
     if (graph_exec) hipGraphExecDestroy(graph_exec);
     graph_exec = nullptr;
@@ -233,6 +251,7 @@ struct Engine {
     dfree(d_p14_eps); dfree(d_p14_qq); dfree(d_espec);
     fft.release();
     dfree(d_grid); dfree(d_cplx); dfree(d_phi); dfree(d_tab); dfree(d_erecip);
+    dfree(d_tile_idx); dfree(d_tiles);
     dfree(d_unit_first); dfree(d_unit_size); dfree(d_kin);
     if (stream) hipStreamDestroy(stream);
     stream = nullptr;
@@ -291,6 +310,7 @@ struct Engine {
     max_places = g.max_clusters * kCluster;
     d_atom_at = dalloc<int>(max_places);
     HIPCHK(hipMemset(d_atom_at, 0xFF, max_places * sizeof(int)));  // -1: no atom
+    HIPCHK(hipDeviceSynchronize());
     d_type_at = dalloc<int>(max_places);
     d_xq = dalloc<float4>(max_places);
     d_F = dalloc<long long>(3 * (size_t)max_places);
@@ -353,6 +373,7 @@ struct Engine {
     HIPCHK(hipMemcpy(d_x, d->positions, 3 * n * sizeof(double), hipMemcpyHostToDevice));
     if (d->velocities)
       HIPCHK(hipMemcpy(d_v, d->velocities, 3 * n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipDeviceSynchronize());
     d_q = dupload(h_q);
     std::vector<double> im(n);
     for (int i = 0; i < n; ++i) im[i] = h_mass[i] > 0 ? 1.0 / h_mass[i] : 0.0;
@@ -507,12 +528,12 @@ struct Engine {
         pmp.box[k] = box[k];
         pmp.inv_box[k] = 1.0 / box[k];
       }
-      if (pme_order > 8 || pme_order < 3) throw HipError("PME order must be in [3, 8]");
+      if (pme_order != 4 && pme_order != 6 && pme_order != 8)
+        throw HipError("PME order must be 4, 6 or 8");
       pmp.order = pme_order;
       fft.make(pme_grid);
       const size_t np = (size_t)pme_grid[0] * pme_grid[1] * pme_grid[2];
       const size_t nh = (size_t)pme_grid[0] * pme_grid[1] * (pme_grid[2] / 2 + 1);
-      d_grid = dalloc<unsigned long long>(np);
       d_cplx = dalloc<cpx<float>>(nh);
       d_phi = dalloc<float>(np);
       make_influence_tables();
@@ -555,6 +576,7 @@ struct Engine {
       }
     }
     configure_list(cutoff + skin);
+    if (use_pme) make_tiles();
     HIPCHK(hipStreamSynchronize(stream));
   }
 
@@ -643,20 +665,105 @@ struct Engine {
         d_place_of, spp, d_F, energy ? d_espec : nullptr);
   }
 
+  // ---- spread tiles of the xy columns ----
+  TileParams tp{};
+  int *d_tile_idx = nullptr;   // gx0 | gy0 | xcov | ycov
+  float *d_tiles = nullptr;
+  size_t tile_lds = 0;
+
+  void make_tiles() {
+    dfree(d_tile_idx);
+    dfree(d_tiles);
+    const double h = 0.5 * skin;
+    int TW[2];
+    std::vector<int> g0[2], cov[2];
+    for (int a = 0; a < 2; ++a) {
+      const int nc = a == 0 ? g.ncx : g.ncy;
+      const double w = a == 0 ? g.wx : g.wy;
+      const int K = pme_grid[a];
+      const double L = box[a];
+      int tw = 0;
+      g0[a].resize(nc);
+      for (int c = 0; c < nc; ++c) {
+        const long lo = (long)floor(K / L * (c * w - h)) - pme_order;      // one point of margin
+        const long hi = (long)floor(K / L * ((c + 1) * w + h)) + 1;
+        g0[a][c] = (int)(((lo % K) + K) % K);
+        tw = std::max(tw, (int)(hi - lo + 1));
+      }
+      if (tw >= K) tw = K;
+      TW[a] = tw;
+      cov[a].assign((size_t)K * kMaxCover, -1);
+      std::vector<int> fill(K, 0);
+      for (int c = 0; c < nc; ++c)
+        for (int o = 0; o < tw; ++o) {
+          const int pnt = (g0[a][c] + o) % K;
+          if (fill[pnt] >= kMaxCover) throw HipError("PME tiles: a grid line is covered by too many columns");
+          cov[a][(size_t)pnt * kMaxCover + fill[pnt]++] = (c << 16) | o;
+        }
+    }
+    std::vector<int> idx;
+    idx.insert(idx.end(), g0[0].begin(), g0[0].end());
+    idx.insert(idx.end(), g0[1].begin(), g0[1].end());
+    idx.insert(idx.end(), cov[0].begin(), cov[0].end());
+    idx.insert(idx.end(), cov[1].begin(), cov[1].end());
+    d_tile_idx = dupload(idx);
+    tp.K0 = pme_grid[0];
+    tp.TWx = TW[0];
+    tp.TWy = TW[1];
+    tp.ncx = g.ncx;
+    tp.gx0 = d_tile_idx;
+    tp.gy0 = d_tile_idx + g.ncx;
+    tp.xcov = d_tile_idx + g.ncx + g.ncy;
+    tp.ycov = tp.xcov + (size_t)pme_grid[0] * kMaxCover;
+    const int K2 = pme_grid[2];
+    const size_t fixed = (size_t)kTileAtoms * (3 * sizeof(int) + 3 * pme_order * sizeof(float));
+    const size_t budget = 64 * 1024 - fixed - 512;
+    int zc = (int)(budget / ((size_t)TW[0] * TW[1] * sizeof(float))) - 1;
+    if (zc < 1) throw HipError("PME tiles do not fit in LDS");
+    zc = std::min(zc, K2);
+    tp.nzc = (K2 + zc - 1) / zc;
+    tp.zc = (K2 + tp.nzc - 1) / tp.nzc;
+    tile_lds = (size_t)TW[0] * TW[1] * (tp.zc + 1) * sizeof(float);
+    d_tiles = dalloc<float>((size_t)g.ncol * TW[0] * TW[1] * K2);
+  }
+
   void pme_spread_launch() {
-    const int nn = pme_order * pme_order;
-    pme_spread<<<blocks_for((size_t)max_places * nn), 256, 0, stream>>>(
-        d_counts, d_atom_at, d_x, d_q, pmp, d_grid);
+    const int blocks = g.ncol * tp.nzc;
+    if (pme_order == 4)
+      pme_spread_tiles<4><<<blocks, 256, tile_lds, stream>>>(d_col_cl_start, d_col_cl_count, d_atom_at, d_x, d_q, pmp, tp, d_tiles, d_flags);
+    else if (pme_order == 6)
+      pme_spread_tiles<6><<<blocks, 256, tile_lds, stream>>>(d_col_cl_start, d_col_cl_count, d_atom_at, d_x, d_q, pmp, tp, d_tiles, d_flags);
+    else
+      pme_spread_tiles<8><<<blocks, 256, tile_lds, stream>>>(d_col_cl_start, d_col_cl_count, d_atom_at, d_x, d_q, pmp, tp, d_tiles, d_flags);
+  }
+
+  void pme_rows_forward() {
+    const int TL = Fft3D<float>::TL;
+    const int nrows = fft.K[0] * fft.K[1];
+    fft_rows_r2c_tiles<float><<<(nrows + 2 * TL - 1) / (2 * TL), 256, fft.lds(fft.K[2]), stream>>>(
+        d_tiles, tp, d_cplx, fft.K[1], fft.ax[2].axis);
+  }
+
+  void pme_gather_launch() {
+    if (pme_order == 4)
+      pme_gather_lanes<4, 4><<<blocks_for((size_t)max_places * 4), 256, 0, stream>>>(d_counts, d_atom_at, d_x, d_q, pmp, d_phi, d_F);
+    else if (pme_order == 6)
+      pme_gather_lanes<6, 8><<<blocks_for((size_t)max_places * 8), 256, 0, stream>>>(d_counts, d_atom_at, d_x, d_q, pmp, d_phi, d_F);
+    else
+      pme_gather_lanes<8, 8><<<blocks_for((size_t)max_places * 8), 256, 0, stream>>>(d_counts, d_atom_at, d_x, d_q, pmp, d_phi, d_F);
   }
 
   void pme(bool energy) {
     if (!use_pme) return;
     pme_spread_launch();
-    fft.forward(d_grid, true, d_cplx, stream);
+    pme_rows_forward();
+    const int TL = Fft3D<float>::TL;
+    const int tiles = (fft.H() + TL - 1) / TL;
+    fft_cols_y<float><<<fft.K[0] * tiles, 256, fft.lds(fft.K[1]), stream>>>(
+        d_cplx, fft.K[1], fft.H(), fft.ax[1].axis, -1);
     fft.x_pass(d_cplx, 0, ct, energy ? d_erecip : nullptr, stream);
     fft.backward(d_cplx, d_phi, stream);
-    pme_gather<<<blocks_for(max_places), 256, 0, stream>>>(d_counts, d_atom_at, d_x, d_q,
-                                                           pmp, d_phi, d_F);
+    pme_gather_launch();
   }
 
   void forces(bool energy) {
@@ -705,6 +812,10 @@ struct Engine {
 
   // Overflow of a column bucket or a row: grow and report true.
   bool grow_if_overflowed(const std::vector<int> &f) {
+    if (f[kFlagPmeRange]) {
+      clear_error_flags();
+      throw HipError("an atom left the PME tile of its column");
+    }
     if (f[kFlagOverflowRow] & 2) throw HipError("more than 128 exclusions within one cluster");
     if (!f[kFlagOverflowColumn] && !f[kFlagOverflowRow]) return false;
     int col_min = f[kFlagOverflowColumn] ? g.col_cap * 2 : g.col_cap;
@@ -744,10 +855,10 @@ struct Engine {
     if (E_out) {
       for (int k = 0; k < MDRT_NUM_ENERGIES; ++k) E_out[k] = 0.0;
       std::vector<int> counts(4);
-      HIPCHK(hipMemcpy(counts.data(), d_counts, 4 * sizeof(int), hipMemcpyDeviceToHost));
+      to_host(counts.data(), d_counts, 4 * sizeof(int));
       const int ncl = counts[1];
       std::vector<double> ep(2 * (size_t)ncl);
-      if (ncl) HIPCHK(hipMemcpy(ep.data(), d_epair, ep.size() * sizeof(double), hipMemcpyDeviceToHost));
+      if (ncl) to_host(ep.data(), d_epair, ep.size() * sizeof(double));
       for (int c = 0; c < ncl; ++c) {
         E_out[MDRT_E_LJ] += ep[2 * c];
         E_out[MDRT_E_COUL_DIRECT] += ep[2 * c + 1];
@@ -755,7 +866,7 @@ struct Engine {
       if (nspecial) {
         const int nb = blocks_for(nspecial);
         std::vector<double> es(3 * (size_t)nb);
-        HIPCHK(hipMemcpy(es.data(), d_espec, es.size() * sizeof(double), hipMemcpyDeviceToHost));
+        to_host(es.data(), d_espec, es.size() * sizeof(double));
         for (int b = 0; b < nb; ++b) {
           E_out[MDRT_E_COUL_EXCL] += es[3 * b];
           E_out[MDRT_E_LJ14] += es[3 * b + 1];
@@ -764,7 +875,7 @@ struct Engine {
       }
       if (use_pme) {
         std::vector<double> er(fft.x_blocks());
-        HIPCHK(hipMemcpy(er.data(), d_erecip, er.size() * sizeof(double), hipMemcpyDeviceToHost));
+        to_host(er.data(), d_erecip, er.size() * sizeof(double));
         for (double e : er) E_out[MDRT_E_COUL_RECIP] += e;
       }
       E_out[MDRT_E_COUL_SELF] = e_self;
@@ -775,7 +886,7 @@ struct Engine {
 
   double kinetic_energy() {
     std::vector<double> v(3 * (size_t)n);
-    HIPCHK(hipMemcpy(v.data(), d_v, v.size() * sizeof(double), hipMemcpyDeviceToHost));
+    to_host(v.data(), d_v, v.size() * sizeof(double));
     double ke = 0;
     for (int i = 0; i < n; ++i)
       ke += 0.5 * h_mass[i] * (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
@@ -846,12 +957,20 @@ struct Engine {
                                                       d_x, d_F, d_place_of, h / fmax, ip);
       std::vector<double> F1(3 * (size_t)n);
       double E1[MDRT_NUM_ENERGIES];
-      compute(F1.data(), E1);
-      const double e1 = potential(E1);
+      double e1;
+      try {
+        compute(F1.data(), E1);
+        e1 = potential(E1);
+      } catch (const HipError &) {
+        // a trial too wild to evaluate (an atom outside its PME tile, a
+        // list that cannot hold it) is a rejected step
+        clear_error_flags();
+        e1 = INFINITY;
+      }
       if (e1 < e0) {
         e0 = e1;
         F.swap(F1);
-        h *= 1.2;
+        h = std::min(1.2 * h, max_step);  // steepest descent: grow, capped
       } else {
         HIPCHK(hipMemcpyAsync(d_x, d_x0, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, stream));
         compute(F.data(), E);
@@ -900,7 +1019,7 @@ struct Engine {
     HIPCHK(hipStreamSynchronize(stream));
     if (cnt > cap2) cnt = cap2;
     std::vector<int2> h(cnt);
-    if (cnt) HIPCHK(hipMemcpy(h.data(), d_pairs, cnt * sizeof(int2), hipMemcpyDeviceToHost));
+    if (cnt) to_host(h.data(), d_pairs, cnt * sizeof(int2));
     dfree(d_pairs);
     dfree(d_count);
     std::vector<long long> keys(cnt);
@@ -919,9 +1038,9 @@ struct Engine {
 
   void stats(int64_t *s) {
     std::vector<int> counts(4);
-    HIPCHK(hipMemcpy(counts.data(), d_counts, 4 * sizeof(int), hipMemcpyDeviceToHost));
+    to_host(counts.data(), d_counts, 4 * sizeof(int));
     std::vector<int> rc(g.max_clusters);
-    HIPCHK(hipMemcpy(rc.data(), d_row_count, rc.size() * sizeof(int), hipMemcpyDeviceToHost));
+    to_host(rc.data(), d_row_count, rc.size() * sizeof(int));
     int64_t entries = 0, largest = 0;
     for (int c = 0; c < counts[1]; ++c) {
       entries += rc[c];
@@ -934,54 +1053,76 @@ struct Engine {
     s[4] = g.row_cap;
     s[5] = g.ncol;
     s[6] = g.col_cap;
-    s[7] = max_places;
+    std::vector<int> f = read_flags();
+    s[7] = f[kFlagBuilds];
   }
 
-  // Device-event timing of the parts of one step.
+  // Device-event timing of every kernel of one step, on the stream the
+  // kernels run on: ms per launch, averaged over `repeats` steps.
+  //  0 integrate_first  1 nb_bin  2 nb_sort  3 nb_search  4 nb_pairs
+  //  5 nb_special  6 pme_spread  7 fft_rows_r2c  8 fft_cols_y (forward)
+  //  9 fft_x_conv  10 fft_cols_y (backward)  11 fft_rows_c2r  12 pme_gather
+  //  13 integrate_second
+  static constexpr int kProfileSlots = 14;
   void profile(int repeats, double *ms) {
     if (!forces_valid) evaluate(false);
-    hipEvent_t ev[8];
+    hipEvent_t ev[kProfileSlots + 1];
     for (auto &e : ev) HIPCHK(hipEventCreate(&e));
-    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    double acc[kProfileSlots] = {0};
+    const int nrows = fft.K[0] * fft.K[1];
+    const int TL = Fft3D<float>::TL;
+    const int tiles = (fft.H() + TL - 1) / TL;
     for (int r = 0; r < repeats; ++r) {
-      HIPCHK(hipEventRecord(ev[0], stream));
+      int k = 0;
+      HIPCHK(hipEventRecord(ev[k++], stream));
       step_first(0.002);
-      HIPCHK(hipEventRecord(ev[1], stream));
-      build();
-      HIPCHK(hipEventRecord(ev[2], stream));
+      HIPCHK(hipEventRecord(ev[k++], stream));
+      nb_bin<<<blocks_for(n), 256, 0, stream>>>(n, d_x, d_shift, d_xref, d_col_count, d_bucket, d_flags, g);
+      HIPCHK(hipEventRecord(ev[k++], stream));
+      nb_sort<<<g.ncol, 256, 0, stream>>>(d_x, d_shift, d_q, d_type, d_col_count, d_bucket,
+                                           d_atom_at, d_place_of, d_xq, d_type_at, d_bb_lo,
+                                           d_bb_hi, d_cluster_col, d_real_mask, d_col_cl_start,
+                                           d_col_cl_count, d_counts, d_flags, g);
+      HIPCHK(hipEventRecord(ev[k++], stream));
+      nb_search<<<blocks_for(g.max_clusters, 4), 256, 0, stream>>>(
+          d_bb_lo, d_bb_hi, d_cluster_col, d_real_mask, d_col_cl_start, d_col_cl_count,
+          d_atom_at, d_place_of, d_excl_start, d_excl_atoms, d_counts, d_col_count, d_rows,
+          d_row_count, d_flags, g);
+      HIPCHK(hipEventRecord(ev[k++], stream));
       pairs(false);
+      HIPCHK(hipEventRecord(ev[k++], stream));
       special(false);
-      HIPCHK(hipEventRecord(ev[3], stream));
+      HIPCHK(hipEventRecord(ev[k++], stream));
       if (use_pme) pme_spread_launch();
-      HIPCHK(hipEventRecord(ev[4], stream));
-      if (use_pme) {
-        fft.forward(d_grid, true, d_cplx, stream);
-        fft.x_pass(d_cplx, 0, ct, nullptr, stream);
-        fft.backward(d_cplx, d_phi, stream);
-      }
-      HIPCHK(hipEventRecord(ev[5], stream));
+      HIPCHK(hipEventRecord(ev[k++], stream));
+      if (use_pme) pme_rows_forward();
+      HIPCHK(hipEventRecord(ev[k++], stream));
       if (use_pme)
-        pme_gather<<<blocks_for(max_places), 256, 0, stream>>>(d_counts, d_atom_at, d_x, d_q,
-                                                               pmp, d_phi, d_F);
-      HIPCHK(hipEventRecord(ev[6], stream));
+        fft_cols_y<float><<<fft.K[0] * tiles, 256, fft.lds(fft.K[1]), stream>>>(
+            d_cplx, fft.K[1], fft.H(), fft.ax[1].axis, -1);
+      HIPCHK(hipEventRecord(ev[k++], stream));
+      if (use_pme) fft.x_pass(d_cplx, 0, ct, nullptr, stream);
+      HIPCHK(hipEventRecord(ev[k++], stream));
+      if (use_pme)
+        fft_cols_y<float><<<fft.K[0] * tiles, 256, fft.lds(fft.K[1]), stream>>>(
+            d_cplx, fft.K[1], fft.H(), fft.ax[1].axis, +1);
+      HIPCHK(hipEventRecord(ev[k++], stream));
+      if (use_pme)
+        fft_rows_c2r<float><<<(nrows + 2 * TL - 1) / (2 * TL), 256, fft.lds(fft.K[2]), stream>>>(
+            d_cplx, d_phi, nrows, fft.ax[2].axis);
+      HIPCHK(hipEventRecord(ev[k++], stream));
+      if (use_pme) pme_gather_launch();
+      HIPCHK(hipEventRecord(ev[k++], stream));
       step_second(0.002, false);
-      HIPCHK(hipEventRecord(ev[7], stream));
-      HIPCHK(hipEventSynchronize(ev[7]));
-      for (int k = 0; k < 7; ++k) {
+      HIPCHK(hipEventRecord(ev[k++], stream));
+      HIPCHK(hipEventSynchronize(ev[kProfileSlots]));
+      for (int j = 0; j < kProfileSlots; ++j) {
         float t = 0;
-        HIPCHK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
-        acc[k] += t;
+        HIPCHK(hipEventElapsedTime(&t, ev[j], ev[j + 1]));
+        acc[j] += t;
       }
     }
-    // integrate (both halves), build, pairs+special, spread, fft, gather
-    ms[0] = (acc[0] + acc[6]) / repeats;
-    ms[1] = acc[1] / repeats;
-    ms[2] = acc[2] / repeats;
-    ms[3] = acc[3] / repeats;
-    ms[4] = acc[4] / repeats;
-    ms[5] = acc[5] / repeats;
-    ms[6] = 0;
-    ms[7] = 0;
+    for (int j = 0; j < kProfileSlots; ++j) ms[j] = acc[j] / repeats;
     for (auto &e : ev) hipEventDestroy(e);
   }
 };
@@ -1134,7 +1275,7 @@ void mdrtHipSystemDestroy(void *system) { delete static_cast<Engine *>(system); 
 int mdrtHipSetPositions(void *system, const double *x) {
   Engine *e = static_cast<Engine *>(system);
   return guarded([&] {
-    HIPCHK(hipMemcpy(e->d_x, x, 3 * e->n * sizeof(double), hipMemcpyHostToDevice));
+    e->to_device(e->d_x, x, 3 * e->n * sizeof(double));
     e->forces_valid = false;
   });
 }
@@ -1142,23 +1283,21 @@ int mdrtHipSetPositions(void *system, const double *x) {
 int mdrtHipSetVelocities(void *system, const double *v) {
   Engine *e = static_cast<Engine *>(system);
   return guarded([&] {
-    HIPCHK(hipMemcpy(e->d_v, v, 3 * e->n * sizeof(double), hipMemcpyHostToDevice));
+    e->to_device(e->d_v, v, 3 * e->n * sizeof(double));
   });
 }
 
 int mdrtHipGetPositions(void *system, double *x) {
   Engine *e = static_cast<Engine *>(system);
   return guarded([&] {
-    HIPCHK(hipStreamSynchronize(e->stream));
-    HIPCHK(hipMemcpy(x, e->d_x, 3 * e->n * sizeof(double), hipMemcpyDeviceToHost));
+    e->to_host(x, e->d_x, 3 * e->n * sizeof(double));
   });
 }
 
 int mdrtHipGetVelocities(void *system, double *v) {
   Engine *e = static_cast<Engine *>(system);
   return guarded([&] {
-    HIPCHK(hipStreamSynchronize(e->stream));
-    HIPCHK(hipMemcpy(v, e->d_v, 3 * e->n * sizeof(double), hipMemcpyDeviceToHost));
+    e->to_host(v, e->d_v, 3 * e->n * sizeof(double));
   });
 }
 

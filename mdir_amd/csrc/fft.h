@@ -46,31 +46,31 @@ struct FftAxis {
   const void *tw;  // cpx<T>[K]: exp(-2 pi i m / K)
 };
 
-// y[k] = sum_r v[r] exp(s 2 pi i r k / R)
-template <typename T>
-__device__ __forceinline__ void butterfly(cpx<T> *v, int R, int s) {
-  if (R == 2) {
+// y[k] = sum_r v[r] exp(s 2 pi i r k / R), R known at compile time so that
+// v stays in registers.
+template <typename T, int R>
+__device__ __forceinline__ void butterfly(cpx<T> *v, int s) {
+  if constexpr (R == 2) {
     cpx<T> a = v[0], b = v[1];
     v[0] = cadd(a, b);
     v[1] = csub(a, b);
-  } else if (R == 4) {
+  } else if constexpr (R == 4) {
     cpx<T> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
     cpx<T> b0 = cadd(v[1], v[3]), b1 = cmuli(csub(v[1], v[3]), s);
     v[0] = cadd(a0, b0);
     v[2] = csub(a0, b0);
     v[1] = cadd(a1, b1);
     v[3] = csub(a1, b1);
-  } else if (R == 8) {
+  } else if constexpr (R == 8) {
     const T h = (T)0.70710678118654752440;
-    // radix-2 split: even / odd, each a radix-4
     cpx<T> e[4] = {v[0], v[2], v[4], v[6]};
     cpx<T> o[4] = {v[1], v[3], v[5], v[7]};
-    butterfly<T>(e, 4, s);
-    butterfly<T>(o, 4, s);
-    // twiddles w8^k, k = 0..3 with sign s
-    cpx<T> t1 = {h * (o[1].x - s * o[1].y), h * (o[1].y + s * o[1].x)};
+    butterfly<T, 4>(e, s);
+    butterfly<T, 4>(o, s);
+    const T ss = (T)s;
+    cpx<T> t1 = {h * (o[1].x - ss * o[1].y), h * (o[1].y + ss * o[1].x)};
     cpx<T> t2 = cmuli(o[2], s);
-    cpx<T> t3 = {h * (-o[3].x - s * o[3].y), h * (-o[3].y + s * o[3].x)};
+    cpx<T> t3 = {h * (-o[3].x - ss * o[3].y), h * (-o[3].y + ss * o[3].x)};
     v[0] = cadd(e[0], o[0]);
     v[4] = csub(e[0], o[0]);
     v[1] = cadd(e[1], t1);
@@ -80,28 +80,58 @@ __device__ __forceinline__ void butterfly(cpx<T> *v, int R, int s) {
     v[3] = cadd(e[3], t3);
     v[7] = csub(e[3], t3);
   } else {
-    // radices 3, 5, 7: direct sum with exact constants
-    cpx<T> in[7];
+    // radices 3, 5, 7: direct sum, constants folded at compile time
+    cpx<T> in[R];
+#pragma unroll
     for (int r = 0; r < R; ++r) in[r] = v[r];
+#pragma unroll
     for (int k = 0; k < R; ++k) {
       cpx<T> acc = in[0];
+#pragma unroll
       for (int r = 1; r < R; ++r) {
         const int m = (r * k) % R;
-        const double ang = 6.283185307179586476925286766559 * m / R;
-        const T c = (T)cos(ang), sn = (T)(s * sin(ang));
-        acc.x += in[r].x * c - in[r].y * sn;
-        acc.y += in[r].x * sn + in[r].y * c;
+        const T c = (T)__builtin_cos(6.283185307179586476925286766559 * m / R);
+        const T sn = (T)__builtin_sin(6.283185307179586476925286766559 * m / R);
+        const T sg = (T)s * sn;
+        acc.x += in[r].x * c - in[r].y * sg;
+        acc.y += in[r].x * sg + in[r].y * c;
       }
       v[k] = acc;
     }
   }
 }
 
+// One Stockham pass of radix R over TL lines (Govindaraju et al., SC'08):
+// after sub-transforms of length Ns, v[r] = in[j + r K/R] is twiddled by
+// w^{(j mod Ns) r K/(Ns R)} and written to out[(j/Ns) Ns R + j mod Ns + r Ns].
+template <typename T, int TL, int R>
+__device__ __forceinline__ void stockham_pass(const cpx<T> *in, cpx<T> *out, int K,
+                                              int Ns, const cpx<T> *tw, int sign) {
+  const int Kr = K / R;
+  const int tstep = K / (Ns * R);
+  for (int w = threadIdx.x; w < TL * Kr; w += blockDim.x) {
+    const int line = w % TL, j = w / TL;
+    cpx<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = in[(j + r * Kr) * TL + line];
+    const int k = j % Ns;
+    if (Ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        cpx<T> t = tw[k * r * tstep];
+        if (sign > 0) t.y = -t.y;
+        v[r] = cmul(v[r], t);
+      }
+    }
+    butterfly<T, R>(v, sign);
+    const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) out[(base + r * Ns) * TL + line] = v[r];
+  }
+}
+
 // Runs all passes of TL lines of length ax.K held in `a` (scratch `b`),
 // sign -1 forward / +1 backward.  Returns the buffer holding the result.
-// Stockham autosort (Govindaraju et al., SC'08): pass with radix R after
-// sub-transforms of length Ns reads v[r] = in[j + r K/R], twiddles it by
-// w^{(j mod Ns) r K/(Ns R)} and writes out[(j/Ns) Ns R + j mod Ns + r Ns].
 template <typename T, int TL>
 __device__ cpx<T> *lds_fft(cpx<T> *a, cpx<T> *b, const FftAxis &ax, int sign) {
   const int K = ax.K;
@@ -110,23 +140,13 @@ __device__ cpx<T> *lds_fft(cpx<T> *a, cpx<T> *b, const FftAxis &ax, int sign) {
   cpx<T> *in = a, *out = b;
   for (int s = 0; s < ax.nrad; ++s) {
     const int R = ax.rad[s];
-    const int Kr = K / R;
-    const int tstep = K / (Ns * R);
-    for (int w = threadIdx.x; w < TL * Kr; w += blockDim.x) {
-      const int line = w % TL, j = w / TL;
-      cpx<T> v[8];
-      for (int r = 0; r < R; ++r) v[r] = in[(j + r * Kr) * TL + line];
-      const int k = j % Ns;
-      if (Ns > 1) {
-        for (int r = 1; r < R; ++r) {
-          cpx<T> t = tw[(k * r * tstep) % K];
-          if (sign > 0) t.y = -t.y;
-          v[r] = cmul(v[r], t);
-        }
-      }
-      butterfly<T>(v, R, sign);
-      const int base = (j / Ns) * Ns * R + k;
-      for (int r = 0; r < R; ++r) out[(base + r * Ns) * TL + line] = v[r];
+    switch (R) {
+      case 8: stockham_pass<T, TL, 8>(in, out, K, Ns, tw, sign); break;
+      case 4: stockham_pass<T, TL, 4>(in, out, K, Ns, tw, sign); break;
+      case 2: stockham_pass<T, TL, 2>(in, out, K, Ns, tw, sign); break;
+      case 3: stockham_pass<T, TL, 3>(in, out, K, Ns, tw, sign); break;
+      case 5: stockham_pass<T, TL, 5>(in, out, K, Ns, tw, sign); break;
+      default: stockham_pass<T, TL, 7>(in, out, K, Ns, tw, sign); break;
     }
     __syncthreads();
     cpx<T> *t = in;

@@ -34,7 +34,7 @@ enum LJModifier { kLJNone = 0, kLJPotentialShift = 1, kLJForceSwitch = 2,
 
 // Device flags word layout (int32 each).
 enum Flag { kFlagRebuild = 0, kFlagOverflowColumn = 1, kFlagOverflowRow = 2,
-            kFlagBuilds = 3, kFlagSettleFail = 4, kNumFlags = 8 };
+            kFlagBuilds = 3, kFlagSettleFail = 4, kFlagPmeRange = 5, kNumFlags = 8 };
 
 // One entry of a cluster-pair row.
 struct __attribute__((aligned(16))) PairEntry {

@@ -38,7 +38,12 @@ __global__ void nb_request_build(int *flags) {
 }
 
 // Thread per atom: wrap into the box, bin into a column bucket, remember
-// the wrap shift and the reference positions of the validity test.
+// the wrap shift and the reference positions of the validity test.  A block
+// first counts its atoms per column in LDS, takes one range of each bucket
+// with one global atomic per column it touches, then places its atoms: the
+// fill order may vary from run to run, the sort of nb_sort makes it not
+// matter.
+constexpr int kBinLdsColumns = 8192;
 __global__ void __launch_bounds__(256) nb_bin(int n, const double *__restrict__ x,
                                              double *__restrict__ shift,
                                              double *__restrict__ xref,
@@ -46,23 +51,41 @@ __global__ void __launch_bounds__(256) nb_bin(int n, const double *__restrict__ 
                                              int *__restrict__ bucket,
                                              int *__restrict__ flags, GridParams g) {
   if (flags[kFlagRebuild] == 0) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double w[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    double xv = x[3 * i + k];
-    double s = floor(xv * g.inv_box[k]) * g.box[k];
-    shift[3 * i + k] = s;
-    xref[3 * i + k] = xv;
-    w[k] = xv - s;
+  __shared__ int s_hist[kBinLdsColumns];
+  const bool lds = g.ncol <= kBinLdsColumns;
+  if (lds) {
+    for (int c = threadIdx.x; c < g.ncol; c += blockDim.x) s_hist[c] = 0;
+    __syncthreads();
   }
-  int cx = (int)(w[0] / g.wx);
-  int cy = (int)(w[1] / g.wy);
-  cx = cx < 0 ? 0 : (cx >= g.ncx ? g.ncx - 1 : cx);
-  cy = cy < 0 ? 0 : (cy >= g.ncy ? g.ncy - 1 : cy);
-  int col = cy * g.ncx + cx;
-  int slot = atomicAdd(&col_count[col], 1);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int col = -1, local = 0;
+  if (i < n) {
+    double w[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double xv = x[3 * i + k];
+      const double sh = floor(xv * g.inv_box[k]) * g.box[k];
+      shift[3 * i + k] = sh;
+      xref[3 * i + k] = xv;
+      w[k] = xv - sh;
+    }
+    int cx = (int)(w[0] / g.wx);
+    int cy = (int)(w[1] / g.wy);
+    cx = cx < 0 ? 0 : (cx >= g.ncx ? g.ncx - 1 : cx);
+    cy = cy < 0 ? 0 : (cy >= g.ncy ? g.ncy - 1 : cy);
+    col = cy * g.ncx + cx;
+    if (lds) local = atomicAdd(&s_hist[col], 1);
+  }
+  if (lds) {
+    __syncthreads();
+    for (int c = threadIdx.x; c < g.ncol; c += blockDim.x) {
+      const int cnt = s_hist[c];
+      if (cnt) s_hist[c] = atomicAdd(&col_count[c], cnt);
+    }
+    __syncthreads();
+  }
+  if (col < 0) return;
+  const int slot = lds ? s_hist[col] + local : atomicAdd(&col_count[col], 1);
   if (slot < g.col_cap)
     bucket[col * g.col_cap + slot] = i;
   else
@@ -82,7 +105,7 @@ __global__ void __launch_bounds__(256) nb_sort(
     float4 *__restrict__ bb_lo, float4 *__restrict__ bb_hi,
     int *__restrict__ cluster_col, uint32_t *__restrict__ real_mask,
     int *__restrict__ col_cl_start, int *__restrict__ col_cl_count,
-    int *__restrict__ counts_out, const int *__restrict__ flags, GridParams g) {
+    int *__restrict__ counts_out, int *__restrict__ flags, GridParams g) {
   if (flags[kFlagRebuild] == 0) return;
   __shared__ int s_red[256];
   __shared__ int s_atom[kSortCapMax];
@@ -147,6 +170,7 @@ __global__ void __launch_bounds__(256) nb_sort(
   if (tid == 0) {
     col_cl_start[col] = cl_start;
     col_cl_count[col] = ncl;
+    if (col == 0) atomicAdd(&flags[kFlagBuilds], 1);
     if (col == g.ncol - 1) {
       counts_out[0] = start + padded;         // places
       counts_out[1] = (start + padded) / kCluster;  // clusters
@@ -238,83 +262,108 @@ __global__ void __launch_bounds__(256) nb_search(
   const float wxf = (float)g.wx, wyf = (float)g.wy;
   int count = 0;
   PairEntry *row = rows + (size_t)ci * g.row_cap;
-
-  for (int dy = -g.ry; dy <= g.ry; ++dy) {
-    int jy = cy + dy, sy = 0;
-    while (jy < 0) { jy += g.ncy; --sy; }
-    while (jy >= g.ncy) { jy -= g.ncy; ++sy; }
-    // column y extent (about the box centre) shifted
-    float ylo = jy * wyf - 0.5f * Ly + sy * Ly, yhi = ylo + wyf;
-    float gy = axis_gap(ilo.y, ihi.y, ylo, yhi);
-    if (gy * gy >= g.rlist2) continue;
-    for (int dx = -g.rx; dx <= g.rx; ++dx) {
-      int jx = cx + dx, sx = 0;
+  // columns within reach: lanes test 64 column offsets at a time, a wave
+  // prefix sum of their candidate counts (clusters x 3 z shifts) lays the
+  // candidates out, and lanes then test 64 candidates at a time; every load
+  // of an iteration is independent of the others.
+  __shared__ int s_cstart[4][64], s_cbase[4][64], s_ccode[4][64];
+  const int ncolx = 2 * g.rx + 1, ncols = ncolx * (2 * g.ry + 1);
+  for (int cbase = 0; cbase < ncols; cbase += 64) {
+    const int c = cbase + lane;
+    int ncand = 0, cl0 = 0, code = 0;
+    if (c < ncols) {
+      const int dy = c / ncolx - g.ry, dx = c % ncolx - g.rx;
+      int jy = cy + dy, sy = 0, jx = cx + dx, sx = 0;
+      while (jy < 0) { jy += g.ncy; --sy; }
+      while (jy >= g.ncy) { jy -= g.ncy; ++sy; }
       while (jx < 0) { jx += g.ncx; --sx; }
       while (jx >= g.ncx) { jx -= g.ncx; ++sx; }
-      float xlo = jx * wxf - 0.5f * Lx + sx * Lx, xhi = xlo + wxf;
-      float gx = axis_gap(ilo.x, ihi.x, xlo, xhi);
-      if (gx * gx + gy * gy >= g.rlist2) continue;
-      if (sx < -1 || sx > 1 || sy < -1 || sy > 1) continue;  // reach < box
-      const int jcol = jy * g.ncx + jx;
-      const int cl0 = col_cl_start[jcol], ncl = col_cl_count[jcol];
-      const int ncand = ncl * 3;
-      for (int base = 0; base < ncand; base += 64) {
-        const int t = base + lane;
-        bool accept = false;
-        int cj = 0;
-        uint32_t code = 0;
-        uint64_t mask = 0;
-        if (t < ncand) {
-          cj = cl0 + t / 3;
-          const int sz = t % 3 - 1;
-          // lexicographic sign of the shift (sz, sy, sx)
-          const int lex = sz != 0 ? sz : (sy != 0 ? sy : sx);
-          const bool ordered = cj > ci || (cj == ci && lex >= 0);
-          if (ordered) {
-            float4 jlo = bb_lo[cj], jhi = bb_hi[cj];
-            float ox = sx * Lx, oy = sy * Ly, oz = sz * Lz;
-            float ex = axis_gap(ilo.x, ihi.x, jlo.x + ox, jhi.x + ox);
-            float ey = axis_gap(ilo.y, ihi.y, jlo.y + oy, jhi.y + oy);
-            float ez = axis_gap(ilo.z, ihi.z, jlo.z + oz, jhi.z + oz);
-            float d2 = ex * ex + ey * ey + ez * ez;
-            if (d2 < g.rlist2) {
-              uint32_t jreal = real_mask[cj];
-              for (int b = 0; b < kCluster; ++b)
-                if (jreal & (1u << b)) mask |= (uint64_t)ireal << (kCluster * b);
-              if (cj == ci) {
-                // a < b at zero shift; a != b for a positive shift
-                uint64_t keep = 0;
-                for (int b = 0; b < kCluster; ++b)
-                  for (int a = 0; a < kCluster; ++a)
-                    if (lex == 0 ? (a < b) : (a != b))
-                      keep |= 1ull << (a + kCluster * b);
-                mask &= keep;
-              }
-              for (int k = 0; k < nex; ++k)
-                if (s_ex_cluster[wave][k] == cj) mask &= ~(1ull << s_ex_bit[wave][k]);
-              code = (uint32_t)(sx + 1) | ((uint32_t)(sy + 1) << 2) |
-                     ((uint32_t)(sz + 1) << 4);
-              accept = mask != 0;
-            }
-          }
-        }
-        const uint64_t ballot = __ballot(accept);
-        const int before = __popcll(ballot & ((1ull << lane) - 1ull));
-        if (accept) {
-          const int slot = count + before;
-          if (slot < g.row_cap) {
-            PairEntry e;
-            e.cj = cj;
-            e.shift = code;
-            e.mask = mask;
-            row[slot] = e;
-          } else {
-            atomicOr(&flags[kFlagOverflowRow], 1);
-          }
-        }
-        count += __popcll(ballot);
+      const float ylo = jy * wyf - 0.5f * Ly + sy * Ly, yhi = ylo + wyf;
+      const float xlo = jx * wxf - 0.5f * Lx + sx * Lx, xhi = xlo + wxf;
+      const float gy = axis_gap(ilo.y, ihi.y, ylo, yhi);
+      const float gx = axis_gap(ilo.x, ihi.x, xlo, xhi);
+      if (gx * gx + gy * gy < g.rlist2 && sx >= -1 && sx <= 1 && sy >= -1 && sy <= 1) {
+        const int jcol = jy * g.ncx + jx;
+        cl0 = col_cl_start[jcol];
+        ncand = col_cl_count[jcol] * 3;
+        code = (sx + 1) | ((sy + 1) << 2);
       }
     }
+    // inclusive wave scan of ncand
+    int scan = ncand;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int up = __shfl_up(scan, off);
+      if (lane >= off) scan += up;
+    }
+    const int total = __shfl(scan, 63);
+    s_cstart[wave][lane] = cl0;
+    s_cbase[wave][lane] = scan - ncand;  // exclusive
+    s_ccode[wave][lane] = code;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int base = 0; base < total; base += 64) {
+      const int t = base + lane;
+      bool accept = false;
+      int cj = 0;
+      uint32_t scode = 0;
+      uint64_t mask = 0;
+      if (t < total) {
+        // the column slot holding candidate t: last slot with cbase <= t
+        int lo = 0, hi = 63;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (s_cbase[wave][mid] <= t) lo = mid; else hi = mid - 1;
+        }
+        const int within = t - s_cbase[wave][lo];
+        cj = s_cstart[wave][lo] + within / 3;
+        const int sz = within % 3 - 1;
+        const int ccode = s_ccode[wave][lo];
+        const int sx = (ccode & 3) - 1, sy = ((ccode >> 2) & 3) - 1;
+        const int lex = sz != 0 ? sz : (sy != 0 ? sy : sx);
+        const bool ordered = cj > ci || (cj == ci && lex >= 0);
+        if (ordered) {
+          const float4 jlo = bb_lo[cj], jhi = bb_hi[cj];
+          const float ox = sx * Lx, oy = sy * Ly, oz = sz * Lz;
+          const float ex = axis_gap(ilo.x, ihi.x, jlo.x + ox, jhi.x + ox);
+          const float ey = axis_gap(ilo.y, ihi.y, jlo.y + oy, jhi.y + oy);
+          const float ez = axis_gap(ilo.z, ihi.z, jlo.z + oz, jhi.z + oz);
+          if (ex * ex + ey * ey + ez * ez < g.rlist2) {
+            const uint32_t jreal = real_mask[cj];
+            for (int b = 0; b < kCluster; ++b)
+              if (jreal & (1u << b)) mask |= (uint64_t)ireal << (kCluster * b);
+            if (cj == ci) {
+              uint64_t keep = 0;
+              for (int b = 0; b < kCluster; ++b)
+                for (int a = 0; a < kCluster; ++a)
+                  if (lex == 0 ? (a < b) : (a != b)) keep |= 1ull << (a + kCluster * b);
+              mask &= keep;
+            }
+            for (int k = 0; k < nex; ++k)
+              if (s_ex_cluster[wave][k] == cj) mask &= ~(1ull << s_ex_bit[wave][k]);
+            scode = (uint32_t)(sx + 1) | ((uint32_t)(sy + 1) << 2) | ((uint32_t)(sz + 1) << 4);
+            accept = mask != 0;
+          }
+        }
+      }
+      const uint64_t ballot = __ballot(accept);
+      const int before = __popcll(ballot & ((1ull << lane) - 1ull));
+      if (accept) {
+        const int slot = count + before;
+        if (slot < g.row_cap) {
+          PairEntry e;
+          e.cj = cj;
+          e.shift = scode;
+          e.mask = mask;
+          row[slot] = e;
+        } else {
+          atomicOr(&flags[kFlagOverflowRow], 1);
+        }
+      }
+      count += __popcll(ballot);
+    }
+    __builtin_amdgcn_wave_barrier();
   }
   if (lane == 0) row_count[ci] = count < g.row_cap ? count : g.row_cap;
 }

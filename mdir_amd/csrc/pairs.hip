@@ -41,6 +41,23 @@ __device__ __forceinline__ void add_fixed(long long *F, int place, float fx,
   if (wz) atomicAdd(u + 2, (unsigned long long)wz);
 }
 
+// erfc(x) = t exp(-x^2) P(t), t = 1/(1 + 0.475 x): a degree-6 polynomial
+// fitted (weighted least squares on the relative error, this project's own
+// fit) on [0, 3.6]; 2e-9 relative in exact arithmetic, ~1e-6 in f32, where
+// the rounding of exp(-x^2) itself dominates as it does for any f32 erfc.
+// It reuses the exp(-x^2) that the force needs anyway.
+__device__ __forceinline__ float erfc_from_exp(float x, float expmx2) {
+  const float t = __frcp_rn(1.0f + 0.475f * x);
+  float pol = 0.09289151430130005f;
+  pol = fmaf(pol, t, -0.4289827346801758f);
+  pol = fmaf(pol, t, 0.6025476455688477f);
+  pol = fmaf(pol, t, -0.1264616847038269f);
+  pol = fmaf(pol, t, 0.3417808711528778f);
+  pol = fmaf(pol, t, 0.24873463809490204f);
+  pol = fmaf(pol, t, 0.26948970556259155f);
+  return t * expmx2 * pol;
+}
+
 // Scalar force -dU/dr / r and energy of one pair in f32.
 template <int MOD, bool COUL, bool ENERGY>
 __device__ __forceinline__ float pair_force(float r2, float qq, float2 lj,
@@ -53,13 +70,13 @@ __device__ __forceinline__ float pair_force(float r2, float qq, float2 lj,
     const float r = r2 * rinv;
     const float br = p.beta * r;
     const float expt = __expf(-br * br);
-    const float erfcv = erfcf(br);
+    const float erfcv = erfc_from_exp(br, expt);
     const float tbs = 1.1283791670955126f * p.beta;  // 2 beta / sqrt(pi)
     fscal = qq * (erfcv * rinv + tbs * expt) * rinv2;
     if (ENERGY) e_c += qq * (erfcv * rinv - p.shift_c);
   }
   const float c6 = lj.x, c12 = lj.y;
-  if (c6 != 0.0f || c12 != 0.0f) {
+  {  // zero parameters give zero: no branch
     const float rinv6 = rinv2 * rinv2 * rinv2;
     float flj = (12.0f * c12 * rinv6 - 6.0f * c6) * rinv6 * rinv2;
     float ulj = (c12 * rinv6 - c6) * rinv6;

@@ -79,89 +79,187 @@ __device__ __forceinline__ void pme_place(double x, double L, double invL, int K
   first = f;
 }
 
-// n^2 threads per atom (in place order): (t0, t1), looping over t2.
-__global__ void __launch_bounds__(256) pme_spread(
-    const int *__restrict__ counts, const int *__restrict__ atom_at,
-    const double *__restrict__ x, const double *__restrict__ q, PmeParams pp,
-    unsigned long long *__restrict__ grid) {
-  const int n = pp.order, nn = n * n;
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  const int place = g / nn, sub = g % nn;
-  if (place >= counts[0]) return;
-  const int atom = atom_at[place];
-  if (atom < 0) return;
-  const double qa = q[atom];
-  if (qa == 0.0) return;
-  int f[3];
-  float M[3][8];
-  for (int a = 0; a < 3; ++a) {
-    float w;
-    pme_place(x[3 * atom + a], pp.box[a], pp.inv_box[a], pp.K[a], n, f[a], w);
-    bspline<float>(w, n, M[a], nullptr);
+// ---- spread: LDS tiles of the xy columns of the cluster sort -----------
+//
+// The atoms of one xy column of the build lie in a contiguous range of
+// places.  Their B-spline stencils cover grid points x in [gx0, gx0 + TWx),
+// y in [gy0, gy0 + TWy) of that column (all z): the column's extent, the
+// distance an atom may move before the validity test rebuilds the list,
+// and the order.  A block accumulates one column's tile in LDS — each
+// thread owns tile points (x, y) and adds the atoms in place order, so the
+// f32 sums have a fixed order and need no atomics — and stores the tile.
+// The first FFT pass sums the tiles that cover each grid row, in a fixed
+// order (fft_rows_r2c_tiles).
+
+struct TileParams {
+  int K0;                // grid points along x (rows are K0 x K1)
+  int TWx, TWy;          // tile extent in grid points (<= K)
+  int ncx;               // columns along x
+  int zc;                // z points per block (a chunk of K3)
+  int nzc;               // chunks
+  const int *gx0;        // first tile point of each column along x (mod K)
+  const int *gy0;
+  const int *xcov;       // per grid x: up to kMaxCover (column x, offset), -1 ends
+  const int *ycov;
+};
+constexpr int kMaxCover = 8;
+constexpr int kTileAtoms = 256;
+
+template <int N>
+__global__ void __launch_bounds__(256) pme_spread_tiles(
+    const int *__restrict__ col_cl_start, const int *__restrict__ col_cl_count,
+    const int *__restrict__ atom_at, const double *__restrict__ x,
+    const double *__restrict__ q, PmeParams pp, TileParams tp,
+    float *__restrict__ tiles, int *__restrict__ flags) {
+  extern __shared__ float s_tile[];
+  __shared__ int s_fx[kTileAtoms], s_fy[kTileAtoms], s_fz[kTileAtoms];
+  __shared__ float s_w[kTileAtoms][3 * N];
+  const int col = blockIdx.x / tp.nzc, zc = blockIdx.x % tp.nzc;
+  const int K0 = pp.K[0], K1 = pp.K[1], K2 = pp.K[2];
+  const int z0 = zc * tp.zc, zlen = min(tp.zc, K2 - z0);
+  const int zs = zlen + 1;  // padded z stride of the LDS tile
+  const int cx = col % tp.ncx, cy = col / tp.ncx;
+  const int gx0 = tp.gx0[cx], gy0 = tp.gy0[cy];
+  const int p0 = col_cl_start[col] * kCluster, np = col_cl_count[col] * kCluster;
+  const int npts = tp.TWx * tp.TWy;
+  for (int k = threadIdx.x; k < npts * zs; k += blockDim.x) s_tile[k] = 0.0f;
+  for (int a0 = 0; a0 < np; a0 += kTileAtoms) {
+    const int na = min(kTileAtoms, np - a0);
+    __syncthreads();
+    for (int t = threadIdx.x; t < na; t += blockDim.x) {
+      const int atom = atom_at[p0 + a0 + t];
+      int fx = -1 << 20, fy = 0, fz = 0;
+      if (atom >= 0 && q[atom] != 0.0) {
+        int f[3];
+        float M[3][N];
+        for (int d = 0; d < 3; ++d) {
+          float w;
+          pme_place(x[3 * atom + d], pp.box[d], pp.inv_box[d], pp.K[d], N, f[d], w);
+          bspline<float>(w, N, M[d], nullptr);
+        }
+        fx = f[0] - gx0;
+        if (fx < 0) fx += K0;
+        fy = f[1] - gy0;
+        if (fy < 0) fy += K1;
+        fz = f[2];
+        // an atom outside its column's tile would lose its charge
+        if ((tp.TWx < K0 && fx > tp.TWx - N) || (tp.TWy < K1 && fy > tp.TWy - N))
+          atomicOr(&flags[kFlagPmeRange], 1);
+        const float qa = (float)q[atom];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          s_w[t][k] = qa * M[0][k];
+          s_w[t][N + k] = M[1][k];
+          s_w[t][2 * N + k] = M[2][k];
+        }
+      }
+      s_fx[t] = fx;
+      s_fy[t] = fy;
+      s_fz[t] = fz;
+    }
+    __syncthreads();
+    for (int xy = threadIdx.x; xy < npts; xy += blockDim.x) {
+      const int tx = xy / tp.TWy, ty = xy % tp.TWy;
+      float *dst = s_tile + xy * zs;
+      for (int a = 0; a < na; ++a) {
+        int ox = tx - s_fx[a];
+        if (tp.TWx == K0 && ox < 0) ox += K0;
+        if (ox < 0 || ox >= N) continue;
+        int oy = ty - s_fy[a];
+        if (tp.TWy == K1 && oy < 0) oy += K1;
+        if (oy < 0 || oy >= N) continue;
+        float wxy = s_w[a][0], wy = s_w[a][N];
+#pragma unroll
+        for (int k = 1; k < N; ++k) {
+          if (k == ox) wxy = s_w[a][k];
+          if (k == oy) wy = s_w[a][N + k];
+        }
+        wxy *= wy;
+        const int fz = s_fz[a];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          int z = fz + k;
+          if (z >= K2) z -= K2;
+          const int rz = z - z0;
+          if (rz >= 0 && rz < zlen) dst[rz] += wxy * s_w[a][2 * N + k];
+        }
+      }
+    }
   }
-  const int t0 = sub / n, t1 = sub % n;
-  int i0 = f[0] + t0;
-  if (i0 >= pp.K[0]) i0 -= pp.K[0];
-  int i1 = f[1] + t1;
-  if (i1 >= pp.K[1]) i1 -= pp.K[1];
-  const float w01 = (float)qa * M[0][t0] * M[1][t1];
-  unsigned long long *row = grid + ((size_t)i0 * pp.K[1] + i1) * pp.K[2];
-  for (int t2 = 0; t2 < n; ++t2) {
-    int i2 = f[2] + t2;
-    if (i2 >= pp.K[2]) i2 -= pp.K[2];
-    const long long word = __double2ll_rn((double)(w01 * M[2][t2]) * kGridScale);
-    if (word) atomicAdd(row + i2, (unsigned long long)word);
+  __syncthreads();
+  float *out = tiles + (size_t)col * npts * K2 + z0;
+  for (int k = threadIdx.x; k < npts * zlen; k += blockDim.x) {
+    const int xy = k / zlen, z = k % zlen;
+    out[(size_t)xy * K2 + z] = s_tile[xy * zs + z];
   }
 }
 
-// Thread per atom (place order): gather phi and add the force words.
-__global__ void __launch_bounds__(256) pme_gather(
+// N lanes per atom (place order): lane t0 gathers plane x = first + t0, the
+// N lanes then sum their three partial gradients.
+template <int N, int LPA>
+__global__ void __launch_bounds__(256) pme_gather_lanes(
     const int *__restrict__ counts, const int *__restrict__ atom_at,
     const double *__restrict__ x, const double *__restrict__ q, PmeParams pp,
     const float *__restrict__ phi, long long *__restrict__ F) {
-  const int place = blockIdx.x * blockDim.x + threadIdx.x;
-  if (place >= counts[0]) return;
-  const int atom = atom_at[place];
-  if (atom < 0) return;
-  const double qa = q[atom];
-  if (qa == 0.0) return;
-  const int n = pp.order;
-  int f[3];
-  float M[3][8], dM[3][8];
-  for (int a = 0; a < 3; ++a) {
-    float w;
-    pme_place(x[3 * atom + a], pp.box[a], pp.inv_box[a], pp.K[a], n, f[a], w);
-    bspline<float>(w, n, M[a], dM[a]);
-  }
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int place = g / LPA, t0 = g % LPA;
+  const bool valid = place < counts[0];
+  const int atom = valid ? atom_at[place] : -1;
+  const double qa = atom >= 0 ? q[atom] : 0.0;
   float g0 = 0.0f, g1 = 0.0f, g2 = 0.0f;
-  for (int t0 = 0; t0 < n; ++t0) {
+  if (atom >= 0 && qa != 0.0 && t0 < N) {
+    int f[3];
+    float M[3][N], dM[3][N];
+    for (int d = 0; d < 3; ++d) {
+      float w;
+      pme_place(x[3 * atom + d], pp.box[d], pp.inv_box[d], pp.K[d], N, f[d], w);
+      bspline<float>(w, N, M[d], dM[d]);
+    }
+    float mx = M[0][0], dmx = dM[0][0];
+#pragma unroll
+    for (int k = 1; k < N; ++k)
+      if (k == t0) {
+        mx = M[0][k];
+        dmx = dM[0][k];
+      }
     int i0 = f[0] + t0;
     if (i0 >= pp.K[0]) i0 -= pp.K[0];
-    for (int t1 = 0; t1 < n; ++t1) {
+    const float *plane = phi + (size_t)i0 * pp.K[1] * pp.K[2];
+#pragma unroll
+    for (int t1 = 0; t1 < N; ++t1) {
       int i1 = f[1] + t1;
       if (i1 >= pp.K[1]) i1 -= pp.K[1];
-      const float *row = phi + ((size_t)i0 * pp.K[1] + i1) * pp.K[2];
+      const float *row = plane + (size_t)i1 * pp.K[2];
       float s = 0.0f, ds = 0.0f;
-      for (int t2 = 0; t2 < n; ++t2) {
+#pragma unroll
+      for (int t2 = 0; t2 < N; ++t2) {
         int i2 = f[2] + t2;
         if (i2 >= pp.K[2]) i2 -= pp.K[2];
         const float p = row[i2];
         s += p * M[2][t2];
         ds += p * dM[2][t2];
       }
-      g0 += dM[0][t0] * M[1][t1] * s;
-      g1 += M[0][t0] * dM[1][t1] * s;
-      g2 += M[0][t0] * M[1][t1] * ds;
+      g0 += dmx * M[1][t1] * s;
+      g1 += mx * dM[1][t1] * s;
+      g2 += mx * M[1][t1] * ds;
     }
   }
-  const float qf = (float)qa;
-  const float fx = -qf * (float)(pp.K[0] * pp.inv_box[0]) * g0;
-  const float fy = -qf * (float)(pp.K[1] * pp.inv_box[1]) * g1;
-  const float fz = -qf * (float)(pp.K[2] * pp.inv_box[2]) * g2;
-  unsigned long long *u = reinterpret_cast<unsigned long long *>(F + 3 * place);
-  atomicAdd(u + 0, (unsigned long long)to_fixed(fx));
-  atomicAdd(u + 1, (unsigned long long)to_fixed(fy));
-  atomicAdd(u + 2, (unsigned long long)to_fixed(fz));
+#pragma unroll
+  for (int m = 1; m < LPA; m <<= 1) {
+    g0 += __shfl_xor(g0, m);
+    g1 += __shfl_xor(g1, m);
+    g2 += __shfl_xor(g2, m);
+  }
+  if (atom >= 0 && qa != 0.0 && t0 == 0) {
+    const float qf = (float)qa;
+    const float fx = -qf * (float)(pp.K[0] * pp.inv_box[0]) * g0;
+    const float fy = -qf * (float)(pp.K[1] * pp.inv_box[1]) * g1;
+    const float fz = -qf * (float)(pp.K[2] * pp.inv_box[2]) * g2;
+    unsigned long long *u = reinterpret_cast<unsigned long long *>(F + 3 * place);
+    atomicAdd(u + 0, (unsigned long long)to_fixed(fx));
+    atomicAdd(u + 1, (unsigned long long)to_fixed(fy));
+    atomicAdd(u + 2, (unsigned long long)to_fixed(fz));
+  }
 }
 
 // ---- FFT passes ---------------------------------------------------------
@@ -212,6 +310,59 @@ __global__ void __launch_bounds__(256) fft_rows_r2c(
     if (r1 < nrows)
       out[(size_t)r1 * H + k] = {(zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5};
     if (r2 < nrows)
+      out[(size_t)r2 * H + k] = {(zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5};
+  }
+}
+
+// Forward along z of the grid rows summed from the column tiles of
+// pme_spread_tiles (in a fixed order: y cover, then x cover).
+template <typename T>
+__global__ void __launch_bounds__(256) fft_rows_r2c_tiles(
+    const float *__restrict__ tiles, TileParams tp, cpx<T> *__restrict__ out,
+    int K1, FftAxis az) {
+  constexpr int TL = tile_lines<T>();
+  extern __shared__ unsigned char smem[];
+  cpx<T> *a = reinterpret_cast<cpx<T> *>(smem);
+  const int K = az.K, H = K / 2 + 1;
+  cpx<T> *b = a + TL * K;
+  const int row0 = blockIdx.x * 2 * TL;
+  const int npts = tp.TWx * tp.TWy;
+  for (int w = threadIdx.x; w < 2 * TL * K; w += blockDim.x) {
+    const int z = w % K, rr = w / K;
+    const int row = row0 + rr;
+    const int ix = row / K1, iy = row % K1;
+    float v = 0.0f;
+    if (ix < tp.K0) {
+      for (int cyi = 0; cyi < kMaxCover; ++cyi) {
+        const int ce = tp.ycov[iy * kMaxCover + cyi];
+        if (ce < 0) break;
+        const int cy = ce >> 16, oy = ce & 0xffff;
+        for (int cxi = 0; cxi < kMaxCover; ++cxi) {
+          const int xe = tp.xcov[ix * kMaxCover + cxi];
+          if (xe < 0) break;
+          const int cx = xe >> 16, ox = xe & 0xffff;
+          const int col = cy * tp.ncx + cx;
+          v += tiles[((size_t)col * npts + ox * tp.TWy + oy) * K + z];
+        }
+      }
+    }
+    const int t = rr >> 1;
+    if (rr & 1)
+      a[z * TL + t].y = (T)v;
+    else
+      a[z * TL + t].x = (T)v;
+  }
+  __syncthreads();
+  cpx<T> *res = lds_fft<T, TL>(a, b, az, -1);
+  const int rows_total = tp.K0 * K1;
+  for (int w = threadIdx.x; w < TL * H; w += blockDim.x) {
+    const int k = w % H, t = w / H;
+    const cpx<T> zk = res[k * TL + t];
+    const cpx<T> zm = res[((K - k) % K) * TL + t];
+    const int r1 = row0 + 2 * t, r2 = r1 + 1;
+    if (r1 < rows_total)
+      out[(size_t)r1 * H + k] = {(zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5};
+    if (r2 < rows_total)
       out[(size_t)r2 * H + k] = {(zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5};
   }
 }
@@ -358,6 +509,13 @@ __global__ void __launch_bounds__(256) fft_x_conv(cpx<T> *__restrict__ data, int
 }
 
 template __global__ void fft_rows_r2c<float, true>(const void *, cpx<float> *, int, FftAxis);
+template __global__ void fft_rows_r2c_tiles<float>(const float *, TileParams, cpx<float> *, int, FftAxis);
+template __global__ void pme_spread_tiles<4>(const int *, const int *, const int *, const double *, const double *, PmeParams, TileParams, float *, int *);
+template __global__ void pme_spread_tiles<6>(const int *, const int *, const int *, const double *, const double *, PmeParams, TileParams, float *, int *);
+template __global__ void pme_spread_tiles<8>(const int *, const int *, const int *, const double *, const double *, PmeParams, TileParams, float *, int *);
+template __global__ void pme_gather_lanes<4, 4>(const int *, const int *, const double *, const double *, PmeParams, const float *, long long *);
+template __global__ void pme_gather_lanes<6, 8>(const int *, const int *, const double *, const double *, PmeParams, const float *, long long *);
+template __global__ void pme_gather_lanes<8, 8>(const int *, const int *, const double *, const double *, PmeParams, const float *, long long *);
 template __global__ void fft_rows_r2c<float, false>(const void *, cpx<float> *, int, FftAxis);
 template __global__ void fft_rows_r2c<double, false>(const void *, cpx<double> *, int, FftAxis);
 template __global__ void fft_rows_c2r<float>(const cpx<float> *, float *, int, FftAxis);

@@ -152,12 +152,18 @@ class Simulation:
         s = np.zeros(8, np.int64)
         rt.check(rt.lib().mdrtHipStats(self._h, rt.ptr(s, C.c_int64)))
         keys = ["places", "clusters", "entries", "largest_row", "row_cap",
-                "columns", "column_cap", "max_places"]
+                "columns", "column_cap", "builds"]
         return dict(zip(keys, (int(v) for v in s)))
 
+    PROFILE_KERNELS = ["integrate_first", "nb_bin", "nb_sort", "nb_search",
+                       "nb_pairs", "nb_special", "pme_spread", "fft_rows_r2c",
+                       "fft_cols_y_fwd", "fft_x_conv", "fft_cols_y_bwd",
+                       "fft_rows_c2r", "pme_gather", "integrate_second"]
+
     def profile_step(self, repeats: int = 20) -> dict:
-        ms = np.zeros(8)
+        """ms per launch of each kernel of a step (HIP events on the stream
+        the kernels run on); the profiled steps advance the state."""
+        ms = np.zeros(len(self.PROFILE_KERNELS))
         rt.check(rt.lib().mdrtHipProfileStep(self._h, int(repeats),
                                              rt.ptr(ms, C.c_double)))
-        keys = ["integrate", "build", "pairs", "spread", "fft", "gather"]
-        return dict(zip(keys, ms[:6]))
+        return dict(zip(self.PROFILE_KERNELS, ms))

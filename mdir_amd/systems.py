@@ -231,3 +231,97 @@ def tip3p_config2(seed: int = 2024) -> System:
     """7,900 TIP3P waters (23,700 atoms) in a 6.2 nm cube, PME rc 0.9 nm,
     tolerance 1e-5, 64^3 grid, order 4."""
     return tip3p_box(7900, 6.2, seed=seed, cutoff=0.9, grid=(64, 64, 64))
+
+
+# ---- initial velocities -----------------------------------------------------
+
+_MASK64 = (1 << 64) - 1
+
+
+def _splitmix64(state: int):
+    state = (state + 0x9E3779B97F4A7C15) & _MASK64
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _MASK64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _MASK64
+    return state, z ^ (z >> 31)
+
+
+def _xoshiro_uniforms(seed: int, count: int) -> np.ndarray:
+    """`count` numbers in (0, 1] of xoshiro256** seeded by splitmix64
+    (lib/Driver/InitialVelocityDraws.h:18-48)."""
+    s = []
+    st = seed & _MASK64
+    for _ in range(4):
+        st, z = _splitmix64(st)
+        s.append(z)
+    out = np.empty(count)
+    rot = lambda x, k: ((x << k) | (x >> (64 - k))) & _MASK64
+    s0, s1, s2, s3 = s
+    for i in range(count):
+        result = (rot((s1 * 5) & _MASK64, 7) * 9) & _MASK64
+        t = (s1 << 17) & _MASK64
+        s2 ^= s0
+        s3 ^= s1
+        s1 ^= s2
+        s0 ^= s3
+        s2 ^= t
+        s3 = rot(s3, 45)
+        out[i] = ((result >> 11) + 1.0) * 2.0 ** -53
+    return out
+
+
+def constrain_water_velocities(x, v, waters, m_o, m_h):
+    """Removes the bond-relative velocities of rigid waters: impulses along
+    the three bonds from a 3x3 solve (ConstraintIR.cpp:392-471,
+    System.cpp:1158-1215).  Vectorised over waters; returns a copy."""
+    v = v.copy()
+    o = np.asarray(waters)
+    if len(o) == 0:
+        return v
+    xa, xb, xc = x[o], x[o + 1], x[o + 2]
+    e0 = xb - xa
+    e1 = xc - xa
+    e2 = xc - xb
+    for e in (e0, e1, e2):
+        e /= np.linalg.norm(e, axis=1)[:, None]
+    va, vb, vc = v[o], v[o + 1], v[o + 2]
+    r = np.stack([-np.sum(e0 * (vb - va), 1), -np.sum(e1 * (vc - va), 1),
+                  -np.sum(e2 * (vc - vb), 1)], 1)
+    io, ih = 1.0 / m_o, 1.0 / m_h
+    A = np.zeros((len(o), 3, 3))
+    A[:, 0, 0] = A[:, 1, 1] = io + ih
+    A[:, 2, 2] = 2 * ih
+    A[:, 0, 1] = A[:, 1, 0] = np.sum(e0 * e1, 1) * io
+    A[:, 0, 2] = A[:, 2, 0] = -np.sum(e0 * e2, 1) * ih
+    A[:, 1, 2] = A[:, 2, 1] = np.sum(e1 * e2, 1) * ih
+    t = np.linalg.solve(A, r[..., None])[..., 0]
+    i0, i1, i2 = t[:, :1] * e0, t[:, 1:2] * e1, t[:, 2:3] * e2
+    v[o] = va - io * (i0 + i1)
+    v[o + 1] = vb + ih * (i0 - i2)
+    v[o + 2] = vc + ih * (i1 + i2)
+    return v
+
+
+def initial_velocities(system: System, seed: int, temperature: float) -> np.ndarray:
+    """Maxwell-Boltzmann velocities as the reference draws them
+    (lib/Driver/System.cpp:1298-1350): six uniforms per particle in order,
+    Box-Muller normals of width sqrt(kT/m), constrained, centre of mass at
+    rest, scaled to the temperature with the constrained degrees of freedom."""
+    n = system.n
+    u = _xoshiro_uniforms(seed, 6 * n).reshape(n, 3, 2)
+    normals = np.sqrt(-2.0 * np.log(u[..., 0])) * np.cos(6.283185307179586 * u[..., 1])
+    m = system.masses
+    width = np.where(m > 0, np.sqrt(BOLTZMANN * temperature / np.where(m > 0, m, 1.0)), 0.0)
+    v = width[:, None] * normals
+    if n < 2:
+        return v
+    if len(system.waters):
+        w0 = int(system.waters[0])
+        v = constrain_water_velocities(system.positions, v, system.waters,
+                                       m[w0], m[w0 + 1])
+    v -= (m[:, None] * v).sum(0) / m.sum()
+    kin = 0.5 * float(np.sum(m[:, None] * v * v))
+    target = 0.5 * system.degrees_of_freedom() * BOLTZMANN * temperature
+    if kin > 0:
+        v *= np.sqrt(target / kin)
+    return v

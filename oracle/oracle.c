@@ -1172,13 +1172,21 @@ int orc_total(const orc_system *s, const double *x, double *F,
   return 0;
 }
 
+int orc_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
 int orc_nve(const orc_system *s, double *x, double *v, double dt,
-            int64_t steps, double *trace) {
+            int64_t steps, double *trace, int32_t mixed) {
   int64_t n = s->n;
   double *F = (double *)malloc(sizeof(double) * 3 * (size_t)n);
   double *xo = (double *)malloc(sizeof(double) * 3 * (size_t)n);
   double e[ORC_NTERMS];
-  orc_total(s, x, F, e, 0);
+  orc_total(s, x, F, e, mixed);
   for (int64_t step = 0; step < steps; ++step) {
     for (int64_t i = 0; i < n; ++i)
       for (int k = 0; k < 3; ++k) v[3 * i + k] += 0.5 * dt * F[3 * i + k] / s->mass[i];
@@ -1192,7 +1200,7 @@ int orc_nve(const orc_system *s, double *x, double *v, double dt,
       for (int64_t i = 0; i < 3 * n; ++i) v[i] += (x[i] - xu[i]) / dt;
       free(xu);
     }
-    orc_total(s, x, F, e, 0);
+    orc_total(s, x, F, e, mixed);
     for (int64_t i = 0; i < n; ++i)
       for (int k = 0; k < 3; ++k) v[3 * i + k] += 0.5 * dt * F[3 * i + k] / s->mass[i];
     if (s->nwaters)

@@ -154,8 +154,10 @@ typedef struct {
 
 int orc_total(const orc_system *s, const double *x, double *F,
               double *energies, int32_t mixed);
+/* mixed != 0 evaluates the pair arithmetic in f32 (timing only). */
 int orc_nve(const orc_system *s, double *x, double *v, double dt,
-            int64_t steps, double *trace);
+            int64_t steps, double *trace, int32_t mixed);
+int orc_threads(void);
 
 /* ---- initial velocities (System.cpp:1298-1350; InitialVelocityDraws.h) ---- */
 uint64_t orc_splitmix64(uint64_t *state);

@@ -101,7 +101,9 @@ def _declare(L):
     L.orc_total.restype = C.c_int
     L.orc_total.argtypes = [C.POINTER(OrcSystem), PD, PD, PD, I32]
     L.orc_nve.restype = C.c_int
-    L.orc_nve.argtypes = [C.POINTER(OrcSystem), PD, PD, D, I64, PD]
+    L.orc_nve.argtypes = [C.POINTER(OrcSystem), PD, PD, D, I64, PD, I32]
+    L.orc_threads.restype = C.c_int
+    L.orc_threads.argtypes = []
     L.orc_splitmix64.restype = C.c_uint64
     L.orc_splitmix64.argtypes = [C.POINTER(C.c_uint64)]
     L.orc_initial_velocities.argtypes = [C.c_uint64, I64, PD, D, PD, I64, PI32,
@@ -267,12 +269,12 @@ class Oracle:
                                     _p(out, D))
         return out
 
-    def nve(self, x, v, dt, steps):
+    def nve(self, x, v, dt, steps, mixed=False):
         x = f64(x).copy()
         v = f64(v).copy()
         trace = np.zeros((steps, 2))
         lib().orc_nve(C.byref(self.st), _p(x, D), _p(v, D), dt, steps,
-                      _p(trace, D))
+                      _p(trace, D), int(mixed))
         return x, v, trace
 
     def initial_velocities(self, seed, temperature):
