@@ -241,16 +241,17 @@ int mdrtHipMinimize(void *system, int64_t steps, double max_step,
 int mdrtHipNeighborPairs(void *system, double reach, int32_t excluded,
                          int64_t capacity, int32_t *pairs, int64_t *count);
 
-/* List statistics: places, clusters, entries, largest row, row capacity,
+/* List statistics: places, clusters, cluster-pair tiles, largest group row (entries), row capacity,
    columns, column capacity, builds done since creation. */
 int mdrtHipStats(void *system, int64_t *stats /* 8 */);
 
 /* Device-event timing of each kernel of a step on the stream it runs on,
    ms per launch averaged over `repeats` steps (the steps advance the
    state): integrate_first, nb_bin, nb_sort, nb_search, nb_pairs,
-   nb_special, pme_spread, fft_rows_r2c, fft_cols_y forward, fft_x_conv,
-   fft_cols_y backward, fft_rows_c2r, pme_gather, integrate_second. */
-int mdrtHipProfileStep(void *system, int32_t repeats, double *ms /* 14 */);
+   nb_special, pme_spread_rows (spread + forward z transform), fft_cols_y
+   forward, fft_x_conv, fft_cols_y backward, fft_rows_c2r, pme_gather,
+   integrate_second. */
+int mdrtHipProfileStep(void *system, int32_t repeats, double *ms /* 13 */);
 
 #ifdef __cplusplus
 }

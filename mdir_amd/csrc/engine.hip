@@ -184,8 +184,11 @@ struct Engine {
   uint32_t *d_real_mask = nullptr;
   int *d_col_cl_start = nullptr, *d_col_cl_count = nullptr;
   int *d_counts = nullptr;
-  PairEntry *d_rows = nullptr;
+  PairEntry *d_rows = nullptr;     // group rows, row_cap entries each
   int *d_row_count = nullptr;
+  uint64_t *d_ex = nullptr;        // explicit masks, kGroup per slot
+  int ex_cap = 0;
+  int max_groups = 0, nseg = 0;    // group rows; pair-loop segments per row
   int *d_excl_start = nullptr, *d_excl_atoms = nullptr;
   int *d_flags = nullptr;
   double list_reach = 0;  // rlist of the current list configuration
@@ -251,7 +254,7 @@ struct Engine {
     dfree(d_p14_eps); dfree(d_p14_qq); dfree(d_espec);
     fft.release();
     dfree(d_grid); dfree(d_cplx); dfree(d_phi); dfree(d_tab); dfree(d_erecip);
-    dfree(d_tile_idx); dfree(d_tiles);
+    dfree(d_brick_idx);
     dfree(d_unit_first); dfree(d_unit_size); dfree(d_kin);
     if (stream) hipStreamDestroy(stream);
     stream = nullptr;
@@ -261,11 +264,12 @@ struct Engine {
     dfree(d_atom_at); dfree(d_type_at); dfree(d_xq); dfree(d_F); dfree(d_col_count);
     dfree(d_bucket); dfree(d_bb_lo); dfree(d_bb_hi); dfree(d_cluster_col);
     dfree(d_real_mask); dfree(d_col_cl_start); dfree(d_col_cl_count); dfree(d_counts);
-    dfree(d_rows); dfree(d_row_count); dfree(d_epair);
+    dfree(d_rows); dfree(d_row_count); dfree(d_epair); dfree(d_ex);
   }
 
   // Column grid, capacities and the list buffers for a search reach.
-  void configure_list(double reach, int col_cap_min = 0, int row_cap_min = 0) {
+  void configure_list(double reach, int col_cap_min = 0, int row_cap_min = 0,
+                      int ex_cap_min = 0) {
     const double volume = box[0] * box[1] * box[2];
     const double rho = n / volume;
     const double target = std::cbrt((double)kCluster / rho);
@@ -290,22 +294,31 @@ struct Engine {
     ng.rlist2 = (float)(rb * rb);
     ng.rx = (int)(rb / ng.wx) + 1;
     ng.ry = (int)(rb / ng.wy) + 1;
-    const int places = n + (kCluster - 1) * ng.ncol + kCluster;
-    ng.max_clusters = places / kCluster + 1;
-    // entries of a row: clusters whose boxes are within the reach
+    // every column is padded to whole groups of kGroup clusters
+    const int gp = kCluster * kGroup;
+    const int places = n + (gp - 1) * ng.ncol;
+    const int ngroups_max = (places + gp - 1) / gp;
+    ng.max_clusters = ngroups_max * kGroup;
+    // entries of a group row: clusters whose boxes are within the reach of
+    // the group's box (all of them: a bound on the half list's share)
     const double h = (double)kCluster / (rho * ng.wx * ng.wy);
-    double est = (ng.wx + 2 * rb) * (ng.wy + 2 * rb) * (std::min(h, box[2]) + 2 * rb) /
-                 (ng.wx * ng.wy * h);
+    double est = (ng.wx + 2 * rb) * (ng.wy + 2 * rb) *
+                 (std::min(kGroup * h, box[2]) + 2 * rb) / (ng.wx * ng.wy * h);
     est = std::min(est, (double)ng.max_clusters * 3.0);
-    int rc_ = (int)(1.5 * est + 64);
+    int rc_ = (int)(1.25 * est + 64);
     rc_ = std::max(rc_, row_cap_min);
-    ng.row_cap = (rc_ + 31) & ~31;
+    ng.row_cap = (rc_ + kSegment - 1) / kSegment * kSegment;
     ng.max_excl = 16;
+    const int nex = std::max(ex_cap_min, ngroups_max * 16 + 1024);
     const bool same = d_rows && ng.ncol == g.ncol && ng.col_cap == g.col_cap &&
-                      ng.row_cap == g.row_cap && ng.max_clusters == g.max_clusters;
+                      ng.row_cap == g.row_cap && ng.max_clusters == g.max_clusters &&
+                      nex == ex_cap;
     g = ng;
     list_reach = reach;
     if (same) return;
+    max_groups = ngroups_max;
+    nseg = g.row_cap / kSegment;
+    ex_cap = nex;
     release_list();
     max_places = g.max_clusters * kCluster;
     d_atom_at = dalloc<int>(max_places);
@@ -323,9 +336,10 @@ struct Engine {
     d_col_cl_start = dalloc<int>(g.ncol);
     d_col_cl_count = dalloc<int>(g.ncol);
     d_counts = dalloc<int>(4);
-    d_rows = dalloc<PairEntry>((size_t)g.max_clusters * g.row_cap);
-    d_row_count = dalloc<int>(g.max_clusters);
-    d_epair = dalloc<double>(2 * (size_t)g.max_clusters);
+    d_rows = dalloc<PairEntry>((size_t)max_groups * g.row_cap);
+    d_row_count = dalloc<int>(max_groups);
+    d_ex = dalloc<uint64_t>((size_t)ex_cap * kGroup);
+    d_epair = dalloc<double>(2 * (size_t)max_groups * nseg);
     forces_valid = false;
     if (graph_exec) {
       hipGraphExecDestroy(graph_exec);
@@ -576,7 +590,7 @@ struct Engine {
       }
     }
     configure_list(cutoff + skin);
-    if (use_pme) make_tiles();
+    if (use_pme) make_bricks();
     HIPCHK(hipStreamSynchronize(stream));
   }
 
@@ -622,31 +636,43 @@ struct Engine {
 
   void request_build() { nb_request_build<<<1, 64, 0, stream>>>(d_flags); }
 
-  void build() {
+  void launch_bin() {
     nb_bin<<<blocks_for(n), 256, 0, stream>>>(n, d_x, d_shift, d_xref, d_col_count,
                                                d_bucket, d_flags, g);
+  }
+  void launch_sort() {
     nb_sort<<<g.ncol, 256, 0, stream>>>(d_x, d_shift, d_q, d_type, d_col_count, d_bucket,
                                          d_atom_at, d_place_of, d_xq, d_type_at, d_bb_lo,
                                          d_bb_hi, d_cluster_col, d_real_mask, d_col_cl_start,
                                          d_col_cl_count, d_counts, d_flags, g);
-    nb_search<<<blocks_for(g.max_clusters, 4), 256, 0, stream>>>(
+  }
+  void launch_search() {
+    nb_search<<<blocks_for(max_groups, 4), 256, 0, stream>>>(
         d_bb_lo, d_bb_hi, d_cluster_col, d_real_mask, d_col_cl_start, d_col_cl_count,
         d_atom_at, d_place_of, d_excl_start, d_excl_atoms, d_counts, d_col_count, d_rows,
-        d_row_count, d_flags, g);
+        d_row_count, d_ex, ex_cap, d_flags, g);
+  }
+  void build() {
+    launch_bin();
+    launch_sort();
+    launch_search();
+  }
+
+  template <int MOD, bool COUL, bool ENERGY>
+  void launch_pairs_t() {
+    const dim3 grid(blocks_for(max_groups, 4), nseg);
+    nb_pairs<MOD, COUL, ENERGY><<<grid, 256, 0, stream>>>(
+        d_xq, d_type_at, d_real_mask, d_rows, d_row_count, d_ex, g.row_cap, max_groups,
+        d_counts, d_lj, nbp, d_F, d_epair, d_flags);
   }
 
   template <int MOD>
   void launch_pairs(bool energy) {
-    const int blocks = blocks_for(g.max_clusters, 4);
     const bool coul = beta > 0;
-    if (coul && energy)
-      nb_pairs<MOD, true, true><<<blocks, 256, 0, stream>>>(d_xq, d_type_at, d_rows, d_row_count, g.row_cap, d_counts, d_lj, nbp, d_F, d_epair, d_flags);
-    else if (coul)
-      nb_pairs<MOD, true, false><<<blocks, 256, 0, stream>>>(d_xq, d_type_at, d_rows, d_row_count, g.row_cap, d_counts, d_lj, nbp, d_F, d_epair, d_flags);
-    else if (energy)
-      nb_pairs<MOD, false, true><<<blocks, 256, 0, stream>>>(d_xq, d_type_at, d_rows, d_row_count, g.row_cap, d_counts, d_lj, nbp, d_F, d_epair, d_flags);
-    else
-      nb_pairs<MOD, false, false><<<blocks, 256, 0, stream>>>(d_xq, d_type_at, d_rows, d_row_count, g.row_cap, d_counts, d_lj, nbp, d_F, d_epair, d_flags);
+    if (coul && energy) launch_pairs_t<MOD, true, true>();
+    else if (coul) launch_pairs_t<MOD, true, false>();
+    else if (energy) launch_pairs_t<MOD, false, true>();
+    else launch_pairs_t<MOD, false, false>();
   }
 
   void pairs(bool energy) {
@@ -665,18 +691,27 @@ struct Engine {
         d_place_of, spp, d_F, energy ? d_espec : nullptr);
   }
 
-  // ---- spread tiles of the xy columns ----
-  TileParams tp{};
-  int *d_tile_idx = nullptr;   // gx0 | gy0 | xcov | ycov
-  float *d_tiles = nullptr;
-  size_t tile_lds = 0;
+  // ---- bricks of the spread ----
+  BrickParams bp{};
+  int *d_brick_idx = nullptr;
+  int brick_tl = 8;            // complex lines of a brick's z transform
 
-  void make_tiles() {
-    dfree(d_tile_idx);
-    dfree(d_tiles);
+  // Tiles of the columns (the grid points their atoms' stencils can cover
+  // while the list is valid) and, per brick along x and y, the columns whose
+  // tiles overlap it.
+  void make_bricks() {
+    dfree(d_brick_idx);
     const double h = 0.5 * skin;
+    const int K2 = pme_grid[2];
+    // complex lines per brick: LDS of 2 TL K2 64-bit accumulators plus two
+    // TL K2 complex buffers within 60 KB
+    brick_tl = 8;
+    while (brick_tl > 1 && (size_t)32 * brick_tl * K2 > 60 * 1024) brick_tl /= 2;
+    if ((size_t)32 * brick_tl * K2 > 60 * 1024) throw HipError("PME grid too long along z");
+    const int R = 2 * brick_tl, BY = R >= 8 ? 4 : 2, BX = R / BY;
+    const int B[2] = {BX, BY};
     int TW[2];
-    std::vector<int> g0[2], cov[2];
+    std::vector<int> g0[2], start[2], cols[2];
     for (int a = 0; a < 2; ++a) {
       const int nc = a == 0 ? g.ncx : g.ncy;
       const double w = a == 0 ? g.wx : g.wy;
@@ -692,56 +727,69 @@ struct Engine {
       }
       if (tw >= K) tw = K;
       TW[a] = tw;
-      cov[a].assign((size_t)K * kMaxCover, -1);
-      std::vector<int> fill(K, 0);
-      for (int c = 0; c < nc; ++c)
-        for (int o = 0; o < tw; ++o) {
-          const int pnt = (g0[a][c] + o) % K;
-          if (fill[pnt] >= kMaxCover) throw HipError("PME tiles: a grid line is covered by too many columns");
-          cov[a][(size_t)pnt * kMaxCover + fill[pnt]++] = (c << 16) | o;
+      const int nb = (K + B[a] - 1) / B[a];
+      start[a].assign(1, 0);
+      for (int bi = 0; bi < nb; ++bi) {
+        for (int c = 0; c < nc; ++c) {
+          bool hit = false;
+          for (int o = 0; o < tw && !hit; ++o) {
+            const int pnt = (g0[a][c] + o) % K;
+            hit = pnt >= bi * B[a] && pnt < (bi + 1) * B[a];
+          }
+          if (hit) cols[a].push_back(c);
         }
+        start[a].push_back((int)cols[a].size());
+      }
     }
+    for (size_t bx = 0; bx + 1 < start[0].size(); ++bx)
+      for (size_t by = 0; by + 1 < start[1].size(); ++by)
+        if ((start[0][bx + 1] - start[0][bx]) * (start[1][by + 1] - start[1][by]) > kBrickMaxCols)
+          throw HipError("PME grid too coarse for the column decomposition");
     std::vector<int> idx;
-    idx.insert(idx.end(), g0[0].begin(), g0[0].end());
-    idx.insert(idx.end(), g0[1].begin(), g0[1].end());
-    idx.insert(idx.end(), cov[0].begin(), cov[0].end());
-    idx.insert(idx.end(), cov[1].begin(), cov[1].end());
-    d_tile_idx = dupload(idx);
-    tp.K0 = pme_grid[0];
-    tp.TWx = TW[0];
-    tp.TWy = TW[1];
-    tp.ncx = g.ncx;
-    tp.gx0 = d_tile_idx;
-    tp.gy0 = d_tile_idx + g.ncx;
-    tp.xcov = d_tile_idx + g.ncx + g.ncy;
-    tp.ycov = tp.xcov + (size_t)pme_grid[0] * kMaxCover;
-    const int K2 = pme_grid[2];
-    const size_t fixed = (size_t)kTileAtoms * (3 * sizeof(int) + 3 * pme_order * sizeof(float));
-    const size_t budget = 64 * 1024 - fixed - 512;
-    int zc = (int)(budget / ((size_t)TW[0] * TW[1] * sizeof(float))) - 1;
-    if (zc < 1) throw HipError("PME tiles do not fit in LDS");
-    zc = std::min(zc, K2);
-    tp.nzc = (K2 + zc - 1) / zc;
-    tp.zc = (K2 + tp.nzc - 1) / tp.nzc;
-    tile_lds = (size_t)TW[0] * TW[1] * (tp.zc + 1) * sizeof(float);
-    d_tiles = dalloc<float>((size_t)g.ncol * TW[0] * TW[1] * K2);
+    auto put = [&](const std::vector<int> &v) {
+      const size_t off = idx.size();
+      idx.insert(idx.end(), v.begin(), v.end());
+      return off;
+    };
+    const size_t o_gx = put(g0[0]), o_gy = put(g0[1]);
+    const size_t o_xs = put(start[0]), o_xc = put(cols[0]);
+    const size_t o_ys = put(start[1]), o_yc = put(cols[1]);
+    d_brick_idx = dupload(idx);
+    bp.ncx = g.ncx;
+    bp.TWx = TW[0];
+    bp.TWy = TW[1];
+    bp.gx0 = d_brick_idx + o_gx;
+    bp.gy0 = d_brick_idx + o_gy;
+    bp.xstart = d_brick_idx + o_xs;
+    bp.xcols = d_brick_idx + o_xc;
+    bp.ystart = d_brick_idx + o_ys;
+    bp.ycols = d_brick_idx + o_yc;
   }
 
+  template <int N, int TL>
+  void spread_rows_t() {
+    constexpr int R = BrickShape<TL>::R, BX = BrickShape<TL>::BX, BY = BrickShape<TL>::BY;
+    static_assert(BX * BY == R, "brick shape");
+    const int blocks = ((pme_grid[0] + BX - 1) / BX) * ((pme_grid[1] + BY - 1) / BY);
+    const size_t lds = (size_t)32 * TL * pme_grid[2];
+    pme_spread_rows<N, TL><<<blocks, 256, lds, stream>>>(d_col_cl_start, d_col_cl_count,
+                                                         d_atom_at, d_x, d_q, pmp, bp, d_cplx,
+                                                         fft.ax[2].axis, d_flags);
+  }
+  template <int N>
+  void spread_rows_n() {
+    switch (brick_tl) {
+      case 8: spread_rows_t<N, 8>(); break;
+      case 4: spread_rows_t<N, 4>(); break;
+      case 2: spread_rows_t<N, 2>(); break;
+      default: spread_rows_t<N, 1>(); break;
+    }
+  }
+  // Spread and forward z transform: the half spectra rows of d_cplx.
   void pme_spread_launch() {
-    const int blocks = g.ncol * tp.nzc;
-    if (pme_order == 4)
-      pme_spread_tiles<4><<<blocks, 256, tile_lds, stream>>>(d_col_cl_start, d_col_cl_count, d_atom_at, d_x, d_q, pmp, tp, d_tiles, d_flags);
-    else if (pme_order == 6)
-      pme_spread_tiles<6><<<blocks, 256, tile_lds, stream>>>(d_col_cl_start, d_col_cl_count, d_atom_at, d_x, d_q, pmp, tp, d_tiles, d_flags);
-    else
-      pme_spread_tiles<8><<<blocks, 256, tile_lds, stream>>>(d_col_cl_start, d_col_cl_count, d_atom_at, d_x, d_q, pmp, tp, d_tiles, d_flags);
-  }
-
-  void pme_rows_forward() {
-    const int TL = Fft3D<float>::TL;
-    const int nrows = fft.K[0] * fft.K[1];
-    fft_rows_r2c_tiles<float><<<(nrows + 2 * TL - 1) / (2 * TL), 256, fft.lds(fft.K[2]), stream>>>(
-        d_tiles, tp, d_cplx, fft.K[1], fft.ax[2].axis);
+    if (pme_order == 4) spread_rows_n<4>();
+    else if (pme_order == 6) spread_rows_n<6>();
+    else spread_rows_n<8>();
   }
 
   void pme_gather_launch() {
@@ -756,7 +804,6 @@ struct Engine {
   void pme(bool energy) {
     if (!use_pme) return;
     pme_spread_launch();
-    pme_rows_forward();
     const int TL = Fft3D<float>::TL;
     const int tiles = (fft.H() + TL - 1) / TL;
     fft_cols_y<float><<<fft.K[0] * tiles, 256, fft.lds(fft.K[1]), stream>>>(
@@ -816,13 +863,14 @@ struct Engine {
       clear_error_flags();
       throw HipError("an atom left the PME tile of its column");
     }
-    if (f[kFlagOverflowRow] & 2) throw HipError("more than 128 exclusions within one cluster");
+    if (f[kFlagOverflowRow] & 2) throw HipError("more than 16 exclusions of one particle");
     if (!f[kFlagOverflowColumn] && !f[kFlagOverflowRow]) return false;
     int col_min = f[kFlagOverflowColumn] ? g.col_cap * 2 : g.col_cap;
-    int row_min = f[kFlagOverflowRow] ? g.row_cap * 2 : g.row_cap;
+    int row_min = (f[kFlagOverflowRow] & 1) ? g.row_cap * 2 : g.row_cap;
+    int ex_min = (f[kFlagOverflowRow] & 4) ? ex_cap * 2 : ex_cap;
     if (col_min > kSortCapMax && f[kFlagOverflowColumn])
       throw HipError("a column holds more than 1024 particles");
-    configure_list(list_reach, col_min, row_min);
+    configure_list(list_reach, col_min, row_min, ex_min);
     clear_error_flags();
     return true;
   }
@@ -854,14 +902,12 @@ struct Engine {
     }
     if (E_out) {
       for (int k = 0; k < MDRT_NUM_ENERGIES; ++k) E_out[k] = 0.0;
-      std::vector<int> counts(4);
-      to_host(counts.data(), d_counts, 4 * sizeof(int));
-      const int ncl = counts[1];
-      std::vector<double> ep(2 * (size_t)ncl);
-      if (ncl) to_host(ep.data(), d_epair, ep.size() * sizeof(double));
-      for (int c = 0; c < ncl; ++c) {
-        E_out[MDRT_E_LJ] += ep[2 * c];
-        E_out[MDRT_E_COUL_DIRECT] += ep[2 * c + 1];
+      // one partial per (segment, group) wave, summed in a fixed order
+      std::vector<double> ep(2 * (size_t)max_groups * nseg);
+      to_host(ep.data(), d_epair, ep.size() * sizeof(double));
+      for (size_t w = 0; w < ep.size() / 2; ++w) {
+        E_out[MDRT_E_LJ] += ep[2 * w];
+        E_out[MDRT_E_COUL_DIRECT] += ep[2 * w + 1];
       }
       if (nspecial) {
         const int nb = blocks_for(nspecial);
@@ -908,28 +954,33 @@ struct Engine {
     dt_graph = dt;
   }
 
+  // Steps in chunks (one graph launch, or up to 64 single steps); after each
+  // chunk the flags are read, and a chunk whose list overflowed is redone
+  // from its saved state with the grown list.
   void run(int64_t steps, double dt, int graph, double *kinetic) {
     if (list_reach != cutoff + skin) configure_list(cutoff + skin);
     if (!forces_valid) evaluate(false);
-    for (int attempt = 0; attempt < 8; ++attempt) {
-      HIPCHK(hipMemcpyAsync(d_save_x, d_x, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, stream));
-      HIPCHK(hipMemcpyAsync(d_save_v, d_v, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, stream));
-      int64_t done = 0;
-      if (graph > 0) {
-        if (!graph_exec || graph_steps != graph || dt_graph != dt) capture(graph, dt);
-        while (done + graph <= steps) {
+    const int64_t chunk = graph > 0 ? graph : 64;
+    int64_t done = 0;
+    while (done < steps) {
+      const int64_t todo = std::min(chunk, steps - done);
+      for (int attempt = 0;; ++attempt) {
+        on_device(d_save_x, d_x, 3 * n * sizeof(double));
+        on_device(d_save_v, d_v, 3 * n * sizeof(double));
+        if (graph > 0 && todo == graph) {
+          if (!graph_exec || graph_steps != graph || dt_graph != dt) capture(graph, dt);
           HIPCHK(hipGraphLaunch(graph_exec, stream));
-          done += graph;
+        } else {
+          for (int64_t k = 0; k < todo; ++k) one_step(dt);
         }
+        const std::vector<int> f = read_flags();
+        if (!grow_if_overflowed(f)) break;
+        if (attempt == 7) throw HipError("the neighbor list kept overflowing");
+        on_device(d_x, d_save_x, 3 * n * sizeof(double));
+        on_device(d_v, d_save_v, 3 * n * sizeof(double));
+        evaluate(false);
       }
-      for (; done < steps; ++done) one_step(dt);
-      const std::vector<int> f = read_flags();
-      if (!grow_if_overflowed(f)) break;
-      // restore and redo from the saved state with the larger list
-      HIPCHK(hipMemcpyAsync(d_x, d_save_x, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, stream));
-      HIPCHK(hipMemcpyAsync(d_v, d_save_v, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, stream));
-      evaluate(false);
-      if (attempt == 7) throw HipError("the neighbor list kept overflowing");
+      done += todo;
     }
     HIPCHK(hipStreamSynchronize(stream));
     if (kinetic) *kinetic = kinetic_energy();
@@ -1011,8 +1062,8 @@ struct Engine {
     const unsigned long long cap2 = (unsigned long long)std::max<int64_t>(capacity, 1) * 4 + 1024;
     int2 *d_pairs = dalloc<int2>(cap2);
     unsigned long long *d_count = dalloc<unsigned long long>(1);
-    nb_export_pairs<<<blocks_for(g.max_clusters, 4), 256, 0, stream>>>(
-        d_x, d_atom_at, d_rows, d_row_count, d_counts, g.row_cap, (float)box[0],
+    nb_export_pairs<<<blocks_for(max_groups, 4), 256, 0, stream>>>(
+        d_x, d_atom_at, d_real_mask, d_rows, d_row_count, d_ex, d_counts, g.row_cap, (float)box[0],
         (float)box[1], (float)box[2], box[0], box[1], box[2], limit2, d_pairs, d_count, cap2);
     unsigned long long cnt = 0;
     HIPCHK(hipMemcpyAsync(&cnt, d_count, sizeof(cnt), hipMemcpyDeviceToHost, stream));
@@ -1039,16 +1090,19 @@ struct Engine {
   void stats(int64_t *s) {
     std::vector<int> counts(4);
     to_host(counts.data(), d_counts, 4 * sizeof(int));
-    std::vector<int> rc(g.max_clusters);
+    std::vector<int> rc(max_groups);
     to_host(rc.data(), d_row_count, rc.size() * sizeof(int));
-    int64_t entries = 0, largest = 0;
-    for (int c = 0; c < counts[1]; ++c) {
-      entries += rc[c];
-      largest = std::max<int64_t>(largest, rc[c]);
+    std::vector<PairEntry> rows((size_t)max_groups * g.row_cap);
+    to_host(rows.data(), d_rows, rows.size() * sizeof(PairEntry));
+    int64_t tiles = 0, largest = 0;
+    for (int gi = 0; gi < counts[1] / kGroup; ++gi) {
+      for (int e = 0; e < rc[gi]; ++e)
+        tiles += __builtin_popcount((rows[(size_t)gi * g.row_cap + e].info >> 8) & 0xFu);
+      largest = std::max<int64_t>(largest, rc[gi]);
     }
     s[0] = counts[0];
     s[1] = counts[1];
-    s[2] = entries;
+    s[2] = tiles;
     s[3] = largest;
     s[4] = g.row_cap;
     s[5] = g.ncol;
@@ -1060,10 +1114,10 @@ struct Engine {
   // Device-event timing of every kernel of one step, on the stream the
   // kernels run on: ms per launch, averaged over `repeats` steps.
   //  0 integrate_first  1 nb_bin  2 nb_sort  3 nb_search  4 nb_pairs
-  //  5 nb_special  6 pme_spread  7 fft_rows_r2c  8 fft_cols_y (forward)
-  //  9 fft_x_conv  10 fft_cols_y (backward)  11 fft_rows_c2r  12 pme_gather
-  //  13 integrate_second
-  static constexpr int kProfileSlots = 14;
+  //  5 nb_special  6 pme_spread_rows (spread + forward z transform)
+  //  7 fft_cols_y (forward)  8 fft_x_conv  9 fft_cols_y (backward)
+  //  10 fft_rows_c2r  11 pme_gather  12 integrate_second
+  static constexpr int kProfileSlots = 13;
   void profile(int repeats, double *ms) {
     if (!forces_valid) evaluate(false);
     hipEvent_t ev[kProfileSlots + 1];
@@ -1077,25 +1131,17 @@ struct Engine {
       HIPCHK(hipEventRecord(ev[k++], stream));
       step_first(0.002);
       HIPCHK(hipEventRecord(ev[k++], stream));
-      nb_bin<<<blocks_for(n), 256, 0, stream>>>(n, d_x, d_shift, d_xref, d_col_count, d_bucket, d_flags, g);
+      launch_bin();
       HIPCHK(hipEventRecord(ev[k++], stream));
-      nb_sort<<<g.ncol, 256, 0, stream>>>(d_x, d_shift, d_q, d_type, d_col_count, d_bucket,
-                                           d_atom_at, d_place_of, d_xq, d_type_at, d_bb_lo,
-                                           d_bb_hi, d_cluster_col, d_real_mask, d_col_cl_start,
-                                           d_col_cl_count, d_counts, d_flags, g);
+      launch_sort();
       HIPCHK(hipEventRecord(ev[k++], stream));
-      nb_search<<<blocks_for(g.max_clusters, 4), 256, 0, stream>>>(
-          d_bb_lo, d_bb_hi, d_cluster_col, d_real_mask, d_col_cl_start, d_col_cl_count,
-          d_atom_at, d_place_of, d_excl_start, d_excl_atoms, d_counts, d_col_count, d_rows,
-          d_row_count, d_flags, g);
+      launch_search();
       HIPCHK(hipEventRecord(ev[k++], stream));
       pairs(false);
       HIPCHK(hipEventRecord(ev[k++], stream));
       special(false);
       HIPCHK(hipEventRecord(ev[k++], stream));
       if (use_pme) pme_spread_launch();
-      HIPCHK(hipEventRecord(ev[k++], stream));
-      if (use_pme) pme_rows_forward();
       HIPCHK(hipEventRecord(ev[k++], stream));
       if (use_pme)
         fft_cols_y<float><<<fft.K[0] * tiles, 256, fft.lds(fft.K[1]), stream>>>(

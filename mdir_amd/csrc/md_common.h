@@ -34,13 +34,22 @@ enum LJModifier { kLJNone = 0, kLJPotentialShift = 1, kLJForceSwitch = 2,
 
 // Device flags word layout (int32 each).
 enum Flag { kFlagRebuild = 0, kFlagOverflowColumn = 1, kFlagOverflowRow = 2,
-            kFlagBuilds = 3, kFlagSettleFail = 4, kFlagPmeRange = 5, kNumFlags = 8 };
+            kFlagBuilds = 3, kFlagSettleFail = 4, kFlagPmeRange = 5,
+            kFlagMaskCount = 6, kNumFlags = 8 };
 
-// One entry of a cluster-pair row.
+constexpr int kGroup = 4;            // i-clusters of a group (one row)
+constexpr int kSegment = 48;         // row entries one wave of the pair loop takes
+
+// One entry of a group row: j cluster cj against the i clusters of the
+// group whose bits are set in imask.  The interaction mask of a tile is
+// the outer product of the real-atom masks of its two clusters, unless
+// `ex` >= 0 points at four explicit 64-bit masks (bit a + 8 b: i atom a,
+// j atom b interact) for tiles with exclusions or i == j.
 struct __attribute__((aligned(16))) PairEntry {
-  int32_t cj;        // j cluster
-  uint32_t shift;    // (sx+1) | (sy+1) << 2 | (sz+1) << 4
-  uint64_t mask;     // bit (a + 8 b): i atom a, j atom b interact
+  int32_t cj;
+  uint32_t info;     // shift (sx+1) | (sy+1) << 2 | (sz+1) << 4; imask << 8
+  int32_t ex;        // -1, or index of uint64[kGroup] masks
+  int32_t pad;
 };
 
 // Per-system constants of the nonbonded and PME kernels (passed by value).

@@ -5,9 +5,10 @@
 // lib/Conversion/MDExecKernels/GroupsBuild.cpp:2515 for the GPU structure)
 // with a layout made for wave64: atoms are binned into xy columns, each
 // column is sorted along z, and every 8 consecutive places form a cluster
-// with a bounding box.  A wave searches the clusters near one i-cluster and
-// writes a row of {j-cluster, periodic shift, 64-bit interaction mask}: an
-// 8 x 8 tile is one pair per lane of a wave64.
+// with a bounding box; columns hold whole groups of 4 clusters.  A wave
+// searches the clusters near one group and writes a row of {j-cluster,
+// periodic shift, the members it pairs with}: an 8 x 8 tile is one pair per
+// lane of a wave64.
 //
 // Three kernels, each a no-op unless the rebuild flag is set, so a captured
 // step graph carries them every step and they cost one empty launch when the
@@ -114,11 +115,14 @@ __global__ void __launch_bounds__(256) nb_sort(
   const int col = blockIdx.x, tid = threadIdx.x;
   const int cap = g.col_cap < kSortCapMax ? g.col_cap : kSortCapMax;
 
+  // every column takes whole groups of clusters, so that no group of the
+  // pair list straddles two columns
+  constexpr int kPad = kCluster * kGroup;
   int sum = 0;
   for (int c = tid; c < col; c += blockDim.x) {
     int cnt = col_count[c];
     cnt = cnt < cap ? cnt : cap;
-    sum += (cnt + kCluster - 1) & ~(kCluster - 1);
+    sum += (cnt + kPad - 1) & ~(kPad - 1);
   }
   s_red[tid] = sum;
   __syncthreads();
@@ -129,7 +133,7 @@ __global__ void __launch_bounds__(256) nb_sort(
   const int start = s_red[0];
   int count = col_count[col];
   count = count < cap ? count : cap;
-  const int padded = (count + kCluster - 1) & ~(kCluster - 1);
+  const int padded = (count + kPad - 1) & ~(kPad - 1);
 
   for (int t = tid; t < count; t += blockDim.x) {
     int a = bucket[col * g.col_cap + t];
@@ -170,7 +174,10 @@ __global__ void __launch_bounds__(256) nb_sort(
   if (tid == 0) {
     col_cl_start[col] = cl_start;
     col_cl_count[col] = ncl;
-    if (col == 0) atomicAdd(&flags[kFlagBuilds], 1);
+    if (col == 0) {
+      atomicAdd(&flags[kFlagBuilds], 1);
+      flags[kFlagMaskCount] = 0;  // explicit-mask slots of the coming search
+    }
     if (col == g.ncol - 1) {
       counts_out[0] = start + padded;         // places
       counts_out[1] = (start + padded) / kCluster;  // clusters
@@ -203,12 +210,16 @@ __device__ __forceinline__ float axis_gap(float lo_i, float hi_i, float lo_j,
   return g > 0.0f ? g : 0.0f;
 }
 
-// Wave per i-cluster: candidates are the clusters of the columns within the
-// reach in x and y, at z shifts -1, 0, +1; a candidate is kept when its box
-// is within the reach of the i box (each unordered cluster pair once, from
-// its lower index).  The 64-bit mask drops padding, i >= j within the same
-// cluster at zero shift, and the excluded pairs (MDExecKernels.cpp:1748-1823
-// removes the same partners from a matrix row).
+// Wave per group of kGroup consecutive i-clusters (one column's).  The
+// candidates are the clusters of the columns within the reach of the group's
+// box, at z shifts -1, 0, +1: lanes test 64 column slots at a time, a wave
+// prefix sum lays out their candidates, and lanes then test 64 candidates at
+// a time, every load of an iteration independent of the others.  Bit m of an
+// entry's imask is set when j is within the reach of member m and the tile
+// (member, j, shift) is ordered, so that each unordered cluster pair occurs
+// once.  Tiles of a cluster with itself, and tiles holding excluded pairs,
+// get four explicit 64-bit masks (MDExecKernels.cpp:1748-1823 removes the
+// same partners from a matrix row).
 __global__ void __launch_bounds__(256) nb_search(
     const float4 *__restrict__ bb_lo, const float4 *__restrict__ bb_hi,
     const int *__restrict__ cluster_col, const uint32_t *__restrict__ real_mask,
@@ -217,33 +228,37 @@ __global__ void __launch_bounds__(256) nb_search(
     const int *__restrict__ excl_start, const int *__restrict__ excl_atoms,
     const int *__restrict__ counts_in, int *__restrict__ col_count,
     PairEntry *__restrict__ rows, int *__restrict__ row_count,
-    int *__restrict__ flags, GridParams g) {
+    uint64_t *__restrict__ ex_masks, int ex_cap, int *__restrict__ flags,
+    GridParams g) {
   if (flags[kFlagRebuild] == 0) return;
   // The next build starts from empty buckets (nb_sort has read them).
   if (blockIdx.x == 0)
     for (int c = threadIdx.x; c < g.ncol; c += blockDim.x) col_count[c] = 0;
-  constexpr int kExMax = 8 * 16;
+  constexpr int kExMax = kGroup * kCluster * 16;  // 16 partners per atom at most
   __shared__ int s_ex_cluster[4][kExMax];
-  __shared__ int s_ex_bit[4][kExMax];
+  __shared__ int s_ex_bit[4][kExMax];             // member << 8 | mask bit
   __shared__ int s_ex_n[4];
+  __shared__ int s_cstart[4][64], s_cbase[4][64], s_ccode[4][64];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int ci = blockIdx.x * 4 + wave;
-  const int nclusters = counts_in[1];
-  const bool active = ci < nclusters;
+  const int gi = blockIdx.x * 4 + wave;
+  const int ngroups = counts_in[1] / kGroup;
+  const bool active = gi < ngroups;
+  const int g0 = gi * kGroup;
   if (lane == 0) s_ex_n[wave] = 0;
   __syncthreads();
 
-  // exclusion table of the i cluster: (partner cluster, mask bit)
-  if (active && lane < kCluster) {
-    int atom = atom_at[ci * kCluster + lane];
+  // exclusion table of the group: (partner cluster, member << 8 | mask bit)
+  if (active && lane < kGroup * kCluster) {
+    const int m = lane >> 3, a = lane & 7;
+    const int atom = atom_at[g0 * kCluster + lane];
     if (atom >= 0) {
-      int b = excl_start[atom], e = excl_start[atom + 1];
-      for (int k = b; k < e; ++k) {
-        int pp = place_of[excl_atoms[k]];
-        int slot = atomicAdd(&s_ex_n[wave], 1);
+      const int b0 = excl_start[atom], b1 = excl_start[atom + 1];
+      for (int k = b0; k < b1; ++k) {
+        const int pp = place_of[excl_atoms[k]];
+        const int slot = atomicAdd(&s_ex_n[wave], 1);
         if (slot < kExMax) {
           s_ex_cluster[wave][slot] = pp / kCluster;
-          s_ex_bit[wave][slot] = lane + kCluster * (pp % kCluster);
+          s_ex_bit[wave][slot] = (m << 8) | (a + kCluster * (pp % kCluster));
         } else {
           atomicOr(&flags[kFlagOverflowRow], 2);
         }
@@ -254,20 +269,30 @@ __global__ void __launch_bounds__(256) nb_search(
   if (!active) return;
   const int nex = s_ex_n[wave] < kExMax ? s_ex_n[wave] : kExMax;
 
-  const float4 ilo = bb_lo[ci], ihi = bb_hi[ci];
-  const uint32_t ireal = real_mask[ci];
-  const int icol = cluster_col[ci];
+  uint32_t mreal[kGroup];
+  float4 mlo[kGroup], mhi[kGroup];
+  float4 glo = make_float4(3.0e38f, 3.0e38f, 3.0e38f, 0.0f);
+  float4 ghi = make_float4(-3.0e38f, -3.0e38f, -3.0e38f, 0.0f);
+#pragma unroll
+  for (int m = 0; m < kGroup; ++m) {
+    mreal[m] = real_mask[g0 + m];
+    mlo[m] = bb_lo[g0 + m];
+    mhi[m] = bb_hi[g0 + m];
+    if (mreal[m]) {
+      glo.x = fminf(glo.x, mlo[m].x); glo.y = fminf(glo.y, mlo[m].y);
+      glo.z = fminf(glo.z, mlo[m].z);
+      ghi.x = fmaxf(ghi.x, mhi[m].x); ghi.y = fmaxf(ghi.y, mhi[m].y);
+      ghi.z = fmaxf(ghi.z, mhi[m].z);
+    }
+  }
+  const bool any_real = (mreal[0] | mreal[1] | mreal[2] | mreal[3]) != 0u;
+  const int icol = cluster_col[g0];
   const int cx = icol % g.ncx, cy = icol / g.ncx;
   const float Lx = (float)g.box[0], Ly = (float)g.box[1], Lz = (float)g.box[2];
   const float wxf = (float)g.wx, wyf = (float)g.wy;
   int count = 0;
-  PairEntry *row = rows + (size_t)ci * g.row_cap;
-  // columns within reach: lanes test 64 column offsets at a time, a wave
-  // prefix sum of their candidate counts (clusters x 3 z shifts) lays the
-  // candidates out, and lanes then test 64 candidates at a time; every load
-  // of an iteration is independent of the others.
-  __shared__ int s_cstart[4][64], s_cbase[4][64], s_ccode[4][64];
-  const int ncolx = 2 * g.rx + 1, ncols = ncolx * (2 * g.ry + 1);
+  PairEntry *row = rows + (size_t)gi * g.row_cap;
+  const int ncolx = 2 * g.rx + 1, ncols = any_real ? ncolx * (2 * g.ry + 1) : 0;
   for (int cbase = 0; cbase < ncols; cbase += 64) {
     const int c = cbase + lane;
     int ncand = 0, cl0 = 0, code = 0;
@@ -280,8 +305,8 @@ __global__ void __launch_bounds__(256) nb_search(
       while (jx >= g.ncx) { jx -= g.ncx; ++sx; }
       const float ylo = jy * wyf - 0.5f * Ly + sy * Ly, yhi = ylo + wyf;
       const float xlo = jx * wxf - 0.5f * Lx + sx * Lx, xhi = xlo + wxf;
-      const float gy = axis_gap(ilo.y, ihi.y, ylo, yhi);
-      const float gx = axis_gap(ilo.x, ihi.x, xlo, xhi);
+      const float gy = axis_gap(glo.y, ghi.y, ylo, yhi);
+      const float gx = axis_gap(glo.x, ghi.x, xlo, xhi);
       if (gx * gx + gy * gy < g.rlist2 && sx >= -1 && sx <= 1 && sy >= -1 && sy <= 1) {
         const int jcol = jy * g.ncx + jx;
         cl0 = col_cl_start[jcol];
@@ -305,10 +330,8 @@ __global__ void __launch_bounds__(256) nb_search(
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     for (int base = 0; base < total; base += 64) {
       const int t = base + lane;
-      bool accept = false;
-      int cj = 0;
-      uint32_t scode = 0;
-      uint64_t mask = 0;
+      uint32_t imask = 0, jreal = 0, scode = 0;
+      int cj = 0, ex = -1;
       if (t < total) {
         // the column slot holding candidate t: last slot with cbase <= t
         int lo = 0, hi = 63;
@@ -322,31 +345,76 @@ __global__ void __launch_bounds__(256) nb_search(
         const int ccode = s_ccode[wave][lo];
         const int sx = (ccode & 3) - 1, sy = ((ccode >> 2) & 3) - 1;
         const int lex = sz != 0 ? sz : (sy != 0 ? sy : sx);
-        const bool ordered = cj > ci || (cj == ci && lex >= 0);
-        if (ordered) {
+        jreal = real_mask[cj];
+        scode = (uint32_t)(sx + 1) | ((uint32_t)(sy + 1) << 2) | ((uint32_t)(sz + 1) << 4);
+        if (jreal) {
           const float4 jlo = bb_lo[cj], jhi = bb_hi[cj];
           const float ox = sx * Lx, oy = sy * Ly, oz = sz * Lz;
-          const float ex = axis_gap(ilo.x, ihi.x, jlo.x + ox, jhi.x + ox);
-          const float ey = axis_gap(ilo.y, ihi.y, jlo.y + oy, jhi.y + oy);
-          const float ez = axis_gap(ilo.z, ihi.z, jlo.z + oz, jhi.z + oz);
-          if (ex * ex + ey * ey + ez * ez < g.rlist2) {
-            const uint32_t jreal = real_mask[cj];
+#pragma unroll
+          for (int m = 0; m < kGroup; ++m) {
+            const int ci = g0 + m;
+            const bool ordered = cj > ci || (cj == ci && lex >= 0);
+            const float ex_ = axis_gap(mlo[m].x, mhi[m].x, jlo.x + ox, jhi.x + ox);
+            const float ey_ = axis_gap(mlo[m].y, mhi[m].y, jlo.y + oy, jhi.y + oy);
+            const float ez_ = axis_gap(mlo[m].z, mhi[m].z, jlo.z + oz, jhi.z + oz);
+            if (mreal[m] && ordered && ex_ * ex_ + ey_ * ey_ + ez_ * ez_ < g.rlist2)
+              imask |= 1u << m;
+          }
+        }
+        if (imask) {
+          // a tile of a member with itself, or one that holds excluded pairs
+          const int self = cj - g0;
+          bool special = self >= 0 && self < kGroup && ((imask >> self) & 1u);
+          for (int k = 0; k < nex && !special; ++k)
+            special = s_ex_cluster[wave][k] == cj &&
+                      ((imask >> (s_ex_bit[wave][k] >> 8)) & 1u);
+          if (special) {
+            uint64_t jcols = 0;  // byte b all ones when j atom b is real
+#pragma unroll
             for (int b = 0; b < kCluster; ++b)
-              if (jreal & (1u << b)) mask |= (uint64_t)ireal << (kCluster * b);
-            if (cj == ci) {
-              uint64_t keep = 0;
-              for (int b = 0; b < kCluster; ++b)
-                for (int a = 0; a < kCluster; ++a)
-                  if (lex == 0 ? (a < b) : (a != b)) keep |= 1ull << (a + kCluster * b);
-              mask &= keep;
+              if ((jreal >> b) & 1u) jcols |= 0xFFull << (kCluster * b);
+            uint64_t mk[kGroup];
+#pragma unroll
+            for (int m = 0; m < kGroup; ++m) {
+              const uint64_t irows = (uint64_t)mreal[m] * 0x0101010101010101ull;
+              mk[m] = ((imask >> m) & 1u) ? (irows & jcols) : 0ull;
+              if (m == self) {
+                // same cluster: i < j at zero shift, i != j under a shift
+                uint64_t keep = 0;
+#pragma unroll
+                for (int b = 0; b < kCluster; ++b)
+#pragma unroll
+                  for (int a = 0; a < kCluster; ++a)
+                    if (lex == 0 ? (a < b) : (a != b)) keep |= 1ull << (a + kCluster * b);
+                mk[m] &= keep;
+              }
             }
             for (int k = 0; k < nex; ++k)
-              if (s_ex_cluster[wave][k] == cj) mask &= ~(1ull << s_ex_bit[wave][k]);
-            scode = (uint32_t)(sx + 1) | ((uint32_t)(sy + 1) << 2) | ((uint32_t)(sz + 1) << 4);
-            accept = mask != 0;
+              if (s_ex_cluster[wave][k] == cj) {
+                const int mm = s_ex_bit[wave][k] >> 8;
+                const uint64_t bit = 1ull << (s_ex_bit[wave][k] & 0xff);
+#pragma unroll
+                for (int m = 0; m < kGroup; ++m)
+                  if (m == mm) mk[m] &= ~bit;
+              }
+#pragma unroll
+            for (int m = 0; m < kGroup; ++m)
+              if (mk[m] == 0ull) imask &= ~(1u << m);
+            if (imask) {
+              const int slot = atomicAdd(&flags[kFlagMaskCount], 1);
+              if (slot < ex_cap) {
+#pragma unroll
+                for (int m = 0; m < kGroup; ++m) ex_masks[(size_t)slot * kGroup + m] = mk[m];
+                ex = slot;
+              } else {
+                atomicOr(&flags[kFlagOverflowRow], 4);
+                imask = 0;
+              }
+            }
           }
         }
       }
+      const bool accept = imask != 0u;
       const uint64_t ballot = __ballot(accept);
       const int before = __popcll(ballot & ((1ull << lane) - 1ull));
       if (accept) {
@@ -354,8 +422,9 @@ __global__ void __launch_bounds__(256) nb_search(
         if (slot < g.row_cap) {
           PairEntry e;
           e.cj = cj;
-          e.shift = scode;
-          e.mask = mask;
+          e.info = scode | (imask << 8) | (jreal << 16);
+          e.ex = ex;
+          e.pad = 0;
           row[slot] = e;
         } else {
           atomicOr(&flags[kFlagOverflowRow], 1);
@@ -365,7 +434,7 @@ __global__ void __launch_bounds__(256) nb_search(
     }
     __builtin_amdgcn_wave_barrier();
   }
-  if (lane == 0) row_count[ci] = count < g.row_cap ? count : g.row_cap;
+  if (lane == 0) row_count[gi] = count < g.row_cap ? count : g.row_cap;
 }
 
 // ---- exports for parity tests -------------------------------------------
@@ -376,54 +445,58 @@ __global__ void __launch_bounds__(256) nb_search(
 // written as (min atom, max atom).  The host removes duplicates.
 __global__ void __launch_bounds__(256) nb_export_pairs(
     const double *__restrict__ x, const int *__restrict__ atom_at,
-    const PairEntry *__restrict__ rows, const int *__restrict__ row_count,
-    const int *__restrict__ counts_in, int row_cap, float Lx, float Ly,
-    float Lz, double bx, double by, double bz, float limit2,
-    int2 *__restrict__ out, unsigned long long *__restrict__ out_count,
-    unsigned long long capacity) {
+    const uint32_t *__restrict__ real_mask, const PairEntry *__restrict__ rows,
+    const int *__restrict__ row_count, const uint64_t *__restrict__ ex_masks,
+    const int *__restrict__ counts_in, int row_cap, float Lx, float Ly, float Lz,
+    double bx, double by, double bz, float limit2, int2 *__restrict__ out,
+    unsigned long long *__restrict__ out_count, unsigned long long capacity) {
 #pragma clang fp contract(off)
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int ci = blockIdx.x * 4 + wave;
-  if (ci >= counts_in[1]) return;
+  const int gi = blockIdx.x * 4 + wave;
+  if (gi >= counts_in[1] / kGroup) return;
   const int a = lane & 7, b = lane >> 3;
-  const int ia = atom_at[ci * kCluster + a];
   const float ilx = __fdiv_rn(1.0f, Lx), ily = __fdiv_rn(1.0f, Ly),
               ilz = __fdiv_rn(1.0f, Lz);
-  float wi[3] = {0, 0, 0};
-  if (ia >= 0) {
-    double L[3] = {bx, by, bz};
-    for (int k = 0; k < 3; ++k) {
-      double v = x[3 * ia + k];
-      wi[k] = (float)(v - floor(v * (1.0 / L[k])) * L[k]);
-    }
-  }
-  const int n = row_count[ci];
-  for (int e = 0; e < n; ++e) {
-    PairEntry E = rows[(size_t)ci * row_cap + e];
-    if (!((E.mask >> lane) & 1ull)) continue;
-    int jb = atom_at[E.cj * kCluster + b];
-    if (ia < 0 || jb < 0) continue;
-    double L[3] = {bx, by, bz};
-    float wj[3];
-    for (int k = 0; k < 3; ++k) {
-      double v = x[3 * jb + k];
-      wj[k] = (float)(v - floor(v * (1.0 / L[k])) * L[k]);
-    }
-    float Lf[3] = {Lx, Ly, Lz}, il[3] = {ilx, ily, ilz};
-    float d2[3];
-    for (int k = 0; k < 3; ++k) {
-      float d = __fsub_rn(wi[k], wj[k]);
-      float images = __fmul_rn(d, il[k]);
-      float nearest = rintf(images);
-      float sh = __fmul_rn(nearest, Lf[k]);
-      float dd = __fsub_rn(d, sh);
-      d2[k] = __fmul_rn(dd, dd);
-    }
-    float r2 = __fadd_rn(__fadd_rn(d2[0], d2[1]), d2[2]);
-    if (r2 < limit2) {
-      unsigned long long slot = atomicAdd(out_count, 1ull);
-      if (slot < capacity)
-        out[slot] = make_int2(ia < jb ? ia : jb, ia < jb ? jb : ia);
+  const double L[3] = {bx, by, bz};
+  const float Lf[3] = {Lx, Ly, Lz}, il[3] = {ilx, ily, ilz};
+  const int n = row_count[gi];
+  for (int m = 0; m < kGroup; ++m) {
+    const int ci = gi * kGroup + m;
+    const int ia = atom_at[ci * kCluster + a];
+    const bool ireal = (real_mask[ci] >> a) & 1u;
+    float wi[3] = {0, 0, 0};
+    if (ia >= 0)
+      for (int k = 0; k < 3; ++k) {
+        const double v = x[3 * ia + k];
+        wi[k] = (float)(v - floor(v * (1.0 / L[k])) * L[k]);
+      }
+    for (int e = 0; e < n; ++e) {
+      const PairEntry E = rows[(size_t)gi * row_cap + e];
+      if (!((E.info >> (8 + m)) & 1u)) continue;
+      const bool on = E.ex >= 0 ? ((ex_masks[(size_t)E.ex * kGroup + m] >> lane) & 1ull)
+                                : (ireal && ((E.info >> (16 + b)) & 1u));
+      if (!on) continue;
+      const int jb = atom_at[E.cj * kCluster + b];
+      if (ia < 0 || jb < 0) continue;
+      float wj[3];
+      for (int k = 0; k < 3; ++k) {
+        const double v = x[3 * jb + k];
+        wj[k] = (float)(v - floor(v * (1.0 / L[k])) * L[k]);
+      }
+      float d2[3];
+      for (int k = 0; k < 3; ++k) {
+        const float d = __fsub_rn(wi[k], wj[k]);
+        const float images = __fmul_rn(d, il[k]);
+        const float nearest = rintf(images);
+        const float sh = __fmul_rn(nearest, Lf[k]);
+        const float dd = __fsub_rn(d, sh);
+        d2[k] = __fmul_rn(dd, dd);
+      }
+      const float r2 = __fadd_rn(__fadd_rn(d2[0], d2[1]), d2[2]);
+      if (r2 < limit2) {
+        const unsigned long long slot = atomicAdd(out_count, 1ull);
+        if (slot < capacity) out[slot] = make_int2(ia < jb ? ia : jb, ia < jb ? jb : ia);
+      }
     }
   }
 }

@@ -7,11 +7,13 @@
 // -f qq erf(beta r)/r (TermNonbonded.cpp:1615-1671) and scaled 1-4 pairs
 // (TermPairs14.cpp:11-17).
 //
-// Pair kernel: one wave per i-cluster row.  Lane l holds i atom a = l & 7 and
-// j atom b = l >> 3 of each 8 x 8 tile, so a tile is one instruction stream
-// with one pair per lane.  The force on j is summed over the 8 lanes of a
-// column by three DPP row operations and added as 64-bit fixed point; the
-// force on i stays in registers across the row and is summed once at the end.
+// Pair kernel: one wave per segment of a group row (kGroup i-clusters against
+// up to kSegment j-clusters).  Lane l holds i atom a = l & 7 and j atom
+// b = l >> 3 of each 8 x 8 tile, so a tile is one instruction stream with one
+// pair per lane.  The force on j is accumulated over the members an entry
+// pairs with, summed over the 8 lanes of a column by three DPP row operations
+// and added once as 64-bit fixed point; the forces on the members stay in
+// registers across the segment and are summed once at the end.
 #include "md_common.h"
 
 namespace mdhip {
@@ -115,73 +117,117 @@ __device__ __forceinline__ float pair_force(float r2, float qq, float2 lj,
 template <int MOD, bool COUL, bool ENERGY>
 __global__ void __launch_bounds__(256) nb_pairs(
     const float4 *__restrict__ xq, const int *__restrict__ type_at,
-    const PairEntry *__restrict__ rows, const int *__restrict__ row_count,
-    int row_cap, const int *__restrict__ counts_in,
+    const uint32_t *__restrict__ real_mask, const PairEntry *__restrict__ rows,
+    const int *__restrict__ row_count, const uint64_t *__restrict__ ex_masks,
+    int row_cap, int max_groups, const int *__restrict__ counts_in,
     const float2 *__restrict__ ljtab, NbParams p, long long *__restrict__ F,
     double *__restrict__ energy_partial, int *__restrict__ flags) {
   __shared__ float2 s_lj[kMaxTypes * kMaxTypes];
   const int nt2 = p.ntypes * p.ntypes;
   for (int k = threadIdx.x; k < nt2; k += blockDim.x) s_lj[k] = ljtab[k];
   // The build kernels of this step have finished: clear the request.
-  if (blockIdx.x == 0 && threadIdx.x == 0) flags[kFlagRebuild] = 0;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) flags[kFlagRebuild] = 0;
   __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int ci = blockIdx.x * 4 + wave;
-  if (ci >= counts_in[1]) return;
+  const int lane = threadIdx.x & 63;
+  const int gi = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  const int e0 = blockIdx.y * kSegment;
+  const int ngroups = counts_in[1] / kGroup;
+  int n = gi < ngroups ? row_count[gi] - e0 : 0;
+  n = __builtin_amdgcn_readfirstlane(n < kSegment ? n : kSegment);
+  const size_t slot = (size_t)blockIdx.y * max_groups + gi;
+  if (n <= 0) {
+    if (ENERGY && gi < max_groups && lane == 0) {
+      energy_partial[2 * slot] = 0.0;
+      energy_partial[2 * slot + 1] = 0.0;
+    }
+    return;
+  }
   const int a = lane & 7, b = lane >> 3;
-  const float4 xi = xq[ci * kCluster + a];
-  const int ti = type_at[ci * kCluster + a];
-  const float qi = xi.w * p.coulomb;
-  float fix = 0.0f, fiy = 0.0f, fiz = 0.0f;
+  const int g0 = gi * kGroup;
+  float4 xi[kGroup];
+  int ti[kGroup];
+  bool ireal[kGroup];
+  float fi[kGroup][3];
+#pragma unroll
+  for (int m = 0; m < kGroup; ++m) {
+    xi[m] = xq[(g0 + m) * kCluster + a];
+    xi[m].w *= p.coulomb;
+    ti[m] = type_at[(g0 + m) * kCluster + a] * p.ntypes;
+    ireal[m] = (real_mask[g0 + m] >> a) & 1u;
+    fi[m][0] = fi[m][1] = fi[m][2] = 0.0f;
+  }
   double e_lj = 0.0, e_c = 0.0;  // energy steps only: f64 partial sums
-  const int n = row_count[ci];
-  const PairEntry *row = rows + (size_t)ci * row_cap;
+  const PairEntry *row = rows + (size_t)gi * row_cap + e0;
+  // Software pipeline: the entry two ahead and the j atoms one ahead are
+  // loaded before the current entry is computed (indices clamped to the
+  // segment, so every load is in bounds).
+  PairEntry E = row[0];
+  PairEntry E1 = row[n > 1 ? 1 : 0];
+  float4 xj = xq[E.cj * kCluster + b];
+  int tj = type_at[E.cj * kCluster + b];
   for (int e = 0; e < n; ++e) {
-    const PairEntry E = row[e];
-    const int pj = E.cj * kCluster + b;
-    const float4 xj = xq[pj];
-    const int tj = type_at[pj];
+    const PairEntry E2 = row[e + 2 < n ? e + 2 : n - 1];
+    const float4 xj1 = xq[E1.cj * kCluster + b];
+    const int tj1 = type_at[E1.cj * kCluster + b];
+
+    const uint32_t info = E.info;
     float sx, sy, sz;
-    shift_vector(E.shift, p.box, sx, sy, sz);
-    const float dx = xi.x - (xj.x + sx);
-    const float dy = xi.y - (xj.y + sy);
-    const float dz = xi.z - (xj.z + sz);
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    const bool on = ((E.mask >> lane) & 1ull) && (r2 < p.rc2);
-    float fscal = 0.0f;
-    if (on) {
+    shift_vector(info & 63u, p.box, sx, sy, sz);
+    const float xjx = xj.x + sx, xjy = xj.y + sy, xjz = xj.z + sz;
+    const bool jreal = (info >> (16 + b)) & 1u;
+    float fjx = 0.0f, fjy = 0.0f, fjz = 0.0f;
+#pragma unroll
+    for (int m = 0; m < kGroup; ++m) {
+      if (!((info >> (8 + m)) & 1u)) continue;  // uniform
+      const float dx = xi[m].x - xjx, dy = xi[m].y - xjy, dz = xi[m].z - xjz;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      bool on = E.ex >= 0 ? ((ex_masks[(size_t)E.ex * kGroup + m] >> lane) & 1ull)
+                          : (ireal[m] && jreal);
+      on = on && r2 < p.rc2;
       float el = 0.0f, ec = 0.0f;
-      fscal = pair_force<MOD, COUL, ENERGY>(r2, qi * xj.w, s_lj[ti * p.ntypes + tj], p, el, ec);
-      if (ENERGY) {
+      float fscal = pair_force<MOD, COUL, ENERGY>(on ? r2 : 1.0f, xi[m].w * xj.w,
+                                                  s_lj[ti[m] + tj], p, el, ec);
+      fscal = on ? fscal : 0.0f;
+      if (ENERGY && on) {
         e_lj += (double)el;
         e_c += (double)ec;
       }
+      const float fx = fscal * dx, fy = fscal * dy, fz = fscal * dz;
+      fi[m][0] += fx;
+      fi[m][1] += fy;
+      fi[m][2] += fz;
+      fjx -= fx;
+      fjy -= fy;
+      fjz -= fz;
     }
-    const float fx = fscal * dx, fy = fscal * dy, fz = fscal * dz;
-    fix += fx;
-    fiy += fy;
-    fiz += fz;
-    const float gx = sum8(fx), gy = sum8(fy), gz = sum8(fz);
-    const bool jreal = ((E.mask >> (kCluster * b)) & 0xFFull) != 0ull;
-    if (a == 0 && jreal) add_fixed(F, pj, -gx, -gy, -gz);
+    // j force: the 8 i lanes of column b, summed across the members
+    const float gx = sum8(fjx), gy = sum8(fjy), gz = sum8(fjz);
+    if (a == 0 && jreal) add_fixed(F, E.cj * kCluster + b, gx, gy, gz);
+    E = E1;
+    E1 = E2;
+    xj = xj1;
+    tj = tj1;
   }
-  // sum the i force over the 8 columns of the wave
+  // i forces: sum over the 8 j columns of the wave
 #pragma unroll
-  for (int m = 8; m < 64; m <<= 1) {
-    fix += __shfl_xor(fix, m);
-    fiy += __shfl_xor(fiy, m);
-    fiz += __shfl_xor(fiz, m);
+  for (int m = 0; m < kGroup; ++m) {
+#pragma unroll
+    for (int s = 8; s < 64; s <<= 1) {
+      fi[m][0] += __shfl_xor(fi[m][0], s);
+      fi[m][1] += __shfl_xor(fi[m][1], s);
+      fi[m][2] += __shfl_xor(fi[m][2], s);
+    }
+    if (b == 0 && ireal[m]) add_fixed(F, (g0 + m) * kCluster + a, fi[m][0], fi[m][1], fi[m][2]);
   }
-  if (b == 0 && n > 0) add_fixed(F, ci * kCluster + a, fix, fiy, fiz);
   if (ENERGY) {
 #pragma unroll
-    for (int m = 1; m < 64; m <<= 1) {
-      e_lj += __shfl_xor(e_lj, m);
-      e_c += __shfl_xor(e_c, m);
+    for (int s = 1; s < 64; s <<= 1) {
+      e_lj += __shfl_xor(e_lj, s);
+      e_c += __shfl_xor(e_c, s);
     }
     if (lane == 0) {
-      energy_partial[2 * ci] = e_lj;
-      energy_partial[2 * ci + 1] = e_c;
+      energy_partial[2 * slot] = e_lj;
+      energy_partial[2 * slot + 1] = e_c;
     }
   }
 }
@@ -299,19 +345,15 @@ __global__ void __launch_bounds__(256) nb_special(
 }
 
 // Instantiations used by the engine.
+#define MDHIP_PAIRS_ONE(MOD, C, E)                                                       \
+  template __global__ void nb_pairs<MOD, C, E>(const float4 *, const int *,            \
+      const uint32_t *, const PairEntry *, const int *, const uint64_t *, int, int,    \
+      const int *, const float2 *, NbParams, long long *, double *, int *);
 #define MDHIP_PAIRS(MOD)                                                                 \
-  template __global__ void nb_pairs<MOD, true, false>(const float4 *, const int *,      \
-      const PairEntry *, const int *, int, const int *, const float2 *, NbParams,       \
-      long long *, double *, int *);                                                    \
-  template __global__ void nb_pairs<MOD, true, true>(const float4 *, const int *,       \
-      const PairEntry *, const int *, int, const int *, const float2 *, NbParams,       \
-      long long *, double *, int *);                                                    \
-  template __global__ void nb_pairs<MOD, false, false>(const float4 *, const int *,     \
-      const PairEntry *, const int *, int, const int *, const float2 *, NbParams,       \
-      long long *, double *, int *);                                                    \
-  template __global__ void nb_pairs<MOD, false, true>(const float4 *, const int *,      \
-      const PairEntry *, const int *, int, const int *, const float2 *, NbParams,       \
-      long long *, double *, int *);
+  MDHIP_PAIRS_ONE(MOD, true, false)                                                      \
+  MDHIP_PAIRS_ONE(MOD, true, true)                                                       \
+  MDHIP_PAIRS_ONE(MOD, false, false)                                                     \
+  MDHIP_PAIRS_ONE(MOD, false, true)
 MDHIP_PAIRS(kLJNone)
 MDHIP_PAIRS(kLJPotentialShift)
 MDHIP_PAIRS(kLJForceSwitch)

@@ -5,7 +5,8 @@
 //             (PMEWeights.cpp:8)
 //   spread    Q(l) = sum_j q_j prod_a M_n(u_aj - l_a)  (PMESpread.cpp; the
 //             deterministic mode adds q prod M scaled by 2^40 with 64-bit
-//             integer atomics, docs/pme.md:310-331) — done that way here
+//             integer atomics, docs/pme.md:310-331) — done that way here,
+//             in LDS, by bricks of grid rows fused with the z transform
 //   R2C FFT   unnormalised, half spectrum along z
 //   product   Q^ <- BC Q^, BC = t1 t2 t3 / m^2, C(0) = 0; energy
 //             sum h BC |Q^|^2 with h = 1/2 on the k3 = 0 and Nyquist planes
@@ -14,10 +15,10 @@
 //   C2R FFT   unnormalised: phi
 //   gather    F_ja = -q_j (K_a/L_a) sum_l phi(l) M'(u_a - l_a) prod M
 //             (PMEGather.cpp)
-// The FFT is the LDS Stockham of fft.h; the forward z pass reads the 64-bit
-// charge words, converts them and clears them for the next step, and the
-// x pass does forward transform, influence product and backward transform
-// of each x line in one LDS residency.
+// The FFT is the LDS Stockham of fft.h; the forward z pass runs in the
+// spread kernel on the brick it has just accumulated, and the x pass does
+// forward transform, influence product and backward transform of each x
+// line in one LDS residency.
 #include "fft.h"
 #include "md_common.h"
 
@@ -79,118 +80,165 @@ __device__ __forceinline__ void pme_place(double x, double L, double invL, int K
   first = f;
 }
 
-// ---- spread: LDS tiles of the xy columns of the cluster sort -----------
+// ---- spread fused with the forward z transform ---------------------------
 //
-// The atoms of one xy column of the build lie in a contiguous range of
-// places.  Their B-spline stencils cover grid points x in [gx0, gx0 + TWx),
-// y in [gy0, gy0 + TWy) of that column (all z): the column's extent, the
-// distance an atom may move before the validity test rebuilds the list,
-// and the order.  A block accumulates one column's tile in LDS — each
-// thread owns tile points (x, y) and adds the atoms in place order, so the
-// f32 sums have a fixed order and need no atomics — and stores the tile.
-// The first FFT pass sums the tiles that cover each grid row, in a fixed
-// order (fft_rows_r2c_tiles).
+// A block owns a brick of BX x BY grid rows (all of z) and produces their
+// half spectra along z.  Its atoms are those of the xy columns of the
+// cluster sort whose PME tiles overlap the brick: a column's tile is the
+// range of grid points the B-spline stencils of its atoms can cover, given
+// the column's extent, the distance an atom may move before the validity
+// test rebuilds the list, and the order.  Each atom adds the part of its
+// stencil that lies in the brick as 64-bit fixed point (2^40 per e) by LDS
+// integer atomics, so the sums do not depend on the order of the adds; the
+// brick's rows are then converted to f32 and transformed in place in LDS.
+// Every grid point belongs to one brick: no global atomics, no grid to
+// clear, no real-space grid in HBM.
 
-struct TileParams {
-  int K0;                // grid points along x (rows are K0 x K1)
-  int TWx, TWy;          // tile extent in grid points (<= K)
-  int ncx;               // columns along x
-  int zc;                // z points per block (a chunk of K3)
-  int nzc;               // chunks
-  const int *gx0;        // first tile point of each column along x (mod K)
-  const int *gy0;
-  const int *xcov;       // per grid x: up to kMaxCover (column x, offset), -1 ends
-  const int *ycov;
+struct BrickParams {
+  int ncx;                  // columns of the cluster sort along x
+  int TWx, TWy;             // column tile extents (grid points)
+  const int *gx0, *gy0;     // first tile point of each column (mod K)
+  const int *xstart, *xcols;  // per brick along x: columns (x index) whose tile overlaps
+  const int *ystart, *ycols;
 };
-constexpr int kMaxCover = 8;
-constexpr int kTileAtoms = 256;
+constexpr int kBrickMaxCols = 256;
 
-template <int N>
-__global__ void __launch_bounds__(256) pme_spread_tiles(
+template <int TL> struct BrickShape {
+  static constexpr int R = 2 * TL;                  // real rows of a brick
+  static constexpr int BY = R >= 8 ? 4 : 2;
+  static constexpr int BX = R / BY;
+};
+
+template <int N, int TL>
+__global__ void __launch_bounds__(256) pme_spread_rows(
     const int *__restrict__ col_cl_start, const int *__restrict__ col_cl_count,
     const int *__restrict__ atom_at, const double *__restrict__ x,
-    const double *__restrict__ q, PmeParams pp, TileParams tp,
-    float *__restrict__ tiles, int *__restrict__ flags) {
-  extern __shared__ float s_tile[];
-  __shared__ int s_fx[kTileAtoms], s_fy[kTileAtoms], s_fz[kTileAtoms];
-  __shared__ float s_w[kTileAtoms][3 * N];
-  const int col = blockIdx.x / tp.nzc, zc = blockIdx.x % tp.nzc;
-  const int K0 = pp.K[0], K1 = pp.K[1], K2 = pp.K[2];
-  const int z0 = zc * tp.zc, zlen = min(tp.zc, K2 - z0);
-  const int zs = zlen + 1;  // padded z stride of the LDS tile
-  const int cx = col % tp.ncx, cy = col / tp.ncx;
-  const int gx0 = tp.gx0[cx], gy0 = tp.gy0[cy];
-  const int p0 = col_cl_start[col] * kCluster, np = col_cl_count[col] * kCluster;
-  const int npts = tp.TWx * tp.TWy;
-  for (int k = threadIdx.x; k < npts * zs; k += blockDim.x) s_tile[k] = 0.0f;
-  for (int a0 = 0; a0 < np; a0 += kTileAtoms) {
-    const int na = min(kTileAtoms, np - a0);
-    __syncthreads();
-    for (int t = threadIdx.x; t < na; t += blockDim.x) {
-      const int atom = atom_at[p0 + a0 + t];
-      int fx = -1 << 20, fy = 0, fz = 0;
-      if (atom >= 0 && q[atom] != 0.0) {
-        int f[3];
-        float M[3][N];
-        for (int d = 0; d < 3; ++d) {
-          float w;
-          pme_place(x[3 * atom + d], pp.box[d], pp.inv_box[d], pp.K[d], N, f[d], w);
-          bspline<float>(w, N, M[d], nullptr);
-        }
-        fx = f[0] - gx0;
-        if (fx < 0) fx += K0;
-        fy = f[1] - gy0;
-        if (fy < 0) fy += K1;
-        fz = f[2];
-        // an atom outside its column's tile would lose its charge
-        if ((tp.TWx < K0 && fx > tp.TWx - N) || (tp.TWy < K1 && fy > tp.TWy - N))
-          atomicOr(&flags[kFlagPmeRange], 1);
-        const float qa = (float)q[atom];
+    const double *__restrict__ q, PmeParams pp, BrickParams bp,
+    cpx<float> *__restrict__ out, FftAxis az, int *__restrict__ flags) {
+  constexpr int R = BrickShape<TL>::R, BX = BrickShape<TL>::BX, BY = BrickShape<TL>::BY;
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  __shared__ int s_p0[kBrickMaxCols], s_cum[kBrickMaxCols + 1], s_cxy[kBrickMaxCols];
+  const int K0 = pp.K[0], K1 = pp.K[1], K2 = pp.K[2], H = K2 / 2 + 1;
+  unsigned long long *acc = reinterpret_cast<unsigned long long *>(smem);
+  cpx<float> *a = reinterpret_cast<cpx<float> *>(acc + (size_t)R * K2);
+  cpx<float> *b = a + TL * K2;
+  const int nbx = (K0 + BX - 1) / BX;
+  const int bxi = blockIdx.x % nbx, byi = blockIdx.x / nbx;
+  const int bx0 = bxi * BX, by0 = byi * BY;
+  for (int w = threadIdx.x; w < R * K2; w += blockDim.x) acc[w] = 0ull;
+
+  // the brick's columns and their place ranges
+  const int xs = bp.xstart[bxi], nxc = bp.xstart[bxi + 1] - xs;
+  const int ys = bp.ystart[byi], nyc = bp.ystart[byi + 1] - ys;
+  const int ncand = nxc * nyc;  // <= kBrickMaxCols (checked by the host)
+  for (int c = threadIdx.x; c < ncand; c += blockDim.x) {
+    const int cx = bp.xcols[xs + c % nxc], cy = bp.ycols[ys + c / nxc];
+    const int col = cy * bp.ncx + cx;
+    s_p0[c] = col_cl_start[col] * kCluster;
+    s_cum[c + 1] = col_cl_count[col] * kCluster;
+    s_cxy[c] = cx | (cy << 16);
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {  // inclusive scan of the counts by one wave
+    const int lane = threadIdx.x;
+    int carry = 0;
+    for (int c0 = 0; c0 < ncand; c0 += 64) {
+      const int c = c0 + lane;
+      int v = c < ncand ? s_cum[c + 1] : 0;
 #pragma unroll
-        for (int k = 0; k < N; ++k) {
-          s_w[t][k] = qa * M[0][k];
-          s_w[t][N + k] = M[1][k];
-          s_w[t][2 * N + k] = M[2][k];
-        }
+      for (int off = 1; off < 64; off <<= 1) {
+        const int up = __shfl_up(v, off);
+        if (lane >= off) v += up;
       }
-      s_fx[t] = fx;
-      s_fy[t] = fy;
-      s_fz[t] = fz;
+      if (c < ncand) s_cum[c + 1] = carry + v;
+      carry += __shfl(v, 63);
     }
-    __syncthreads();
-    for (int xy = threadIdx.x; xy < npts; xy += blockDim.x) {
-      const int tx = xy / tp.TWy, ty = xy % tp.TWy;
-      float *dst = s_tile + xy * zs;
-      for (int a = 0; a < na; ++a) {
-        int ox = tx - s_fx[a];
-        if (tp.TWx == K0 && ox < 0) ox += K0;
-        if (ox < 0 || ox >= N) continue;
-        int oy = ty - s_fy[a];
-        if (tp.TWy == K1 && oy < 0) oy += K1;
-        if (oy < 0 || oy >= N) continue;
-        float wxy = s_w[a][0], wy = s_w[a][N];
+    if (lane == 0) s_cum[0] = 0;
+  }
+  __syncthreads();
+
+  const int total = s_cum[ncand];
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    int lo = 0, hi = ncand - 1;  // last column with s_cum <= i
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_cum[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    const int atom = atom_at[s_p0[lo] + (i - s_cum[lo])];
+    if (atom < 0) continue;
+    const double qd = q[atom];
+    if (qd == 0.0) continue;
+    int f[3];
+    float M[3][N];
 #pragma unroll
-        for (int k = 1; k < N; ++k) {
-          if (k == ox) wxy = s_w[a][k];
-          if (k == oy) wy = s_w[a][N + k];
-        }
-        wxy *= wy;
-        const int fz = s_fz[a];
+    for (int d = 0; d < 3; ++d) {
+      float w;
+      pme_place(x[3 * atom + d], pp.box[d], pp.inv_box[d], pp.K[d], N, f[d], w);
+      bspline<float>(w, N, M[d], nullptr);
+    }
+    // an atom outside its column's tile would be missed by some brick
+    const int cx = s_cxy[lo] & 0xffff, cy = s_cxy[lo] >> 16;
+    int fx = f[0] - bp.gx0[cx];
+    if (fx < 0) fx += K0;
+    int fy = f[1] - bp.gy0[cy];
+    if (fy < 0) fy += K1;
+    if ((bp.TWx < K0 && fx > bp.TWx - N) || (bp.TWy < K1 && fy > bp.TWy - N))
+      atomicOr(&flags[kFlagPmeRange], 1);
+    const float qf = (float)qd;
+#pragma unroll
+    for (int tx = 0; tx < BX; ++tx) {
+      int kx = bx0 + tx - f[0];
+      if (kx < 0) kx += K0;
+      if (kx >= N || bx0 + tx >= K0) continue;
+      float wx = M[0][0];
+#pragma unroll
+      for (int k = 1; k < N; ++k)
+        if (k == kx) wx = M[0][k];
+      wx *= qf;
+#pragma unroll
+      for (int ty = 0; ty < BY; ++ty) {
+        int ky = by0 + ty - f[1];
+        if (ky < 0) ky += K1;
+        if (ky >= N || by0 + ty >= K1) continue;
+        float wxy = M[1][0];
+#pragma unroll
+        for (int k = 1; k < N; ++k)
+          if (k == ky) wxy = M[1][k];
+        wxy *= wx;
+        unsigned long long *row = acc + (tx * BY + ty) * K2;
 #pragma unroll
         for (int k = 0; k < N; ++k) {
-          int z = fz + k;
+          int z = f[2] + k;
           if (z >= K2) z -= K2;
-          const int rz = z - z0;
-          if (rz >= 0 && rz < zlen) dst[rz] += wxy * s_w[a][2 * N + k];
+          const long long v = __double2ll_rn((double)(wxy * M[2][k]) * kGridScale);
+          atomicAdd(row + z, (unsigned long long)v);
         }
       }
     }
   }
   __syncthreads();
-  float *out = tiles + (size_t)col * npts * K2 + z0;
-  for (int k = threadIdx.x; k < npts * zlen; k += blockDim.x) {
-    const int xy = k / zlen, z = k % zlen;
-    out[(size_t)xy * K2 + z] = s_tile[xy * zs + z];
+  // rows 2t and 2t+1 of the brick as the real and imaginary parts of line t
+  for (int w = threadIdx.x; w < R * K2; w += blockDim.x) {
+    const int rr = w / K2, z = w % K2;
+    const float v = (float)((double)(long long)acc[w] * kGridUnscale);
+    if (rr & 1)
+      a[z * TL + (rr >> 1)].y = v;
+    else
+      a[z * TL + (rr >> 1)].x = v;
+  }
+  __syncthreads();
+  cpx<float> *res = lds_fft<float, TL>(a, b, az, -1);
+  for (int w = threadIdx.x; w < TL * H; w += blockDim.x) {
+    const int k = w % H, t = w / H;
+    const cpx<float> zk = res[k * TL + t];
+    const cpx<float> zm = res[((K2 - k) % K2) * TL + t];
+    const int r1 = 2 * t, r2 = r1 + 1;
+    const int ix1 = bx0 + r1 / BY, iy1 = by0 + r1 % BY;
+    const int ix2 = bx0 + r2 / BY, iy2 = by0 + r2 % BY;
+    if (ix1 < K0 && iy1 < K1)
+      out[((size_t)ix1 * K1 + iy1) * H + k] = {(zk.x + zm.x) * 0.5f, (zk.y - zm.y) * 0.5f};
+    if (ix2 < K0 && iy2 < K1)
+      out[((size_t)ix2 * K1 + iy2) * H + k] = {(zk.y + zm.y) * 0.5f, (zm.x - zk.x) * 0.5f};
   }
 }
 
@@ -310,59 +358,6 @@ __global__ void __launch_bounds__(256) fft_rows_r2c(
     if (r1 < nrows)
       out[(size_t)r1 * H + k] = {(zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5};
     if (r2 < nrows)
-      out[(size_t)r2 * H + k] = {(zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5};
-  }
-}
-
-// Forward along z of the grid rows summed from the column tiles of
-// pme_spread_tiles (in a fixed order: y cover, then x cover).
-template <typename T>
-__global__ void __launch_bounds__(256) fft_rows_r2c_tiles(
-    const float *__restrict__ tiles, TileParams tp, cpx<T> *__restrict__ out,
-    int K1, FftAxis az) {
-  constexpr int TL = tile_lines<T>();
-  extern __shared__ unsigned char smem[];
-  cpx<T> *a = reinterpret_cast<cpx<T> *>(smem);
-  const int K = az.K, H = K / 2 + 1;
-  cpx<T> *b = a + TL * K;
-  const int row0 = blockIdx.x * 2 * TL;
-  const int npts = tp.TWx * tp.TWy;
-  for (int w = threadIdx.x; w < 2 * TL * K; w += blockDim.x) {
-    const int z = w % K, rr = w / K;
-    const int row = row0 + rr;
-    const int ix = row / K1, iy = row % K1;
-    float v = 0.0f;
-    if (ix < tp.K0) {
-      for (int cyi = 0; cyi < kMaxCover; ++cyi) {
-        const int ce = tp.ycov[iy * kMaxCover + cyi];
-        if (ce < 0) break;
-        const int cy = ce >> 16, oy = ce & 0xffff;
-        for (int cxi = 0; cxi < kMaxCover; ++cxi) {
-          const int xe = tp.xcov[ix * kMaxCover + cxi];
-          if (xe < 0) break;
-          const int cx = xe >> 16, ox = xe & 0xffff;
-          const int col = cy * tp.ncx + cx;
-          v += tiles[((size_t)col * npts + ox * tp.TWy + oy) * K + z];
-        }
-      }
-    }
-    const int t = rr >> 1;
-    if (rr & 1)
-      a[z * TL + t].y = (T)v;
-    else
-      a[z * TL + t].x = (T)v;
-  }
-  __syncthreads();
-  cpx<T> *res = lds_fft<T, TL>(a, b, az, -1);
-  const int rows_total = tp.K0 * K1;
-  for (int w = threadIdx.x; w < TL * H; w += blockDim.x) {
-    const int k = w % H, t = w / H;
-    const cpx<T> zk = res[k * TL + t];
-    const cpx<T> zm = res[((K - k) % K) * TL + t];
-    const int r1 = row0 + 2 * t, r2 = r1 + 1;
-    if (r1 < rows_total)
-      out[(size_t)r1 * H + k] = {(zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5};
-    if (r2 < rows_total)
       out[(size_t)r2 * H + k] = {(zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5};
   }
 }
@@ -509,10 +504,13 @@ __global__ void __launch_bounds__(256) fft_x_conv(cpx<T> *__restrict__ data, int
 }
 
 template __global__ void fft_rows_r2c<float, true>(const void *, cpx<float> *, int, FftAxis);
-template __global__ void fft_rows_r2c_tiles<float>(const float *, TileParams, cpx<float> *, int, FftAxis);
-template __global__ void pme_spread_tiles<4>(const int *, const int *, const int *, const double *, const double *, PmeParams, TileParams, float *, int *);
-template __global__ void pme_spread_tiles<6>(const int *, const int *, const int *, const double *, const double *, PmeParams, TileParams, float *, int *);
-template __global__ void pme_spread_tiles<8>(const int *, const int *, const int *, const double *, const double *, PmeParams, TileParams, float *, int *);
+#define MDHIP_SPREAD(N, TL)                                                              \
+  template __global__ void pme_spread_rows<N, TL>(const int *, const int *, const int *, \
+      const double *, const double *, PmeParams, BrickParams, cpx<float> *, FftAxis, int *);
+#define MDHIP_SPREAD_N(N) MDHIP_SPREAD(N, 8) MDHIP_SPREAD(N, 4) MDHIP_SPREAD(N, 2) MDHIP_SPREAD(N, 1)
+MDHIP_SPREAD_N(4)
+MDHIP_SPREAD_N(6)
+MDHIP_SPREAD_N(8)
 template __global__ void pme_gather_lanes<4, 4>(const int *, const int *, const double *, const double *, PmeParams, const float *, long long *);
 template __global__ void pme_gather_lanes<6, 8>(const int *, const int *, const double *, const double *, PmeParams, const float *, long long *);
 template __global__ void pme_gather_lanes<8, 8>(const int *, const int *, const double *, const double *, PmeParams, const float *, long long *);

@@ -151,12 +151,12 @@ class Simulation:
     def stats(self) -> dict:
         s = np.zeros(8, np.int64)
         rt.check(rt.lib().mdrtHipStats(self._h, rt.ptr(s, C.c_int64)))
-        keys = ["places", "clusters", "entries", "largest_row", "row_cap",
+        keys = ["places", "clusters", "tiles", "largest_row", "row_cap",
                 "columns", "column_cap", "builds"]
         return dict(zip(keys, (int(v) for v in s)))
 
     PROFILE_KERNELS = ["integrate_first", "nb_bin", "nb_sort", "nb_search",
-                       "nb_pairs", "nb_special", "pme_spread", "fft_rows_r2c",
+                       "nb_pairs", "nb_special", "pme_spread_rows",
                        "fft_cols_y_fwd", "fft_x_conv", "fft_cols_y_bwd",
                        "fft_rows_c2r", "pme_gather", "integrate_second"]
 
