@@ -43,6 +43,8 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--profile-repeats", type=int, default=50)
+    p.add_argument("--drift-ps", type=float, default=100.0,
+                   help="length of the NVE run the energy drift is measured on")
     return p.parse_args()
 
 
@@ -52,20 +54,20 @@ def total_energy(sim):
 
 
 def pme_bytes(n_atoms, K):
-    """Algorithmic HBM bytes per launch of the PME kernels (64^3 f32 grid):
-    spread reads N x 28 B (x f64 x3 + q f32) and writes K^3 x 8 B words;
-    each FFT pass reads and writes the half spectrum (8 B per point) or the
-    real grid; gather reads K^3 x 4 B + N x 28 B and writes N x 24 B."""
+    """Algorithmic HBM bytes per launch of the PME kernels (f32 grid): the
+    spread (fused with the z transform) reads N x 36 B (x f64 x3, q f64,
+    place index) and writes the half spectrum (8 B per point); each FFT
+    pass reads and writes the half spectrum or the real grid; gather reads
+    K^3 x 4 B + N x 36 B and writes N x 24 B (64-bit force words)."""
     k3 = K[0] * K[1] * K[2]
     half = K[0] * K[1] * (K[2] // 2 + 1) * 8
     return {
-        "pme_spread": n_atoms * 28 + k3 * 8,
-        "fft_rows_r2c": k3 * 8 + half,
+        "pme_spread_rows": n_atoms * 36 + half,
         "fft_cols_y_fwd": 2 * half,
         "fft_x_conv": 2 * half,
         "fft_cols_y_bwd": 2 * half,
         "fft_rows_c2r": half + k3 * 4,
-        "pme_gather": k3 * 4 + n_atoms * 28 + n_atoms * 24,
+        "pme_gather": k3 * 4 + n_atoms * 36 + n_atoms * 24,
     }
 
 
@@ -124,17 +126,30 @@ def main():
           flush=True)
 
     sim.run(args.warmup, dt, graph=args.graph)
-    e0, _ = total_energy(sim)
     t0 = time.perf_counter()
     sim.run(args.steps, dt, graph=args.graph)   # returns after a device sync
     t1 = time.perf_counter()
-    e1, terms = total_energy(sim)
     elapsed = t1 - t0
     ms_per_step = elapsed / args.steps * 1e3
     ns_day = args.steps * dt * 1e-3 / (elapsed / 86400.0)
-    drift = (e1 - e0) / (args.steps * dt) / system.n  # kJ/mol/ps per atom
     print(f"timed: {args.steps} steps in {elapsed:.3f} s, {ns_day:.1f} ns/day",
           file=sys.stderr, flush=True)
+
+    # conserved-energy drift of an NVE run (untimed): end minus start over
+    # the run, and the least-squares slope of one sample per picosecond
+    per_ps = int(round(1.0 / dt))
+    samples = max(1, int(round(args.drift_ps)))
+    times, energies = [0.0], [total_energy(sim)[0]]
+    for k in range(samples):
+        sim.run(per_ps, dt, graph=args.graph)
+        times.append((k + 1) * per_ps * dt)
+        energies.append(total_energy(sim)[0])
+    terms = total_energy(sim)[1]
+    times, energies = np.array(times), np.array(energies)
+    drift = (energies[-1] - energies[0]) / times[-1] / system.n  # kJ/mol/ps per atom
+    slope = float(np.polyfit(times, energies, 1)[0]) / system.n
+    print(f"drift: {samples * per_ps} steps, {drift:.3e} kJ/mol/ps/atom "
+          f"(fit {slope:.3e})", file=sys.stderr, flush=True)
 
     # per-kernel device time (HIP events on the kernels' stream)
     prof = sim.profile_step(args.profile_repeats)
@@ -189,6 +204,8 @@ def main():
         "pme_hbm_gbs": hbm,
         "pair_tflops": round(pair_tflops, 3),
         "nve_drift_kj_mol_ps_atom": drift,
+        "nve_drift_fit_kj_mol_ps_atom": slope,
+        "nve_drift_ps": float(times[-1]),
         "energy_terms_kj_mol": [round(float(t), 4) for t in terms],
     }
     if not args.no_cpu_baseline:

@@ -173,7 +173,7 @@ __global__ void __launch_bounds__(256) nb_sort(
   const int cl_start = start / kCluster, ncl = padded / kCluster;
   if (tid == 0) {
     col_cl_start[col] = cl_start;
-    col_cl_count[col] = ncl;
+    col_cl_count[col] = (count + kCluster - 1) / kCluster;  // clusters holding atoms
     if (col == 0) {
       atomicAdd(&flags[kFlagBuilds], 1);
       flags[kFlagMaskCount] = 0;  // explicit-mask slots of the coming search
@@ -212,9 +212,12 @@ __device__ __forceinline__ float axis_gap(float lo_i, float hi_i, float lo_j,
 
 // Wave per group of kGroup consecutive i-clusters (one column's).  The
 // candidates are the clusters of the columns within the reach of the group's
-// box, at z shifts -1, 0, +1: lanes test 64 column slots at a time, a wave
-// prefix sum lays out their candidates, and lanes then test 64 candidates at
-// a time, every load of an iteration independent of the others.  Bit m of an
+// box, at z shifts -1, 0, +1, whose z range meets the group's z range
+// widened by the reach (a contiguous range of each column's z-sorted
+// clusters, found by binary search).  Lanes test 64 column slots at a time,
+// a wave prefix sum lays out their candidates, and lanes then test 64
+// candidates at a time, every load of an iteration independent of the
+// others.  Bit m of an
 // entry's imask is set when j is within the reach of member m and the tile
 // (member, j, shift) is ordered, so that each unordered cluster pair occurs
 // once.  Tiles of a cluster with itself, and tiles holding excluded pairs,
@@ -239,6 +242,7 @@ __global__ void __launch_bounds__(256) nb_search(
   __shared__ int s_ex_bit[4][kExMax];             // member << 8 | mask bit
   __shared__ int s_ex_n[4];
   __shared__ int s_cstart[4][64], s_cbase[4][64], s_ccode[4][64];
+  __shared__ int s_k0[4][3][64], s_n[4][2][64];  // per z shift: first cluster, count
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int gi = blockIdx.x * 4 + wave;
   const int ngroups = counts_in[1] / kGroup;
@@ -290,12 +294,14 @@ __global__ void __launch_bounds__(256) nb_search(
   const int cx = icol % g.ncx, cy = icol / g.ncx;
   const float Lx = (float)g.box[0], Ly = (float)g.box[1], Lz = (float)g.box[2];
   const float wxf = (float)g.wx, wyf = (float)g.wy;
+  const float zreach = sqrtf(g.rlist2) * 1.0001f + 1.0e-5f;
   int count = 0;
   PairEntry *row = rows + (size_t)gi * g.row_cap;
   const int ncolx = 2 * g.rx + 1, ncols = any_real ? ncolx * (2 * g.ry + 1) : 0;
   for (int cbase = 0; cbase < ncols; cbase += 64) {
     const int c = cbase + lane;
     int ncand = 0, cl0 = 0, code = 0;
+    int k0[3] = {0, 0, 0}, nz[3] = {0, 0, 0};
     if (c < ncols) {
       const int dy = c / ncolx - g.ry, dx = c % ncolx - g.rx;
       int jy = cy + dy, sy = 0, jx = cx + dx, sx = 0;
@@ -310,8 +316,31 @@ __global__ void __launch_bounds__(256) nb_search(
       if (gx * gx + gy * gy < g.rlist2 && sx >= -1 && sx <= 1 && sy >= -1 && sy <= 1) {
         const int jcol = jy * g.ncx + jx;
         cl0 = col_cl_start[jcol];
-        ncand = col_cl_count[jcol] * 3;
+        const int nreal = col_cl_count[jcol];
+        // a tile is ordered only when cj >= ci: clusters before the group
+        // never pair with it
+        const int kmin = g0 - cl0 > 0 ? g0 - cl0 : 0;
         code = (sx + 1) | ((sy + 1) << 2);
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          // the z window in j's own frame: bb_hi.z >= zlo and bb_lo.z <= zhi
+          const float oz = (float)(s - 1) * Lz;
+          const float zlo = glo.z - zreach - oz, zhi = ghi.z + zreach - oz;
+          int lo = kmin < nreal ? kmin : nreal, hi = nreal;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (bb_hi[cl0 + mid].z < zlo) lo = mid + 1; else hi = mid;
+          }
+          const int first = lo;
+          hi = nreal;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (bb_lo[cl0 + mid].z <= zhi) lo = mid + 1; else hi = mid;
+          }
+          k0[s] = first;
+          nz[s] = lo - first;
+        }
+        ncand = nz[0] + nz[1] + nz[2];
       }
     }
     // inclusive wave scan of ncand
@@ -325,6 +354,10 @@ __global__ void __launch_bounds__(256) nb_search(
     s_cstart[wave][lane] = cl0;
     s_cbase[wave][lane] = scan - ncand;  // exclusive
     s_ccode[wave][lane] = code;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) s_k0[wave][s][lane] = k0[s];
+    s_n[wave][0][lane] = nz[0];
+    s_n[wave][1][lane] = nz[1];
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -339,9 +372,18 @@ __global__ void __launch_bounds__(256) nb_search(
           const int mid = (lo + hi + 1) >> 1;
           if (s_cbase[wave][mid] <= t) lo = mid; else hi = mid - 1;
         }
-        const int within = t - s_cbase[wave][lo];
-        cj = s_cstart[wave][lo] + within / 3;
-        const int sz = within % 3 - 1;
+        int within = t - s_cbase[wave][lo];
+        const int n0 = s_n[wave][0][lo], n1 = s_n[wave][1][lo];
+        int sz = -1;
+        if (within >= n0) {
+          within -= n0;
+          sz = 0;
+          if (within >= n1) {
+            within -= n1;
+            sz = 1;
+          }
+        }
+        cj = s_cstart[wave][lo] + s_k0[wave][sz + 1][lo] + within;
         const int ccode = s_ccode[wave][lo];
         const int sx = (ccode & 3) - 1, sy = ((ccode >> 2) & 3) - 1;
         const int lex = sz != 0 ? sz : (sy != 0 ? sy : sx);

@@ -1224,7 +1224,8 @@ int orc_nve(const orc_system *s, double *x, double *v, double dt,
  * splitmix64 seeds xoshiro256**, six uniforms in (0,1] per particle at
  * counter 6 id, Box-Muller normals, constrained, centre of mass at rest,
  * scaled to the temperature.  splitmix64 is pinned by its published first
- * output for seed 0; the draws by their statistics.
+ * output for seed 0, xoshiro256** by its published first outputs from the
+ * state {1, 2, 3, 4}, the draws by their statistics.
  * ====================================================================== */
 
 uint64_t orc_splitmix64(uint64_t *state) {
@@ -1251,6 +1252,15 @@ static uint64_t xoshiro_next(xoshiro *g) {
   g->s[2] ^= t;
   g->s[3] = rotl64(g->s[3], 45);
   return result;
+}
+
+/* One draw of xoshiro256** (Blackman and Vigna) on a caller's state. */
+uint64_t orc_xoshiro256ss(uint64_t state[4]) {
+  xoshiro g;
+  for (int k = 0; k < 4; ++k) g.s[k] = state[k];
+  const uint64_t r = xoshiro_next(&g);
+  for (int k = 0; k < 4; ++k) state[k] = g.s[k];
+  return r;
 }
 
 void orc_initial_velocities(uint64_t seed, int64_t n, const double *mass,

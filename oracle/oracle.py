@@ -106,6 +106,8 @@ def _declare(L):
     L.orc_threads.argtypes = []
     L.orc_splitmix64.restype = C.c_uint64
     L.orc_splitmix64.argtypes = [C.POINTER(C.c_uint64)]
+    L.orc_xoshiro256ss.restype = C.c_uint64
+    L.orc_xoshiro256ss.argtypes = [C.POINTER(C.c_uint64)]
     L.orc_initial_velocities.argtypes = [C.c_uint64, I64, PD, D, PD, I64, PI32,
                                          I64, PD]
 

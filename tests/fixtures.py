@@ -87,3 +87,48 @@ def lcg_positions(count, edge):
             state = (state * 1103515245 + 12345) % 2147483648
             out[i, k] = state / 2147483648.0 * edge
     return out
+
+
+def chain_box(nchains=60, edge=3.2, seed=11, beta=3.47045919371208278,
+              grid=(32, 32, 32), order=4, lj_modifier=0, cutoff=0.9,
+              switch_from=0.7):
+    """Four-atom chains (bonds 0.153 nm, bends near 112 degrees) on a
+    jittered lattice in random orientations: every pair within a chain is excluded and the
+    1-4 pair (0, 3) is scaled (s_LJ = 0.5, s_C = 1/1.2, the Amber factors).
+    Two types, charges +-0.25 e."""
+    rng = np.random.default_rng(seed)
+    d = 0.153
+    theta = math.radians(112.0)
+    local = np.array([[0.0, 0.0, 0.0], [d, 0.0, 0.0],
+                      [d - d * math.cos(theta), d * math.sin(theta), 0.0]])
+    local = np.vstack([local, local[2] + np.array([d, 0.0, 0.02])])
+    local -= local.mean(axis=0)
+    m = int(math.ceil(nchains ** (1 / 3) - 1e-9))
+    spacing = edge / m
+    slots = rng.permutation(m ** 3)[:nchains]
+    xs, q, t, exc, p14 = [], [], [], [], []
+    for c in range(nchains):
+        a, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        ix = np.array([slots[c] % m, (slots[c] // m) % m, slots[c] // (m * m)])
+        base = (ix + 0.5) * spacing + rng.uniform(-0.03, 0.03, 3)
+        xs.append(base + local @ a.T)
+        q += [0.25, -0.25, 0.25, -0.25]
+        t += [0, 1, 1, 0]
+        o = 4 * c
+        exc += [(o + i, o + j) for i in range(4) for j in range(i + 1, 4)]
+        p14.append((o, o + 3))
+    x = np.vstack(xs)
+    n = x.shape[0]
+    sigma = np.array([[0.34, 0.33], [0.33, 0.32]])
+    eps = np.array([[0.36, 0.40], [0.40, 0.45]])
+    types = np.array(t, np.int32)
+    charges = np.array(q)
+    p14 = np.array(p14, np.int32)
+    ti, tj = types[p14[:, 0]], types[p14[:, 1]]
+    return System(positions=x, masses=np.full(n, 12.011), charges=charges,
+                  types=types, sigma=sigma, epsilon=eps, box=np.full(3, edge),
+                  cutoff=cutoff, switch_from=switch_from, lj_modifier=lj_modifier,
+                  beta=beta, pme_grid=grid, pme_order=order,
+                  exclusions=np.array(exc, np.int32), pairs14=p14,
+                  pairs14_sigma=sigma[ti, tj], pairs14_eps=0.5 * eps[ti, tj],
+                  pairs14_qq=COULOMB * charges[p14[:, 0]] * charges[p14[:, 1]] / 1.2)

@@ -101,6 +101,26 @@ def test_tip3p_small_box_forces():
         assert abs(e[k] - eo[k]) <= bar, (k, e[k], eo[k])
 
 
+@pytest.mark.parametrize("modifier", [S.LJ_NONE, S.LJ_FORCE_SWITCH, S.LJ_POTENTIAL_SHIFT])
+def test_chain_box_scaled_pairs(modifier):
+    """Exclusions beyond water, the excluded-pair Ewald correction and scaled
+    1-4 pairs under each LJ modifier, with PME order 6."""
+    from fixtures import chain_box
+    say(f"case: 216 four-atom chains, 3.6 nm, 36^3 order 6, modifier {modifier}")
+    c = chain_box(nchains=216, edge=3.6, grid=(36, 36, 36), order=6, lj_modifier=modifier,
+                  cutoff=0.9, switch_from=0.7)
+    F, e = sim(c).forces()
+    Fo, eo = O.Oracle(c).total()
+    err = O.rel_rms(F, Fo)
+    say(f"  rel rms force error {err:.3e}")
+    assert err < 5e-6
+    for k in (O.E_LJ, O.E_COUL_DIRECT, O.E_COUL_EXCL, O.E_COUL_RECIP, O.E_COUL_SELF,
+              O.E_LJ14, O.E_COUL14):
+        bar = 2e-6 * abs(eo[k]) + 5e-7 * c.n
+        say(f"  term {O.TERM_NAMES[k]}: {e[k]:.8f} vs {eo[k]:.8f}")
+        assert abs(e[k] - eo[k]) <= bar, (k, e[k], eo[k])
+
+
 def test_pme_against_ewald_sum():
     say("case: PME reciprocal vs Ewald sum, 64 waters")
     w = S.tip3p_box(64, 2.2, seed=3, grid=(32, 32, 32), order=6, cutoff=0.9)

@@ -104,3 +104,12 @@ def test_nve_water_energy_conservation():
     assert np.ptp(total) < 2e-3 * abs(total).mean() + 0.5
     f = w.waters
     assert np.allclose(np.linalg.norm(x[f + 1] - x[f], axis=1), 0.09572, atol=1e-12)
+
+
+def test_xoshiro256ss_published_values():
+    """xoshiro256** from the state {1, 2, 3, 4}: 11520, 0, 1509978240,
+    1215971899390074240 (the published first outputs of the generator)."""
+    import ctypes as C
+    st = (C.c_uint64 * 4)(1, 2, 3, 4)
+    got = [O.lib().orc_xoshiro256ss(st) for _ in range(4)]
+    assert got == [11520, 0, 1509978240, 1215971899390074240]

@@ -123,3 +123,44 @@ def test_pme_virial_trace_against_volume_derivative():
         es.append(O.Oracle(t).pme()[1])
     dE_dlnL = (es[0] - es[1]) / (2 * eps)
     assert (w[0] + w[1] + w[2]) == pytest.approx(-dE_dlnL, rel=1e-6)
+
+
+def test_pairs14_special_values():
+    """A lone scaled 1-4 pair: U_LJ(sigma) = 0 and U_LJ(2^(1/6) sigma) =
+    -eps s_LJ with zero force (the minimum); Coulomb qq/r with its -qq/r^2
+    force (TermPairs14.cpp:11-17, the pair excluded from the nonbonded sum)."""
+    sig, eps = 0.31, 0.7
+    for r, e_lj in ((sig, 0.0), (2 ** (1 / 6) * sig, -0.5 * eps)):
+        x = np.array([[1.0, 1.0, 1.0], [1.0 + r, 1.0, 1.0]])
+        s = S.System(positions=x, masses=np.ones(2), charges=np.zeros(2),
+                     types=np.zeros(2, np.int32), sigma=np.array([[0.3]]),
+                     epsilon=np.array([[0.0]]), box=np.full(3, 5.0), cutoff=1.0,
+                     exclusions=np.array([[0, 1]], np.int32),
+                     pairs14=np.array([[0, 1]], np.int32), pairs14_sigma=np.array([sig]),
+                     pairs14_eps=np.array([0.5 * eps]), pairs14_qq=np.array([0.0]))
+        F, e = O.Oracle(s).total()
+        assert e[O.E_LJ14] == pytest.approx(e_lj, abs=1e-12)
+        if e_lj != 0.0:
+            assert abs(F[0, 0]) < 1e-9
+    x = np.array([[1.0, 1.0, 1.0], [1.0, 1.0 + 0.4, 1.0]])
+    s = S.System(positions=x, masses=np.ones(2), charges=np.zeros(2),
+                 types=np.zeros(2, np.int32), sigma=np.array([[0.3]]),
+                 epsilon=np.array([[0.0]]), box=np.full(3, 5.0), cutoff=1.0,
+                 exclusions=np.array([[0, 1]], np.int32),
+                 pairs14=np.array([[0, 1]], np.int32), pairs14_sigma=np.array([0.3]),
+                 pairs14_eps=np.array([0.0]), pairs14_qq=np.array([-7.0]))
+    F, e = O.Oracle(s).total()
+    assert e[O.E_COUL14] == pytest.approx(-7.0 / 0.4, rel=1e-14)
+    assert F[1, 1] == pytest.approx(-7.0 / 0.16, rel=1e-12)  # attraction pulls j back
+
+
+@pytest.mark.parametrize("modifier", [S.LJ_NONE, S.LJ_FORCE_SWITCH])
+def test_chain_forces_are_gradients(modifier):
+    """Exclusions, excluded-pair Ewald correction, 1-4 pairs and PME together:
+    forces are the negative gradient of the total energy."""
+    from fixtures import chain_box
+    c = chain_box(nchains=20, edge=2.0, grid=(20, 20, 20), lj_modifier=modifier,
+                  cutoff=0.8, switch_from=0.6)
+    o = O.Oracle(c)
+    fd_check(lambda y: o.total(y), c.positions.copy(), lambda y: o.total(y)[1].sum(),
+             atoms=[0, 3, 41], h=1e-5, rtol=5e-6)
