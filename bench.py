@@ -45,6 +45,9 @@ def parse():
     p.add_argument("--profile-repeats", type=int, default=50)
     p.add_argument("--drift-ps", type=float, default=100.0,
                    help="length of the NVE run the energy drift is measured on")
+    p.add_argument("--calibrate-copy-mib", type=int, default=0,
+                   help="after the bench, run a plain device copy of this many MiB "
+                        "5 times (the reference kernel for counter-based traffic)")
     return p.parse_args()
 
 
@@ -56,18 +59,16 @@ def total_energy(sim):
 def pme_bytes(n_atoms, K):
     """Algorithmic HBM bytes per launch of the PME kernels (f32 grid): the
     spread (fused with the z transform) reads N x 36 B (x f64 x3, q f64,
-    place index) and writes the half spectrum (8 B per point); each FFT
-    pass reads and writes the half spectrum or the real grid; gather reads
-    K^3 x 4 B + N x 36 B and writes N x 24 B (64-bit force words)."""
+    place index) and writes the half spectrum (8 B per point); the y and x
+    passes with the influence product read and write the half spectrum
+    once; the backward z pass fused with the gather reads it and N x 36 B
+    and writes N x 24 B (64-bit force words)."""
     k3 = K[0] * K[1] * K[2]
     half = K[0] * K[1] * (K[2] // 2 + 1) * 8
     return {
         "pme_spread_rows": n_atoms * 36 + half,
-        "fft_cols_y_fwd": 2 * half,
-        "fft_x_conv": 2 * half,
-        "fft_cols_y_bwd": 2 * half,
-        "fft_rows_c2r": half + k3 * 4,
-        "pme_gather": k3 * 4 + n_atoms * 36 + n_atoms * 24,
+        "pme_yx": 2 * half,
+        "pme_gather_rows": half + n_atoms * 36 + n_atoms * 24,
     }
 
 
@@ -126,9 +127,11 @@ def main():
           flush=True)
 
     sim.run(args.warmup, dt, graph=args.graph)
+    builds0 = sim.stats()["builds"]
     t0 = time.perf_counter()
     sim.run(args.steps, dt, graph=args.graph)   # returns after a device sync
     t1 = time.perf_counter()
+    builds = sim.stats()["builds"] - builds0
     elapsed = t1 - t0
     ms_per_step = elapsed / args.steps * 1e3
     ns_day = args.steps * dt * 1e-3 / (elapsed / 86400.0)
@@ -203,6 +206,7 @@ def main():
         "kernels_ms": kernels_ms,
         "pme_hbm_gbs": hbm,
         "pair_tflops": round(pair_tflops, 3),
+        "list_builds_in_timed_steps": int(builds),
         "nve_drift_kj_mol_ps_atom": drift,
         "nve_drift_fit_kj_mol_ps_atom": slope,
         "nve_drift_ps": float(times[-1]),
@@ -211,7 +215,22 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(system, sim.positions, sim.velocities, dt,
                                            args.cpu_seconds)
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
+    if args.calibrate_copy_mib:
+        calibrate_copy(args.calibrate_copy_mib)
+
+
+def calibrate_copy(mib):
+    """A plain copy kernel of known size: read mib MiB, write mib MiB."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(f"copy calibration: 5 x {mib} MiB read + {mib} MiB written", file=sys.stderr,
+          flush=True)
 
 
 if __name__ == "__main__":

@@ -247,11 +247,11 @@ int mdrtHipStats(void *system, int64_t *stats /* 8 */);
 
 /* Device-event timing of each kernel of a step on the stream it runs on,
    ms per launch averaged over `repeats` steps (the steps advance the
-   state): integrate_first, nb_bin, nb_sort, nb_search, nb_pairs,
-   nb_special, pme_spread_rows (spread + forward z transform), fft_cols_y
-   forward, fft_x_conv, fft_cols_y backward, fft_rows_c2r, pme_gather,
-   integrate_second. */
-int mdrtHipProfileStep(void *system, int32_t repeats, double *ms /* 13 */);
+   state): integrate_first, nb_bin, nb_sort, nb_search, nb_pairs (with the
+   excluded and 1-4 pairs), pme_spread_rows (spread + forward z transform),
+   pme_yx (y and x transforms with the influence product), pme_gather_rows
+   (backward z transform + gather), integrate_second. */
+int mdrtHipProfileStep(void *system, int32_t repeats, double *ms /* 9 */);
 
 #ifdef __cplusplus
 }

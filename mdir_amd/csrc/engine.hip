@@ -146,7 +146,7 @@ struct Fft3D {
     fft_cols_y<T><<<K[0] * tiles, 256, lds(K[1]), s>>>(cplx, K[1], H(), ax[1].axis, +1);
     const int nrows = K[0] * K[1];
     const int blocks = (nrows + 2 * TL - 1) / (2 * TL);
-    fft_rows_c2r<T><<<blocks, 256, lds(K[2]), s>>>(cplx, dst, nrows, ax[2].axis);
+    fft_rows_c2r<T><<<blocks, 256, lds(K[2]), s>>>(cplx, dst, nrows, ax[2].axis, 0);
   }
 };
 
@@ -220,7 +220,6 @@ struct Engine {
   // ---- integration ----
   int *d_unit_first = nullptr, *d_unit_size = nullptr;
   IntParams ip{};
-  double *d_kin = nullptr;
   double dt_graph = -1;
 
   bool forces_valid = false;
@@ -255,7 +254,7 @@ struct Engine {
     fft.release();
     dfree(d_grid); dfree(d_cplx); dfree(d_phi); dfree(d_tab); dfree(d_erecip);
     dfree(d_brick_idx);
-    dfree(d_unit_first); dfree(d_unit_size); dfree(d_kin);
+    dfree(d_unit_first); dfree(d_unit_size);
     if (stream) hipStreamDestroy(stream);
     stream = nullptr;
   }
@@ -417,11 +416,13 @@ struct Engine {
     d_excl_start = dupload(es);
     d_excl_atoms = dupload(ea);
 
-    // Lennard-Jones table (c6 = 4 eps sigma^6, c12 = 4 eps sigma^12)
+    // Lennard-Jones table (6 c6, 12 c12) with c6 = 4 eps sigma^6, c12 = 4 eps
+    // sigma^12: the factors of the force, so that it needs no scaling
     std::vector<float2> lj(ntypes * ntypes);
     for (int k = 0; k < ntypes * ntypes; ++k) {
       const double s6 = pow(h_sigma[k], 6.0);
-      lj[k] = make_float2((float)(4.0 * h_eps[k] * s6), (float)(4.0 * h_eps[k] * s6 * s6));
+      lj[k] = make_float2((float)(6.0 * 4.0 * h_eps[k] * s6),
+                          (float)(12.0 * 4.0 * h_eps[k] * s6 * s6));
     }
     d_lj = dupload(lj);
 
@@ -434,12 +435,14 @@ struct Engine {
     nbp.rc2 = (float)(cutoff * cutoff);
     nbp.rs = (float)switch_from;
     nbp.beta = (float)beta;
+    nbp.mb2l2e = (float)(-beta * beta / M_LN2);
+    nbp.tbs = (float)(2.0 * beta / sqrt(M_PI));
     nbp.coulomb = (float)kCoulomb;
     nbp.shift_c = (coulomb_shift && beta > 0) ? (float)(erfc(beta * cutoff) / cutoff) : 0.0f;
     nbp.ntypes = ntypes;
     nbp.lj_modifier = lj_modifier;
-    nbp.shift12 = (float)pow(cutoff, -12.0);
-    nbp.shift6 = (float)pow(cutoff, -6.0);
+    nbp.shift12 = (float)(pow(cutoff, -12.0) / 12.0);
+    nbp.shift6 = (float)(pow(cutoff, -6.0) / 6.0);
     if (lj_modifier == kLJForceSwitch) {
       const double rc = cutoff, rs = switch_from, w = rc - rs;
       if (!(w > 0)) throw HipError("force switch needs switch_from < cutoff");
@@ -451,8 +454,10 @@ struct Engine {
         Bc[k] = nn * ((nn + 3.0) * rc - (nn + 1.0) * rs) / (pow(rc, nn + 2.0) * w * w * w);
         Cc[k] = pow(rc, -nn) - A[k] / 3.0 * pow(w, 3.0) - Bc[k] / 4.0 * pow(w, 4.0);
       }
-      nbp.sw_a12 = (float)A[0]; nbp.sw_b12 = (float)Bc[0]; nbp.sw_c12 = (float)Cc[0];
-      nbp.sw_a6 = (float)A[1]; nbp.sw_b6 = (float)Bc[1]; nbp.sw_c6 = (float)Cc[1];
+      nbp.sw_a12 = (float)(A[0] / 12.0); nbp.sw_b12 = (float)(Bc[0] / 12.0);
+      nbp.sw_c12 = (float)(Cc[0] / 12.0);
+      nbp.sw_a6 = (float)(A[1] / 6.0); nbp.sw_b6 = (float)(Bc[1] / 6.0);
+      nbp.sw_c6 = (float)(Cc[1] / 6.0);
     }
     if (lj_modifier == kLJPotentialSwitch && !(switch_from < cutoff))
       throw HipError("potential switch needs switch_from < cutoff");
@@ -551,7 +556,7 @@ struct Engine {
       d_cplx = dalloc<cpx<float>>(nh);
       d_phi = dalloc<float>(np);
       make_influence_tables();
-      d_erecip = dalloc<double>(fft.x_blocks());
+      d_erecip = dalloc<double>(std::max(fft.x_blocks(), fft.H()));
     }
 
     // integration units: waters, then the other particles
@@ -573,7 +578,6 @@ struct Engine {
       nunits = (int)uf.size();
       d_unit_first = dupload(uf);
       d_unit_size = dupload(us);
-      d_kin = dalloc<double>(blocks_for(nunits));
       for (int k = 0; k < 3; ++k) ip.half_box[k] = 0.5 * box[k];
       ip.skin_half2 = 0.25 * skin * skin;
       if (d->nwaters > 0) {
@@ -658,12 +662,26 @@ struct Engine {
     launch_search();
   }
 
+  // One launch: plane y = 0 of the grid takes the excluded and 1-4 pairs,
+  // the planes above the segments of the group rows.
   template <int MOD, bool COUL, bool ENERGY>
   void launch_pairs_t() {
-    const dim3 grid(blocks_for(max_groups, 4), nseg);
-    nb_pairs<MOD, COUL, ENERGY><<<grid, 256, 0, stream>>>(
+    SpecialArgs sa{};
+    sa.npairs = nspecial;
+    sa.pairs = d_special;
+    sa.is14 = d_is14;
+    sa.p14_sigma = d_p14_sigma;
+    sa.p14_eps = d_p14_eps;
+    sa.p14_qq = d_p14_qq;
+    sa.x = d_x;
+    sa.q = d_q;
+    sa.place_of = d_place_of;
+    sa.energy_partial = ENERGY ? d_espec : nullptr;
+    sa.p = spp;
+    const dim3 grid(std::max(blocks_for(max_groups, 4), blocks_for(nspecial)), nseg + 1);
+    nb_pairs<MOD, COUL, ENERGY><<<grid, 256, ntypes * ntypes * sizeof(float2), stream>>>(
         d_xq, d_type_at, d_real_mask, d_rows, d_row_count, d_ex, g.row_cap, max_groups,
-        d_counts, d_lj, nbp, d_F, d_epair, d_flags);
+        d_counts, d_lj, nbp, d_F, d_epair, d_flags, sa);
   }
 
   template <int MOD>
@@ -684,12 +702,6 @@ struct Engine {
     }
   }
 
-  void special(bool energy) {
-    if (nspecial == 0) return;
-    nb_special<<<blocks_for(nspecial), 256, 0, stream>>>(
-        nspecial, d_special, d_is14, d_p14_sigma, d_p14_eps, d_p14_qq, d_x, d_q,
-        d_place_of, spp, d_F, energy ? d_espec : nullptr);
-  }
 
   // ---- bricks of the spread ----
   BrickParams bp{};
@@ -764,6 +776,7 @@ struct Engine {
     bp.xcols = d_brick_idx + o_xc;
     bp.ystart = d_brick_idx + o_ys;
     bp.ycols = d_brick_idx + o_yc;
+    bp.plane_major = plane_fft() ? 1 : 0;
   }
 
   template <int N, int TL>
@@ -785,6 +798,31 @@ struct Engine {
       default: spread_rows_t<N, 1>(); break;
     }
   }
+  template <int N, int TL>
+  void gather_rows_t() {
+    constexpr int BX = BrickShape<TL>::BX, BY = BrickShape<TL>::BY;
+    const int blocks = ((pme_grid[0] + BX - 1) / BX) * ((pme_grid[1] + BY - 1) / BY);
+    const size_t lds = (size_t)16 * TL * pme_grid[2];
+    pme_gather_rows<N, TL><<<blocks, 256, lds, stream>>>(d_col_cl_start, d_col_cl_count,
+                                                         d_atom_at, d_x, d_q, pmp, bp, d_cplx,
+                                                         fft.ax[2].axis, d_F);
+  }
+  template <int N>
+  void gather_rows_n() {
+    switch (brick_tl) {
+      case 8: gather_rows_t<N, 8>(); break;
+      case 4: gather_rows_t<N, 4>(); break;
+      case 2: gather_rows_t<N, 2>(); break;
+      default: gather_rows_t<N, 1>(); break;
+    }
+  }
+  // Backward z transform and gather, by the spread's bricks.
+  void pme_gather_rows_launch() {
+    if (pme_order == 4) gather_rows_n<4>();
+    else if (pme_order == 6) gather_rows_n<6>();
+    else gather_rows_n<8>();
+  }
+
   // Spread and forward z transform: the half spectra rows of d_cplx.
   void pme_spread_launch() {
     if (pme_order == 4) spread_rows_n<4>();
@@ -801,21 +839,44 @@ struct Engine {
       pme_gather_lanes<8, 8><<<blocks_for((size_t)max_places * 8), 256, 0, stream>>>(d_counts, d_atom_at, d_x, d_q, pmp, d_phi, d_F);
   }
 
-  void pme(bool energy) {
-    if (!use_pme) return;
-    pme_spread_launch();
+  // The y and x passes with the influence product: one kernel per kz plane
+  // when a plane fits in LDS, else the y pass, the fused x pass and the y
+  // pass again over tiles of lines.
+  bool plane_fft() const { return use_pme && pme_grid[0] * pme_grid[1] <= kPlaneMax; }
+  static constexpr int kPlaneMax = 4096;
+  int recip_blocks() const { return plane_fft() ? fft.H() : fft.x_blocks(); }
+  void pme_yx(bool energy) {
+    double *ep = energy ? d_erecip : nullptr;
+    if (plane_fft()) {
+      const size_t lds = (size_t)2 * pme_grid[0] * pme_grid[1] * sizeof(cpx<float>);
+      fft_plane_conv<<<fft.H(), 512, lds, stream>>>(d_cplx, fft.K[0], fft.K[1], fft.ax[0].axis,
+                                                     fft.ax[1].axis, ct, ep);
+      return;
+    }
     const int TL = Fft3D<float>::TL;
     const int tiles = (fft.H() + TL - 1) / TL;
     fft_cols_y<float><<<fft.K[0] * tiles, 256, fft.lds(fft.K[1]), stream>>>(
         d_cplx, fft.K[1], fft.H(), fft.ax[1].axis, -1);
-    fft.x_pass(d_cplx, 0, ct, energy ? d_erecip : nullptr, stream);
-    fft.backward(d_cplx, d_phi, stream);
-    pme_gather_launch();
+    fft.x_pass(d_cplx, 0, ct, ep, stream);
+    fft_cols_y<float><<<fft.K[0] * tiles, 256, fft.lds(fft.K[1]), stream>>>(
+        d_cplx, fft.K[1], fft.H(), fft.ax[1].axis, +1);
+  }
+  void pme_c2r() {
+    const int TL = Fft3D<float>::TL;
+    const int nrows = fft.K[0] * fft.K[1];
+    fft_rows_c2r<float><<<(nrows + 2 * TL - 1) / (2 * TL), 256, fft.lds(fft.K[2]), stream>>>(
+        d_cplx, d_phi, nrows, fft.ax[2].axis, plane_fft() ? 1 : 0);
+  }
+
+  void pme(bool energy) {
+    if (!use_pme) return;
+    pme_spread_launch();
+    pme_yx(energy);
+    pme_gather_rows_launch();
   }
 
   void forces(bool energy) {
     pairs(energy);
-    special(energy);
     pme(energy);
   }
 
@@ -823,25 +884,28 @@ struct Engine {
     clear_words<<<256, 256, 0, stream>>>(d_F, 3 * (size_t)max_places);
   }
 
-  void step_first(double dt) {
+  template <int PART>
+  void integrate(double dt) {
     ip.dt = dt;
-    integrate_first<<<blocks_for(nunits), 256, 0, stream>>>(
+    integrate_part<PART><<<blocks_for(nunits, 64), 64, 0, stream>>>(
         nunits, d_unit_first, d_unit_size, d_x, d_v, d_inv_mass, d_F, d_place_of, d_shift,
         d_xref, d_q, d_xq, ip, d_flags);
   }
+  void step_first(double dt) { integrate<kFirstHalf>(dt); }
+  void step_second(double dt) { integrate<kSecondHalf>(dt); }
+  void step_between(double dt) { integrate<kBetween>(dt); }
 
-  void step_second(double dt, bool kinetic) {
-    ip.dt = dt;
-    integrate_second<<<blocks_for(nunits), 256, 0, stream>>>(
-        nunits, d_unit_first, d_unit_size, d_x, d_v, d_inv_mass, d_F, d_place_of, ip,
-        kinetic ? d_kin : nullptr);
-  }
-
-  void one_step(double dt) {
+  // `count` whole steps; between two steps the second half of one and the
+  // first half of the next run as one kernel.
+  void steps_seq(int64_t count, double dt) {
+    if (count <= 0) return;
     step_first(dt);
-    build();
-    forces(false);
-    step_second(dt, false);
+    for (int64_t s = 0; s < count; ++s) {
+      build();
+      forces(false);
+      if (s + 1 < count) step_between(dt);
+      else step_second(dt);
+    }
   }
 
   std::vector<int> read_flags() {
@@ -920,7 +984,7 @@ struct Engine {
         }
       }
       if (use_pme) {
-        std::vector<double> er(fft.x_blocks());
+        std::vector<double> er(recip_blocks());
         to_host(er.data(), d_erecip, er.size() * sizeof(double));
         for (double e : er) E_out[MDRT_E_COUL_RECIP] += e;
       }
@@ -946,7 +1010,7 @@ struct Engine {
     }
     hipGraph_t graph;
     HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    for (int s = 0; s < steps; ++s) one_step(dt);
+    steps_seq(steps, dt);
     HIPCHK(hipStreamEndCapture(stream, &graph));
     HIPCHK(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
     HIPCHK(hipGraphDestroy(graph));
@@ -971,7 +1035,7 @@ struct Engine {
           if (!graph_exec || graph_steps != graph || dt_graph != dt) capture(graph, dt);
           HIPCHK(hipGraphLaunch(graph_exec, stream));
         } else {
-          for (int64_t k = 0; k < todo; ++k) one_step(dt);
+          steps_seq(todo, dt);
         }
         const std::vector<int> f = read_flags();
         if (!grow_if_overflowed(f)) break;
@@ -1113,19 +1177,17 @@ struct Engine {
 
   // Device-event timing of every kernel of one step, on the stream the
   // kernels run on: ms per launch, averaged over `repeats` steps.
-  //  0 integrate_first  1 nb_bin  2 nb_sort  3 nb_search  4 nb_pairs
-  //  5 nb_special  6 pme_spread_rows (spread + forward z transform)
-  //  7 fft_cols_y (forward)  8 fft_x_conv  9 fft_cols_y (backward)
-  //  10 fft_rows_c2r  11 pme_gather  12 integrate_second
-  static constexpr int kProfileSlots = 13;
+  //  0 integrate_first  1 nb_bin  2 nb_sort  3 nb_search
+  //  4 nb_pairs (with the excluded and 1-4 pairs)
+  //  5 pme_spread_rows (spread + forward z transform)
+  //  6 pme_yx (y and x transforms and the influence product)
+  //  7 pme_gather_rows (backward z transform + gather)  8 integrate_second
+  static constexpr int kProfileSlots = 9;
   void profile(int repeats, double *ms) {
     if (!forces_valid) evaluate(false);
     hipEvent_t ev[kProfileSlots + 1];
     for (auto &e : ev) HIPCHK(hipEventCreate(&e));
     double acc[kProfileSlots] = {0};
-    const int nrows = fft.K[0] * fft.K[1];
-    const int TL = Fft3D<float>::TL;
-    const int tiles = (fft.H() + TL - 1) / TL;
     for (int r = 0; r < repeats; ++r) {
       int k = 0;
       HIPCHK(hipEventRecord(ev[k++], stream));
@@ -1139,27 +1201,13 @@ struct Engine {
       HIPCHK(hipEventRecord(ev[k++], stream));
       pairs(false);
       HIPCHK(hipEventRecord(ev[k++], stream));
-      special(false);
-      HIPCHK(hipEventRecord(ev[k++], stream));
       if (use_pme) pme_spread_launch();
       HIPCHK(hipEventRecord(ev[k++], stream));
-      if (use_pme)
-        fft_cols_y<float><<<fft.K[0] * tiles, 256, fft.lds(fft.K[1]), stream>>>(
-            d_cplx, fft.K[1], fft.H(), fft.ax[1].axis, -1);
+      if (use_pme) pme_yx(false);
       HIPCHK(hipEventRecord(ev[k++], stream));
-      if (use_pme) fft.x_pass(d_cplx, 0, ct, nullptr, stream);
+      if (use_pme) pme_gather_rows_launch();
       HIPCHK(hipEventRecord(ev[k++], stream));
-      if (use_pme)
-        fft_cols_y<float><<<fft.K[0] * tiles, 256, fft.lds(fft.K[1]), stream>>>(
-            d_cplx, fft.K[1], fft.H(), fft.ax[1].axis, +1);
-      HIPCHK(hipEventRecord(ev[k++], stream));
-      if (use_pme)
-        fft_rows_c2r<float><<<(nrows + 2 * TL - 1) / (2 * TL), 256, fft.lds(fft.K[2]), stream>>>(
-            d_cplx, d_phi, nrows, fft.ax[2].axis);
-      HIPCHK(hipEventRecord(ev[k++], stream));
-      if (use_pme) pme_gather_launch();
-      HIPCHK(hipEventRecord(ev[k++], stream));
-      step_second(0.002, false);
+      step_second(0.002);
       HIPCHK(hipEventRecord(ev[k++], stream));
       HIPCHK(hipEventSynchronize(ev[kProfileSlots]));
       for (int j = 0; j < kProfileSlots; ++j) {

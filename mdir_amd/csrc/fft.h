@@ -157,4 +157,68 @@ __device__ cpx<T> *lds_fft(cpx<T> *a, cpx<T> *b, const FftAxis &ax, int sign) {
   return in;
 }
 
+// Stockham pass over `nlines` lines whose element e sits at e * es + line * ls
+// (a plane of a 2D transform held whole in LDS).  Threads run along the
+// unit-stride index (elements when es == 1, lines otherwise), so that a
+// wave's LDS accesses are consecutive.
+template <typename T, int R>
+__device__ __forceinline__ void stockham_pass_strided(const cpx<T> *in, cpx<T> *out, int K,
+                                                      int Ns, const cpx<T> *tw, int sign,
+                                                      int nlines, int es, int ls) {
+  const int Kr = K / R;
+  const int tstep = K / (Ns * R);
+  for (int w = threadIdx.x; w < nlines * Kr; w += blockDim.x) {
+    int line, j;
+    if (es == 1) {
+      j = w % Kr;
+      line = w / Kr;
+    } else {
+      line = w % nlines;
+      j = w / nlines;
+    }
+    cpx<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = in[(j + r * Kr) * es + line * ls];
+    const int k = j % Ns;
+    if (Ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        cpx<T> t = tw[k * r * tstep];
+        if (sign > 0) t.y = -t.y;
+        v[r] = cmul(v[r], t);
+      }
+    }
+    butterfly<T, R>(v, sign);
+    const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) out[(base + r * Ns) * es + line * ls] = v[r];
+  }
+}
+
+template <typename T>
+__device__ cpx<T> *lds_fft_strided(cpx<T> *a, cpx<T> *b, const FftAxis &ax, int sign,
+                                   int nlines, int es, int ls) {
+  const int K = ax.K;
+  const cpx<T> *tw = static_cast<const cpx<T> *>(ax.tw);
+  int Ns = 1;
+  cpx<T> *in = a, *out = b;
+  for (int s = 0; s < ax.nrad; ++s) {
+    const int R = ax.rad[s];
+    switch (R) {
+      case 8: stockham_pass_strided<T, 8>(in, out, K, Ns, tw, sign, nlines, es, ls); break;
+      case 4: stockham_pass_strided<T, 4>(in, out, K, Ns, tw, sign, nlines, es, ls); break;
+      case 2: stockham_pass_strided<T, 2>(in, out, K, Ns, tw, sign, nlines, es, ls); break;
+      case 3: stockham_pass_strided<T, 3>(in, out, K, Ns, tw, sign, nlines, es, ls); break;
+      case 5: stockham_pass_strided<T, 5>(in, out, K, Ns, tw, sign, nlines, es, ls); break;
+      default: stockham_pass_strided<T, 7>(in, out, K, Ns, tw, sign, nlines, es, ls); break;
+    }
+    __syncthreads();
+    cpx<T> *t = in;
+    in = out;
+    out = t;
+    Ns *= R;
+  }
+  return in;
+}
+
 }  // namespace mdhip

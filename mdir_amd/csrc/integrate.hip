@@ -9,6 +9,7 @@
 //   (ConstraintIR.cpp:392-471).
 // A unit is a SETTLE water (3 consecutive atoms) or a free atom; one thread
 // owns a unit, reads its force words and clears them for the next step.
+// Blocks of 64 threads: a few thousand units still spread over every CU.
 #include "md_common.h"
 
 namespace mdhip {
@@ -136,10 +137,142 @@ __device__ void settle_velocities(const double *xa, const double *xb, const doub
   }
 }
 
-// Unit u: first atom unit_first[u], size 3 (water) or 1.
-// Half kick with the force words (cleared), drift, SETTLE, v += dx_c/dt,
-// f32 positions of the pair loop, validity flag.
-__global__ void __launch_bounds__(256) integrate_first(
+// The state of one unit of S atoms (a SETTLE water, S = 3, or a free
+// atom, S = 1) in registers: S is a compile-time constant so that the
+// arrays stay in registers.
+template <int S>
+struct UnitState {
+  int first;
+  int pl[S];
+  double im[S], x[S][3], v[S][3], f[S][3];
+};
+
+// Loads the unit at `first`; with clear, the force words are zeroed for the
+// next evaluation once read.
+template <int S>
+__device__ __forceinline__ void load_unit(int first, const double *__restrict__ x,
+                                          const double *__restrict__ v,
+                                          const double *__restrict__ inv_mass,
+                                          long long *__restrict__ F,
+                                          const int *__restrict__ place_of, bool clear,
+                                          UnitState<S> &s) {
+  s.first = first;
+#pragma unroll
+  for (int m = 0; m < S; ++m) {
+    const int at = first + m;
+    s.pl[m] = place_of[at];
+    s.im[m] = inv_mass[at];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      s.f[m][k] = (double)F[3 * s.pl[m] + k] * kForceUnscale;
+      if (clear) F[3 * s.pl[m] + k] = 0;
+      s.v[m][k] = v[3 * at + k];
+      s.x[m][k] = x[3 * at + k];
+    }
+  }
+}
+
+template <int S>
+__device__ __forceinline__ void half_kick(UnitState<S> &s, double dt) {
+#pragma unroll
+  for (int m = 0; m < S; ++m)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s.v[m][k] = s.v[m][k] + 0.5 * dt * s.f[m][k] * s.im[m];
+}
+
+template <int S>
+__device__ __forceinline__ void constrain_velocities(UnitState<S> &s, const IntParams &p) {
+  if constexpr (S == 3) settle_velocities(s.x[0], s.x[1], s.x[2], s.v[0], s.v[1], s.v[2], p);
+}
+
+// Drift, SETTLE, v += dx_c/dt; stores x, v and the f32 positions of the
+// pair loop; returns whether an atom moved beyond half the skin.
+template <int S>
+__device__ __forceinline__ bool drift_store(UnitState<S> &s, double *__restrict__ x,
+                                           double *__restrict__ v,
+                                           const double *__restrict__ shift,
+                                           const double *__restrict__ xref,
+                                           const double *__restrict__ q,
+                                           float4 *__restrict__ xq, const IntParams &p) {
+  double xn[S][3];
+#pragma unroll
+  for (int m = 0; m < S; ++m)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) xn[m][k] = s.x[m][k] + p.dt * s.v[m][k];
+  if constexpr (S == 3) {
+    double xu[3][3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) xu[m][k] = xn[m][k];
+    settle_positions(s.x[0], s.x[1], s.x[2], xn[0], xn[1], xn[2], p);
+    const double idt = 1.0 / p.dt;
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) s.v[m][k] += (xn[m][k] - xu[m][k]) * idt;
+  }
+  bool moved = false;
+#pragma unroll
+  for (int m = 0; m < S; ++m) {
+    const int at = s.first + m;
+    double d2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      x[3 * at + k] = xn[m][k];
+      v[3 * at + k] = s.v[m][k];
+      const double d = xn[m][k] - xref[3 * at + k];
+      d2 += d * d;
+    }
+    moved |= d2 > p.skin_half2;
+    xq[s.pl[m]] = make_float4((float)(xn[m][0] - shift[3 * at + 0] - p.half_box[0]),
+                              (float)(xn[m][1] - shift[3 * at + 1] - p.half_box[1]),
+                              (float)(xn[m][2] - shift[3 * at + 2] - p.half_box[2]),
+                              (float)q[at]);
+  }
+  return moved;
+}
+
+// Which halves of velocity Verlet a launch runs on each unit.
+enum StepPart { kFirstHalf = 1, kSecondHalf = 2, kBetween = 3 };
+
+template <int PART, int S>
+__device__ __forceinline__ void step_unit(int first, double *__restrict__ x,
+                                          double *__restrict__ v,
+                                          const double *__restrict__ inv_mass,
+                                          long long *__restrict__ F,
+                                          const int *__restrict__ place_of,
+                                          const double *__restrict__ shift,
+                                          const double *__restrict__ xref,
+                                          const double *__restrict__ q,
+                                          float4 *__restrict__ xq, const IntParams &p,
+                                          int *__restrict__ flags) {
+  UnitState<S> s;
+  // the force words are cleared by the launch that reads them last
+  load_unit<S>(first, x, v, inv_mass, F, place_of, PART != kSecondHalf, s);
+  if (PART & kSecondHalf) {  // end of a step: half kick, velocity constraint
+    half_kick<S>(s, p.dt);
+    constrain_velocities<S>(s, p);
+  }
+  if (PART & kFirstHalf) {   // start of a step: half kick, drift, SETTLE
+    half_kick<S>(s, p.dt);
+    if (drift_store<S>(s, x, v, shift, xref, q, xq, p)) atomicOr(&flags[kFlagRebuild], 1);
+  } else {
+#pragma unroll
+    for (int m = 0; m < S; ++m)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) v[3 * (first + m) + k] = s.v[m][k];
+  }
+}
+
+// PART kFirstHalf: half kick with the force words (cleared), drift, SETTLE,
+// v += dx_c/dt, f32 positions of the pair loop, validity flag.
+// kSecondHalf: half kick with the new forces (not cleared: the next step's
+// first half reads them) and the velocity constraint.  kBetween: both, the
+// second half of one step and the first half of the next, in one pass over
+// the state with the same operations in the same order.
+template <int PART>
+__global__ void __launch_bounds__(64) integrate_part(
     int nunits, const int *__restrict__ unit_first, const int *__restrict__ unit_size,
     double *__restrict__ x, double *__restrict__ v, const double *__restrict__ inv_mass,
     long long *__restrict__ F, const int *__restrict__ place_of,
@@ -148,89 +281,11 @@ __global__ void __launch_bounds__(256) integrate_first(
     int *__restrict__ flags) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= nunits) return;
-  const int first = unit_first[u], size = unit_size[u];
-  double xo[3][3], xn[3][3], vv[3][3];
-  for (int m = 0; m < size; ++m) {
-    const int at = first + m;
-    const int pl = place_of[at];
-    const double im = inv_mass[at];
-    for (int k = 0; k < 3; ++k) {
-      const double f = (double)F[3 * pl + k] * kForceUnscale;
-      F[3 * pl + k] = 0;
-      const double vk = v[3 * at + k] + 0.5 * p.dt * f * im;
-      vv[m][k] = vk;
-      xo[m][k] = x[3 * at + k];
-      xn[m][k] = xo[m][k] + p.dt * vk;
-    }
-  }
-  if (size == 3) {
-    double xu[3][3];
-    for (int m = 0; m < 3; ++m)
-      for (int k = 0; k < 3; ++k) xu[m][k] = xn[m][k];
-    settle_positions(xo[0], xo[1], xo[2], xn[0], xn[1], xn[2], p);
-    const double idt = 1.0 / p.dt;
-    for (int m = 0; m < 3; ++m)
-      for (int k = 0; k < 3; ++k) vv[m][k] += (xn[m][k] - xu[m][k]) * idt;
-  }
-  bool moved = false;
-  for (int m = 0; m < size; ++m) {
-    const int at = first + m;
-    double d2 = 0.0;
-    for (int k = 0; k < 3; ++k) {
-      x[3 * at + k] = xn[m][k];
-      v[3 * at + k] = vv[m][k];
-      const double d = xn[m][k] - xref[3 * at + k];
-      d2 += d * d;
-    }
-    moved |= d2 > p.skin_half2;
-    const int pl = place_of[at];
-    xq[pl] = make_float4((float)(xn[m][0] - shift[3 * at + 0] - p.half_box[0]),
-                         (float)(xn[m][1] - shift[3 * at + 1] - p.half_box[1]),
-                         (float)(xn[m][2] - shift[3 * at + 2] - p.half_box[2]),
-                         (float)q[at]);
-  }
-  if (moved) atomicOr(&flags[kFlagRebuild], 1);
-}
-
-// Half kick with the new forces (not cleared: the next step's first half
-// kick reads them) and the velocity constraint of the waters.  With
-// kinetic != null, each block adds its kinetic energy (f64) to a partial.
-__global__ void __launch_bounds__(256) integrate_second(
-    int nunits, const int *__restrict__ unit_first, const int *__restrict__ unit_size,
-    const double *__restrict__ x, double *__restrict__ v,
-    const double *__restrict__ inv_mass, const long long *__restrict__ F,
-    const int *__restrict__ place_of, IntParams p, double *__restrict__ kinetic) {
-  __shared__ double s_k[256];
-  const int u = blockIdx.x * blockDim.x + threadIdx.x;
-  double ke = 0.0;
-  if (u < nunits) {
-    const int first = unit_first[u], size = unit_size[u];
-    double vv[3][3];
-    for (int m = 0; m < size; ++m) {
-      const int at = first + m;
-      const int pl = place_of[at];
-      for (int k = 0; k < 3; ++k)
-        vv[m][k] = v[3 * at + k] +
-                   0.5 * p.dt * ((double)F[3 * pl + k] * kForceUnscale) * inv_mass[at];
-    }
-    if (size == 3)
-      settle_velocities(x + 3 * first, x + 3 * first + 3, x + 3 * first + 6, vv[0],
-                        vv[1], vv[2], p);
-    for (int m = 0; m < size; ++m) {
-      const int at = first + m;
-      for (int k = 0; k < 3; ++k) v[3 * at + k] = vv[m][k];
-      ke += 0.5 / inv_mass[at] * (vv[m][0] * vv[m][0] + vv[m][1] * vv[m][1] + vv[m][2] * vv[m][2]);
-    }
-  }
-  if (kinetic) {
-    s_k[threadIdx.x] = ke;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (threadIdx.x < s) s_k[threadIdx.x] += s_k[threadIdx.x + s];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) kinetic[blockIdx.x] = s_k[0];
-  }
+  const int first = unit_first[u];
+  if (unit_size[u] == 3)
+    step_unit<PART, 3>(first, x, v, inv_mass, F, place_of, shift, xref, q, xq, p, flags);
+  else
+    step_unit<PART, 1>(first, x, v, inv_mass, F, place_of, shift, xref, q, xq, p, flags);
 }
 
 // Force words (place order) -> f64 forces (atom order).

@@ -62,6 +62,9 @@ struct NbParams {
   float beta;          // Ewald splitting, 0 = no Coulomb
   float coulomb;       // f
   float shift_c;       // erfc(beta rc)/rc under POTENTIAL_SHIFT, else 0
+  float mb2l2e;        // -beta^2 log2(e): exp(-beta^2 r^2) = exp2(mb2l2e r^2)
+  float tbs;           // 2 beta / sqrt(pi)
+  // modifier constants, with the 1/12 and 1/6 of the (6 c6, 12 c12) table
   float sw_a12, sw_b12, sw_c12, sw_a6, sw_b6, sw_c6;   // force switch
   float shift12, shift6;                               // potential shift
   int ntypes;

@@ -47,9 +47,10 @@ __device__ __forceinline__ void add_fixed(long long *F, int place, float fx,
 // fitted (weighted least squares on the relative error, this project's own
 // fit) on [0, 3.6]; 2e-9 relative in exact arithmetic, ~1e-6 in f32, where
 // the rounding of exp(-x^2) itself dominates as it does for any f32 erfc.
-// It reuses the exp(-x^2) that the force needs anyway.
+// It reuses the exp(-x^2) that the force needs anyway.  t comes from the
+// hardware reciprocal (1 ulp).
 __device__ __forceinline__ float erfc_from_exp(float x, float expmx2) {
-  const float t = __frcp_rn(1.0f + 0.475f * x);
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.475f, x, 1.0f));
   float pol = 0.09289151430130005f;
   pol = fmaf(pol, t, -0.4289827346801758f);
   pol = fmaf(pol, t, 0.6025476455688477f);
@@ -60,41 +61,45 @@ __device__ __forceinline__ float erfc_from_exp(float x, float expmx2) {
   return t * expmx2 * pol;
 }
 
-// Scalar force -dU/dr / r and energy of one pair in f32.
+// Scalar force -dU/dr / r and energy of one pair in f32.  lj holds
+// (6 c6, 12 c12), the force's own scaling; the energy divides it back out
+// and the modifier constants carry the 1/6 and 1/12.  r2 is never zero or
+// denormal (padding is masked, overlapping atoms are not a state), so the
+// hardware rsq and exp2 are used without range fix-ups.
 template <int MOD, bool COUL, bool ENERGY>
 __device__ __forceinline__ float pair_force(float r2, float qq, float2 lj,
                                             const NbParams &p, float &e_lj,
                                             float &e_c) {
-  const float rinv = rsqrtf(r2);
+  const float rinv = __builtin_amdgcn_rsqf(r2);
   const float rinv2 = rinv * rinv;
   float fscal = 0.0f;
   if (COUL) {
-    const float r = r2 * rinv;
-    const float br = p.beta * r;
-    const float expt = __expf(-br * br);
+    const float br = p.beta * (r2 * rinv);
+    const float expt = __builtin_amdgcn_exp2f(p.mb2l2e * r2);  // exp(-beta^2 r^2)
     const float erfcv = erfc_from_exp(br, expt);
-    const float tbs = 1.1283791670955126f * p.beta;  // 2 beta / sqrt(pi)
-    fscal = qq * (erfcv * rinv + tbs * expt) * rinv2;
+    fscal = qq * fmaf(erfcv, rinv, p.tbs * expt) * rinv2;
     if (ENERGY) e_c += qq * (erfcv * rinv - p.shift_c);
   }
-  const float c6 = lj.x, c12 = lj.y;
+  const float c6x = lj.x, c12x = lj.y;
   {  // zero parameters give zero: no branch
     const float rinv6 = rinv2 * rinv2 * rinv2;
-    float flj = (12.0f * c12 * rinv6 - 6.0f * c6) * rinv6 * rinv2;
-    float ulj = (c12 * rinv6 - c6) * rinv6;
+    float flj = fmaf(c12x, rinv6, -c6x) * rinv6 * rinv2;
+    float ulj = 0.0f;
+    if (ENERGY || MOD == kLJPotentialSwitch)
+      ulj = fmaf(c12x * (1.0f / 12.0f), rinv6, -c6x * (1.0f / 6.0f)) * rinv6;
     if (MOD == kLJPotentialShift) {
-      ulj -= c12 * p.shift12 - c6 * p.shift6;
+      if (ENERGY) ulj -= c12x * p.shift12 - c6x * p.shift6;
     } else if (MOD == kLJForceSwitch) {
       const float r = r2 * rinv;
       const float u = fmaxf(r - p.rs, 0.0f);
       const float u2 = u * u, u3 = u2 * u;
       const float dp12 = p.sw_a12 * u2 + p.sw_b12 * u3;
       const float dp6 = p.sw_a6 * u2 + p.sw_b6 * u3;
-      flj += (c12 * dp12 - c6 * dp6) * rinv;
+      flj += (c12x * dp12 - c6x * dp6) * rinv;
       if (ENERGY) {
         const float p12 = p.sw_c12 + u3 * (p.sw_a12 * (1.0f / 3.0f) + p.sw_b12 * 0.25f * u);
         const float p6 = p.sw_c6 + u3 * (p.sw_a6 * (1.0f / 3.0f) + p.sw_b6 * 0.25f * u);
-        ulj -= c12 * p12 - c6 * p6;
+        ulj -= c12x * p12 - c6x * p6;
       }
     } else if (MOD == kLJPotentialSwitch) {
       const float r = r2 * rinv;
@@ -114,126 +119,9 @@ __device__ __forceinline__ float pair_force(float r2, float qq, float2 lj,
   return fscal;
 }
 
-template <int MOD, bool COUL, bool ENERGY>
-__global__ void __launch_bounds__(256) nb_pairs(
-    const float4 *__restrict__ xq, const int *__restrict__ type_at,
-    const uint32_t *__restrict__ real_mask, const PairEntry *__restrict__ rows,
-    const int *__restrict__ row_count, const uint64_t *__restrict__ ex_masks,
-    int row_cap, int max_groups, const int *__restrict__ counts_in,
-    const float2 *__restrict__ ljtab, NbParams p, long long *__restrict__ F,
-    double *__restrict__ energy_partial, int *__restrict__ flags) {
-  __shared__ float2 s_lj[kMaxTypes * kMaxTypes];
-  const int nt2 = p.ntypes * p.ntypes;
-  for (int k = threadIdx.x; k < nt2; k += blockDim.x) s_lj[k] = ljtab[k];
-  // The build kernels of this step have finished: clear the request.
-  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) flags[kFlagRebuild] = 0;
-  __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const int gi = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-  const int e0 = blockIdx.y * kSegment;
-  const int ngroups = counts_in[1] / kGroup;
-  int n = gi < ngroups ? row_count[gi] - e0 : 0;
-  n = __builtin_amdgcn_readfirstlane(n < kSegment ? n : kSegment);
-  const size_t slot = (size_t)blockIdx.y * max_groups + gi;
-  if (n <= 0) {
-    if (ENERGY && gi < max_groups && lane == 0) {
-      energy_partial[2 * slot] = 0.0;
-      energy_partial[2 * slot + 1] = 0.0;
-    }
-    return;
-  }
-  const int a = lane & 7, b = lane >> 3;
-  const int g0 = gi * kGroup;
-  float4 xi[kGroup];
-  int ti[kGroup];
-  bool ireal[kGroup];
-  float fi[kGroup][3];
-#pragma unroll
-  for (int m = 0; m < kGroup; ++m) {
-    xi[m] = xq[(g0 + m) * kCluster + a];
-    xi[m].w *= p.coulomb;
-    ti[m] = type_at[(g0 + m) * kCluster + a] * p.ntypes;
-    ireal[m] = (real_mask[g0 + m] >> a) & 1u;
-    fi[m][0] = fi[m][1] = fi[m][2] = 0.0f;
-  }
-  double e_lj = 0.0, e_c = 0.0;  // energy steps only: f64 partial sums
-  const PairEntry *row = rows + (size_t)gi * row_cap + e0;
-  // Software pipeline: the entry two ahead and the j atoms one ahead are
-  // loaded before the current entry is computed (indices clamped to the
-  // segment, so every load is in bounds).
-  PairEntry E = row[0];
-  PairEntry E1 = row[n > 1 ? 1 : 0];
-  float4 xj = xq[E.cj * kCluster + b];
-  int tj = type_at[E.cj * kCluster + b];
-  for (int e = 0; e < n; ++e) {
-    const PairEntry E2 = row[e + 2 < n ? e + 2 : n - 1];
-    const float4 xj1 = xq[E1.cj * kCluster + b];
-    const int tj1 = type_at[E1.cj * kCluster + b];
-
-    const uint32_t info = E.info;
-    float sx, sy, sz;
-    shift_vector(info & 63u, p.box, sx, sy, sz);
-    const float xjx = xj.x + sx, xjy = xj.y + sy, xjz = xj.z + sz;
-    const bool jreal = (info >> (16 + b)) & 1u;
-    float fjx = 0.0f, fjy = 0.0f, fjz = 0.0f;
-#pragma unroll
-    for (int m = 0; m < kGroup; ++m) {
-      if (!((info >> (8 + m)) & 1u)) continue;  // uniform
-      const float dx = xi[m].x - xjx, dy = xi[m].y - xjy, dz = xi[m].z - xjz;
-      const float r2 = dx * dx + dy * dy + dz * dz;
-      bool on = E.ex >= 0 ? ((ex_masks[(size_t)E.ex * kGroup + m] >> lane) & 1ull)
-                          : (ireal[m] && jreal);
-      on = on && r2 < p.rc2;
-      float el = 0.0f, ec = 0.0f;
-      float fscal = pair_force<MOD, COUL, ENERGY>(on ? r2 : 1.0f, xi[m].w * xj.w,
-                                                  s_lj[ti[m] + tj], p, el, ec);
-      fscal = on ? fscal : 0.0f;
-      if (ENERGY && on) {
-        e_lj += (double)el;
-        e_c += (double)ec;
-      }
-      const float fx = fscal * dx, fy = fscal * dy, fz = fscal * dz;
-      fi[m][0] += fx;
-      fi[m][1] += fy;
-      fi[m][2] += fz;
-      fjx -= fx;
-      fjy -= fy;
-      fjz -= fz;
-    }
-    // j force: the 8 i lanes of column b, summed across the members
-    const float gx = sum8(fjx), gy = sum8(fjy), gz = sum8(fjz);
-    if (a == 0 && jreal) add_fixed(F, E.cj * kCluster + b, gx, gy, gz);
-    E = E1;
-    E1 = E2;
-    xj = xj1;
-    tj = tj1;
-  }
-  // i forces: sum over the 8 j columns of the wave
-#pragma unroll
-  for (int m = 0; m < kGroup; ++m) {
-#pragma unroll
-    for (int s = 8; s < 64; s <<= 1) {
-      fi[m][0] += __shfl_xor(fi[m][0], s);
-      fi[m][1] += __shfl_xor(fi[m][1], s);
-      fi[m][2] += __shfl_xor(fi[m][2], s);
-    }
-    if (b == 0 && ireal[m]) add_fixed(F, (g0 + m) * kCluster + a, fi[m][0], fi[m][1], fi[m][2]);
-  }
-  if (ENERGY) {
-#pragma unroll
-    for (int s = 1; s < 64; s <<= 1) {
-      e_lj += __shfl_xor(e_lj, s);
-      e_c += __shfl_xor(e_c, s);
-    }
-    if (lane == 0) {
-      energy_partial[2 * slot] = e_lj;
-      energy_partial[2 * slot + 1] = e_c;
-    }
-  }
-}
-
 // Excluded pairs (every excluded pair, no cutoff) and scaled 1-4 pairs, in
 // f64: thread per pair.  pairs14 params may be null when there are none.
+// They run as the first plane of blocks of the pair kernel's launch.
 struct SpecialParams {
   double box[3];
   double beta;
@@ -281,14 +169,30 @@ __device__ void lj_pair_f64(double sigma, double eps, double r,
   dudr = draw;
 }
 
-__global__ void __launch_bounds__(256) nb_special(
-    int npairs, const int2 *__restrict__ pairs, const int *__restrict__ is14,
-    const double *__restrict__ p14_sigma, const double *__restrict__ p14_eps,
-    const double *__restrict__ p14_qq, const double *__restrict__ x,
-    const double *__restrict__ q, const int *__restrict__ place_of,
-    SpecialParams p, long long *__restrict__ F, double *__restrict__ energy_partial) {
-  __shared__ double s_e[3][256];
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+// The excluded and 1-4 pairs: thread per pair, blocks of 256 threads.
+struct SpecialArgs {
+  int npairs;
+  const int2 *pairs;
+  const int *is14;           // index into the 1-4 parameters, or -1
+  const double *p14_sigma, *p14_eps, *p14_qq;
+  const double *x, *q;
+  const int *place_of;
+  double *energy_partial;    // 3 per block, or null
+  SpecialParams p;
+};
+
+__device__ __forceinline__ void special_block(int block, const SpecialArgs &sa,
+                                              long long *__restrict__ F,
+                                              double (*s_e)[256]) {
+  const int npairs = sa.npairs;
+  const int2 *__restrict__ pairs = sa.pairs;
+  const int *__restrict__ is14 = sa.is14;
+  const double *__restrict__ p14_sigma = sa.p14_sigma, *__restrict__ p14_eps = sa.p14_eps,
+               *__restrict__ p14_qq = sa.p14_qq, *__restrict__ x = sa.x, *__restrict__ q = sa.q;
+  const int *__restrict__ place_of = sa.place_of;
+  double *__restrict__ energy_partial = sa.energy_partial;
+  const SpecialParams &p = sa.p;
+  const int t = block * blockDim.x + threadIdx.x;
   double e_ex = 0.0, e_lj14 = 0.0, e_c14 = 0.0;
   if (t < npairs) {
     const int2 ij = pairs[t];
@@ -340,7 +244,164 @@ __global__ void __launch_bounds__(256) nb_special(
       __syncthreads();
     }
     if (threadIdx.x == 0)
-      for (int k = 0; k < 3; ++k) energy_partial[3 * blockIdx.x + k] = s_e[k][0];
+      for (int k = 0; k < 3; ++k) energy_partial[3 * block + k] = s_e[k][0];
+  }
+}
+
+// Grid: x over groups (4 waves per block), y = 0 the excluded and 1-4
+// pairs, y = 1 + segment the pair segments.
+template <int MOD, bool COUL, bool ENERGY>
+__global__ void __launch_bounds__(256) nb_pairs(
+    const float4 *__restrict__ xq, const int *__restrict__ type_at,
+    const uint32_t *__restrict__ real_mask, const PairEntry *__restrict__ rows,
+    const int *__restrict__ row_count, const uint64_t *__restrict__ ex_masks,
+    int row_cap, int max_groups, const int *__restrict__ counts_in,
+    const float2 *__restrict__ ljtab, NbParams p, long long *__restrict__ F,
+    double *__restrict__ energy_partial, int *__restrict__ flags, SpecialArgs sa) {
+  extern __shared__ float2 s_lj[];                   // ntypes^2 (launch size)
+  __shared__ union {
+    struct {
+      float fj[4][kSegment * kCluster * 3];          // j forces of the entries
+      int cj[4][kSegment];
+    } pr;
+    double e[3][256];                                // special-pair energies
+  } sm;
+  const int nt2 = p.ntypes * p.ntypes;
+  for (int k = threadIdx.x; k < nt2; k += blockDim.x) s_lj[k] = ljtab[k];
+  // The build kernels of this step have finished: clear the request.
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) flags[kFlagRebuild] = 0;
+  __syncthreads();
+  if (blockIdx.y == 0) {  // the excluded and 1-4 pairs
+    if ((int)blockIdx.x * (int)blockDim.x < sa.npairs) special_block(blockIdx.x, sa, F, sm.e);
+    return;
+  }
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int gi = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
+  const int seg = blockIdx.y - 1;
+  const int e0 = seg * kSegment;
+  const int ngroups = counts_in[1] / kGroup;
+  int n = gi < ngroups ? row_count[gi] - e0 : 0;
+  n = __builtin_amdgcn_readfirstlane(n < kSegment ? n : kSegment);
+  const size_t slot = (size_t)seg * max_groups + gi;
+  if (n <= 0) {
+    if (ENERGY && gi < max_groups && lane == 0) {
+      energy_partial[2 * slot] = 0.0;
+      energy_partial[2 * slot + 1] = 0.0;
+    }
+    return;
+  }
+  const int a = lane & 7, b = lane >> 3;
+  const int g0 = gi * kGroup;
+  float4 xi[kGroup];
+  int ti[kGroup];
+  bool ireal[kGroup];
+  float fi[kGroup][3];
+#pragma unroll
+  for (int m = 0; m < kGroup; ++m) {
+    xi[m] = xq[(g0 + m) * kCluster + a];
+    xi[m].w *= p.coulomb;
+    ti[m] = type_at[(g0 + m) * kCluster + a] * p.ntypes;
+    ireal[m] = (real_mask[g0 + m] >> a) & 1u;
+    fi[m][0] = fi[m][1] = fi[m][2] = 0.0f;
+  }
+  double e_lj = 0.0, e_c = 0.0;  // energy steps only: f64 partial sums
+  const PairEntry *row = rows + (size_t)gi * row_cap + e0;
+  // Software pipeline: the entry two ahead and the j atoms one ahead are
+  // loaded before the current entry is computed (indices clamped to the
+  // segment, so every load is in bounds).
+  PairEntry E = row[0];
+  PairEntry E1 = row[n > 1 ? 1 : 0];
+  float4 xj = xq[E.cj * kCluster + b];
+  int tj = type_at[E.cj * kCluster + b];
+  for (int e = 0; e < n; ++e) {
+    const PairEntry E2 = row[e + 2 < n ? e + 2 : n - 1];
+    const float4 xj1 = xq[E1.cj * kCluster + b];
+    const int tj1 = type_at[E1.cj * kCluster + b];
+
+    const uint32_t info = E.info;
+    float2 ljm[kGroup];  // issued early: LDS latency overlaps the set-up
+#pragma unroll
+    for (int m = 0; m < kGroup; ++m) ljm[m] = s_lj[ti[m] + tj];
+    float sx, sy, sz;
+    shift_vector(info & 63u, p.box, sx, sy, sz);
+    const float xjx = xj.x + sx, xjy = xj.y + sy, xjz = xj.z + sz;
+    const bool jreal = (info >> (16 + b)) & 1u;
+    float fjx = 0.0f, fjy = 0.0f, fjz = 0.0f;
+#pragma unroll
+    for (int m = 0; m < kGroup; ++m) {
+      if (!((info >> (8 + m)) & 1u)) continue;  // uniform
+      const float dx = xi[m].x - xjx, dy = xi[m].y - xjy, dz = xi[m].z - xjz;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      bool on = E.ex >= 0 ? ((ex_masks[(size_t)E.ex * kGroup + m] >> lane) & 1ull)
+                          : (ireal[m] && jreal);
+      on = on && r2 < p.rc2;
+      float el = 0.0f, ec = 0.0f;
+      float fscal = pair_force<MOD, COUL, ENERGY>(on ? r2 : 1.0f, xi[m].w * xj.w,
+                                                  ljm[m], p, el, ec);
+      fscal = on ? fscal : 0.0f;
+      if (ENERGY && on) {
+        e_lj += (double)el;
+        e_c += (double)ec;
+      }
+      const float fx = fscal * dx, fy = fscal * dy, fz = fscal * dz;
+      fi[m][0] += fx;
+      fi[m][1] += fy;
+      fi[m][2] += fz;
+      fjx -= fx;
+      fjy -= fy;
+      fjz -= fz;
+    }
+    // j force: the 8 i lanes of column b, summed across the members, kept
+    // in LDS; the atomics are issued after the loop, so that the loads of
+    // the next entries never wait behind them
+    const float gx = sum8(fjx), gy = sum8(fjy), gz = sum8(fjz);
+    if (a == 0) {
+      float *dst = &sm.pr.fj[wave][(e * kCluster + b) * 3];
+      dst[0] = gx;
+      dst[1] = gy;
+      dst[2] = gz;
+    }
+    if (lane == 0) sm.pr.cj[wave][e] = E.cj;
+    E = E1;
+    E1 = E2;
+    xj = xj1;
+    tj = tj1;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // j forces of the segment: 24 words per entry, zero words skipped
+  // (padding atoms and pairs beyond the cutoff give exact zeros)
+  for (int k = lane; k < n * kCluster * 3; k += 64) {
+    const int e = k / (kCluster * 3), rem = k - e * (kCluster * 3);
+    const long long w = to_fixed(sm.pr.fj[wave][k]);
+    if (w) {
+      unsigned long long *u = reinterpret_cast<unsigned long long *>(F);
+      atomicAdd(u + 3 * (sm.pr.cj[wave][e] * kCluster) + rem, (unsigned long long)w);
+    }
+  }
+  // i forces: sum over the 8 j columns of the wave
+#pragma unroll
+  for (int m = 0; m < kGroup; ++m) {
+#pragma unroll
+    for (int s = 8; s < 64; s <<= 1) {
+      fi[m][0] += __shfl_xor(fi[m][0], s);
+      fi[m][1] += __shfl_xor(fi[m][1], s);
+      fi[m][2] += __shfl_xor(fi[m][2], s);
+    }
+    if (b == 0 && ireal[m]) add_fixed(F, (g0 + m) * kCluster + a, fi[m][0], fi[m][1], fi[m][2]);
+  }
+  if (ENERGY) {
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+      e_lj += __shfl_xor(e_lj, s);
+      e_c += __shfl_xor(e_c, s);
+    }
+    if (lane == 0) {
+      energy_partial[2 * slot] = e_lj;
+      energy_partial[2 * slot + 1] = e_c;
+    }
   }
 }
 
@@ -348,7 +409,7 @@ __global__ void __launch_bounds__(256) nb_special(
 #define MDHIP_PAIRS_ONE(MOD, C, E)                                                       \
   template __global__ void nb_pairs<MOD, C, E>(const float4 *, const int *,            \
       const uint32_t *, const PairEntry *, const int *, const uint64_t *, int, int,    \
-      const int *, const float2 *, NbParams, long long *, double *, int *);
+      const int *, const float2 *, NbParams, long long *, double *, int *, SpecialArgs);
 #define MDHIP_PAIRS(MOD)                                                                 \
   MDHIP_PAIRS_ONE(MOD, true, false)                                                      \
   MDHIP_PAIRS_ONE(MOD, true, true)                                                       \

@@ -100,6 +100,7 @@ struct BrickParams {
   const int *gx0, *gy0;     // first tile point of each column (mod K)
   const int *xstart, *xcols;  // per brick along x: columns (x index) whose tile overlaps
   const int *ystart, *ycols;
+  int plane_major;          // half spectrum as [kz][x][y] instead of [x][y][kz]
 };
 constexpr int kBrickMaxCols = 256;
 
@@ -169,13 +170,10 @@ __global__ void __launch_bounds__(256) pme_spread_rows(
     const double qd = q[atom];
     if (qd == 0.0) continue;
     int f[3];
-    float M[3][N];
+    float w[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      float w;
-      pme_place(x[3 * atom + d], pp.box[d], pp.inv_box[d], pp.K[d], N, f[d], w);
-      bspline<float>(w, N, M[d], nullptr);
-    }
+    for (int d = 0; d < 2; ++d)
+      pme_place(x[3 * atom + d], pp.box[d], pp.inv_box[d], pp.K[d], N, f[d], w[d]);
     // an atom outside its column's tile would be missed by some brick
     const int cx = s_cxy[lo] & 0xffff, cy = s_cxy[lo] >> 16;
     int fx = f[0] - bp.gx0[cx];
@@ -184,6 +182,16 @@ __global__ void __launch_bounds__(256) pme_spread_rows(
     if (fy < 0) fy += K1;
     if ((bp.TWx < K0 && fx > bp.TWx - N) || (bp.TWy < K1 && fy > bp.TWy - N))
       atomicOr(&flags[kFlagPmeRange], 1);
+    // atoms whose stencil misses the brick end here
+    int kx0 = bx0 - f[0];
+    if (kx0 < 0) kx0 += K0;
+    int ky0 = by0 - f[1];
+    if (ky0 < 0) ky0 += K1;
+    if (!(kx0 < N || kx0 > K0 - BX) || !(ky0 < N || ky0 > K1 - BY)) continue;
+    pme_place(x[3 * atom + 2], pp.box[2], pp.inv_box[2], pp.K[2], N, f[2], w[2]);
+    float M[3][N];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) bspline<float>(w[d], N, M[d], nullptr);
     const float qf = (float)qd;
 #pragma unroll
     for (int tx = 0; tx < BX; ++tx) {
@@ -235,10 +243,175 @@ __global__ void __launch_bounds__(256) pme_spread_rows(
     const int r1 = 2 * t, r2 = r1 + 1;
     const int ix1 = bx0 + r1 / BY, iy1 = by0 + r1 % BY;
     const int ix2 = bx0 + r2 / BY, iy2 = by0 + r2 % BY;
-    if (ix1 < K0 && iy1 < K1)
-      out[((size_t)ix1 * K1 + iy1) * H + k] = {(zk.x + zm.x) * 0.5f, (zk.y - zm.y) * 0.5f};
-    if (ix2 < K0 && iy2 < K1)
-      out[((size_t)ix2 * K1 + iy2) * H + k] = {(zk.y + zm.y) * 0.5f, (zm.x - zk.x) * 0.5f};
+    const size_t rows = (size_t)K0 * K1;
+    if (ix1 < K0 && iy1 < K1) {
+      const size_t row = (size_t)ix1 * K1 + iy1;
+      out[bp.plane_major ? k * rows + row : row * H + k] = {(zk.x + zm.x) * 0.5f,
+                                                            (zk.y - zm.y) * 0.5f};
+    }
+    if (ix2 < K0 && iy2 < K1) {
+      const size_t row = (size_t)ix2 * K1 + iy2;
+      out[bp.plane_major ? k * rows + row : row * H + k] = {(zk.y + zm.y) * 0.5f,
+                                                            (zm.x - zk.x) * 0.5f};
+    }
+  }
+}
+
+// ---- backward z transform fused with the gather ---------------------------
+//
+// A block owns the same brick of grid rows as the spread: it transforms the
+// brick's half-spectrum rows back to real potential rows in LDS, and every
+// atom of the brick's columns adds the part of its force gradient that comes
+// from the brick's points, as 64-bit fixed point (each part is a fixed f32
+// sum, so the total does not depend on the order of the adds).  The real
+// potential grid never goes to HBM.
+template <int N, int TL>
+__global__ void __launch_bounds__(256) pme_gather_rows(
+    const int *__restrict__ col_cl_start, const int *__restrict__ col_cl_count,
+    const int *__restrict__ atom_at, const double *__restrict__ x,
+    const double *__restrict__ q, PmeParams pp, BrickParams bp,
+    const cpx<float> *__restrict__ in, FftAxis az, long long *__restrict__ F) {
+  constexpr int BX = BrickShape<TL>::BX, BY = BrickShape<TL>::BY;
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  __shared__ int s_p0[kBrickMaxCols], s_cum[kBrickMaxCols + 1];
+  const int K0 = pp.K[0], K1 = pp.K[1], K2 = pp.K[2], H = K2 / 2 + 1;
+  cpx<float> *a = reinterpret_cast<cpx<float> *>(smem);
+  cpx<float> *b = a + TL * K2;
+  const int nbx = (K0 + BX - 1) / BX;
+  const int bxi = blockIdx.x % nbx, byi = blockIdx.x / nbx;
+  const int bx0 = bxi * BX, by0 = byi * BY;
+  const size_t rows = (size_t)K0 * K1;
+
+  // half spectra of rows 2t and 2t+1 as one complex line t
+  for (int w = threadIdx.x; w < TL * K2; w += blockDim.x) {
+    const int k = w % K2, t = w / K2;
+    const int r1 = 2 * t, r2 = r1 + 1;
+    const int ix1 = bx0 + r1 / BY, iy1 = by0 + r1 % BY;
+    const int ix2 = bx0 + r2 / BY, iy2 = by0 + r2 % BY;
+    const int m = k < H ? k : K2 - k;
+    cpx<float> x1 = {0.0f, 0.0f}, x2 = {0.0f, 0.0f};
+    if (ix1 < K0 && iy1 < K1) {
+      const size_t row = (size_t)ix1 * K1 + iy1;
+      x1 = in[bp.plane_major ? m * rows + row : row * H + m];
+    }
+    if (ix2 < K0 && iy2 < K1) {
+      const size_t row = (size_t)ix2 * K1 + iy2;
+      x2 = in[bp.plane_major ? m * rows + row : row * H + m];
+    }
+    if (m == 0 || (K2 % 2 == 0 && m == K2 / 2)) {
+      x1.y = 0.0f;
+      x2.y = 0.0f;
+    }
+    a[k * TL + t] = k < H ? cpx<float>{x1.x - x2.y, x1.y + x2.x}
+                          : cpx<float>{x1.x + x2.y, -x1.y + x2.x};
+  }
+  // the brick's columns and their place ranges
+  const int xs = bp.xstart[bxi], nxc = bp.xstart[bxi + 1] - xs;
+  const int ys = bp.ystart[byi], nyc = bp.ystart[byi + 1] - ys;
+  const int ncand = nxc * nyc;
+  for (int c = threadIdx.x; c < ncand; c += blockDim.x) {
+    const int col = bp.ycols[ys + c / nxc] * bp.ncx + bp.xcols[xs + c % nxc];
+    s_p0[c] = col_cl_start[col] * kCluster;
+    s_cum[c + 1] = col_cl_count[col] * kCluster;
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    int carry = 0;
+    for (int c0 = 0; c0 < ncand; c0 += 64) {
+      const int c = c0 + lane;
+      int v = c < ncand ? s_cum[c + 1] : 0;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int up = __shfl_up(v, off);
+        if (lane >= off) v += up;
+      }
+      if (c < ncand) s_cum[c + 1] = carry + v;
+      carry += __shfl(v, 63);
+    }
+    if (lane == 0) s_cum[0] = 0;
+  }
+  // (the scan above is read only after the transform's first pass barrier)
+  const cpx<float> *phi = lds_fft<float, TL>(a, b, az, +1);
+  // phi of brick row r at z: phi[z * TL + r / 2], .x for even r, .y for odd
+  const int total = s_cum[ncand];
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    int lo = 0, hi = ncand - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_cum[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    const int place = s_p0[lo] + (i - s_cum[lo]);
+    const int atom = atom_at[place];
+    if (atom < 0) continue;
+    const double qd = q[atom];
+    if (qd == 0.0) continue;
+    int f[3];
+    float w[3];
+#pragma unroll
+    for (int d = 0; d < 2; ++d)
+      pme_place(x[3 * atom + d], pp.box[d], pp.inv_box[d], pp.K[d], N, f[d], w[d]);
+    // atoms whose stencil misses the brick end here
+    int kx0 = bx0 - f[0];
+    if (kx0 < 0) kx0 += K0;
+    int ky0 = by0 - f[1];
+    if (ky0 < 0) ky0 += K1;
+    const bool hit_x = kx0 < N || kx0 > K0 - BX;
+    const bool hit_y = ky0 < N || ky0 > K1 - BY;
+    if (!hit_x || !hit_y) continue;
+    pme_place(x[3 * atom + 2], pp.box[2], pp.inv_box[2], pp.K[2], N, f[2], w[2]);
+    float M[3][N], dM[3][N];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) bspline<float>(w[d], N, M[d], dM[d]);
+    float g0 = 0.0f, g1 = 0.0f, g2 = 0.0f;
+    bool any = false;
+#pragma unroll
+    for (int tx = 0; tx < BX; ++tx) {
+      int kx = bx0 + tx - f[0];
+      if (kx < 0) kx += K0;
+      if (kx >= N || bx0 + tx >= K0) continue;
+      float mx = M[0][0], dmx = dM[0][0];
+#pragma unroll
+      for (int k = 1; k < N; ++k)
+        if (k == kx) {
+          mx = M[0][k];
+          dmx = dM[0][k];
+        }
+#pragma unroll
+      for (int ty = 0; ty < BY; ++ty) {
+        int ky = by0 + ty - f[1];
+        if (ky < 0) ky += K1;
+        if (ky >= N || by0 + ty >= K1) continue;
+        float my = M[1][0], dmy = dM[1][0];
+#pragma unroll
+        for (int k = 1; k < N; ++k)
+          if (k == ky) {
+            my = M[1][k];
+            dmy = dM[1][k];
+          }
+        const int r = tx * BY + ty;
+        float sz = 0.0f, dsz = 0.0f;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          int z = f[2] + k;
+          if (z >= K2) z -= K2;
+          const cpx<float> c = phi[z * TL + (r >> 1)];
+          const float pv = (r & 1) ? c.y : c.x;
+          sz += pv * M[2][k];
+          dsz += pv * dM[2][k];
+        }
+        g0 += dmx * my * sz;
+        g1 += mx * dmy * sz;
+        g2 += mx * my * dsz;
+        any = true;
+      }
+    }
+    if (!any) continue;
+    const float qf = (float)qd;
+    unsigned long long *u = reinterpret_cast<unsigned long long *>(F + 3 * place);
+    atomicAdd(u + 0, (unsigned long long)to_fixed(-qf * (float)(pp.K[0] * pp.inv_box[0]) * g0));
+    atomicAdd(u + 1, (unsigned long long)to_fixed(-qf * (float)(pp.K[1] * pp.inv_box[1]) * g1));
+    atomicAdd(u + 2, (unsigned long long)to_fixed(-qf * (float)(pp.K[2] * pp.inv_box[2]) * g2));
   }
 }
 
@@ -368,7 +541,7 @@ __global__ void __launch_bounds__(256) fft_rows_r2c(
 template <typename T>
 __global__ void __launch_bounds__(256) fft_rows_c2r(const cpx<T> *__restrict__ in,
                                                     T *__restrict__ dst, int nrows,
-                                                    FftAxis az) {
+                                                    FftAxis az, int plane_major) {
   constexpr int TL = tile_lines<T>();
   extern __shared__ unsigned char smem[];
   cpx<T> *a = reinterpret_cast<cpx<T> *>(smem);
@@ -380,8 +553,9 @@ __global__ void __launch_bounds__(256) fft_rows_c2r(const cpx<T> *__restrict__ i
     const int r1 = row0 + 2 * t, r2 = r1 + 1;
     const int m = k < H ? k : K - k;
     cpx<T> x1 = {(T)0, (T)0}, x2 = {(T)0, (T)0};
-    if (r1 < nrows) x1 = in[(size_t)r1 * H + m];
-    if (r2 < nrows) x2 = in[(size_t)r2 * H + m];
+    // half spectrum [row][k], or kz-plane major [k][row]
+    if (r1 < nrows) x1 = in[plane_major ? (size_t)m * nrows + r1 : (size_t)r1 * H + m];
+    if (r2 < nrows) x2 = in[plane_major ? (size_t)m * nrows + r2 : (size_t)r2 * H + m];
     if (m == 0 || (K % 2 == 0 && m == K / 2)) {
       x1.y = (T)0;
       x2.y = (T)0;
@@ -503,11 +677,68 @@ __global__ void __launch_bounds__(256) fft_x_conv(cpx<T> *__restrict__ data, int
   }
 }
 
+// The y and x passes of one kz plane, whole in LDS, for grids whose planes
+// fit (K1 K2 <= 4096 points): forward y, forward x, influence product (and
+// its energy), backward x, backward y.  The half spectrum is kz-plane major
+// ([kz][x][y]), so a plane is one contiguous read and write.
+__global__ void __launch_bounds__(512) fft_plane_conv(cpx<float> *__restrict__ data, int K1,
+                                                      int K2, FftAxis ax, FftAxis ay,
+                                                      ConvTables ct,
+                                                      double *__restrict__ energy_partial) {
+  extern __shared__ unsigned char smem[];
+  __shared__ double s_e[512];
+  const int np = K1 * K2;
+  cpx<float> *a = reinterpret_cast<cpx<float> *>(smem);
+  cpx<float> *b = a + np;
+  const int kz = blockIdx.x;
+  cpx<float> *plane = data + (size_t)kz * np;
+  for (int w = threadIdx.x; w < np; w += blockDim.x) a[w] = plane[w];
+  __syncthreads();
+  // element (x, y) at x K2 + y
+  cpx<float> *r = lds_fft_strided<float>(a, b, ay, -1, K1, 1, K2);          // along y
+  r = lds_fft_strided<float>(r, r == a ? b : a, ax, -1, K2, K2, 1);        // along x
+  const float *t0 = (const float *)ct.t[0], *t1 = (const float *)ct.t[1],
+              *t2 = (const float *)ct.t[2];
+  const float *m0 = (const float *)ct.m[0], *m1 = (const float *)ct.m[1],
+              *m2 = (const float *)ct.m[2];
+  const double h = (kz == 0 || (ct.K3 % 2 == 0 && kz == ct.K3 / 2)) ? 0.5 : 1.0;
+  double e = 0.0;
+  for (int w = threadIdx.x; w < np; w += blockDim.x) {
+    const int kx = w / K2, ky = w % K2;
+    const cpx<float> v = r[w];
+    float bc = 0.0f;
+    if ((kx | ky | kz) != 0) {
+      const float msq = m0[kx] * m0[kx] + m1[ky] * m1[ky] + m2[kz] * m2[kz];
+      bc = t0[kx] * t1[ky] * t2[kz] / msq;
+      if (energy_partial) e += h * (double)bc * ((double)v.x * v.x + (double)v.y * v.y);
+    }
+    r[w] = {v.x * bc, v.y * bc};
+  }
+  __syncthreads();
+  r = lds_fft_strided<float>(r, r == a ? b : a, ax, +1, K2, K2, 1);        // along x
+  r = lds_fft_strided<float>(r, r == a ? b : a, ay, +1, K1, 1, K2);        // along y
+  for (int w = threadIdx.x; w < np; w += blockDim.x) plane[w] = r[w];
+  if (energy_partial) {
+    s_e[threadIdx.x] = e;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) s_e[threadIdx.x] += s_e[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) energy_partial[blockIdx.x] = s_e[0];
+  }
+}
+
 template __global__ void fft_rows_r2c<float, true>(const void *, cpx<float> *, int, FftAxis);
 #define MDHIP_SPREAD(N, TL)                                                              \
   template __global__ void pme_spread_rows<N, TL>(const int *, const int *, const int *, \
       const double *, const double *, PmeParams, BrickParams, cpx<float> *, FftAxis, int *);
-#define MDHIP_SPREAD_N(N) MDHIP_SPREAD(N, 8) MDHIP_SPREAD(N, 4) MDHIP_SPREAD(N, 2) MDHIP_SPREAD(N, 1)
+#define MDHIP_SPREAD_N(N) MDHIP_SPREAD(N, 8) MDHIP_SPREAD(N, 4) MDHIP_SPREAD(N, 2) MDHIP_SPREAD(N, 1) \
+  MDHIP_GATHER(N, 8) MDHIP_GATHER(N, 4) MDHIP_GATHER(N, 2) MDHIP_GATHER(N, 1)
+#define MDHIP_GATHER(N, TL)                                                              \
+  template __global__ void pme_gather_rows<N, TL>(const int *, const int *, const int *, \
+      const double *, const double *, PmeParams, BrickParams, const cpx<float> *, FftAxis, \
+      long long *);
 MDHIP_SPREAD_N(4)
 MDHIP_SPREAD_N(6)
 MDHIP_SPREAD_N(8)
@@ -516,8 +747,8 @@ template __global__ void pme_gather_lanes<6, 8>(const int *, const int *, const 
 template __global__ void pme_gather_lanes<8, 8>(const int *, const int *, const double *, const double *, PmeParams, const float *, long long *);
 template __global__ void fft_rows_r2c<float, false>(const void *, cpx<float> *, int, FftAxis);
 template __global__ void fft_rows_r2c<double, false>(const void *, cpx<double> *, int, FftAxis);
-template __global__ void fft_rows_c2r<float>(const cpx<float> *, float *, int, FftAxis);
-template __global__ void fft_rows_c2r<double>(const cpx<double> *, double *, int, FftAxis);
+template __global__ void fft_rows_c2r<float>(const cpx<float> *, float *, int, FftAxis, int);
+template __global__ void fft_rows_c2r<double>(const cpx<double> *, double *, int, FftAxis, int);
 template __global__ void fft_cols_y<float>(cpx<float> *, int, int, FftAxis, int);
 template __global__ void fft_cols_y<double>(cpx<double> *, int, int, FftAxis, int);
 template __global__ void fft_x_conv<float>(cpx<float> *, int, int, int, FftAxis, ConvTables, int, double *);

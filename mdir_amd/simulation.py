@@ -156,9 +156,8 @@ class Simulation:
         return dict(zip(keys, (int(v) for v in s)))
 
     PROFILE_KERNELS = ["integrate_first", "nb_bin", "nb_sort", "nb_search",
-                       "nb_pairs", "nb_special", "pme_spread_rows",
-                       "fft_cols_y_fwd", "fft_x_conv", "fft_cols_y_bwd",
-                       "fft_rows_c2r", "pme_gather", "integrate_second"]
+                       "nb_pairs", "pme_spread_rows", "pme_yx",
+                       "pme_gather_rows", "integrate_second"]
 
     def profile_step(self, repeats: int = 20) -> dict:
         """ms per launch of each kernel of a step (HIP events on the stream
