@@ -204,6 +204,8 @@ typedef struct {
   const int32_t *water_first; /* first atom (oxygen) of each water      */
   double d_oh, d_hh;          /* SETTLE geometry                        */
   double skin;                /* list buffer beyond rc (nm)             */
+  double prune_skin;          /* buffer of the pruned list the pair loop
+                                 runs over (nm, < skin); 0: no pruning   */
 } mdrtHipSystemDesc;
 
 /* Creates a system on the current device; NULL on error (see
@@ -241,9 +243,11 @@ int mdrtHipMinimize(void *system, int64_t steps, double max_step,
 int mdrtHipNeighborPairs(void *system, double reach, int32_t excluded,
                          int64_t capacity, int32_t *pairs, int64_t *count);
 
-/* List statistics: places, clusters, cluster-pair tiles, largest group row (entries), row capacity,
-   columns, column capacity, builds done since creation. */
-int mdrtHipStats(void *system, int64_t *stats /* 8 */);
+/* List statistics: places, clusters, cluster-pair tiles of the list,
+   largest group row (entries), row capacity, columns, column capacity,
+   builds done since creation, tiles of the rows the pair loop runs over
+   (the pruned list when pruning), prunes done since creation. */
+int mdrtHipStats(void *system, int64_t *stats /* 10 */);
 
 /* Device-event timing of each kernel of a step on the stream it runs on,
    ms per launch averaged over `repeats` steps (the steps advance the

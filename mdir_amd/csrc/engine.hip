@@ -157,6 +157,7 @@ struct Engine {
   int n = 0, ntypes = 0, nunits = 0;
   double box[3] = {0, 0, 0};
   double cutoff = 0, switch_from = 0, skin = 0.1, beta = 0;
+  double prune_skin = 0;       // > 0: the pair loop runs over a pruned list
   int lj_modifier = 0, coulomb_shift = 0, dispersion = 0;
   int pme_grid[3] = {0, 0, 0}, pme_order = 4;
   std::vector<double> h_q, h_mass, h_sigma, h_eps;
@@ -177,6 +178,7 @@ struct Engine {
   int max_places = 0;
   int *d_atom_at = nullptr, *d_type_at = nullptr;
   float4 *d_xq = nullptr;
+  double4 *d_xpq = nullptr;   // f64 positions and charges in place order (PME)
   long long *d_F = nullptr;
   int *d_col_count = nullptr, *d_bucket = nullptr;
   float4 *d_bb_lo = nullptr, *d_bb_hi = nullptr;
@@ -187,6 +189,9 @@ struct Engine {
   PairEntry *d_rows = nullptr;     // group rows, row_cap entries each
   int *d_row_count = nullptr;
   uint64_t *d_ex = nullptr;        // explicit masks, kGroup per slot
+  PairEntry *d_rows_in = nullptr;  // pruned rows (same capacity)
+  int *d_row_count_in = nullptr;
+  double *d_xprune = nullptr;      // positions at the last prune
   int ex_cap = 0;
   int max_groups = 0, nseg = 0;    // group rows; pair-loop segments per row
   int *d_excl_start = nullptr, *d_excl_atoms = nullptr;
@@ -247,6 +252,7 @@ struct Engine {
     graph_exec = nullptr;
     dfree(d_x); dfree(d_v); dfree(d_xref); dfree(d_shift); dfree(d_q);
     dfree(d_inv_mass); dfree(d_type); dfree(d_place_of); dfree(d_save_x); dfree(d_save_v);
+    dfree(d_xprune);
     release_list();
     dfree(d_excl_start); dfree(d_excl_atoms); dfree(d_flags); dfree(d_lj);
     dfree(d_special); dfree(d_is14); dfree(d_isexcl); dfree(d_p14_sigma);
@@ -260,10 +266,11 @@ struct Engine {
   }
 
   void release_list() {
-    dfree(d_atom_at); dfree(d_type_at); dfree(d_xq); dfree(d_F); dfree(d_col_count);
+    dfree(d_atom_at); dfree(d_type_at); dfree(d_xq); dfree(d_xpq); dfree(d_F); dfree(d_col_count);
     dfree(d_bucket); dfree(d_bb_lo); dfree(d_bb_hi); dfree(d_cluster_col);
     dfree(d_real_mask); dfree(d_col_cl_start); dfree(d_col_cl_count); dfree(d_counts);
     dfree(d_rows); dfree(d_row_count); dfree(d_epair); dfree(d_ex);
+    dfree(d_rows_in); dfree(d_row_count_in);
   }
 
   // Column grid, capacities and the list buffers for a search reach.
@@ -325,6 +332,7 @@ struct Engine {
     HIPCHK(hipDeviceSynchronize());
     d_type_at = dalloc<int>(max_places);
     d_xq = dalloc<float4>(max_places);
+    d_xpq = dalloc<double4>(max_places);
     d_F = dalloc<long long>(3 * (size_t)max_places);
     d_col_count = dalloc<int>(g.ncol);
     d_bucket = dalloc<int>((size_t)g.ncol * g.col_cap);
@@ -338,6 +346,10 @@ struct Engine {
     d_rows = dalloc<PairEntry>((size_t)max_groups * g.row_cap);
     d_row_count = dalloc<int>(max_groups);
     d_ex = dalloc<uint64_t>((size_t)ex_cap * kGroup);
+    if (prune_skin > 0) {
+      d_rows_in = dalloc<PairEntry>((size_t)max_groups * g.row_cap);
+      d_row_count_in = dalloc<int>(max_groups);
+    }
     d_epair = dalloc<double>(2 * (size_t)max_groups * nseg);
     forces_valid = false;
     if (graph_exec) {
@@ -356,6 +368,8 @@ struct Engine {
     cutoff = d->cutoff;
     switch_from = d->switch_from;
     skin = d->skin > 0 ? d->skin : 0.1;
+    prune_skin = d->prune_skin > 0 ? d->prune_skin : 0.0;
+    if (prune_skin >= skin) prune_skin = 0.0;  // nothing to prune
     beta = d->beta;
     lj_modifier = d->lj_modifier;
     coulomb_shift = d->coulomb_shift;
@@ -580,6 +594,14 @@ struct Engine {
       d_unit_size = dupload(us);
       for (int k = 0; k < 3; ++k) ip.half_box[k] = 0.5 * box[k];
       ip.skin_half2 = 0.25 * skin * skin;
+      if (prune_skin > 0) {
+        d_xprune = dalloc<double>(3 * (size_t)n);
+        ip.prune_half2 = 0.25 * prune_skin * prune_skin;
+        ip.x_prune = d_xprune;
+      } else {
+        ip.prune_half2 = 0.0;
+        ip.x_prune = nullptr;
+      }
       if (d->nwaters > 0) {
         const int o = d->water_first[0];
         ip.m_o = h_mass[o];
@@ -646,7 +668,7 @@ struct Engine {
   }
   void launch_sort() {
     nb_sort<<<g.ncol, 256, 0, stream>>>(d_x, d_shift, d_q, d_type, d_col_count, d_bucket,
-                                         d_atom_at, d_place_of, d_xq, d_type_at, d_bb_lo,
+                                         d_atom_at, d_place_of, d_xq, d_xpq, d_type_at, d_bb_lo,
                                          d_bb_hi, d_cluster_col, d_real_mask, d_col_cl_start,
                                          d_col_cl_count, d_counts, d_flags, g);
   }
@@ -656,10 +678,19 @@ struct Engine {
         d_atom_at, d_place_of, d_excl_start, d_excl_atoms, d_counts, d_col_count, d_rows,
         d_row_count, d_ex, ex_cap, d_flags, g);
   }
+  void launch_prune() {
+    if (prune_skin <= 0) return;
+    const double reach = cutoff + prune_skin + 1e-4;
+    nb_prune<<<blocks_for(max_groups, 4), 256, 0, stream>>>(
+        d_xq, d_rows, d_row_count, d_rows_in, d_row_count_in, d_counts, d_atom_at, d_x,
+        d_xprune, g.row_cap, (float)(reach * reach), (float)box[0], (float)box[1],
+        (float)box[2], d_flags);
+  }
   void build() {
     launch_bin();
     launch_sort();
     launch_search();
+    launch_prune();
   }
 
   // One launch: plane y = 0 of the grid takes the excluded and 1-4 pairs,
@@ -679,9 +710,11 @@ struct Engine {
     sa.energy_partial = ENERGY ? d_espec : nullptr;
     sa.p = spp;
     const dim3 grid(std::max(blocks_for(max_groups, 4), blocks_for(nspecial)), nseg + 1);
+    const bool pruned = prune_skin > 0;
     nb_pairs<MOD, COUL, ENERGY><<<grid, 256, ntypes * ntypes * sizeof(float2), stream>>>(
-        d_xq, d_type_at, d_real_mask, d_rows, d_row_count, d_ex, g.row_cap, max_groups,
-        d_counts, d_lj, nbp, d_F, d_epair, d_flags, sa);
+        d_xq, d_type_at, d_real_mask, pruned ? d_rows_in : d_rows,
+        pruned ? d_row_count_in : d_row_count, d_ex, g.row_cap, max_groups, d_counts, d_lj,
+        nbp, d_F, d_epair, d_flags, sa);
   }
 
   template <int MOD>
@@ -786,7 +819,7 @@ struct Engine {
     const int blocks = ((pme_grid[0] + BX - 1) / BX) * ((pme_grid[1] + BY - 1) / BY);
     const size_t lds = (size_t)32 * TL * pme_grid[2];
     pme_spread_rows<N, TL><<<blocks, 256, lds, stream>>>(d_col_cl_start, d_col_cl_count,
-                                                         d_atom_at, d_x, d_q, pmp, bp, d_cplx,
+                                                         d_xpq, pmp, bp, d_cplx,
                                                          fft.ax[2].axis, d_flags);
   }
   template <int N>
@@ -804,7 +837,7 @@ struct Engine {
     const int blocks = ((pme_grid[0] + BX - 1) / BX) * ((pme_grid[1] + BY - 1) / BY);
     const size_t lds = (size_t)16 * TL * pme_grid[2];
     pme_gather_rows<N, TL><<<blocks, 256, lds, stream>>>(d_col_cl_start, d_col_cl_count,
-                                                         d_atom_at, d_x, d_q, pmp, bp, d_cplx,
+                                                         d_xpq, pmp, bp, d_cplx,
                                                          fft.ax[2].axis, d_F);
   }
   template <int N>
@@ -889,7 +922,7 @@ struct Engine {
     ip.dt = dt;
     integrate_part<PART><<<blocks_for(nunits, 64), 64, 0, stream>>>(
         nunits, d_unit_first, d_unit_size, d_x, d_v, d_inv_mass, d_F, d_place_of, d_shift,
-        d_xref, d_q, d_xq, ip, d_flags);
+        d_xref, d_q, d_xq, d_xpq, ip, d_flags);
   }
   void step_first(double dt) { integrate<kFirstHalf>(dt); }
   void step_second(double dt) { integrate<kSecondHalf>(dt); }
@@ -1173,6 +1206,19 @@ struct Engine {
     s[6] = g.col_cap;
     std::vector<int> f = read_flags();
     s[7] = f[kFlagBuilds];
+    s[8] = tiles;
+    s[9] = f[kFlagPrunes];
+    if (prune_skin > 0) {
+      std::vector<int> rci(max_groups);
+      to_host(rci.data(), d_row_count_in, rci.size() * sizeof(int));
+      std::vector<PairEntry> rin((size_t)max_groups * g.row_cap);
+      to_host(rin.data(), d_rows_in, rin.size() * sizeof(PairEntry));
+      int64_t t = 0;
+      for (int gi = 0; gi < counts[1] / kGroup; ++gi)
+        for (int e = 0; e < rci[gi]; ++e)
+          t += __builtin_popcount((rin[(size_t)gi * g.row_cap + e].info >> 8) & 0xFu);
+      s[8] = t;
+    }
   }
 
   // Device-event timing of every kernel of one step, on the stream the

@@ -16,7 +16,9 @@ namespace mdhip {
 
 struct IntParams {
   double dt;
-  double skin_half2;
+  double skin_half2;     // (outer list buffer / 2)^2: rebuild beyond it
+  double prune_half2;    // (pruned list buffer / 2)^2: prune again beyond it
+  const double *x_prune; // positions at the last prune (atom order), or null
   double half_box[3];
   double m_o, m_h;
   double ra, rb, rc;   // SETTLE canonical triangle
@@ -186,14 +188,17 @@ __device__ __forceinline__ void constrain_velocities(UnitState<S> &s, const IntP
 }
 
 // Drift, SETTLE, v += dx_c/dt; stores x, v and the f32 positions of the
-// pair loop; returns whether an atom moved beyond half the skin.
+// pair loop; returns bit 0 when an atom moved beyond half the list buffer
+// since the build, bit 1 beyond half the pruned list's buffer since the
+// last prune.
 template <int S>
-__device__ __forceinline__ bool drift_store(UnitState<S> &s, double *__restrict__ x,
+__device__ __forceinline__ int drift_store(UnitState<S> &s, double *__restrict__ x,
                                            double *__restrict__ v,
                                            const double *__restrict__ shift,
                                            const double *__restrict__ xref,
                                            const double *__restrict__ q,
-                                           float4 *__restrict__ xq, const IntParams &p) {
+                                           float4 *__restrict__ xq,
+                                           double4 *__restrict__ xpq, const IntParams &p) {
   double xn[S][3];
 #pragma unroll
   for (int m = 0; m < S; ++m)
@@ -212,23 +217,29 @@ __device__ __forceinline__ bool drift_store(UnitState<S> &s, double *__restrict_
 #pragma unroll
       for (int k = 0; k < 3; ++k) s.v[m][k] += (xn[m][k] - xu[m][k]) * idt;
   }
-  bool moved = false;
+  int moved = 0;  // bit 0: beyond the list buffer, bit 1: beyond the pruned one
 #pragma unroll
   for (int m = 0; m < S; ++m) {
     const int at = s.first + m;
-    double d2 = 0.0;
+    double d2 = 0.0, d2p = 0.0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       x[3 * at + k] = xn[m][k];
       v[3 * at + k] = s.v[m][k];
       const double d = xn[m][k] - xref[3 * at + k];
       d2 += d * d;
+      if (p.x_prune) {
+        const double dp = xn[m][k] - p.x_prune[3 * at + k];
+        d2p += dp * dp;
+      }
     }
-    moved |= d2 > p.skin_half2;
+    if (d2 > p.skin_half2) moved |= 1;
+    if (d2p > p.prune_half2) moved |= 2;
     xq[s.pl[m]] = make_float4((float)(xn[m][0] - shift[3 * at + 0] - p.half_box[0]),
                               (float)(xn[m][1] - shift[3 * at + 1] - p.half_box[1]),
                               (float)(xn[m][2] - shift[3 * at + 2] - p.half_box[2]),
                               (float)q[at]);
+    xpq[s.pl[m]] = make_double4(xn[m][0], xn[m][1], xn[m][2], q[at]);
   }
   return moved;
 }
@@ -245,7 +256,8 @@ __device__ __forceinline__ void step_unit(int first, double *__restrict__ x,
                                           const double *__restrict__ shift,
                                           const double *__restrict__ xref,
                                           const double *__restrict__ q,
-                                          float4 *__restrict__ xq, const IntParams &p,
+                                          float4 *__restrict__ xq,
+                                          double4 *__restrict__ xpq, const IntParams &p,
                                           int *__restrict__ flags) {
   UnitState<S> s;
   // the force words are cleared by the launch that reads them last
@@ -256,7 +268,9 @@ __device__ __forceinline__ void step_unit(int first, double *__restrict__ x,
   }
   if (PART & kFirstHalf) {   // start of a step: half kick, drift, SETTLE
     half_kick<S>(s, p.dt);
-    if (drift_store<S>(s, x, v, shift, xref, q, xq, p)) atomicOr(&flags[kFlagRebuild], 1);
+    const int moved = drift_store<S>(s, x, v, shift, xref, q, xq, xpq, p);
+    if (moved & 1) atomicOr(&flags[kFlagRebuild], 1);
+    if (moved & 2) atomicOr(&flags[kFlagPrune], 1);
   } else {
 #pragma unroll
     for (int m = 0; m < S; ++m)
@@ -277,15 +291,15 @@ __global__ void __launch_bounds__(64) integrate_part(
     double *__restrict__ x, double *__restrict__ v, const double *__restrict__ inv_mass,
     long long *__restrict__ F, const int *__restrict__ place_of,
     const double *__restrict__ shift, const double *__restrict__ xref,
-    const double *__restrict__ q, float4 *__restrict__ xq, IntParams p,
-    int *__restrict__ flags) {
+    const double *__restrict__ q, float4 *__restrict__ xq, double4 *__restrict__ xpq,
+    IntParams p, int *__restrict__ flags) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= nunits) return;
   const int first = unit_first[u];
   if (unit_size[u] == 3)
-    step_unit<PART, 3>(first, x, v, inv_mass, F, place_of, shift, xref, q, xq, p, flags);
+    step_unit<PART, 3>(first, x, v, inv_mass, F, place_of, shift, xref, q, xq, xpq, p, flags);
   else
-    step_unit<PART, 1>(first, x, v, inv_mass, F, place_of, shift, xref, q, xq, p, flags);
+    step_unit<PART, 1>(first, x, v, inv_mass, F, place_of, shift, xref, q, xq, xpq, p, flags);
 }
 
 // Force words (place order) -> f64 forces (atom order).

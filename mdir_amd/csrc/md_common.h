@@ -6,8 +6,9 @@
 //   place order  xq: float4 (x, y, z, q) relative to the box centre, type_at,
 //                atom_at (-1 for padding), force words: int64[3 np]
 //   clusters     8 consecutive places; bbox lo/hi float4; column of cluster
-//   pair list    rows of capacity row_cap entries per i-cluster: {cj, shift,
-//                64-bit interaction mask}
+//   pair list    rows of capacity row_cap entries per group of 4 i-clusters:
+//                {cj, shift | member mask | j real mask, explicit-mask slot};
+//                the pruned rows have the same layout
 // Forces are accumulated as 64-bit fixed point, 2^32 words per kJ/mol/nm,
 // by integer atomics: the sum does not depend on the order of the adds.
 #pragma once
@@ -35,7 +36,7 @@ enum LJModifier { kLJNone = 0, kLJPotentialShift = 1, kLJForceSwitch = 2,
 // Device flags word layout (int32 each).
 enum Flag { kFlagRebuild = 0, kFlagOverflowColumn = 1, kFlagOverflowRow = 2,
             kFlagBuilds = 3, kFlagSettleFail = 4, kFlagPmeRange = 5,
-            kFlagMaskCount = 6, kNumFlags = 8 };
+            kFlagMaskCount = 6, kFlagPrune = 7, kFlagPrunes = 8, kNumFlags = 16 };
 
 constexpr int kGroup = 4;            // i-clusters of a group (one row)
 constexpr int kSegment = 48;         // row entries one wave of the pair loop takes

@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(256) nb_sort(
     const double *__restrict__ q, const int *__restrict__ type,
     const int *__restrict__ col_count, const int *__restrict__ bucket,
     int *__restrict__ atom_at, int *__restrict__ place_of,
-    float4 *__restrict__ xq, int *__restrict__ type_at,
+    float4 *__restrict__ xq, double4 *__restrict__ xpq, int *__restrict__ type_at,
     float4 *__restrict__ bb_lo, float4 *__restrict__ bb_hi,
     int *__restrict__ cluster_col, uint32_t *__restrict__ real_mask,
     int *__restrict__ col_cl_start, int *__restrict__ col_cl_count,
@@ -159,6 +159,7 @@ __global__ void __launch_bounds__(256) nb_sort(
     p.z = (float)(x[3 * ia + 2] - shift[3 * ia + 2] - g.half_box[2]);
     p.w = (float)q[ia];
     xq[place] = p;
+    xpq[place] = make_double4(x[3 * ia + 0], x[3 * ia + 1], x[3 * ia + 2], q[ia]);
     type_at[place] = type[ia];
     s_pos[rank] = p;
   }
@@ -167,6 +168,7 @@ __global__ void __launch_bounds__(256) nb_sort(
     atom_at[place] = -1;
     // far from everything and from each other; masked out of every tile
     xq[place] = make_float4(1.0e5f, 1.0e5f, 1.0e5f + (float)(place & 1023), 0.0f);
+    xpq[place] = make_double4(0.0, 0.0, 0.0, 0.0);  // zero charge: no PME work
     type_at[place] = 0;
   }
   __syncthreads();
@@ -241,6 +243,12 @@ __global__ void __launch_bounds__(256) nb_search(
   __shared__ int s_ex_cluster[4][kExMax];
   __shared__ int s_ex_bit[4][kExMax];             // member << 8 | mask bit
   __shared__ int s_ex_n[4];
+  // partner clusters of the group: open-addressing set, key -1 = empty,
+  // value = the members that have a partner there
+  constexpr int kPartnerSlots = kExMax;  // a power of two
+  static_assert((kPartnerSlots & (kPartnerSlots - 1)) == 0, "partner set size");
+  __shared__ int s_pkey[4][kPartnerSlots];
+  __shared__ int s_pmem[4][kPartnerSlots];
   __shared__ int s_cstart[4][64], s_cbase[4][64], s_ccode[4][64];
   __shared__ int s_k0[4][3][64], s_n[4][2][64];  // per z shift: first cluster, count
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -249,6 +257,10 @@ __global__ void __launch_bounds__(256) nb_search(
   const bool active = gi < ngroups;
   const int g0 = gi * kGroup;
   if (lane == 0) s_ex_n[wave] = 0;
+  for (int k = lane; k < kPartnerSlots; k += 64) {
+    s_pkey[wave][k] = -1;
+    s_pmem[wave][k] = 0;
+  }
   __syncthreads();
 
   // exclusion table of the group: (partner cluster, member << 8 | mask bit)
@@ -261,8 +273,17 @@ __global__ void __launch_bounds__(256) nb_search(
         const int pp = place_of[excl_atoms[k]];
         const int slot = atomicAdd(&s_ex_n[wave], 1);
         if (slot < kExMax) {
-          s_ex_cluster[wave][slot] = pp / kCluster;
+          const int pc = pp / kCluster;
+          s_ex_cluster[wave][slot] = pc;
           s_ex_bit[wave][slot] = (m << 8) | (a + kCluster * (pp % kCluster));
+          // at most kExMax distinct keys: the probe always finds a slot
+          for (int h = pc & (kPartnerSlots - 1);; h = (h + 1) & (kPartnerSlots - 1)) {
+            const int prev = atomicCAS(&s_pkey[wave][h], -1, pc);
+            if (prev == -1 || prev == pc) {
+              atomicOr(&s_pmem[wave][h], 1 << m);
+              break;
+            }
+          }
         } else {
           atomicOr(&flags[kFlagOverflowRow], 2);
         }
@@ -407,9 +428,17 @@ __global__ void __launch_bounds__(256) nb_search(
           // a tile of a member with itself, or one that holds excluded pairs
           const int self = cj - g0;
           bool special = self >= 0 && self < kGroup && ((imask >> self) & 1u);
-          for (int k = 0; k < nex && !special; ++k)
-            special = s_ex_cluster[wave][k] == cj &&
-                      ((imask >> (s_ex_bit[wave][k] >> 8)) & 1u);
+          if (!special && nex > 0) {
+            int h = cj & (kPartnerSlots - 1);
+            for (int probe = 0; probe < kPartnerSlots; ++probe, h = (h + 1) & (kPartnerSlots - 1)) {
+              const int key = s_pkey[wave][h];
+              if (key == cj) {
+                special = (s_pmem[wave][h] & (int)imask) != 0;
+                break;
+              }
+              if (key == -1) break;
+            }
+          }
           if (special) {
             uint64_t jcols = 0;  // byte b all ones when j atom b is real
 #pragma unroll
@@ -477,6 +506,91 @@ __global__ void __launch_bounds__(256) nb_search(
     __builtin_amdgcn_wave_barrier();
   }
   if (lane == 0) row_count[gi] = count < g.row_cap ? count : g.row_cap;
+}
+
+// ---- the pruned list ------------------------------------------------------
+//
+// The list above is searched with a wide buffer and rebuilt rarely; the pair
+// loop runs over a pruned copy that keeps, of each entry, the members with
+// at least one atom pair within rc + the pruned buffer (the dual list of the
+// reference, 04-neighbors.md).  Pruning runs when the list was rebuilt this
+// step or an atom moved more than half the pruned buffer since the last
+// prune (the flags the integration sets); it also records the positions the
+// next test measures from.  Wave per group; a lane per entry tests its
+// j-cluster against every member atom, so the 64 entries' loads are in
+// flight together.  Row order is kept, so the pruned rows are deterministic.
+__global__ void __launch_bounds__(256) nb_prune(
+    const float4 *__restrict__ xq, const PairEntry *__restrict__ rows,
+    const int *__restrict__ row_count, PairEntry *__restrict__ rows_in,
+    int *__restrict__ row_count_in, const int *__restrict__ counts_in,
+    const int *__restrict__ atom_at, const double *__restrict__ x,
+    double *__restrict__ x_prune, int row_cap, float reach2, float Lx, float Ly,
+    float Lz, int *__restrict__ flags) {
+  if (flags[kFlagRebuild] == 0 && flags[kFlagPrune] == 0) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&flags[kFlagPrunes], 1);
+  __shared__ float4 s_xi[4][kGroup * kCluster];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int gi = blockIdx.x * 4 + wave;
+  const int ngroups = counts_in[1] / kGroup;
+  if (gi >= ngroups) return;  // whole waves: no block barrier below
+  const int g0 = gi * kGroup;
+  if (lane < kGroup * kCluster) {
+    s_xi[wave][lane] = xq[g0 * kCluster + lane];
+    const int atom = atom_at[g0 * kCluster + lane];
+    if (atom >= 0)
+      for (int k = 0; k < 3; ++k) x_prune[3 * atom + k] = x[3 * atom + k];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int n = row_count[gi];
+  const PairEntry *row = rows + (size_t)gi * row_cap;
+  PairEntry *out = rows_in + (size_t)gi * row_cap;
+  int count = 0;
+  for (int base = 0; base < n; base += 64) {
+    const int e = base + lane;
+    PairEntry E = {0, 0u, -1, 0};
+    uint32_t inner = 0;
+    if (e < n) {
+      E = row[e];
+      const uint32_t info = E.info;
+      float4 xj[kCluster];
+#pragma unroll
+      for (int b = 0; b < kCluster; ++b) xj[b] = xq[E.cj * kCluster + b];
+      const float sx = (float)((int)(info & 3u) - 1) * Lx;
+      const float sy = (float)((int)((info >> 2) & 3u) - 1) * Ly;
+      const float sz = (float)((int)((info >> 4) & 3u) - 1) * Lz;
+#pragma unroll
+      for (int b = 0; b < kCluster; ++b) {
+        xj[b].x += sx;
+        xj[b].y += sy;
+        xj[b].z += sz;
+      }
+#pragma unroll
+      for (int m = 0; m < kGroup; ++m) {
+        if (!((info >> (8 + m)) & 1u)) continue;
+        bool near = false;
+        for (int a = 0; a < kCluster && !near; ++a) {
+          const float4 xi = s_xi[wave][m * kCluster + a];
+#pragma unroll
+          for (int b = 0; b < kCluster; ++b) {
+            const float dx = xi.x - xj[b].x, dy = xi.y - xj[b].y, dz = xi.z - xj[b].z;
+            near |= dx * dx + dy * dy + dz * dz < reach2;
+          }
+        }
+        if (near) inner |= 1u << m;
+      }
+    }
+    const bool keep = inner != 0u;
+    const uint64_t ballot = __ballot(keep);
+    if (keep) {
+      PairEntry o = E;
+      o.info = (E.info & ~0xF00u) | (inner << 8);
+      out[count + __popcll(ballot & ((1ull << lane) - 1ull))] = o;
+    }
+    count += __popcll(ballot);
+  }
+  if (lane == 0) row_count_in[gi] = count;
 }
 
 // ---- exports for parity tests -------------------------------------------

@@ -269,7 +269,10 @@ __global__ void __launch_bounds__(256) nb_pairs(
   const int nt2 = p.ntypes * p.ntypes;
   for (int k = threadIdx.x; k < nt2; k += blockDim.x) s_lj[k] = ljtab[k];
   // The build kernels of this step have finished: clear the request.
-  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) flags[kFlagRebuild] = 0;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+    flags[kFlagRebuild] = 0;
+    flags[kFlagPrune] = 0;
+  }
   __syncthreads();
   if (blockIdx.y == 0) {  // the excluded and 1-4 pairs
     if ((int)blockIdx.x * (int)blockDim.x < sa.npairs) special_block(blockIdx.x, sa, F, sm.e);

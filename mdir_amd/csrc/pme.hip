@@ -110,11 +110,85 @@ template <int TL> struct BrickShape {
   static constexpr int BX = R / BY;
 };
 
+// The column slot holding flattened atom index i: the last slot whose
+// cumulative count is <= i.
+__device__ __forceinline__ int column_slot(const int *s_cum, int ncand, int i) {
+  int lo = 0, hi = ncand - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (s_cum[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// The part of one atom's stencil (place-ordered position and charge pq)
+// that lies in the brick at (bx0, by0), added to the brick's accumulators.
+template <int N, int BX, int BY>
+__device__ __forceinline__ void spread_atom(double4 pq, int cxy, const PmeParams &pp,
+                                            const BrickParams &bp, int bx0, int by0,
+                                            unsigned long long *acc, int *flags) {
+  const int K0 = pp.K[0], K1 = pp.K[1], K2 = pp.K[2];
+  const double qd = pq.w;
+  if (qd == 0.0) return;  // padding, or an uncharged atom
+  int f[3];
+  float w[3];
+  pme_place(pq.x, pp.box[0], pp.inv_box[0], K0, N, f[0], w[0]);
+  pme_place(pq.y, pp.box[1], pp.inv_box[1], K1, N, f[1], w[1]);
+  // an atom outside its column's tile would be missed by some brick
+  const int cx = cxy & 0xffff, cy = cxy >> 16;
+  int fx = f[0] - bp.gx0[cx];
+  if (fx < 0) fx += K0;
+  int fy = f[1] - bp.gy0[cy];
+  if (fy < 0) fy += K1;
+  if ((bp.TWx < K0 && fx > bp.TWx - N) || (bp.TWy < K1 && fy > bp.TWy - N))
+    atomicOr(&flags[kFlagPmeRange], 1);
+  // atoms whose stencil misses the brick end here
+  int kx0 = bx0 - f[0];
+  if (kx0 < 0) kx0 += K0;
+  int ky0 = by0 - f[1];
+  if (ky0 < 0) ky0 += K1;
+  if (!(kx0 < N || kx0 > K0 - BX) || !(ky0 < N || ky0 > K1 - BY)) return;
+  pme_place(pq.z, pp.box[2], pp.inv_box[2], K2, N, f[2], w[2]);
+  float M[3][N];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) bspline<float>(w[d], N, M[d], nullptr);
+  const float qf = (float)qd;
+#pragma unroll
+  for (int tx = 0; tx < BX; ++tx) {
+    int kx = bx0 + tx - f[0];
+    if (kx < 0) kx += K0;
+    if (kx >= N || bx0 + tx >= K0) continue;
+    float wx = M[0][0];
+#pragma unroll
+    for (int k = 1; k < N; ++k)
+      if (k == kx) wx = M[0][k];
+    wx *= qf;
+#pragma unroll
+    for (int ty = 0; ty < BY; ++ty) {
+      int ky = by0 + ty - f[1];
+      if (ky < 0) ky += K1;
+      if (ky >= N || by0 + ty >= K1) continue;
+      float wxy = M[1][0];
+#pragma unroll
+      for (int k = 1; k < N; ++k)
+        if (k == ky) wxy = M[1][k];
+      wxy *= wx;
+      unsigned long long *row = acc + (tx * BY + ty) * K2;
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        int z = f[2] + k;
+        if (z >= K2) z -= K2;
+        const long long v = __double2ll_rn((double)(wxy * M[2][k]) * kGridScale);
+        atomicAdd(row + z, (unsigned long long)v);
+      }
+    }
+  }
+}
+
 template <int N, int TL>
 __global__ void __launch_bounds__(256) pme_spread_rows(
     const int *__restrict__ col_cl_start, const int *__restrict__ col_cl_count,
-    const int *__restrict__ atom_at, const double *__restrict__ x,
-    const double *__restrict__ q, PmeParams pp, BrickParams bp,
+    const double4 *__restrict__ xpq, PmeParams pp, BrickParams bp,
     cpx<float> *__restrict__ out, FftAxis az, int *__restrict__ flags) {
   constexpr int R = BrickShape<TL>::R, BX = BrickShape<TL>::BX, BY = BrickShape<TL>::BY;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -158,71 +232,16 @@ __global__ void __launch_bounds__(256) pme_spread_rows(
   }
   __syncthreads();
 
+  // two atoms per thread in flight: their loads are independent
   const int total = s_cum[ncand];
-  for (int i = threadIdx.x; i < total; i += blockDim.x) {
-    int lo = 0, hi = ncand - 1;  // last column with s_cum <= i
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (s_cum[mid] <= i) lo = mid; else hi = mid - 1;
-    }
-    const int atom = atom_at[s_p0[lo] + (i - s_cum[lo])];
-    if (atom < 0) continue;
-    const double qd = q[atom];
-    if (qd == 0.0) continue;
-    int f[3];
-    float w[3];
-#pragma unroll
-    for (int d = 0; d < 2; ++d)
-      pme_place(x[3 * atom + d], pp.box[d], pp.inv_box[d], pp.K[d], N, f[d], w[d]);
-    // an atom outside its column's tile would be missed by some brick
-    const int cx = s_cxy[lo] & 0xffff, cy = s_cxy[lo] >> 16;
-    int fx = f[0] - bp.gx0[cx];
-    if (fx < 0) fx += K0;
-    int fy = f[1] - bp.gy0[cy];
-    if (fy < 0) fy += K1;
-    if ((bp.TWx < K0 && fx > bp.TWx - N) || (bp.TWy < K1 && fy > bp.TWy - N))
-      atomicOr(&flags[kFlagPmeRange], 1);
-    // atoms whose stencil misses the brick end here
-    int kx0 = bx0 - f[0];
-    if (kx0 < 0) kx0 += K0;
-    int ky0 = by0 - f[1];
-    if (ky0 < 0) ky0 += K1;
-    if (!(kx0 < N || kx0 > K0 - BX) || !(ky0 < N || ky0 > K1 - BY)) continue;
-    pme_place(x[3 * atom + 2], pp.box[2], pp.inv_box[2], pp.K[2], N, f[2], w[2]);
-    float M[3][N];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) bspline<float>(w[d], N, M[d], nullptr);
-    const float qf = (float)qd;
-#pragma unroll
-    for (int tx = 0; tx < BX; ++tx) {
-      int kx = bx0 + tx - f[0];
-      if (kx < 0) kx += K0;
-      if (kx >= N || bx0 + tx >= K0) continue;
-      float wx = M[0][0];
-#pragma unroll
-      for (int k = 1; k < N; ++k)
-        if (k == kx) wx = M[0][k];
-      wx *= qf;
-#pragma unroll
-      for (int ty = 0; ty < BY; ++ty) {
-        int ky = by0 + ty - f[1];
-        if (ky < 0) ky += K1;
-        if (ky >= N || by0 + ty >= K1) continue;
-        float wxy = M[1][0];
-#pragma unroll
-        for (int k = 1; k < N; ++k)
-          if (k == ky) wxy = M[1][k];
-        wxy *= wx;
-        unsigned long long *row = acc + (tx * BY + ty) * K2;
-#pragma unroll
-        for (int k = 0; k < N; ++k) {
-          int z = f[2] + k;
-          if (z >= K2) z -= K2;
-          const long long v = __double2ll_rn((double)(wxy * M[2][k]) * kGridScale);
-          atomicAdd(row + z, (unsigned long long)v);
-        }
-      }
-    }
+  for (int i0 = threadIdx.x; i0 < total; i0 += 2 * blockDim.x) {
+    const int i1 = i0 + blockDim.x;
+    const int c0 = column_slot(s_cum, ncand, i0);
+    const int c1 = i1 < total ? column_slot(s_cum, ncand, i1) : c0;
+    const double4 a0 = xpq[s_p0[c0] + (i0 - s_cum[c0])];
+    const double4 a1 = i1 < total ? xpq[s_p0[c1] + (i1 - s_cum[c1])] : make_double4(0, 0, 0, 0);
+    spread_atom<N, BX, BY>(a0, s_cxy[c0], pp, bp, bx0, by0, acc, flags);
+    spread_atom<N, BX, BY>(a1, s_cxy[c1], pp, bp, bx0, by0, acc, flags);
   }
   __syncthreads();
   // rows 2t and 2t+1 of the brick as the real and imaginary parts of line t
@@ -265,11 +284,85 @@ __global__ void __launch_bounds__(256) pme_spread_rows(
 // from the brick's points, as 64-bit fixed point (each part is a fixed f32
 // sum, so the total does not depend on the order of the adds).  The real
 // potential grid never goes to HBM.
+// One atom's force from the potential rows of the brick at (bx0, by0):
+// -q (K/L) sum phi M' M M over the stencil points in the brick, added as
+// 64-bit fixed point.  phi of brick row r at z is phi[z TL + r/2], .x for
+// even r and .y for odd r.
+template <int N, int TL, int BX, int BY>
+__device__ __forceinline__ void gather_atom(double4 pq, int place, const PmeParams &pp,
+                                            int bx0, int by0, const cpx<float> *phi,
+                                            long long *__restrict__ F) {
+  const int K0 = pp.K[0], K1 = pp.K[1], K2 = pp.K[2];
+  const double qd = pq.w;
+  if (qd == 0.0) return;
+  int f[3];
+  float w[3];
+  pme_place(pq.x, pp.box[0], pp.inv_box[0], K0, N, f[0], w[0]);
+  pme_place(pq.y, pp.box[1], pp.inv_box[1], K1, N, f[1], w[1]);
+  int kx0 = bx0 - f[0];
+  if (kx0 < 0) kx0 += K0;
+  int ky0 = by0 - f[1];
+  if (ky0 < 0) ky0 += K1;
+  if (!(kx0 < N || kx0 > K0 - BX) || !(ky0 < N || ky0 > K1 - BY)) return;
+  pme_place(pq.z, pp.box[2], pp.inv_box[2], K2, N, f[2], w[2]);
+  float M[3][N], dM[3][N];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) bspline<float>(w[d], N, M[d], dM[d]);
+  float g0 = 0.0f, g1 = 0.0f, g2 = 0.0f;
+  bool any = false;
+#pragma unroll
+  for (int tx = 0; tx < BX; ++tx) {
+    int kx = bx0 + tx - f[0];
+    if (kx < 0) kx += K0;
+    if (kx >= N || bx0 + tx >= K0) continue;
+    float mx = M[0][0], dmx = dM[0][0];
+#pragma unroll
+    for (int k = 1; k < N; ++k)
+      if (k == kx) {
+        mx = M[0][k];
+        dmx = dM[0][k];
+      }
+#pragma unroll
+    for (int ty = 0; ty < BY; ++ty) {
+      int ky = by0 + ty - f[1];
+      if (ky < 0) ky += K1;
+      if (ky >= N || by0 + ty >= K1) continue;
+      float my = M[1][0], dmy = dM[1][0];
+#pragma unroll
+      for (int k = 1; k < N; ++k)
+        if (k == ky) {
+          my = M[1][k];
+          dmy = dM[1][k];
+        }
+      const int r = tx * BY + ty;
+      float sz = 0.0f, dsz = 0.0f;
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        int z = f[2] + k;
+        if (z >= K2) z -= K2;
+        const cpx<float> c = phi[z * TL + (r >> 1)];
+        const float pv = (r & 1) ? c.y : c.x;
+        sz += pv * M[2][k];
+        dsz += pv * dM[2][k];
+      }
+      g0 += dmx * my * sz;
+      g1 += mx * dmy * sz;
+      g2 += mx * my * dsz;
+      any = true;
+    }
+  }
+  if (!any) return;
+  const float qf = (float)qd;
+  unsigned long long *u = reinterpret_cast<unsigned long long *>(F + 3 * place);
+  atomicAdd(u + 0, (unsigned long long)to_fixed(-qf * (float)(K0 * pp.inv_box[0]) * g0));
+  atomicAdd(u + 1, (unsigned long long)to_fixed(-qf * (float)(K1 * pp.inv_box[1]) * g1));
+  atomicAdd(u + 2, (unsigned long long)to_fixed(-qf * (float)(K2 * pp.inv_box[2]) * g2));
+}
+
 template <int N, int TL>
 __global__ void __launch_bounds__(256) pme_gather_rows(
     const int *__restrict__ col_cl_start, const int *__restrict__ col_cl_count,
-    const int *__restrict__ atom_at, const double *__restrict__ x,
-    const double *__restrict__ q, PmeParams pp, BrickParams bp,
+    const double4 *__restrict__ xpq, PmeParams pp, BrickParams bp,
     const cpx<float> *__restrict__ in, FftAxis az, long long *__restrict__ F) {
   constexpr int BX = BrickShape<TL>::BX, BY = BrickShape<TL>::BY;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -335,83 +428,16 @@ __global__ void __launch_bounds__(256) pme_gather_rows(
   const cpx<float> *phi = lds_fft<float, TL>(a, b, az, +1);
   // phi of brick row r at z: phi[z * TL + r / 2], .x for even r, .y for odd
   const int total = s_cum[ncand];
-  for (int i = threadIdx.x; i < total; i += blockDim.x) {
-    int lo = 0, hi = ncand - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (s_cum[mid] <= i) lo = mid; else hi = mid - 1;
-    }
-    const int place = s_p0[lo] + (i - s_cum[lo]);
-    const int atom = atom_at[place];
-    if (atom < 0) continue;
-    const double qd = q[atom];
-    if (qd == 0.0) continue;
-    int f[3];
-    float w[3];
-#pragma unroll
-    for (int d = 0; d < 2; ++d)
-      pme_place(x[3 * atom + d], pp.box[d], pp.inv_box[d], pp.K[d], N, f[d], w[d]);
-    // atoms whose stencil misses the brick end here
-    int kx0 = bx0 - f[0];
-    if (kx0 < 0) kx0 += K0;
-    int ky0 = by0 - f[1];
-    if (ky0 < 0) ky0 += K1;
-    const bool hit_x = kx0 < N || kx0 > K0 - BX;
-    const bool hit_y = ky0 < N || ky0 > K1 - BY;
-    if (!hit_x || !hit_y) continue;
-    pme_place(x[3 * atom + 2], pp.box[2], pp.inv_box[2], pp.K[2], N, f[2], w[2]);
-    float M[3][N], dM[3][N];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) bspline<float>(w[d], N, M[d], dM[d]);
-    float g0 = 0.0f, g1 = 0.0f, g2 = 0.0f;
-    bool any = false;
-#pragma unroll
-    for (int tx = 0; tx < BX; ++tx) {
-      int kx = bx0 + tx - f[0];
-      if (kx < 0) kx += K0;
-      if (kx >= N || bx0 + tx >= K0) continue;
-      float mx = M[0][0], dmx = dM[0][0];
-#pragma unroll
-      for (int k = 1; k < N; ++k)
-        if (k == kx) {
-          mx = M[0][k];
-          dmx = dM[0][k];
-        }
-#pragma unroll
-      for (int ty = 0; ty < BY; ++ty) {
-        int ky = by0 + ty - f[1];
-        if (ky < 0) ky += K1;
-        if (ky >= N || by0 + ty >= K1) continue;
-        float my = M[1][0], dmy = dM[1][0];
-#pragma unroll
-        for (int k = 1; k < N; ++k)
-          if (k == ky) {
-            my = M[1][k];
-            dmy = dM[1][k];
-          }
-        const int r = tx * BY + ty;
-        float sz = 0.0f, dsz = 0.0f;
-#pragma unroll
-        for (int k = 0; k < N; ++k) {
-          int z = f[2] + k;
-          if (z >= K2) z -= K2;
-          const cpx<float> c = phi[z * TL + (r >> 1)];
-          const float pv = (r & 1) ? c.y : c.x;
-          sz += pv * M[2][k];
-          dsz += pv * dM[2][k];
-        }
-        g0 += dmx * my * sz;
-        g1 += mx * dmy * sz;
-        g2 += mx * my * dsz;
-        any = true;
-      }
-    }
-    if (!any) continue;
-    const float qf = (float)qd;
-    unsigned long long *u = reinterpret_cast<unsigned long long *>(F + 3 * place);
-    atomicAdd(u + 0, (unsigned long long)to_fixed(-qf * (float)(pp.K[0] * pp.inv_box[0]) * g0));
-    atomicAdd(u + 1, (unsigned long long)to_fixed(-qf * (float)(pp.K[1] * pp.inv_box[1]) * g1));
-    atomicAdd(u + 2, (unsigned long long)to_fixed(-qf * (float)(pp.K[2] * pp.inv_box[2]) * g2));
+  for (int i0 = threadIdx.x; i0 < total; i0 += 2 * blockDim.x) {
+    const int i1 = i0 + blockDim.x;
+    const int c0 = column_slot(s_cum, ncand, i0);
+    const int c1 = i1 < total ? column_slot(s_cum, ncand, i1) : c0;
+    const int p0 = s_p0[c0] + (i0 - s_cum[c0]);
+    const int p1 = i1 < total ? s_p0[c1] + (i1 - s_cum[c1]) : p0;
+    const double4 a0 = xpq[p0];
+    const double4 a1 = i1 < total ? xpq[p1] : make_double4(0, 0, 0, 0);
+    gather_atom<N, TL, BX, BY>(a0, p0, pp, bx0, by0, phi, F);
+    gather_atom<N, TL, BX, BY>(a1, p1, pp, bx0, by0, phi, F);
   }
 }
 
@@ -731,14 +757,13 @@ __global__ void __launch_bounds__(512) fft_plane_conv(cpx<float> *__restrict__ d
 
 template __global__ void fft_rows_r2c<float, true>(const void *, cpx<float> *, int, FftAxis);
 #define MDHIP_SPREAD(N, TL)                                                              \
-  template __global__ void pme_spread_rows<N, TL>(const int *, const int *, const int *, \
-      const double *, const double *, PmeParams, BrickParams, cpx<float> *, FftAxis, int *);
+  template __global__ void pme_spread_rows<N, TL>(const int *, const int *,              \
+      const double4 *, PmeParams, BrickParams, cpx<float> *, FftAxis, int *);
 #define MDHIP_SPREAD_N(N) MDHIP_SPREAD(N, 8) MDHIP_SPREAD(N, 4) MDHIP_SPREAD(N, 2) MDHIP_SPREAD(N, 1) \
   MDHIP_GATHER(N, 8) MDHIP_GATHER(N, 4) MDHIP_GATHER(N, 2) MDHIP_GATHER(N, 1)
 #define MDHIP_GATHER(N, TL)                                                              \
-  template __global__ void pme_gather_rows<N, TL>(const int *, const int *, const int *, \
-      const double *, const double *, PmeParams, BrickParams, const cpx<float> *, FftAxis, \
-      long long *);
+  template __global__ void pme_gather_rows<N, TL>(const int *, const int *,              \
+      const double4 *, PmeParams, BrickParams, const cpx<float> *, FftAxis, long long *);
 MDHIP_SPREAD_N(4)
 MDHIP_SPREAD_N(6)
 MDHIP_SPREAD_N(8)

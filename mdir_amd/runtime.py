@@ -35,7 +35,8 @@ class SystemDesc(C.Structure):
                 ("pme_order", I32), ("nexclusions", I64), ("exclusions", PI32),
                 ("npairs14", I64), ("pairs14", PI32), ("pairs14_sigma", PD),
                 ("pairs14_eps", PD), ("pairs14_qq", PD), ("nwaters", I64),
-                ("water_first", PI32), ("d_oh", D), ("d_hh", D), ("skin", D)]
+                ("water_first", PI32), ("d_oh", D), ("d_hh", D), ("skin", D),
+                ("prune_skin", D)]
 
 
 class Buffer1(C.Structure):

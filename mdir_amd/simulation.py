@@ -16,7 +16,7 @@ from .systems import System
 
 
 class Simulation:
-    def __init__(self, system: System, skin: float = 0.1):
+    def __init__(self, system: System, skin: float = 0.1, prune_skin: float = 0.0):
         self.system = system
         s = system
         L = rt.lib()
@@ -63,6 +63,7 @@ class Simulation:
         d.water_first = rt.ptr(w, C.c_int32)
         d.d_oh, d.d_hh = s.water_geometry
         d.skin = skin
+        d.prune_skin = prune_skin
         self._desc = d
         h = L.mdrtHipSystemCreate(C.byref(d))
         if not h:
@@ -149,10 +150,10 @@ class Simulation:
         return out[:count.value].copy()
 
     def stats(self) -> dict:
-        s = np.zeros(8, np.int64)
+        s = np.zeros(10, np.int64)
         rt.check(rt.lib().mdrtHipStats(self._h, rt.ptr(s, C.c_int64)))
         keys = ["places", "clusters", "tiles", "largest_row", "row_cap",
-                "columns", "column_cap", "builds"]
+                "columns", "column_cap", "builds", "pair_loop_tiles", "prunes"]
         return dict(zip(keys, (int(v) for v in s)))
 
     PROFILE_KERNELS = ["integrate_first", "nb_bin", "nb_sort", "nb_search",

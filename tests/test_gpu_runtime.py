@@ -91,14 +91,16 @@ def test_memory_pool_and_flags():
     assert L.mdrtFlagTake(flag.aligned) == 0
 
 
-def test_nve_settle_water():
+@pytest.mark.parametrize("skin,prune", [(0.1, 0.0), (0.2, 0.05)])
+def test_nve_settle_water(skin, prune):
     """NVE with SETTLE: constraints hold to round-off, and the conserved
     energy shows no drift beyond its fluctuation (slope of a least-squares
-    line through 21 samples over 2,000 steps after 200 of equilibration)."""
-    say("case: NVE TIP3P 512 waters, 2,200 steps of 2 fs")
+    line through 21 samples over 2,000 steps after 200 of equilibration),
+    with the single list and with the pruned dual list."""
+    say(f"case: NVE TIP3P 512 waters, 2,200 steps of 2 fs, skin {skin}, pruned {prune}")
     from mdir_amd.simulation import Simulation
     w = S.tip3p_box(512, 2.5, seed=21, grid=(24, 24, 24))
-    sim = Simulation(w)
+    sim = Simulation(w, skin=skin, prune_skin=prune)
     sim.minimize(50, 0.01)
     relaxed = w.copy()
     relaxed.positions = sim.positions
