@@ -252,10 +252,11 @@ int mdrtHipStats(void *system, int64_t *stats /* 10 */);
 /* Device-event timing of each kernel of a step on the stream it runs on,
    ms per launch averaged over `repeats` steps (the steps advance the
    state): integrate_first, nb_bin, nb_sort, nb_search, nb_pairs (with the
-   excluded and 1-4 pairs), pme_spread_rows (spread + forward z transform),
-   pme_yx (y and x transforms with the influence product), pme_gather_rows
-   (backward z transform + gather), integrate_second. */
-int mdrtHipProfileStep(void *system, int32_t repeats, double *ms /* 9 */);
+   excluded and 1-4 pairs), pme_bin (atoms listed per brick),
+   pme_spread_rows (spread + forward z transform), pme_yx (y and x transforms
+   with the influence product), pme_gather_rows (backward z transform +
+   gather), integrate_second. */
+int mdrtHipProfileStep(void *system, int32_t repeats, double *ms /* 10 */);
 
 #ifdef __cplusplus
 }

@@ -166,6 +166,12 @@ struct Engine {
   double e_self = 0, e_disp = 0;
 
   hipStream_t stream = nullptr;
+  // The PME chain runs on a second stream, concurrently with the pair loop
+  // (both add integer force words, so the order does not matter); events
+  // fork it after the list build and join it before the integration.
+  hipStream_t side = nullptr;
+  hipStream_t pstream = nullptr;   // where the PME launches go
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
   // ---- state (atom order) ----
   double *d_x = nullptr, *d_v = nullptr, *d_xref = nullptr, *d_shift = nullptr;
@@ -215,9 +221,7 @@ struct Engine {
   bool use_pme = false;
   PmeParams pmp{};
   Fft3D<float> fft;
-  unsigned long long *d_grid = nullptr;
   cpx<float> *d_cplx = nullptr;
-  float *d_phi = nullptr;
   float *d_tab = nullptr;
   ConvTables ct{};
   double *d_erecip = nullptr;
@@ -258,9 +262,14 @@ struct Engine {
     dfree(d_special); dfree(d_is14); dfree(d_isexcl); dfree(d_p14_sigma);
     dfree(d_p14_eps); dfree(d_p14_qq); dfree(d_espec);
     fft.release();
-    dfree(d_grid); dfree(d_cplx); dfree(d_phi); dfree(d_tab); dfree(d_erecip);
-    dfree(d_brick_idx);
+    dfree(d_cplx); dfree(d_tab); dfree(d_erecip);
+    dfree(bp.count); dfree(bp.list);
     dfree(d_unit_first); dfree(d_unit_size);
+    if (ev_fork) hipEventDestroy(ev_fork);
+    if (ev_join) hipEventDestroy(ev_join);
+    ev_fork = ev_join = nullptr;
+    if (side) hipStreamDestroy(side);
+    side = nullptr;
     if (stream) hipStreamDestroy(stream);
     stream = nullptr;
   }
@@ -382,6 +391,10 @@ struct Engine {
         throw HipError("every box edge must exceed 2 (cutoff + skin)");
     if (lj_modifier < 0 || lj_modifier > 3) throw HipError("unknown Lennard-Jones modifier");
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    pstream = side;
 
     h_q.assign(d->charges, d->charges + n);
     h_mass.assign(d->masses, d->masses + n);
@@ -565,10 +578,8 @@ struct Engine {
         throw HipError("PME order must be 4, 6 or 8");
       pmp.order = pme_order;
       fft.make(pme_grid);
-      const size_t np = (size_t)pme_grid[0] * pme_grid[1] * pme_grid[2];
       const size_t nh = (size_t)pme_grid[0] * pme_grid[1] * (pme_grid[2] / 2 + 1);
       d_cplx = dalloc<cpx<float>>(nh);
-      d_phi = dalloc<float>(np);
       make_influence_tables();
       d_erecip = dalloc<double>(std::max(fft.x_blocks(), fft.H()));
     }
@@ -736,80 +747,57 @@ struct Engine {
   }
 
 
-  // ---- bricks of the spread ----
+  // ---- bricks of the spread and gather ----
   BrickParams bp{};
-  int *d_brick_idx = nullptr;
   int brick_tl = 8;            // complex lines of a brick's z transform
+  int nbricks = 0;
 
-  // Tiles of the columns (the grid points their atoms' stencils can cover
-  // while the list is valid) and, per brick along x and y, the columns whose
-  // tiles overlap it.
-  void make_bricks() {
-    dfree(d_brick_idx);
-    const double h = 0.5 * skin;
+  // Brick shape from the z length (LDS of 2 TL K3 64-bit accumulators, two
+  // TL K3 complex buffers and the K3 twiddles within 60 KB) and the
+  // per-brick atom lists,
+  // sized from the mean count with room for fluctuations (grown on
+  // overflow).
+  void make_bricks(int cap_min = 0) {
     const int K2 = pme_grid[2];
-    // complex lines per brick: LDS of 2 TL K2 64-bit accumulators plus two
-    // TL K2 complex buffers within 60 KB
     brick_tl = 8;
-    while (brick_tl > 1 && (size_t)32 * brick_tl * K2 > 60 * 1024) brick_tl /= 2;
-    if ((size_t)32 * brick_tl * K2 > 60 * 1024) throw HipError("PME grid too long along z");
+    while (brick_tl > 1 && (size_t)(32 * brick_tl + 8) * K2 > 60 * 1024) brick_tl /= 2;
+    if ((size_t)(32 * brick_tl + 8) * K2 > 60 * 1024) throw HipError("PME grid too long along z");
     const int R = 2 * brick_tl, BY = R >= 8 ? 4 : 2, BX = R / BY;
-    const int B[2] = {BX, BY};
-    int TW[2];
-    std::vector<int> g0[2], start[2], cols[2];
-    for (int a = 0; a < 2; ++a) {
-      const int nc = a == 0 ? g.ncx : g.ncy;
-      const double w = a == 0 ? g.wx : g.wy;
-      const int K = pme_grid[a];
-      const double L = box[a];
-      int tw = 0;
-      g0[a].resize(nc);
-      for (int c = 0; c < nc; ++c) {
-        const long lo = (long)floor(K / L * (c * w - h)) - pme_order;      // one point of margin
-        const long hi = (long)floor(K / L * ((c + 1) * w + h)) + 1;
-        g0[a][c] = (int)(((lo % K) + K) % K);
-        tw = std::max(tw, (int)(hi - lo + 1));
-      }
-      if (tw >= K) tw = K;
-      TW[a] = tw;
-      const int nb = (K + B[a] - 1) / B[a];
-      start[a].assign(1, 0);
-      for (int bi = 0; bi < nb; ++bi) {
-        for (int c = 0; c < nc; ++c) {
-          bool hit = false;
-          for (int o = 0; o < tw && !hit; ++o) {
-            const int pnt = (g0[a][c] + o) % K;
-            hit = pnt >= bi * B[a] && pnt < (bi + 1) * B[a];
-          }
-          if (hit) cols[a].push_back(c);
-        }
-        start[a].push_back((int)cols[a].size());
-      }
-    }
-    for (size_t bx = 0; bx + 1 < start[0].size(); ++bx)
-      for (size_t by = 0; by + 1 < start[1].size(); ++by)
-        if ((start[0][bx + 1] - start[0][bx]) * (start[1][by + 1] - start[1][by]) > kBrickMaxCols)
-          throw HipError("PME grid too coarse for the column decomposition");
-    std::vector<int> idx;
-    auto put = [&](const std::vector<int> &v) {
-      const size_t off = idx.size();
-      idx.insert(idx.end(), v.begin(), v.end());
-      return off;
-    };
-    const size_t o_gx = put(g0[0]), o_gy = put(g0[1]);
-    const size_t o_xs = put(start[0]), o_xc = put(cols[0]);
-    const size_t o_ys = put(start[1]), o_yc = put(cols[1]);
-    d_brick_idx = dupload(idx);
-    bp.ncx = g.ncx;
-    bp.TWx = TW[0];
-    bp.TWy = TW[1];
-    bp.gx0 = d_brick_idx + o_gx;
-    bp.gy0 = d_brick_idx + o_gy;
-    bp.xstart = d_brick_idx + o_xs;
-    bp.xcols = d_brick_idx + o_xc;
-    bp.ystart = d_brick_idx + o_ys;
-    bp.ycols = d_brick_idx + o_yc;
+    bp.nbx = (pme_grid[0] + BX - 1) / BX;
+    bp.nby = (pme_grid[1] + BY - 1) / BY;
+    nbricks = bp.nbx * bp.nby;
+    const double touched = ((pme_order - 1.0) / BX + 1.0) * ((pme_order - 1.0) / BY + 1.0);
+    const double mean = n * touched / nbricks;
+    int cap = (int)(1.5 * mean + 128.0);
+    cap = std::max(cap, cap_min);
+    cap = (cap + 63) & ~63;
+    dfree(bp.count);
+    dfree(bp.list);
+    bp.cap = cap;
+    bp.count = dalloc<int>(nbricks);
+    bp.list = dalloc<int>((size_t)nbricks * cap);
     bp.plane_major = plane_fft() ? 1 : 0;
+  }
+
+  // Lists the atoms of every brick their stencils touch.
+  template <int N, int TL>
+  void bin_t() {
+    pme_bin<N, TL><<<blocks_for(max_places), 256, 0, pstream>>>(d_counts, d_xpq, pmp, bp,
+                                                               d_flags);
+  }
+  template <int N>
+  void bin_n() {
+    switch (brick_tl) {
+      case 8: bin_t<N, 8>(); break;
+      case 4: bin_t<N, 4>(); break;
+      case 2: bin_t<N, 2>(); break;
+      default: bin_t<N, 1>(); break;
+    }
+  }
+  void pme_bin_launch() {
+    if (pme_order == 4) bin_n<4>();
+    else if (pme_order == 6) bin_n<6>();
+    else bin_n<8>();
   }
 
   template <int N, int TL>
@@ -817,10 +805,9 @@ struct Engine {
     constexpr int R = BrickShape<TL>::R, BX = BrickShape<TL>::BX, BY = BrickShape<TL>::BY;
     static_assert(BX * BY == R, "brick shape");
     const int blocks = ((pme_grid[0] + BX - 1) / BX) * ((pme_grid[1] + BY - 1) / BY);
-    const size_t lds = (size_t)32 * TL * pme_grid[2];
-    pme_spread_rows<N, TL><<<blocks, 256, lds, stream>>>(d_col_cl_start, d_col_cl_count,
-                                                         d_xpq, pmp, bp, d_cplx,
-                                                         fft.ax[2].axis, d_flags);
+    const size_t lds = (size_t)(32 * TL + 8) * pme_grid[2];
+    pme_spread_rows<N, TL><<<blocks, 256, lds, pstream>>>(d_xpq, pmp, bp, d_cplx,
+                                                         fft.ax[2].axis);
   }
   template <int N>
   void spread_rows_n() {
@@ -835,9 +822,8 @@ struct Engine {
   void gather_rows_t() {
     constexpr int BX = BrickShape<TL>::BX, BY = BrickShape<TL>::BY;
     const int blocks = ((pme_grid[0] + BX - 1) / BX) * ((pme_grid[1] + BY - 1) / BY);
-    const size_t lds = (size_t)16 * TL * pme_grid[2];
-    pme_gather_rows<N, TL><<<blocks, 256, lds, stream>>>(d_col_cl_start, d_col_cl_count,
-                                                         d_xpq, pmp, bp, d_cplx,
+    const size_t lds = (size_t)(16 * TL + 8) * pme_grid[2];
+    pme_gather_rows<N, TL><<<blocks, 256, lds, pstream>>>(d_xpq, pmp, bp, d_cplx,
                                                          fft.ax[2].axis, d_F);
   }
   template <int N>
@@ -863,54 +849,51 @@ struct Engine {
     else spread_rows_n<8>();
   }
 
-  void pme_gather_launch() {
-    if (pme_order == 4)
-      pme_gather_lanes<4, 4><<<blocks_for((size_t)max_places * 4), 256, 0, stream>>>(d_counts, d_atom_at, d_x, d_q, pmp, d_phi, d_F);
-    else if (pme_order == 6)
-      pme_gather_lanes<6, 8><<<blocks_for((size_t)max_places * 8), 256, 0, stream>>>(d_counts, d_atom_at, d_x, d_q, pmp, d_phi, d_F);
-    else
-      pme_gather_lanes<8, 8><<<blocks_for((size_t)max_places * 8), 256, 0, stream>>>(d_counts, d_atom_at, d_x, d_q, pmp, d_phi, d_F);
-  }
 
   // The y and x passes with the influence product: one kernel per kz plane
   // when a plane fits in LDS, else the y pass, the fused x pass and the y
   // pass again over tiles of lines.
-  bool plane_fft() const { return use_pme && pme_grid[0] * pme_grid[1] <= kPlaneMax; }
+  bool plane_fft() const {
+    return use_pme && pme_grid[0] * pme_grid[1] <= kPlaneMax &&
+           pme_grid[0] <= kPlaneAxisMax && pme_grid[1] <= kPlaneAxisMax;
+  }
   static constexpr int kPlaneMax = 4096;
   int recip_blocks() const { return plane_fft() ? fft.H() : fft.x_blocks(); }
   void pme_yx(bool energy) {
     double *ep = energy ? d_erecip : nullptr;
     if (plane_fft()) {
       const size_t lds = (size_t)2 * pme_grid[0] * pme_grid[1] * sizeof(cpx<float>);
-      fft_plane_conv<<<fft.H(), 512, lds, stream>>>(d_cplx, fft.K[0], fft.K[1], fft.ax[0].axis,
+      fft_plane_conv<<<fft.H(), 512, lds, pstream>>>(d_cplx, fft.K[0], fft.K[1], fft.ax[0].axis,
                                                      fft.ax[1].axis, ct, ep);
       return;
     }
     const int TL = Fft3D<float>::TL;
     const int tiles = (fft.H() + TL - 1) / TL;
-    fft_cols_y<float><<<fft.K[0] * tiles, 256, fft.lds(fft.K[1]), stream>>>(
+    fft_cols_y<float><<<fft.K[0] * tiles, 256, fft.lds(fft.K[1]), pstream>>>(
         d_cplx, fft.K[1], fft.H(), fft.ax[1].axis, -1);
-    fft.x_pass(d_cplx, 0, ct, ep, stream);
-    fft_cols_y<float><<<fft.K[0] * tiles, 256, fft.lds(fft.K[1]), stream>>>(
+    fft.x_pass(d_cplx, 0, ct, ep, pstream);
+    fft_cols_y<float><<<fft.K[0] * tiles, 256, fft.lds(fft.K[1]), pstream>>>(
         d_cplx, fft.K[1], fft.H(), fft.ax[1].axis, +1);
-  }
-  void pme_c2r() {
-    const int TL = Fft3D<float>::TL;
-    const int nrows = fft.K[0] * fft.K[1];
-    fft_rows_c2r<float><<<(nrows + 2 * TL - 1) / (2 * TL), 256, fft.lds(fft.K[2]), stream>>>(
-        d_cplx, d_phi, nrows, fft.ax[2].axis, plane_fft() ? 1 : 0);
   }
 
   void pme(bool energy) {
     if (!use_pme) return;
+    pme_bin_launch();
     pme_spread_launch();
     pme_yx(energy);
     pme_gather_rows_launch();
   }
 
+  // Pair loop on the engine's stream, PME on the side stream, joined.
   void forces(bool energy) {
+    if (use_pme) {
+      HIPCHK(hipEventRecord(ev_fork, stream));
+      HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
+      pme(energy);
+      HIPCHK(hipEventRecord(ev_join, side));
+    }
     pairs(energy);
-    pme(energy);
+    if (use_pme) HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
   }
 
   void clear_forces() {
@@ -956,9 +939,10 @@ struct Engine {
 
   // Overflow of a column bucket or a row: grow and report true.
   bool grow_if_overflowed(const std::vector<int> &f) {
-    if (f[kFlagPmeRange]) {
+    if (f[kFlagBrickOverflow]) {  // a brick listed more atoms than it holds
+      make_bricks(bp.cap * 2);
       clear_error_flags();
-      throw HipError("an atom left the PME tile of its column");
+      return true;
     }
     if (f[kFlagOverflowRow] & 2) throw HipError("more than 16 exclusions of one particle");
     if (!f[kFlagOverflowColumn] && !f[kFlagOverflowRow]) return false;
@@ -1110,8 +1094,8 @@ struct Engine {
         compute(F1.data(), E1);
         e1 = potential(E1);
       } catch (const HipError &) {
-        // a trial too wild to evaluate (an atom outside its PME tile, a
-        // list that cannot hold it) is a rejected step
+        // a trial too wild to evaluate (a list that cannot hold it) is a
+        // rejected step
         clear_error_flags();
         e1 = INFINITY;
       }
@@ -1224,13 +1208,14 @@ struct Engine {
   // Device-event timing of every kernel of one step, on the stream the
   // kernels run on: ms per launch, averaged over `repeats` steps.
   //  0 integrate_first  1 nb_bin  2 nb_sort  3 nb_search
-  //  4 nb_pairs (with the excluded and 1-4 pairs)
-  //  5 pme_spread_rows (spread + forward z transform)
-  //  6 pme_yx (y and x transforms and the influence product)
-  //  7 pme_gather_rows (backward z transform + gather)  8 integrate_second
-  static constexpr int kProfileSlots = 9;
+  //  4 nb_pairs (with the excluded and 1-4 pairs)  5 pme_bin
+  //  6 pme_spread_rows (spread + forward z transform)
+  //  7 pme_yx (y and x transforms and the influence product)
+  //  8 pme_gather_rows (backward z transform + gather)  9 integrate_second
+  static constexpr int kProfileSlots = 10;
   void profile(int repeats, double *ms) {
     if (!forces_valid) evaluate(false);
+    pstream = stream;  // kernel by kernel on one stream
     hipEvent_t ev[kProfileSlots + 1];
     for (auto &e : ev) HIPCHK(hipEventCreate(&e));
     double acc[kProfileSlots] = {0};
@@ -1246,6 +1231,8 @@ struct Engine {
       launch_search();
       HIPCHK(hipEventRecord(ev[k++], stream));
       pairs(false);
+      HIPCHK(hipEventRecord(ev[k++], stream));
+      if (use_pme) pme_bin_launch();
       HIPCHK(hipEventRecord(ev[k++], stream));
       if (use_pme) pme_spread_launch();
       HIPCHK(hipEventRecord(ev[k++], stream));
@@ -1264,6 +1251,7 @@ struct Engine {
     }
     for (int j = 0; j < kProfileSlots; ++j) ms[j] = acc[j] / repeats;
     for (auto &e : ev) hipEventDestroy(e);
+    pstream = side;
   }
 };
 

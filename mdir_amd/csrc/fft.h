@@ -132,10 +132,12 @@ __device__ __forceinline__ void stockham_pass(const cpx<T> *in, cpx<T> *out, int
 
 // Runs all passes of TL lines of length ax.K held in `a` (scratch `b`),
 // sign -1 forward / +1 backward.  Returns the buffer holding the result.
+// tw: the twiddles to read (a copy in LDS), or null for the axis' own.
 template <typename T, int TL>
-__device__ cpx<T> *lds_fft(cpx<T> *a, cpx<T> *b, const FftAxis &ax, int sign) {
+__device__ cpx<T> *lds_fft(cpx<T> *a, cpx<T> *b, const FftAxis &ax, int sign,
+                           const cpx<T> *tw = nullptr) {
   const int K = ax.K;
-  const cpx<T> *tw = static_cast<const cpx<T> *>(ax.tw);
+  if (!tw) tw = static_cast<const cpx<T> *>(ax.tw);
   int Ns = 1;
   cpx<T> *in = a, *out = b;
   for (int s = 0; s < ax.nrad; ++s) {
@@ -157,19 +159,28 @@ __device__ cpx<T> *lds_fft(cpx<T> *a, cpx<T> *b, const FftAxis &ax, int sign) {
   return in;
 }
 
-// Stockham pass over `nlines` lines whose element e sits at e * es + line * ls
-// (a plane of a 2D transform held whole in LDS).  Threads run along the
-// unit-stride index (elements when es == 1, lines otherwise), so that a
-// wave's LDS accesses are consecutive.
-template <typename T, int R>
-__device__ __forceinline__ void stockham_pass_strided(const cpx<T> *in, cpx<T> *out, int K,
-                                                      int Ns, const cpx<T> *tw, int sign,
-                                                      int nlines, int es, int ls) {
+// A K1 x K2 plane of complex numbers held whole in LDS, element (x, y) at
+// x K2 + (y ^ s(x)), s(x) = (x & 1) << 3 | (x >> 1) & 7 when 16 divides K2
+// (swz != 0), else 0: the rows a y-pass wave touches land in all 16 LDS
+// bank pairs instead of 8.
+__device__ __forceinline__ int plane_index(int x, int y, int K2, int swz) {
+  return x * K2 + (y ^ (swz & (((x & 1) << 3) | ((x >> 1) & 7))));
+}
+
+// One Stockham pass over the lines of a plane: along y (AXIS 1: K1 lines of
+// K2) or along x (AXIS 0: K2 lines of K1).  Threads run along the elements
+// of a line for y and along the lines for x, so a wave's accesses are
+// consecutive up to the swizzle.
+template <typename T, int R, int AXIS>
+__device__ __forceinline__ void stockham_pass_plane(const cpx<T> *in, cpx<T> *out, int K1,
+                                                    int K2, int swz, int Ns,
+                                                    const cpx<T> *tw, int sign) {
+  const int K = AXIS == 1 ? K2 : K1, nlines = AXIS == 1 ? K1 : K2;
   const int Kr = K / R;
   const int tstep = K / (Ns * R);
   for (int w = threadIdx.x; w < nlines * Kr; w += blockDim.x) {
     int line, j;
-    if (es == 1) {
+    if (AXIS == 1) {
       j = w % Kr;
       line = w / Kr;
     } else {
@@ -178,7 +189,10 @@ __device__ __forceinline__ void stockham_pass_strided(const cpx<T> *in, cpx<T> *
     }
     cpx<T> v[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = in[(j + r * Kr) * es + line * ls];
+    for (int r = 0; r < R; ++r) {
+      const int e = j + r * Kr;
+      v[r] = in[AXIS == 1 ? plane_index(line, e, K2, swz) : plane_index(e, line, K2, swz)];
+    }
     const int k = j % Ns;
     if (Ns > 1) {
 #pragma unroll
@@ -191,26 +205,29 @@ __device__ __forceinline__ void stockham_pass_strided(const cpx<T> *in, cpx<T> *
     butterfly<T, R>(v, sign);
     const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
-    for (int r = 0; r < R; ++r) out[(base + r * Ns) * es + line * ls] = v[r];
+    for (int r = 0; r < R; ++r) {
+      const int e = base + r * Ns;
+      out[AXIS == 1 ? plane_index(line, e, K2, swz) : plane_index(e, line, K2, swz)] = v[r];
+    }
   }
 }
 
-template <typename T>
-__device__ cpx<T> *lds_fft_strided(cpx<T> *a, cpx<T> *b, const FftAxis &ax, int sign,
-                                   int nlines, int es, int ls) {
-  const int K = ax.K;
-  const cpx<T> *tw = static_cast<const cpx<T> *>(ax.tw);
+// All passes along one axis of the plane in `a` (scratch `b`); returns the
+// buffer holding the result.  tw: the axis' twiddles (in LDS).
+template <typename T, int AXIS>
+__device__ cpx<T> *lds_fft_plane(cpx<T> *a, cpx<T> *b, const FftAxis &ax, int sign,
+                                 int K1, int K2, int swz, const cpx<T> *tw) {
   int Ns = 1;
   cpx<T> *in = a, *out = b;
   for (int s = 0; s < ax.nrad; ++s) {
     const int R = ax.rad[s];
     switch (R) {
-      case 8: stockham_pass_strided<T, 8>(in, out, K, Ns, tw, sign, nlines, es, ls); break;
-      case 4: stockham_pass_strided<T, 4>(in, out, K, Ns, tw, sign, nlines, es, ls); break;
-      case 2: stockham_pass_strided<T, 2>(in, out, K, Ns, tw, sign, nlines, es, ls); break;
-      case 3: stockham_pass_strided<T, 3>(in, out, K, Ns, tw, sign, nlines, es, ls); break;
-      case 5: stockham_pass_strided<T, 5>(in, out, K, Ns, tw, sign, nlines, es, ls); break;
-      default: stockham_pass_strided<T, 7>(in, out, K, Ns, tw, sign, nlines, es, ls); break;
+      case 8: stockham_pass_plane<T, 8, AXIS>(in, out, K1, K2, swz, Ns, tw, sign); break;
+      case 4: stockham_pass_plane<T, 4, AXIS>(in, out, K1, K2, swz, Ns, tw, sign); break;
+      case 2: stockham_pass_plane<T, 2, AXIS>(in, out, K1, K2, swz, Ns, tw, sign); break;
+      case 3: stockham_pass_plane<T, 3, AXIS>(in, out, K1, K2, swz, Ns, tw, sign); break;
+      case 5: stockham_pass_plane<T, 5, AXIS>(in, out, K1, K2, swz, Ns, tw, sign); break;
+      default: stockham_pass_plane<T, 7, AXIS>(in, out, K1, K2, swz, Ns, tw, sign); break;
     }
     __syncthreads();
     cpx<T> *t = in;

@@ -35,7 +35,7 @@ enum LJModifier { kLJNone = 0, kLJPotentialShift = 1, kLJForceSwitch = 2,
 
 // Device flags word layout (int32 each).
 enum Flag { kFlagRebuild = 0, kFlagOverflowColumn = 1, kFlagOverflowRow = 2,
-            kFlagBuilds = 3, kFlagSettleFail = 4, kFlagPmeRange = 5,
+            kFlagBuilds = 3, kFlagSettleFail = 4, kFlagBrickOverflow = 5,
             kFlagMaskCount = 6, kFlagPrune = 7, kFlagPrunes = 8, kNumFlags = 16 };
 
 constexpr int kGroup = 4;            // i-clusters of a group (one row)

@@ -74,8 +74,9 @@ __device__ __forceinline__ void pme_place(double x, double L, double invL, int K
   const double u = (double)K * frac;
   const double fl = floor(u);
   w = (float)(u - fl);
+  // fl is in [0, K] (K only when frac rounds up to 1), so f is in
+  // [1 - order, K - order + 1] and one wrap suffices
   int f = (int)fl - order + 1;
-  f %= K;
   if (f < 0) f += K;
   first = f;
 }
@@ -83,26 +84,22 @@ __device__ __forceinline__ void pme_place(double x, double L, double invL, int K
 // ---- spread fused with the forward z transform ---------------------------
 //
 // A block owns a brick of BX x BY grid rows (all of z) and produces their
-// half spectra along z.  Its atoms are those of the xy columns of the
-// cluster sort whose PME tiles overlap the brick: a column's tile is the
-// range of grid points the B-spline stencils of its atoms can cover, given
-// the column's extent, the distance an atom may move before the validity
-// test rebuilds the list, and the order.  Each atom adds the part of its
-// stencil that lies in the brick as 64-bit fixed point (2^40 per e) by LDS
-// integer atomics, so the sums do not depend on the order of the adds; the
-// brick's rows are then converted to f32 and transformed in place in LDS.
-// Every grid point belongs to one brick: no global atomics, no grid to
-// clear, no real-space grid in HBM.
+// half spectra along z.  pme_bin first lists, for every brick, the atoms
+// whose B-spline stencil touches it (thread per atom, from the atom's own
+// stencil, so no atom can be missed); each listed atom then adds the part
+// of its stencil that lies in the brick as 64-bit fixed point (2^40 per e)
+// by LDS integer atomics, so the sums do not depend on the order of the
+// adds or of the list; the brick's rows are converted to f32 and
+// transformed in place in LDS.  Every grid point belongs to one brick: no
+// global atomics on the grid, no grid to clear, no real-space grid in HBM.
 
 struct BrickParams {
-  int ncx;                  // columns of the cluster sort along x
-  int TWx, TWy;             // column tile extents (grid points)
-  const int *gx0, *gy0;     // first tile point of each column (mod K)
-  const int *xstart, *xcols;  // per brick along x: columns (x index) whose tile overlaps
-  const int *ystart, *ycols;
+  int nbx, nby;             // bricks along x and y
+  int cap;                  // list capacity per brick
+  int *count;               // atoms listed per brick; the gather clears it
+  int *list;                // places of each brick's atoms, cap per brick
   int plane_major;          // half spectrum as [kz][x][y] instead of [x][y][kz]
 };
-constexpr int kBrickMaxCols = 256;
 
 template <int TL> struct BrickShape {
   static constexpr int R = 2 * TL;                  // real rows of a brick
@@ -110,44 +107,87 @@ template <int TL> struct BrickShape {
   static constexpr int BX = R / BY;
 };
 
-// The column slot holding flattened atom index i: the last slot whose
-// cumulative count is <= i.
-__device__ __forceinline__ int column_slot(const int *s_cum, int ncand, int i) {
-  int lo = 0, hi = ncand - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (s_cum[mid] <= i) lo = mid; else hi = mid - 1;
+// The bricks along one axis that the stencil f, f+1, .., f+N-1 (mod K)
+// touches, each once.
+template <int N, int B>
+__device__ __forceinline__ int stencil_bricks(int f, int K, int *out) {
+  int n = 0;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    int p = f + k;
+    if (p >= K) p -= K;
+    const int b = p / B;
+    bool seen = false;
+    for (int j = 0; j < n; ++j) seen |= out[j] == b;
+    if (!seen) out[n++] = b;
   }
-  return lo;
+  return n;
+}
+
+// Thread per place: list the place with every brick its stencil touches.
+// Places run in column order, so a block touches few bricks: it counts its
+// entries per brick in LDS, takes one range of each brick's list with one
+// global atomic, then writes (the order within a list may vary from run to
+// run; nothing downstream depends on it).
+constexpr int kBinLdsBricks = 8192;
+template <int N, int TL>
+__global__ void __launch_bounds__(256) pme_bin(const int *__restrict__ counts,
+                                               const double4 *__restrict__ xpq,
+                                               PmeParams pp, BrickParams bp,
+                                               int *__restrict__ flags) {
+  constexpr int BX = BrickShape<TL>::BX, BY = BrickShape<TL>::BY;
+  __shared__ int s_count[kBinLdsBricks];
+  const int nbricks = bp.nbx * bp.nby;
+  const bool lds = nbricks <= kBinLdsBricks;
+  if (lds)
+    for (int k = threadIdx.x; k < nbricks; k += blockDim.x) s_count[k] = 0;
+  const int place = blockIdx.x * blockDim.x + threadIdx.x;
+  int bx[N], by[N], nx = 0, ny = 0;
+  if (place < counts[0]) {
+    const double4 pq = xpq[place];
+    if (pq.w != 0.0) {  // not padding, not an uncharged atom
+      int f0, f1;
+      float w;
+      pme_place(pq.x, pp.box[0], pp.inv_box[0], pp.K[0], N, f0, w);
+      pme_place(pq.y, pp.box[1], pp.inv_box[1], pp.K[1], N, f1, w);
+      nx = stencil_bricks<N, BX>(f0, pp.K[0], bx);
+      ny = stencil_bricks<N, BY>(f1, pp.K[1], by);
+    }
+  }
+  int local[N * N];
+  if (lds) {
+    __syncthreads();
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) local[j * N + i] = atomicAdd(&s_count[by[j] * bp.nbx + bx[i]], 1);
+    __syncthreads();
+    for (int k = threadIdx.x; k < nbricks; k += blockDim.x) {
+      const int c = s_count[k];
+      if (c) s_count[k] = atomicAdd(&bp.count[k], c);
+    }
+    __syncthreads();
+  }
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      const int brick = by[j] * bp.nbx + bx[i];
+      const int slot = lds ? s_count[brick] + local[j * N + i] : atomicAdd(&bp.count[brick], 1);
+      if (slot < bp.cap)
+        bp.list[(size_t)brick * bp.cap + slot] = place;
+      else
+        atomicOr(&flags[kFlagBrickOverflow], 1);
+    }
 }
 
 // The part of one atom's stencil (place-ordered position and charge pq)
 // that lies in the brick at (bx0, by0), added to the brick's accumulators.
 template <int N, int BX, int BY>
-__device__ __forceinline__ void spread_atom(double4 pq, int cxy, const PmeParams &pp,
-                                            const BrickParams &bp, int bx0, int by0,
-                                            unsigned long long *acc, int *flags) {
+__device__ __forceinline__ void spread_atom(double4 pq, const PmeParams &pp, int bx0,
+                                            int by0, unsigned long long *acc) {
   const int K0 = pp.K[0], K1 = pp.K[1], K2 = pp.K[2];
   const double qd = pq.w;
-  if (qd == 0.0) return;  // padding, or an uncharged atom
   int f[3];
   float w[3];
   pme_place(pq.x, pp.box[0], pp.inv_box[0], K0, N, f[0], w[0]);
   pme_place(pq.y, pp.box[1], pp.inv_box[1], K1, N, f[1], w[1]);
-  // an atom outside its column's tile would be missed by some brick
-  const int cx = cxy & 0xffff, cy = cxy >> 16;
-  int fx = f[0] - bp.gx0[cx];
-  if (fx < 0) fx += K0;
-  int fy = f[1] - bp.gy0[cy];
-  if (fy < 0) fy += K1;
-  if ((bp.TWx < K0 && fx > bp.TWx - N) || (bp.TWy < K1 && fy > bp.TWy - N))
-    atomicOr(&flags[kFlagPmeRange], 1);
-  // atoms whose stencil misses the brick end here
-  int kx0 = bx0 - f[0];
-  if (kx0 < 0) kx0 += K0;
-  int ky0 = by0 - f[1];
-  if (ky0 < 0) ky0 += K1;
-  if (!(kx0 < N || kx0 > K0 - BX) || !(ky0 < N || ky0 > K1 - BY)) return;
   pme_place(pq.z, pp.box[2], pp.inv_box[2], K2, N, f[2], w[2]);
   float M[3][N];
 #pragma unroll
@@ -187,61 +227,31 @@ __device__ __forceinline__ void spread_atom(double4 pq, int cxy, const PmeParams
 
 template <int N, int TL>
 __global__ void __launch_bounds__(256) pme_spread_rows(
-    const int *__restrict__ col_cl_start, const int *__restrict__ col_cl_count,
     const double4 *__restrict__ xpq, PmeParams pp, BrickParams bp,
-    cpx<float> *__restrict__ out, FftAxis az, int *__restrict__ flags) {
+    cpx<float> *__restrict__ out, FftAxis az) {
   constexpr int R = BrickShape<TL>::R, BX = BrickShape<TL>::BX, BY = BrickShape<TL>::BY;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-  __shared__ int s_p0[kBrickMaxCols], s_cum[kBrickMaxCols + 1], s_cxy[kBrickMaxCols];
   const int K0 = pp.K[0], K1 = pp.K[1], K2 = pp.K[2], H = K2 / 2 + 1;
   unsigned long long *acc = reinterpret_cast<unsigned long long *>(smem);
   cpx<float> *a = reinterpret_cast<cpx<float> *>(acc + (size_t)R * K2);
   cpx<float> *b = a + TL * K2;
-  const int nbx = (K0 + BX - 1) / BX;
-  const int bxi = blockIdx.x % nbx, byi = blockIdx.x / nbx;
-  const int bx0 = bxi * BX, by0 = byi * BY;
+  cpx<float> *tw = b + TL * K2;  // the z twiddles, read by every pass
+  const int brick = blockIdx.x;
+  const int bx0 = (brick % bp.nbx) * BX, by0 = (brick / bp.nbx) * BY;
   for (int w = threadIdx.x; w < R * K2; w += blockDim.x) acc[w] = 0ull;
-
-  // the brick's columns and their place ranges
-  const int xs = bp.xstart[bxi], nxc = bp.xstart[bxi + 1] - xs;
-  const int ys = bp.ystart[byi], nyc = bp.ystart[byi + 1] - ys;
-  const int ncand = nxc * nyc;  // <= kBrickMaxCols (checked by the host)
-  for (int c = threadIdx.x; c < ncand; c += blockDim.x) {
-    const int cx = bp.xcols[xs + c % nxc], cy = bp.ycols[ys + c / nxc];
-    const int col = cy * bp.ncx + cx;
-    s_p0[c] = col_cl_start[col] * kCluster;
-    s_cum[c + 1] = col_cl_count[col] * kCluster;
-    s_cxy[c] = cx | (cy << 16);
-  }
+  for (int w = threadIdx.x; w < K2; w += blockDim.x)
+    tw[w] = static_cast<const cpx<float> *>(az.tw)[w];
+  int nat = bp.count[brick];
+  nat = nat < bp.cap ? nat : bp.cap;
+  const int *lst = bp.list + (size_t)brick * bp.cap;
   __syncthreads();
-  if (threadIdx.x < 64) {  // inclusive scan of the counts by one wave
-    const int lane = threadIdx.x;
-    int carry = 0;
-    for (int c0 = 0; c0 < ncand; c0 += 64) {
-      const int c = c0 + lane;
-      int v = c < ncand ? s_cum[c + 1] : 0;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const int up = __shfl_up(v, off);
-        if (lane >= off) v += up;
-      }
-      if (c < ncand) s_cum[c + 1] = carry + v;
-      carry += __shfl(v, 63);
-    }
-    if (lane == 0) s_cum[0] = 0;
-  }
-  __syncthreads();
-
   // two atoms per thread in flight: their loads are independent
-  const int total = s_cum[ncand];
-  for (int i0 = threadIdx.x; i0 < total; i0 += 2 * blockDim.x) {
+  for (int i0 = threadIdx.x; i0 < nat; i0 += 2 * blockDim.x) {
     const int i1 = i0 + blockDim.x;
-    const int c0 = column_slot(s_cum, ncand, i0);
-    const int c1 = i1 < total ? column_slot(s_cum, ncand, i1) : c0;
-    const double4 a0 = xpq[s_p0[c0] + (i0 - s_cum[c0])];
-    const double4 a1 = i1 < total ? xpq[s_p0[c1] + (i1 - s_cum[c1])] : make_double4(0, 0, 0, 0);
-    spread_atom<N, BX, BY>(a0, s_cxy[c0], pp, bp, bx0, by0, acc, flags);
-    spread_atom<N, BX, BY>(a1, s_cxy[c1], pp, bp, bx0, by0, acc, flags);
+    const double4 a0 = xpq[lst[i0]];
+    const double4 a1 = i1 < nat ? xpq[lst[i1]] : make_double4(0, 0, 0, 0);
+    spread_atom<N, BX, BY>(a0, pp, bx0, by0, acc);
+    if (i1 < nat) spread_atom<N, BX, BY>(a1, pp, bx0, by0, acc);
   }
   __syncthreads();
   // rows 2t and 2t+1 of the brick as the real and imaginary parts of line t
@@ -254,7 +264,7 @@ __global__ void __launch_bounds__(256) pme_spread_rows(
       a[z * TL + (rr >> 1)].x = v;
   }
   __syncthreads();
-  cpx<float> *res = lds_fft<float, TL>(a, b, az, -1);
+  cpx<float> *res = lds_fft<float, TL>(a, b, az, -1, tw);
   for (int w = threadIdx.x; w < TL * H; w += blockDim.x) {
     const int k = w % H, t = w / H;
     const cpx<float> zk = res[k * TL + t];
@@ -280,10 +290,11 @@ __global__ void __launch_bounds__(256) pme_spread_rows(
 //
 // A block owns the same brick of grid rows as the spread: it transforms the
 // brick's half-spectrum rows back to real potential rows in LDS, and every
-// atom of the brick's columns adds the part of its force gradient that comes
-// from the brick's points, as 64-bit fixed point (each part is a fixed f32
-// sum, so the total does not depend on the order of the adds).  The real
-// potential grid never goes to HBM.
+// listed atom adds the part of its force that comes from the brick's points,
+// as 64-bit fixed point (each part is a fixed f32 sum, so the total does not
+// depend on the order of the adds).  The real potential grid never goes to
+// HBM.  The block clears its brick's list count for the next step.
+
 // One atom's force from the potential rows of the brick at (bx0, by0):
 // -q (K/L) sum phi M' M M over the stencil points in the brick, added as
 // 64-bit fixed point.  phi of brick row r at z is phi[z TL + r/2], .x for
@@ -294,22 +305,15 @@ __device__ __forceinline__ void gather_atom(double4 pq, int place, const PmePara
                                             long long *__restrict__ F) {
   const int K0 = pp.K[0], K1 = pp.K[1], K2 = pp.K[2];
   const double qd = pq.w;
-  if (qd == 0.0) return;
   int f[3];
   float w[3];
   pme_place(pq.x, pp.box[0], pp.inv_box[0], K0, N, f[0], w[0]);
   pme_place(pq.y, pp.box[1], pp.inv_box[1], K1, N, f[1], w[1]);
-  int kx0 = bx0 - f[0];
-  if (kx0 < 0) kx0 += K0;
-  int ky0 = by0 - f[1];
-  if (ky0 < 0) ky0 += K1;
-  if (!(kx0 < N || kx0 > K0 - BX) || !(ky0 < N || ky0 > K1 - BY)) return;
   pme_place(pq.z, pp.box[2], pp.inv_box[2], K2, N, f[2], w[2]);
   float M[3][N], dM[3][N];
 #pragma unroll
   for (int d = 0; d < 3; ++d) bspline<float>(w[d], N, M[d], dM[d]);
   float g0 = 0.0f, g1 = 0.0f, g2 = 0.0f;
-  bool any = false;
 #pragma unroll
   for (int tx = 0; tx < BX; ++tx) {
     int kx = bx0 + tx - f[0];
@@ -348,10 +352,8 @@ __device__ __forceinline__ void gather_atom(double4 pq, int place, const PmePara
       g0 += dmx * my * sz;
       g1 += mx * dmy * sz;
       g2 += mx * my * dsz;
-      any = true;
     }
   }
-  if (!any) return;
   const float qf = (float)qd;
   unsigned long long *u = reinterpret_cast<unsigned long long *>(F + 3 * place);
   atomicAdd(u + 0, (unsigned long long)to_fixed(-qf * (float)(K0 * pp.inv_box[0]) * g0));
@@ -361,19 +363,22 @@ __device__ __forceinline__ void gather_atom(double4 pq, int place, const PmePara
 
 template <int N, int TL>
 __global__ void __launch_bounds__(256) pme_gather_rows(
-    const int *__restrict__ col_cl_start, const int *__restrict__ col_cl_count,
     const double4 *__restrict__ xpq, PmeParams pp, BrickParams bp,
     const cpx<float> *__restrict__ in, FftAxis az, long long *__restrict__ F) {
   constexpr int BX = BrickShape<TL>::BX, BY = BrickShape<TL>::BY;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-  __shared__ int s_p0[kBrickMaxCols], s_cum[kBrickMaxCols + 1];
   const int K0 = pp.K[0], K1 = pp.K[1], K2 = pp.K[2], H = K2 / 2 + 1;
   cpx<float> *a = reinterpret_cast<cpx<float> *>(smem);
   cpx<float> *b = a + TL * K2;
-  const int nbx = (K0 + BX - 1) / BX;
-  const int bxi = blockIdx.x % nbx, byi = blockIdx.x / nbx;
-  const int bx0 = bxi * BX, by0 = byi * BY;
+  cpx<float> *tw = b + TL * K2;  // the z twiddles, read by every pass
+  const int brick = blockIdx.x;
+  const int bx0 = (brick % bp.nbx) * BX, by0 = (brick / bp.nbx) * BY;
   const size_t rows = (size_t)K0 * K1;
+  for (int w = threadIdx.x; w < K2; w += blockDim.x)
+    tw[w] = static_cast<const cpx<float> *>(az.tw)[w];
+  int nat = bp.count[brick];
+  nat = nat < bp.cap ? nat : bp.cap;
+  const int *lst = bp.list + (size_t)brick * bp.cap;
 
   // half spectra of rows 2t and 2t+1 as one complex line t
   for (int w = threadIdx.x; w < TL * K2; w += blockDim.x) {
@@ -398,115 +403,19 @@ __global__ void __launch_bounds__(256) pme_gather_rows(
     a[k * TL + t] = k < H ? cpx<float>{x1.x - x2.y, x1.y + x2.x}
                           : cpx<float>{x1.x + x2.y, -x1.y + x2.x};
   }
-  // the brick's columns and their place ranges
-  const int xs = bp.xstart[bxi], nxc = bp.xstart[bxi + 1] - xs;
-  const int ys = bp.ystart[byi], nyc = bp.ystart[byi + 1] - ys;
-  const int ncand = nxc * nyc;
-  for (int c = threadIdx.x; c < ncand; c += blockDim.x) {
-    const int col = bp.ycols[ys + c / nxc] * bp.ncx + bp.xcols[xs + c % nxc];
-    s_p0[c] = col_cl_start[col] * kCluster;
-    s_cum[c + 1] = col_cl_count[col] * kCluster;
-  }
   __syncthreads();
-  if (threadIdx.x < 64) {
-    const int lane = threadIdx.x;
-    int carry = 0;
-    for (int c0 = 0; c0 < ncand; c0 += 64) {
-      const int c = c0 + lane;
-      int v = c < ncand ? s_cum[c + 1] : 0;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const int up = __shfl_up(v, off);
-        if (lane >= off) v += up;
-      }
-      if (c < ncand) s_cum[c + 1] = carry + v;
-      carry += __shfl(v, 63);
-    }
-    if (lane == 0) s_cum[0] = 0;
-  }
-  // (the scan above is read only after the transform's first pass barrier)
-  const cpx<float> *phi = lds_fft<float, TL>(a, b, az, +1);
-  // phi of brick row r at z: phi[z * TL + r / 2], .x for even r, .y for odd
-  const int total = s_cum[ncand];
-  for (int i0 = threadIdx.x; i0 < total; i0 += 2 * blockDim.x) {
+  const cpx<float> *phi = lds_fft<float, TL>(a, b, az, +1, tw);
+  for (int i0 = threadIdx.x; i0 < nat; i0 += 2 * blockDim.x) {
     const int i1 = i0 + blockDim.x;
-    const int c0 = column_slot(s_cum, ncand, i0);
-    const int c1 = i1 < total ? column_slot(s_cum, ncand, i1) : c0;
-    const int p0 = s_p0[c0] + (i0 - s_cum[c0]);
-    const int p1 = i1 < total ? s_p0[c1] + (i1 - s_cum[c1]) : p0;
+    const int p0 = lst[i0];
+    const int p1 = i1 < nat ? lst[i1] : p0;
     const double4 a0 = xpq[p0];
-    const double4 a1 = i1 < total ? xpq[p1] : make_double4(0, 0, 0, 0);
+    const double4 a1 = xpq[p1];
     gather_atom<N, TL, BX, BY>(a0, p0, pp, bx0, by0, phi, F);
-    gather_atom<N, TL, BX, BY>(a1, p1, pp, bx0, by0, phi, F);
+    if (i1 < nat) gather_atom<N, TL, BX, BY>(a1, p1, pp, bx0, by0, phi, F);
   }
-}
-
-// N lanes per atom (place order): lane t0 gathers plane x = first + t0, the
-// N lanes then sum their three partial gradients.
-template <int N, int LPA>
-__global__ void __launch_bounds__(256) pme_gather_lanes(
-    const int *__restrict__ counts, const int *__restrict__ atom_at,
-    const double *__restrict__ x, const double *__restrict__ q, PmeParams pp,
-    const float *__restrict__ phi, long long *__restrict__ F) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  const int place = g / LPA, t0 = g % LPA;
-  const bool valid = place < counts[0];
-  const int atom = valid ? atom_at[place] : -1;
-  const double qa = atom >= 0 ? q[atom] : 0.0;
-  float g0 = 0.0f, g1 = 0.0f, g2 = 0.0f;
-  if (atom >= 0 && qa != 0.0 && t0 < N) {
-    int f[3];
-    float M[3][N], dM[3][N];
-    for (int d = 0; d < 3; ++d) {
-      float w;
-      pme_place(x[3 * atom + d], pp.box[d], pp.inv_box[d], pp.K[d], N, f[d], w);
-      bspline<float>(w, N, M[d], dM[d]);
-    }
-    float mx = M[0][0], dmx = dM[0][0];
-#pragma unroll
-    for (int k = 1; k < N; ++k)
-      if (k == t0) {
-        mx = M[0][k];
-        dmx = dM[0][k];
-      }
-    int i0 = f[0] + t0;
-    if (i0 >= pp.K[0]) i0 -= pp.K[0];
-    const float *plane = phi + (size_t)i0 * pp.K[1] * pp.K[2];
-#pragma unroll
-    for (int t1 = 0; t1 < N; ++t1) {
-      int i1 = f[1] + t1;
-      if (i1 >= pp.K[1]) i1 -= pp.K[1];
-      const float *row = plane + (size_t)i1 * pp.K[2];
-      float s = 0.0f, ds = 0.0f;
-#pragma unroll
-      for (int t2 = 0; t2 < N; ++t2) {
-        int i2 = f[2] + t2;
-        if (i2 >= pp.K[2]) i2 -= pp.K[2];
-        const float p = row[i2];
-        s += p * M[2][t2];
-        ds += p * dM[2][t2];
-      }
-      g0 += dmx * M[1][t1] * s;
-      g1 += mx * dM[1][t1] * s;
-      g2 += mx * M[1][t1] * ds;
-    }
-  }
-#pragma unroll
-  for (int m = 1; m < LPA; m <<= 1) {
-    g0 += __shfl_xor(g0, m);
-    g1 += __shfl_xor(g1, m);
-    g2 += __shfl_xor(g2, m);
-  }
-  if (atom >= 0 && qa != 0.0 && t0 == 0) {
-    const float qf = (float)qa;
-    const float fx = -qf * (float)(pp.K[0] * pp.inv_box[0]) * g0;
-    const float fy = -qf * (float)(pp.K[1] * pp.inv_box[1]) * g1;
-    const float fz = -qf * (float)(pp.K[2] * pp.inv_box[2]) * g2;
-    unsigned long long *u = reinterpret_cast<unsigned long long *>(F + 3 * place);
-    atomicAdd(u + 0, (unsigned long long)to_fixed(fx));
-    atomicAdd(u + 1, (unsigned long long)to_fixed(fy));
-    atomicAdd(u + 2, (unsigned long long)to_fixed(fz));
-  }
+  __syncthreads();  // every thread has read the count
+  if (threadIdx.x == 0) bp.count[brick] = 0;
 }
 
 // ---- FFT passes ---------------------------------------------------------
@@ -703,47 +612,67 @@ __global__ void __launch_bounds__(256) fft_x_conv(cpx<T> *__restrict__ data, int
   }
 }
 
+constexpr int kPlaneAxisMax = 256;
+
 // The y and x passes of one kz plane, whole in LDS, for grids whose planes
-// fit (K1 K2 <= 4096 points): forward y, forward x, influence product (and
-// its energy), backward x, backward y.  The half spectrum is kz-plane major
-// ([kz][x][y]), so a plane is one contiguous read and write.
+// fit (K1 K2 <= 4096 points, K1, K2 <= 256): forward y, forward x,
+// influence product (and its energy), backward x, backward y.  The half
+// spectrum is kz-plane major ([kz][x][y]), so a plane is one contiguous read
+// and write.
 __global__ void __launch_bounds__(512) fft_plane_conv(cpx<float> *__restrict__ data, int K1,
                                                       int K2, FftAxis ax, FftAxis ay,
                                                       ConvTables ct,
                                                       double *__restrict__ energy_partial) {
   extern __shared__ unsigned char smem[];
   __shared__ double s_e[512];
+  __shared__ cpx<float> s_twx[kPlaneAxisMax], s_twy[kPlaneAxisMax];
+  __shared__ float s_t0[kPlaneAxisMax], s_m0[kPlaneAxisMax];
+  __shared__ float s_t1[kPlaneAxisMax], s_m1[kPlaneAxisMax];
   const int np = K1 * K2;
   cpx<float> *a = reinterpret_cast<cpx<float> *>(smem);
   cpx<float> *b = a + np;
   const int kz = blockIdx.x;
   cpx<float> *plane = data + (size_t)kz * np;
-  for (int w = threadIdx.x; w < np; w += blockDim.x) a[w] = plane[w];
+  const int swz = K2 % 16 == 0 ? 15 : 0;
+  // the plane (unrolled, so that the loads are issued together)
+#pragma unroll 8
+  for (int w = threadIdx.x; w < np; w += blockDim.x)
+    a[plane_index(w / K2, w % K2, K2, swz)] = plane[w];
+  // twiddles and influence tables in LDS: every pass reads them
+  for (int w = threadIdx.x; w < K1; w += blockDim.x) {
+    s_twx[w] = static_cast<const cpx<float> *>(ax.tw)[w];
+    s_t0[w] = static_cast<const float *>(ct.t[0])[w];
+    s_m0[w] = static_cast<const float *>(ct.m[0])[w];
+  }
+  for (int w = threadIdx.x; w < K2; w += blockDim.x) {
+    s_twy[w] = static_cast<const cpx<float> *>(ay.tw)[w];
+    s_t1[w] = static_cast<const float *>(ct.t[1])[w];
+    s_m1[w] = static_cast<const float *>(ct.m[1])[w];
+  }
   __syncthreads();
-  // element (x, y) at x K2 + y
-  cpx<float> *r = lds_fft_strided<float>(a, b, ay, -1, K1, 1, K2);          // along y
-  r = lds_fft_strided<float>(r, r == a ? b : a, ax, -1, K2, K2, 1);        // along x
-  const float *t0 = (const float *)ct.t[0], *t1 = (const float *)ct.t[1],
-              *t2 = (const float *)ct.t[2];
-  const float *m0 = (const float *)ct.m[0], *m1 = (const float *)ct.m[1],
-              *m2 = (const float *)ct.m[2];
+  cpx<float> *r = lds_fft_plane<float, 1>(a, b, ay, -1, K1, K2, swz, s_twy);
+  r = lds_fft_plane<float, 0>(r, r == a ? b : a, ax, -1, K1, K2, swz, s_twx);
+  const float t2 = static_cast<const float *>(ct.t[2])[kz];
+  const float m2 = static_cast<const float *>(ct.m[2])[kz];
   const double h = (kz == 0 || (ct.K3 % 2 == 0 && kz == ct.K3 / 2)) ? 0.5 : 1.0;
   double e = 0.0;
   for (int w = threadIdx.x; w < np; w += blockDim.x) {
     const int kx = w / K2, ky = w % K2;
-    const cpx<float> v = r[w];
+    const int i = plane_index(kx, ky, K2, swz);
+    const cpx<float> v = r[i];
     float bc = 0.0f;
     if ((kx | ky | kz) != 0) {
-      const float msq = m0[kx] * m0[kx] + m1[ky] * m1[ky] + m2[kz] * m2[kz];
-      bc = t0[kx] * t1[ky] * t2[kz] / msq;
+      const float msq = s_m0[kx] * s_m0[kx] + s_m1[ky] * s_m1[ky] + m2 * m2;
+      bc = s_t0[kx] * s_t1[ky] * t2 / msq;
       if (energy_partial) e += h * (double)bc * ((double)v.x * v.x + (double)v.y * v.y);
     }
-    r[w] = {v.x * bc, v.y * bc};
+    r[i] = {v.x * bc, v.y * bc};
   }
   __syncthreads();
-  r = lds_fft_strided<float>(r, r == a ? b : a, ax, +1, K2, K2, 1);        // along x
-  r = lds_fft_strided<float>(r, r == a ? b : a, ay, +1, K1, 1, K2);        // along y
-  for (int w = threadIdx.x; w < np; w += blockDim.x) plane[w] = r[w];
+  r = lds_fft_plane<float, 0>(r, r == a ? b : a, ax, +1, K1, K2, swz, s_twx);
+  r = lds_fft_plane<float, 1>(r, r == a ? b : a, ay, +1, K1, K2, swz, s_twy);
+  for (int w = threadIdx.x; w < np; w += blockDim.x)
+    plane[w] = r[plane_index(w / K2, w % K2, K2, swz)];
   if (energy_partial) {
     s_e[threadIdx.x] = e;
     __syncthreads();
@@ -757,19 +686,16 @@ __global__ void __launch_bounds__(512) fft_plane_conv(cpx<float> *__restrict__ d
 
 template __global__ void fft_rows_r2c<float, true>(const void *, cpx<float> *, int, FftAxis);
 #define MDHIP_SPREAD(N, TL)                                                              \
-  template __global__ void pme_spread_rows<N, TL>(const int *, const int *,              \
-      const double4 *, PmeParams, BrickParams, cpx<float> *, FftAxis, int *);
-#define MDHIP_SPREAD_N(N) MDHIP_SPREAD(N, 8) MDHIP_SPREAD(N, 4) MDHIP_SPREAD(N, 2) MDHIP_SPREAD(N, 1) \
-  MDHIP_GATHER(N, 8) MDHIP_GATHER(N, 4) MDHIP_GATHER(N, 2) MDHIP_GATHER(N, 1)
-#define MDHIP_GATHER(N, TL)                                                              \
-  template __global__ void pme_gather_rows<N, TL>(const int *, const int *,              \
-      const double4 *, PmeParams, BrickParams, const cpx<float> *, FftAxis, long long *);
+  template __global__ void pme_bin<N, TL>(const int *, const double4 *, PmeParams,       \
+      BrickParams, int *);                                                               \
+  template __global__ void pme_spread_rows<N, TL>(const double4 *, PmeParams,            \
+      BrickParams, cpx<float> *, FftAxis);                                               \
+  template __global__ void pme_gather_rows<N, TL>(const double4 *, PmeParams,            \
+      BrickParams, const cpx<float> *, FftAxis, long long *);
+#define MDHIP_SPREAD_N(N) MDHIP_SPREAD(N, 8) MDHIP_SPREAD(N, 4) MDHIP_SPREAD(N, 2) MDHIP_SPREAD(N, 1)
 MDHIP_SPREAD_N(4)
 MDHIP_SPREAD_N(6)
 MDHIP_SPREAD_N(8)
-template __global__ void pme_gather_lanes<4, 4>(const int *, const int *, const double *, const double *, PmeParams, const float *, long long *);
-template __global__ void pme_gather_lanes<6, 8>(const int *, const int *, const double *, const double *, PmeParams, const float *, long long *);
-template __global__ void pme_gather_lanes<8, 8>(const int *, const int *, const double *, const double *, PmeParams, const float *, long long *);
 template __global__ void fft_rows_r2c<float, false>(const void *, cpx<float> *, int, FftAxis);
 template __global__ void fft_rows_r2c<double, false>(const void *, cpx<double> *, int, FftAxis);
 template __global__ void fft_rows_c2r<float>(const cpx<float> *, float *, int, FftAxis, int);

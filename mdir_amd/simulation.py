@@ -157,7 +157,7 @@ class Simulation:
         return dict(zip(keys, (int(v) for v in s)))
 
     PROFILE_KERNELS = ["integrate_first", "nb_bin", "nb_sort", "nb_search",
-                       "nb_pairs", "pme_spread_rows", "pme_yx",
+                       "nb_pairs", "pme_bin", "pme_spread_rows", "pme_yx",
                        "pme_gather_rows", "integrate_second"]
 
     def profile_step(self, repeats: int = 20) -> dict:
