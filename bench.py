@@ -30,6 +30,11 @@ FLOP_PER_COULOMB_PAIR = 50.0     # force-only erfc Ewald pair
 FLOP_PER_LJ_COULOMB_PAIR = 60.0  # LJ + erfc Ewald pair
 PEAK_FP32_TFLOPS = 157.3         # MI355X FP32 (vector and matrix)
 PEAK_HBM_GBS = 8000.0            # MI355X HBM3E
+# HBM bytes per launch of the pair kernel on this workload, from two
+# rocprofv3 --pmc runs of this script (FETCH_SIZE x 2.000 and WRITE_SIZE x
+# 1.000, the factors that make a 256 MiB device copy read and write 256 MiB):
+# profiles/nb_pairs_tip3p_7900_pme64.md
+NB_PAIRS_TRAFFIC_BYTES = 30.15e6
 
 
 def parse():
@@ -38,11 +43,17 @@ def parse():
     p.add_argument("--steps", type=int, default=5000)
     p.add_argument("--warmup", type=int, default=500)
     p.add_argument("--graph", type=int, default=50, help="steps per captured graph")
-    p.add_argument("--skin", type=float, default=0.1)
+    p.add_argument("--skin", type=float, default=0.15,
+                   help="list buffer beyond the cutoff (nm): the list is rebuilt when an "
+                        "atom moved half of it")
+    p.add_argument("--prune-skin", type=float, default=0.0,
+                   help="buffer of a pruned inner list (nm, below --skin); 0: none")
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--profile-repeats", type=int, default=50)
+    p.add_argument("--measure-steps", type=int, default=1000,
+                   help="steps of the run that times the pair kernel in place")
     p.add_argument("--drift-ps", type=float, default=100.0,
                    help="length of the NVE run the energy drift is measured on")
     p.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -116,7 +127,7 @@ def main():
 
     dt = 0.002
     system = S.tip3p_config2(seed=args.seed)
-    sim = Simulation(system, skin=args.skin)
+    sim = Simulation(system, skin=args.skin, prune_skin=args.prune_skin)
     sim.minimize(200, 0.01)
     x = sim.positions
     relaxed = system.copy()
@@ -154,7 +165,13 @@ def main():
     print(f"drift: {samples * per_ps} steps, {drift:.3e} kJ/mol/ps/atom "
           f"(fit {slope:.3e})", file=sys.stderr, flush=True)
 
-    # per-kernel device time (HIP events on the kernels' stream)
+    # The pair kernel's duration inside the replayed steps, on its stream:
+    # device-clock readings bracket it in every step of a run of the same
+    # kind right after the timed one (events give no elapsed times inside
+    # graphs, and the readings cost a few per cent of a step, so they stay
+    # out of the timed run).
+    pair_ms = sim.run_measured(args.measure_steps, dt, graph=args.graph)
+    # per-kernel device time, kernel by kernel on one stream (HIP events)
     prof = sim.profile_step(args.profile_repeats)
     # pairs within rc for the algorithmic FLOP count (f32 test of the
     # reference matrix, exclusions removed)
@@ -164,25 +181,21 @@ def main():
     n_lj = int(both_o.sum())
     n_pairs = int(len(pairs))
     flops = FLOP_PER_LJ_COULOMB_PAIR * n_lj + FLOP_PER_COULOMB_PAIR * (n_pairs - n_lj)
-    t_pairs = prof["nb_pairs"] * 1e-3
+    t_pairs = pair_ms * 1e-3
     pair_tflops = flops / t_pairs / 1e12
     kernels_ms = {k: round(v_, 5) for k, v_ in prof.items()}
-    dominant = max(prof, key=prof.get)
     pmeb = pme_bytes(system.n, system.pme_grid)
     hbm = {k: round(b / (prof[k] * 1e-3) / 1e9, 1) for k, b in pmeb.items()}
 
+    # the dominant kernel: the pair loop, against the FP32 vector peak
     roofline = {"kernel": "nb_pairs", "bound": "mfma", "pipe": "fp32 vector ALU",
+                "duration_us": round(pair_ms * 1e3, 2),
                 "achieved": round(pair_tflops, 3), "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(pair_tflops / PEAK_FP32_TFLOPS, 4),
-                "traffic": None,
+                "traffic": NB_PAIRS_TRAFFIC_BYTES,
                 "work": {"pairs_within_rc": n_pairs, "lj_pairs": n_lj,
                          "flop_per_pair": [FLOP_PER_COULOMB_PAIR, FLOP_PER_LJ_COULOMB_PAIR],
                          "flop_per_launch": flops}}
-    if dominant != "nb_pairs" and dominant in pmeb:
-        roofline = {"kernel": dominant, "bound": "hbm",
-                    "achieved": hbm[dominant], "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": round(hbm[dominant] / PEAK_HBM_GBS, 4), "traffic": None,
-                    "work": {"bytes_per_launch": pmeb[dominant]}}
 
     out = {
         "metric": "ns/day on ~24k-atom PME water box (1 GPU)",
@@ -201,7 +214,8 @@ def main():
                    "box_nm": 6.2, "cutoff_nm": system.cutoff, "pme_grid": list(system.pme_grid),
                    "pme_order": system.pme_order, "ewald_beta": system.beta,
                    "dt_fs": 2.0, "ensemble": "NVE", "constraints": "SETTLE",
-                   "list_skin_nm": args.skin, "graph_steps": args.graph},
+                   "list_skin_nm": args.skin, "prune_skin_nm": args.prune_skin,
+                   "graph_steps": args.graph},
         "roofline": roofline,
         "kernels_ms": kernels_ms,
         "pme_hbm_gbs": hbm,
@@ -221,14 +235,24 @@ def main():
 
 
 def calibrate_copy(mib):
-    """A plain copy kernel of known size: read mib MiB, write mib MiB."""
-    import torch
-    n = mib * (1 << 20) // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
+    """A plain device copy of known size through the runtime ABI
+    (mgpuMemcpy device to device, the HIP runtime's copy kernel): 5 copies
+    of mib MiB, each reading and writing mib MiB, for scaling hardware
+    counters of memory traffic."""
+    from mdir_amd import runtime as rt
+    L = rt.lib()
+    size = mib << 20
+    s = L.mgpuStreamCreate()
+    a = L.mgpuMemAlloc(size, s, False)
+    b = L.mgpuMemAlloc(size, s, False)
+    L.mgpuMemset32(a, 0x3F800000, size // 4, s)
     for _ in range(5):
-        torch.mul(a, 1.0, out=b)
-    torch.cuda.synchronize()
+        L.mgpuMemcpy(b, a, size, s)
+    L.mgpuStreamSynchronize(s)
+    L.mgpuMemFree(a, s)
+    L.mgpuMemFree(b, s)
+    L.mgpuStreamSynchronize(s)
+    L.mgpuStreamDestroy(s)
     print(f"copy calibration: 5 x {mib} MiB read + {mib} MiB written", file=sys.stderr,
           flush=True)
 

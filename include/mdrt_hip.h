@@ -229,6 +229,15 @@ int mdrtHipComputeForces(void *system, double *forces, double *energies);
 int mdrtHipRun(void *system, int64_t steps, double dt, int32_t graph,
                double *kinetic);
 
+/* mdrtHipRun without the kinetic energy, timing the pair loop live: device
+   clock (wall_clock64) readings by one-wave kernels just before and after
+   the pair kernel of every step, and once more right after, on its stream,
+   inside the captured graphs too; *pair_ms receives the mean of
+   (after - before) - (again - after) (ms).  No counterpart in the
+   reference: it is the measurement hook of the benchmark. */
+int mdrtHipRunMeasured(void *system, int64_t steps, double dt, int32_t graph,
+                       double *pair_ms);
+
 /* Steepest descent of `steps` accepted-or-rejected moves, rigid waters. */
 int mdrtHipMinimize(void *system, int64_t steps, double max_step,
                     double *energy);

@@ -254,6 +254,8 @@ struct Engine {
   void release() {
     if (graph_exec) hipGraphExecDestroy(graph_exec);
     graph_exec = nullptr;
+    dfree(d_timing);
+    timing_slots = 0;
     dfree(d_x); dfree(d_v); dfree(d_xref); dfree(d_shift); dfree(d_q);
     dfree(d_inv_mass); dfree(d_type); dfree(d_place_of); dfree(d_save_x); dfree(d_save_v);
     dfree(d_xprune);
@@ -885,14 +887,21 @@ struct Engine {
   }
 
   // Pair loop on the engine's stream, PME on the side stream, joined.
-  void forces(bool energy) {
+  // With timed >= 0, the pair kernel records its span in slot `timed` of
+  // the timing buffer.
+  void forces(bool energy, int timed = -1) {
     if (use_pme) {
       HIPCHK(hipEventRecord(ev_fork, stream));
       HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
       pme(energy);
       HIPCHK(hipEventRecord(ev_join, side));
     }
+    if (timed >= 0) stamp_clock<<<1, 64, 0, stream>>>(d_timing + 3 * timed);
     pairs(energy);
+    if (timed >= 0) {
+      stamp_clock<<<1, 64, 0, stream>>>(d_timing + 3 * timed + 1);
+      stamp_clock<<<1, 64, 0, stream>>>(d_timing + 3 * timed + 2);
+    }
     if (use_pme) HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
   }
 
@@ -918,9 +927,54 @@ struct Engine {
     step_first(dt);
     for (int64_t s = 0; s < count; ++s) {
       build();
-      forces(false);
+      forces(false, measure ? (int)s : -1);
       if (s + 1 < count) step_between(dt);
       else step_second(dt);
+    }
+  }
+
+  // Live timing of the pair loop inside the steps: device-clock readings
+  // just before and after the pair kernel of step s land in slots 3 s and
+  // 3 s + 1, and a third right after the second in 3 s + 2 (in the captured
+  // graphs too): the pair kernel took (t1 - t0) - (t2 - t1), the second
+  // difference being what one reading adds.  run() resets the slots before
+  // each chunk and reads them after its flag check.
+  bool measure = false;
+  unsigned long long *d_timing = nullptr;
+  int64_t timing_slots = 0;
+  double clock_khz = 0.0;
+  double pair_ms_sum = 0.0;
+  int64_t pair_ms_count = 0;
+  void ensure_timing(int64_t count) {
+    if (timing_slots < count) {
+      dfree(d_timing);
+      d_timing = dalloc<unsigned long long>(3 * (size_t)count);
+      timing_slots = count;
+      if (graph_exec) {  // a captured graph points at the old buffer
+        hipGraphExecDestroy(graph_exec);
+        graph_exec = nullptr;
+      }
+    }
+    if (clock_khz == 0.0) {
+      int dev = 0, khz = 0;
+      HIPCHK(hipGetDevice(&dev));
+      HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+      if (khz <= 0) throw HipError("the device reports no wall-clock rate");
+      clock_khz = (double)khz;
+    }
+  }
+  void reset_timing(int64_t count) {
+    std::vector<unsigned long long> init(3 * (size_t)count, 0ull);
+    to_device(d_timing, init.data(), init.size() * sizeof(unsigned long long));
+  }
+  void collect_pair_times(int64_t count) {
+    std::vector<unsigned long long> t(3 * (size_t)count);
+    to_host(t.data(), d_timing, t.size() * sizeof(unsigned long long));
+    for (int64_t s = 0; s < count; ++s) {
+      const unsigned long long t0 = t[3 * s], t1 = t[3 * s + 1], t2 = t[3 * s + 2];
+      if (t0 == 0 || t1 < t0 || t2 < t1) continue;  // not launched
+      pair_ms_sum += ((double)(t1 - t0) - (double)(t2 - t1)) / clock_khz;
+      ++pair_ms_count;
     }
   }
 
@@ -1033,29 +1087,40 @@ struct Engine {
     HIPCHK(hipGraphDestroy(graph));
     graph_steps = steps;
     dt_graph = dt;
+    graph_measure = measure;
   }
 
   // Steps in chunks (one graph launch, or up to 64 single steps); after each
   // chunk the flags are read, and a chunk whose list overflowed is redone
   // from its saved state with the grown list.
-  void run(int64_t steps, double dt, int graph, double *kinetic) {
+  bool graph_measure = false;
+  void run(int64_t steps, double dt, int graph, double *kinetic, double *pair_ms = nullptr) {
     if (list_reach != cutoff + skin) configure_list(cutoff + skin);
     if (!forces_valid) evaluate(false);
     const int64_t chunk = graph > 0 ? graph : 64;
+    measure = pair_ms != nullptr;
+    pair_ms_sum = 0.0;
+    pair_ms_count = 0;
+    if (measure) ensure_timing(chunk);
     int64_t done = 0;
     while (done < steps) {
       const int64_t todo = std::min(chunk, steps - done);
       for (int attempt = 0;; ++attempt) {
+        if (measure) reset_timing(todo);
         on_device(d_save_x, d_x, 3 * n * sizeof(double));
         on_device(d_save_v, d_v, 3 * n * sizeof(double));
         if (graph > 0 && todo == graph) {
-          if (!graph_exec || graph_steps != graph || dt_graph != dt) capture(graph, dt);
+          if (!graph_exec || graph_steps != graph || dt_graph != dt || graph_measure != measure)
+            capture(graph, dt);
           HIPCHK(hipGraphLaunch(graph_exec, stream));
         } else {
           steps_seq(todo, dt);
         }
-        const std::vector<int> f = read_flags();
-        if (!grow_if_overflowed(f)) break;
+        const std::vector<int> f = read_flags();  // synchronises the stream
+        if (!grow_if_overflowed(f)) {
+          if (measure) collect_pair_times(todo);
+          break;
+        }
         if (attempt == 7) throw HipError("the neighbor list kept overflowing");
         on_device(d_x, d_save_x, 3 * n * sizeof(double));
         on_device(d_v, d_save_v, 3 * n * sizeof(double));
@@ -1064,6 +1129,8 @@ struct Engine {
       done += todo;
     }
     HIPCHK(hipStreamSynchronize(stream));
+    measure = false;
+    if (pair_ms) *pair_ms = pair_ms_count ? pair_ms_sum / (double)pair_ms_count : 0.0;
     if (kinetic) *kinetic = kinetic_energy();
   }
 
@@ -1437,6 +1504,12 @@ int mdrtHipComputeForces(void *system, double *forces, double *energies) {
 int mdrtHipRun(void *system, int64_t steps, double dt, int32_t graph, double *kinetic) {
   Engine *e = static_cast<Engine *>(system);
   return guarded([&] { e->run(steps, dt, graph, kinetic); });
+}
+
+int mdrtHipRunMeasured(void *system, int64_t steps, double dt, int32_t graph,
+                       double *pair_ms) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] { e->run(steps, dt, graph, nullptr, pair_ms); });
 }
 
 int mdrtHipMinimize(void *system, int64_t steps, double max_step, double *energy) {

@@ -251,13 +251,13 @@ __device__ __forceinline__ void special_block(int block, const SpecialArgs &sa,
 // Grid: x over groups (4 waves per block), y = 0 the excluded and 1-4
 // pairs, y = 1 + segment the pair segments.
 template <int MOD, bool COUL, bool ENERGY>
-__global__ void __launch_bounds__(256) nb_pairs(
+__device__ __forceinline__ void nb_pairs_body(
     const float4 *__restrict__ xq, const int *__restrict__ type_at,
     const uint32_t *__restrict__ real_mask, const PairEntry *__restrict__ rows,
     const int *__restrict__ row_count, const uint64_t *__restrict__ ex_masks,
     int row_cap, int max_groups, const int *__restrict__ counts_in,
-    const float2 *__restrict__ ljtab, NbParams p, long long *__restrict__ F,
-    double *__restrict__ energy_partial, int *__restrict__ flags, SpecialArgs sa) {
+    const float2 *__restrict__ ljtab, const NbParams &p, long long *__restrict__ F,
+    double *__restrict__ energy_partial, int *__restrict__ flags, const SpecialArgs &sa) {
   extern __shared__ float2 s_lj[];                   // ntypes^2 (launch size)
   __shared__ union {
     struct {
@@ -406,6 +406,28 @@ __global__ void __launch_bounds__(256) nb_pairs(
       energy_partial[2 * slot + 1] = e_c;
     }
   }
+}
+
+// The pair kernel.
+template <int MOD, bool COUL, bool ENERGY>
+__global__ void __launch_bounds__(256) nb_pairs(
+    const float4 *__restrict__ xq, const int *__restrict__ type_at,
+    const uint32_t *__restrict__ real_mask, const PairEntry *__restrict__ rows,
+    const int *__restrict__ row_count, const uint64_t *__restrict__ ex_masks,
+    int row_cap, int max_groups, const int *__restrict__ counts_in,
+    const float2 *__restrict__ ljtab, NbParams p, long long *__restrict__ F,
+    double *__restrict__ energy_partial, int *__restrict__ flags, SpecialArgs sa) {
+  nb_pairs_body<MOD, COUL, ENERGY>(xq, type_at, real_mask, rows, row_count, ex_masks, row_cap,
+                                   max_groups, counts_in, ljtab, p, F, energy_partial, flags,
+                                   sa);
+}
+
+// One device-clock reading (wall_clock64), stored by lane 0 with a vector
+// store: launched on a stream before and after a kernel, the two readings
+// bracket it inside captured graphs, where events give no elapsed times.
+__global__ void stamp_clock(unsigned long long *__restrict__ slot) {
+  const unsigned long long t = (unsigned long long)wall_clock64();
+  if (threadIdx.x == 0) slot[0] = t;
 }
 
 // Instantiations used by the engine.

@@ -115,6 +115,7 @@ SIGNATURES = [
     ("mdrtHipGetVelocities", C.c_int, [C.c_void_p, PD]),
     ("mdrtHipComputeForces", C.c_int, [C.c_void_p, PD, PD]),
     ("mdrtHipRun", C.c_int, [C.c_void_p, I64, D, I32, PD]),
+    ("mdrtHipRunMeasured", C.c_int, [C.c_void_p, I64, D, I32, PD]),
     ("mdrtHipMinimize", C.c_int, [C.c_void_p, I64, D, PD]),
     ("mdrtHipNeighborPairs", C.c_int, [C.c_void_p, D, I32, I64, PI32, PI64]),
     ("mdrtHipStats", C.c_int, [C.c_void_p, PI64]),

@@ -126,6 +126,14 @@ class Simulation:
                                      C.byref(ke)))
         return ke.value
 
+    def run_measured(self, steps: int, dt: float, graph: int = 50) -> float:
+        """Runs like run() and returns the mean duration (ms) of the pair
+        kernel, timed live on the device clock in every step."""
+        ms = C.c_double(0.0)
+        rt.check(rt.lib().mdrtHipRunMeasured(self._h, int(steps), float(dt), int(graph),
+                                             C.byref(ms)))
+        return ms.value
+
     def minimize(self, steps: int = 200, max_step: float = 0.01) -> float:
         e = C.c_double(0.0)
         rt.check(rt.lib().mdrtHipMinimize(self._h, int(steps), float(max_step),
