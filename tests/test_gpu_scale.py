@@ -1,0 +1,77 @@
+"""The other workloads of the force path on the GPU: the small argon NVE run,
+a PME grid too wide for the plane transform (the tiled y / x passes), and
+the 999,000-atom water box at full size through size-independent checks."""
+import numpy as np
+import pytest
+
+from mdir_amd import systems as S
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+KINETIC = 8  # MDRT_E_KINETIC
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def test_argon_nve_config():
+    """864 argon atoms, LJ cutoff 1.0 nm with dispersion correction,
+    Maxwell-Boltzmann velocities at 120 K, NVE 2 fs for 2,000 steps:
+    forces match the oracle and the total energy stays within a small
+    fraction of the kinetic energy."""
+    from mdir_amd.simulation import Simulation
+    say("case: argon 864, NVE 2,000 steps of 2 fs")
+    a = S.argon_config1()
+    # off the lattice, where forces do not cancel by symmetry
+    a.positions = a.positions + np.random.default_rng(4).normal(0, 0.01, a.positions.shape)
+    sim = Simulation(a)
+    F, e = sim.forces()
+    Fo, eo = O.Oracle(a).total()
+    assert O.rel_rms(F, Fo) < 5e-6
+    assert abs(e[O.E_LJ] - eo[O.E_LJ]) <= 2e-6 * abs(eo[O.E_LJ]) + 5e-7 * a.n
+    sim.velocities = S.initial_velocities(a, 7, 120.0)
+    _, e0 = sim.forces()
+    total0 = e0.sum()
+    ke0 = e0[KINETIC]
+    sim.run(2000, 0.002, graph=50)
+    _, e1 = sim.forces()
+    say(f"  total energy {total0:.4f} -> {e1.sum():.4f} kJ/mol, kinetic {ke0:.2f}")
+    assert abs(e1.sum() - total0) < 2e-2 * ke0
+
+
+def test_wide_grid_tiled_passes():
+    """A 72 x 72 x 36 grid (the xy plane exceeds what the plane transform
+    holds): forces and the reciprocal energy against the oracle."""
+    from mdir_amd.simulation import Simulation
+    say("case: TIP3P 2,000 waters, 3.9 nm, grid 72 x 72 x 36")
+    w = S.tip3p_box(2000, 3.9, seed=17, grid=(72, 72, 36))
+    F, e = Simulation(w).forces()
+    Fo, eo = O.Oracle(w).total()
+    err = O.rel_rms(F, Fo)
+    say(f"  rel rms force error {err:.3e}; recip {e[O.E_COUL_RECIP]:.6f} vs {eo[O.E_COUL_RECIP]:.6f}")
+    assert err < 5e-6
+    assert abs(e[O.E_COUL_RECIP] - eo[O.E_COUL_RECIP]) <= 2e-6 * abs(eo[O.E_COUL_RECIP]) + 5e-7 * w.n
+
+
+def test_million_atom_box():
+    """333,000 waters (999,000 atoms), 21.5 nm, PME 216^3: the list, the
+    bricks and the graphs at full size.  Forces are finite with a net force
+    near zero, and SETTLE keeps the geometry through 20 steps."""
+    from mdir_amd.simulation import Simulation
+    say("case: 999,000 atoms, 21.5 nm, PME 216^3")
+    w = S.tip3p_box(333000, 21.5, seed=5, grid=(216, 216, 216), jitter=0.02)
+    sim = Simulation(w)
+    F, e = sim.forces()
+    assert np.isfinite(F).all() and np.isfinite(e).all()
+    net = np.abs(F.sum(axis=0)).max() / np.sqrt((F * F).sum(axis=1).mean())
+    say(f"  net force / rms atom force {net:.3e}")
+    assert net < 1e-3 * np.sqrt(w.n)
+    st = sim.stats()
+    say(f"  stats {st}")
+    sim.velocities = S.initial_velocities(w, 3, 300.0) * 0.1
+    sim.run(20, 0.002, graph=10)
+    x = sim.positions
+    f = w.waters
+    doh = np.linalg.norm(x[f + 1] - x[f], axis=1)
+    assert np.abs(doh - 0.09572).max() < 1e-9
