@@ -267,6 +267,72 @@ int mdrtHipStats(void *system, int64_t *stats /* 10 */);
    gather), integrate_second. */
 int mdrtHipProfileStep(void *system, int32_t repeats, double *ms /* 10 */);
 
+/* ---------------------------------------------------------------------
+ * 3. Frame evaluator: energies, forces and reverse products of many frames
+ *    of one system (shared topology and box) in one launch per kernel.
+ *
+ *    Replaces the device frame batch of the reference's compiled entry
+ *    (lib/Compiler/Simulation.cpp:3189 evaluateFrames; per-frame body
+ *    lib/Driver/FrameBatchEvaluationIR.cpp:20-60) behind the Python
+ *    FrameEvaluator (python/mdir/_frames.py:458, evaluate :715,
+ *    FrameEnergies.vjp :327) and the torch operators mdir::frame_vjp and
+ *    mdir::frame_hvp (python/mdir/torch.py:139-364).
+ *
+ *    The parameters theta are the per-atom charges and the per-type sigma
+ *    and epsilon of the Lennard-Jones pairs, combined by the
+ *    Lorentz-Berthelot rules (the type-pair tables of the description must
+ *    follow those rules from their diagonal wherever epsilon is not zero).
+ *    The 1-4 pair parameters are given per pair and are not tunable.
+ *    Arrays of frames are frame-major: positions [count][n][3].  Frame
+ *    arrays (positions, forces, cotangents, reference forces, v, out) may be
+ *    host or device pointers; the others are host pointers.
+ * ------------------------------------------------------------------- */
+
+/* A frame evaluator for up to max_frames frames per call; NULL on error
+   (mdrtHipLastError).  The description's positions, velocities, waters and
+   skins are not used. */
+void *mdrtHipFramesCreate(const mdrtHipSystemDesc *desc, int32_t max_frames);
+void mdrtHipFramesDestroy(void *frames);
+
+/* Sets (any non-NULL of) the charges [n] and the per-type sigma [ntypes]
+   and epsilon [ntypes]; the loaded frames must be evaluated again. */
+int mdrtHipFramesSetParameters(void *frames, const double *charges, const double *sigma,
+                               const double *epsilon);
+int mdrtHipFramesGetParameters(void *frames, double *charges, double *sigma,
+                               double *epsilon);
+
+/* Energies [count][MDRT_NUM_ENERGIES] (kinetic 0) and forces [count][n][3]
+   of `count` frames (either output may be NULL).  Loads the frames for
+   mdrtHipFramesVjp and mdrtHipFramesHvp. */
+int mdrtHipFramesEvaluate(void *frames, int32_t count, const double *positions,
+                          double *energies, double *forces);
+
+/* The product with the parameters, summed over the loaded frames,
+     p = sum_n [ g_n dU_n/dtheta + sum_i G_ni . d2U_n/(dx_i dtheta) ],
+   for energy cotangents g [count] (NULL: zeros) and cotangents G
+   [count][n][3] on the position gradients dU/dx (NULL: zeros; the
+   cotangent on the forces with the opposite sign): the reference's
+   FrameEnergies.vjp(cotangent, positions=G) (_frames.py:327).  Outputs
+   d_charges [n], d_sigma [ntypes], d_epsilon [ntypes] (any may be NULL;
+   d_epsilon of a type with epsilon 0 is reported as 0). */
+int mdrtHipFramesVjp(void *frames, int32_t count, const double *energy_cotangent,
+                     const double *position_cotangent, double *d_charges, double *d_sigma,
+                     double *d_epsilon);
+
+/* out[n] = H_n v[n], H_n the Hessian of U_n in the positions, for the
+   loaded frames (the reference's frame_hvp, torch.py:341; with the
+   reciprocal PME sum included). */
+int mdrtHipFramesHvp(void *frames, int32_t count, const double *v, double *out);
+
+/* Force matching in one call: evaluates the frames, the loss
+   loss[n] = sum_i |F_ni - F_ref,ni|^2 and its parameter gradient (the
+   product above with G = -2 (F - F_ref), g = 0).  graph != 0 captures the
+   sequence in a HIP graph at the first call with this count and replays
+   it afterwards. */
+int mdrtHipFramesForceMatching(void *frames, int32_t count, const double *positions,
+                               const double *reference_forces, int32_t graph, double *loss,
+                               double *d_charges, double *d_sigma, double *d_epsilon);
+
 #ifdef __cplusplus
 }
 #endif

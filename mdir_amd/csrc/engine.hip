@@ -231,6 +231,93 @@ struct Engine {
   IntParams ip{};
   double dt_graph = -1;
 
+  // ---- frame batches ----
+  // The frame-path launches run nz frames (blockIdx.z); the per-frame
+  // buffers of frame f lie f * fs bytes after frame 0's (one system: 1, 0).
+  int nz = 1;
+  size_t fs = 0;
+  char *d_arena = nullptr;             // nz slabs of fs bytes
+  std::vector<size_t> frame_bytes;     // per-frame size of each buffer
+  std::vector<void **> frame_extra;    // buffers the frame evaluator adds
+
+  // Every buffer a frame's build, pair loop and PME chain own.
+  std::vector<void **> frame_buffers() {
+    std::vector<void **> v = {
+        (void **)&d_x, (void **)&d_xref, (void **)&d_shift, (void **)&d_place_of,
+        (void **)&d_atom_at, (void **)&d_type_at, (void **)&d_xq, (void **)&d_xpq,
+        (void **)&d_F, (void **)&d_col_count, (void **)&d_bucket, (void **)&d_bb_lo,
+        (void **)&d_bb_hi, (void **)&d_cluster_col, (void **)&d_real_mask,
+        (void **)&d_col_cl_start, (void **)&d_col_cl_count, (void **)&d_counts,
+        (void **)&d_rows, (void **)&d_row_count, (void **)&d_ex, (void **)&d_epair,
+        (void **)&d_flags};
+    if (nspecial) v.push_back((void **)&d_espec);
+    if (use_pme) {
+      v.push_back((void **)&d_cplx);
+      v.push_back((void **)&d_erecip);
+      v.push_back((void **)&bp.count);
+      v.push_back((void **)&bp.list);
+    }
+    for (void **p : frame_extra) v.push_back(p);
+    return v;
+  }
+
+  // Moves the per-frame buffers into one arena of `frames` slabs, each slab
+  // starting as a copy of the single system's buffers.
+  void batch(int frames) {
+    if (nz != 1 || d_arena) throw HipError("already batched");
+    if (frames < 1) throw HipError("a batch needs at least one frame");
+    if (use_pme && !plane_fft())
+      throw HipError("frame batches need a PME grid whose xy plane the plane transform holds");
+    if (prune_skin > 0) throw HipError("frame batches build their lists without pruning");
+    std::vector<void **> bufs = frame_buffers();
+    std::vector<size_t> off(bufs.size());
+    frame_bytes.assign(bufs.size(), 0);
+    size_t total = 0;
+    for (size_t i = 0; i < bufs.size(); ++i) {
+      void *base = nullptr;
+      size_t size = 0;
+      HIPCHK(hipMemGetAddressRange(&base, &size, *bufs[i]));
+      if (base != *bufs[i]) throw HipError("a per-frame buffer is not an allocation");
+      frame_bytes[i] = size;
+      off[i] = total;
+      total += (size + 255) & ~(size_t)255;
+    }
+    char *arena = dalloc<char>((size_t)frames * total);
+    for (int f = 0; f < frames; ++f)
+      for (size_t i = 0; i < bufs.size(); ++i)
+        on_device(arena + (size_t)f * total + off[i], *bufs[i], frame_bytes[i]);
+    HIPCHK(hipStreamSynchronize(stream));
+    for (size_t i = 0; i < bufs.size(); ++i) {
+      dfree(*bufs[i]);
+      *bufs[i] = arena + off[i];
+    }
+    d_arena = arena;
+    nz = frames;
+    fs = total;
+    invalidate_graph();
+  }
+
+  // Back to one system: the buffers of frame 0, each in its own allocation.
+  void unbatch() {
+    if (!d_arena) return;
+    std::vector<void **> bufs = frame_buffers();
+    for (size_t i = 0; i < bufs.size(); ++i) {
+      char *p = dalloc<char>(frame_bytes[i]);
+      on_device(p, *bufs[i], frame_bytes[i]);
+      *bufs[i] = p;
+    }
+    HIPCHK(hipStreamSynchronize(stream));
+    dfree(d_arena);
+    nz = 1;
+    fs = 0;
+    invalidate_graph();
+  }
+
+  void invalidate_graph() {
+    if (graph_exec) hipGraphExecDestroy(graph_exec);
+    graph_exec = nullptr;
+  }
+
   bool forces_valid = false;
   hipGraphExec_t graph_exec = nullptr;
   int graph_steps = 0;
@@ -252,6 +339,12 @@ struct Engine {
   }
 
   void release() {
+    if (d_arena) {  // per-frame pointers lie inside the arena
+      for (void **b : frame_buffers()) *b = nullptr;
+      dfree(d_arena);
+      nz = 1;
+      fs = 0;
+    }
     if (graph_exec) hipGraphExecDestroy(graph_exec);
     graph_exec = nullptr;
     dfree(d_timing);
@@ -445,15 +538,7 @@ struct Engine {
     d_excl_start = dupload(es);
     d_excl_atoms = dupload(ea);
 
-    // Lennard-Jones table (6 c6, 12 c12) with c6 = 4 eps sigma^6, c12 = 4 eps
-    // sigma^12: the factors of the force, so that it needs no scaling
-    std::vector<float2> lj(ntypes * ntypes);
-    for (int k = 0; k < ntypes * ntypes; ++k) {
-      const double s6 = pow(h_sigma[k], 6.0);
-      lj[k] = make_float2((float)(6.0 * 4.0 * h_eps[k] * s6),
-                          (float)(12.0 * 4.0 * h_eps[k] * s6 * s6));
-    }
-    d_lj = dupload(lj);
+    upload_lj();
 
     // nonbonded constants
     for (int k = 0; k < 3; ++k) {
@@ -535,36 +620,7 @@ struct Engine {
       d_espec = dalloc<double>(3 * (size_t)std::max(1, blocks_for(nspecial)));
     }
 
-    // host constants: self + background, dispersion correction
-    {
-      double sq = 0, net = 0;
-      for (double qq : h_q) {
-        sq += qq * qq;
-        net += qq;
-      }
-      const double volume = box[0] * box[1] * box[2];
-      e_self = beta > 0 ? -kCoulomb * beta / sqrt(M_PI) * sq -
-                              kCoulomb * M_PI * net * net / (2.0 * volume * beta * beta)
-                        : 0.0;
-      e_disp = 0;
-      if (dispersion) {
-        std::vector<double> num(ntypes, 0.0);
-        for (int t : h_type) num[t] += 1.0;
-        double sum = 0;
-        for (int a = 0; a < ntypes; ++a)
-          for (int b = 0; b < ntypes; ++b)
-            sum += num[a] * (num[b] - (a == b ? 1.0 : 0.0)) * 4.0 * h_eps[a * ntypes + b] *
-                   pow(h_sigma[a * ntypes + b], 6.0);
-        const size_t ne = h_excl_pairs.size() / 2;
-        for (size_t e = 0; e < ne; ++e) {
-          const int a = h_type[h_excl_pairs[2 * e]], b = h_type[h_excl_pairs[2 * e + 1]];
-          sum -= 2.0 * 4.0 * h_eps[a * ntypes + b] * pow(h_sigma[a * ntypes + b], 6.0);
-        }
-        const double nn = n, pairs = nn * (nn - 1.0) - 2.0 * (double)ne;
-        const double mean = pairs > 0 ? sum / pairs : 0.0;
-        e_disp = -2.0 * M_PI / (3.0 * volume) * nn * nn * mean / (cutoff * cutoff * cutoff);
-      }
-    }
+    host_constants();
 
     // PME
     use_pme = beta > 0;
@@ -633,6 +689,54 @@ struct Engine {
     HIPCHK(hipStreamSynchronize(stream));
   }
 
+  // Lennard-Jones table (6 c6, 12 c12) with c6 = 4 eps sigma^6, c12 = 4 eps
+  // sigma^12: the factors of the force, so that it needs no scaling.  Written
+  // in place once allocated (captured graphs hold the pointer).
+  void upload_lj() {
+    std::vector<float2> lj(ntypes * ntypes);
+    for (int k = 0; k < ntypes * ntypes; ++k) {
+      const double s6 = pow(h_sigma[k], 6.0);
+      lj[k] = make_float2((float)(6.0 * 4.0 * h_eps[k] * s6),
+                          (float)(12.0 * 4.0 * h_eps[k] * s6 * s6));
+    }
+    if (!d_lj) d_lj = dalloc<float2>(lj.size());
+    to_device(d_lj, lj.data(), lj.size() * sizeof(float2));
+  }
+
+  // Host constants: the Ewald self and net-charge terms, the dispersion
+  // correction.
+  void host_constants() {
+    {
+      double sq = 0, net = 0;
+      for (double qq : h_q) {
+        sq += qq * qq;
+        net += qq;
+      }
+      const double volume = box[0] * box[1] * box[2];
+      e_self = beta > 0 ? -kCoulomb * beta / sqrt(M_PI) * sq -
+                              kCoulomb * M_PI * net * net / (2.0 * volume * beta * beta)
+                        : 0.0;
+      e_disp = 0;
+      if (dispersion) {
+        std::vector<double> num(ntypes, 0.0);
+        for (int t : h_type) num[t] += 1.0;
+        double sum = 0;
+        for (int a = 0; a < ntypes; ++a)
+          for (int b = 0; b < ntypes; ++b)
+            sum += num[a] * (num[b] - (a == b ? 1.0 : 0.0)) * 4.0 * h_eps[a * ntypes + b] *
+                   pow(h_sigma[a * ntypes + b], 6.0);
+        const size_t ne = h_excl_pairs.size() / 2;
+        for (size_t e = 0; e < ne; ++e) {
+          const int a = h_type[h_excl_pairs[2 * e]], b = h_type[h_excl_pairs[2 * e + 1]];
+          sum -= 2.0 * 4.0 * h_eps[a * ntypes + b] * pow(h_sigma[a * ntypes + b], 6.0);
+        }
+        const double nn = n, pairs = nn * (nn - 1.0) - 2.0 * (double)ne;
+        const double mean = pairs > 0 ? sum / pairs : 0.0;
+        e_disp = -2.0 * M_PI / (3.0 * volume) * nn * nn * mean / (cutoff * cutoff * cutoff);
+      }
+    }
+  }
+
   void make_influence_tables() {
     const int Kmax = std::max({pme_grid[0], pme_grid[1], pme_grid[2]});
     std::vector<float> tab(6 * (size_t)Kmax, 0.0f);
@@ -673,23 +777,23 @@ struct Engine {
 
   // ---- launches ----
 
-  void request_build() { nb_request_build<<<1, 64, 0, stream>>>(d_flags); }
+  void request_build() { nb_request_build<<<dim3(1, 1, nz), 64, 0, stream>>>(d_flags, fs); }
 
   void launch_bin() {
-    nb_bin<<<blocks_for(n), 256, 0, stream>>>(n, d_x, d_shift, d_xref, d_col_count,
-                                               d_bucket, d_flags, g);
+    nb_bin<<<dim3(blocks_for(n), 1, nz), 256, 0, stream>>>(n, d_x, d_shift, d_xref,
+                                                          d_col_count, d_bucket, d_flags, g, fs);
   }
   void launch_sort() {
-    nb_sort<<<g.ncol, 256, 0, stream>>>(d_x, d_shift, d_q, d_type, d_col_count, d_bucket,
-                                         d_atom_at, d_place_of, d_xq, d_xpq, d_type_at, d_bb_lo,
-                                         d_bb_hi, d_cluster_col, d_real_mask, d_col_cl_start,
-                                         d_col_cl_count, d_counts, d_flags, g);
+    nb_sort<<<dim3(g.ncol, 1, nz), 256, 0, stream>>>(
+        d_x, d_shift, d_q, d_type, d_col_count, d_bucket, d_atom_at, d_place_of, d_xq, d_xpq,
+        d_type_at, d_bb_lo, d_bb_hi, d_cluster_col, d_real_mask, d_col_cl_start,
+        d_col_cl_count, d_counts, d_flags, g, fs);
   }
   void launch_search() {
-    nb_search<<<blocks_for(max_groups, 4), 256, 0, stream>>>(
+    nb_search<<<dim3(blocks_for(max_groups, 4), 1, nz), 256, 0, stream>>>(
         d_bb_lo, d_bb_hi, d_cluster_col, d_real_mask, d_col_cl_start, d_col_cl_count,
         d_atom_at, d_place_of, d_excl_start, d_excl_atoms, d_counts, d_col_count, d_rows,
-        d_row_count, d_ex, ex_cap, d_flags, g);
+        d_row_count, d_ex, ex_cap, d_flags, g, fs);
   }
   void launch_prune() {
     if (prune_skin <= 0) return;
@@ -722,12 +826,12 @@ struct Engine {
     sa.place_of = d_place_of;
     sa.energy_partial = ENERGY ? d_espec : nullptr;
     sa.p = spp;
-    const dim3 grid(std::max(blocks_for(max_groups, 4), blocks_for(nspecial)), nseg + 1);
+    const dim3 grid(std::max(blocks_for(max_groups, 4), blocks_for(nspecial)), nseg + 1, nz);
     const bool pruned = prune_skin > 0;
     nb_pairs<MOD, COUL, ENERGY><<<grid, 256, ntypes * ntypes * sizeof(float2), stream>>>(
         d_xq, d_type_at, d_real_mask, pruned ? d_rows_in : d_rows,
         pruned ? d_row_count_in : d_row_count, d_ex, g.row_cap, max_groups, d_counts, d_lj,
-        nbp, d_F, d_epair, d_flags, sa);
+        nbp, d_F, d_epair, d_flags, sa, fs);
   }
 
   template <int MOD>
@@ -784,8 +888,8 @@ struct Engine {
   // Lists the atoms of every brick their stencils touch.
   template <int N, int TL>
   void bin_t() {
-    pme_bin<N, TL><<<blocks_for(max_places), 256, 0, pstream>>>(d_counts, d_xpq, pmp, bp,
-                                                               d_flags);
+    pme_bin<N, TL><<<dim3(blocks_for(max_places), 1, nz), 256, 0, pstream>>>(
+        d_counts, d_xpq, pmp, bp, d_flags, fs);
   }
   template <int N>
   void bin_n() {
@@ -808,8 +912,8 @@ struct Engine {
     static_assert(BX * BY == R, "brick shape");
     const int blocks = ((pme_grid[0] + BX - 1) / BX) * ((pme_grid[1] + BY - 1) / BY);
     const size_t lds = (size_t)(32 * TL + 8) * pme_grid[2];
-    pme_spread_rows<N, TL><<<blocks, 256, lds, pstream>>>(d_xpq, pmp, bp, d_cplx,
-                                                         fft.ax[2].axis);
+    pme_spread_rows<N, TL><<<dim3(blocks, 1, nz), 256, lds, pstream>>>(
+        d_xpq, pmp, bp, d_cplx, fft.ax[2].axis, fs);
   }
   template <int N>
   void spread_rows_n() {
@@ -825,8 +929,8 @@ struct Engine {
     constexpr int BX = BrickShape<TL>::BX, BY = BrickShape<TL>::BY;
     const int blocks = ((pme_grid[0] + BX - 1) / BX) * ((pme_grid[1] + BY - 1) / BY);
     const size_t lds = (size_t)(16 * TL + 8) * pme_grid[2];
-    pme_gather_rows<N, TL><<<blocks, 256, lds, pstream>>>(d_xpq, pmp, bp, d_cplx,
-                                                         fft.ax[2].axis, d_F);
+    pme_gather_rows<N, TL><<<dim3(blocks, 1, nz), 256, lds, pstream>>>(
+        d_xpq, pmp, bp, d_cplx, fft.ax[2].axis, d_F, fs);
   }
   template <int N>
   void gather_rows_n() {
@@ -865,8 +969,8 @@ struct Engine {
     double *ep = energy ? d_erecip : nullptr;
     if (plane_fft()) {
       const size_t lds = (size_t)2 * pme_grid[0] * pme_grid[1] * sizeof(cpx<float>);
-      fft_plane_conv<<<fft.H(), 512, lds, pstream>>>(d_cplx, fft.K[0], fft.K[1], fft.ax[0].axis,
-                                                     fft.ax[1].axis, ct, ep);
+      fft_plane_conv<<<dim3(fft.H(), 1, nz), 512, lds, pstream>>>(
+          d_cplx, fft.K[0], fft.K[1], fft.ax[0].axis, fft.ax[1].axis, ct, ep, fs);
       return;
     }
     const int TL = Fft3D<float>::TL;
@@ -906,7 +1010,7 @@ struct Engine {
   }
 
   void clear_forces() {
-    clear_words<<<256, 256, 0, stream>>>(d_F, 3 * (size_t)max_places);
+    clear_words<<<dim3(256, 1, nz), 256, 0, stream>>>(d_F, 3 * (size_t)max_places, fs);
   }
 
   template <int PART>
@@ -1030,7 +1134,7 @@ struct Engine {
     evaluate(true);
     if (F_out) {
       double *d_out = dalloc<double>(3 * (size_t)n);
-      forces_to_atoms<<<blocks_for(n), 256, 0, stream>>>(n, d_place_of, d_F, d_out);
+      forces_to_atoms<<<blocks_for(n), 256, 0, stream>>>(n, d_place_of, d_F, d_out, 0);
       HIPCHK(hipMemcpyAsync(F_out, d_out, 3 * n * sizeof(double), hipMemcpyDeviceToHost, stream));
       HIPCHK(hipStreamSynchronize(stream));
       dfree(d_out);
@@ -1324,7 +1428,10 @@ struct Engine {
 
 }  // namespace mdhip
 
+#include "frames.hip"
+
 using mdhip::Engine;
+using mdhip::FrameBatch;
 using mdhip::g_last_error;
 
 template <typename F>
@@ -1526,6 +1633,62 @@ int mdrtHipNeighborPairs(void *system, double reach, int32_t excluded, int64_t c
 int mdrtHipStats(void *system, int64_t *stats) {
   Engine *e = static_cast<Engine *>(system);
   return guarded([&] { e->stats(stats); });
+}
+
+void *mdrtHipFramesCreate(const mdrtHipSystemDesc *desc, int32_t max_frames) {
+  FrameBatch *f = new FrameBatch();
+  if (guarded([&] { f->create(desc, max_frames); }) != 0) {
+    delete f;
+    return nullptr;
+  }
+  return f;
+}
+
+void mdrtHipFramesDestroy(void *frames) { delete static_cast<FrameBatch *>(frames); }
+
+int mdrtHipFramesSetParameters(void *frames, const double *charges, const double *sigma,
+                               const double *epsilon) {
+  FrameBatch *f = static_cast<FrameBatch *>(frames);
+  return guarded([&] { f->set_parameters(charges, sigma, epsilon); });
+}
+
+int mdrtHipFramesGetParameters(void *frames, double *charges, double *sigma, double *epsilon) {
+  FrameBatch *f = static_cast<FrameBatch *>(frames);
+  return guarded([&] {
+    if (charges) std::copy(f->e.h_q.begin(), f->e.h_q.end(), charges);
+    if (sigma) std::copy(f->sig_t.begin(), f->sig_t.end(), sigma);
+    if (epsilon) std::copy(f->eps_t.begin(), f->eps_t.end(), epsilon);
+  });
+}
+
+int mdrtHipFramesEvaluate(void *frames, int32_t count, const double *positions,
+                          double *energies, double *forces) {
+  FrameBatch *f = static_cast<FrameBatch *>(frames);
+  return guarded([&] { f->evaluate(count, positions, energies, forces); });
+}
+
+int mdrtHipFramesVjp(void *frames, int32_t count, const double *energy_cotangent,
+                     const double *position_cotangent, double *d_charges, double *d_sigma,
+                     double *d_epsilon) {
+  FrameBatch *f = static_cast<FrameBatch *>(frames);
+  return guarded([&] {
+    f->vjp(count, energy_cotangent, position_cotangent, d_charges, d_sigma, d_epsilon);
+  });
+}
+
+int mdrtHipFramesHvp(void *frames, int32_t count, const double *v, double *out) {
+  FrameBatch *f = static_cast<FrameBatch *>(frames);
+  return guarded([&] { f->hvp(count, v, out); });
+}
+
+int mdrtHipFramesForceMatching(void *frames, int32_t count, const double *positions,
+                               const double *reference_forces, int32_t graph, double *loss,
+                               double *d_charges, double *d_sigma, double *d_epsilon) {
+  FrameBatch *f = static_cast<FrameBatch *>(frames);
+  return guarded([&] {
+    f->force_matching(count, positions, reference_forces, loss, d_charges, d_sigma, d_epsilon,
+                      graph);
+  });
 }
 
 int mdrtHipProfileStep(void *system, int32_t repeats, double *ms) {

@@ -304,14 +304,19 @@ __global__ void __launch_bounds__(64) integrate_part(
 
 // Force words (place order) -> f64 forces (atom order).
 __global__ void forces_to_atoms(int n, const int *__restrict__ place_of,
-                                const long long *__restrict__ F, double *__restrict__ out) {
+                                const long long *__restrict__ F, double *__restrict__ out,
+                                size_t fs) {
+  place_of = frame_at(place_of, fs);
+  F = frame_at(F, fs);
+  out = frame_at(out, fs);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int pl = place_of[i];
   for (int k = 0; k < 3; ++k) out[3 * i + k] = (double)F[3 * pl + k] * kForceUnscale;
 }
 
-__global__ void clear_words(long long *__restrict__ w, size_t count) {
+__global__ void clear_words(long long *__restrict__ w, size_t count, size_t fs) {
+  w = frame_at(w, fs);
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (; i < count; i += stride) w[i] = 0;

@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace mdhip {
 
 // Coulomb constant f = 138.93545755023302 kJ nm mol^-1 e^-2 as the exact
@@ -74,6 +76,18 @@ struct NbParams {
 
 __device__ __forceinline__ long long to_fixed(float f) {
   return __double2ll_rn((double)f * kForceScale);
+}
+
+// Frame batches: a launch over K frames of one system runs frame blockIdx.z,
+// whose per-frame buffers lie fs bytes after those of the frame before
+// (fs = 0 for a single system).  Null pointers stay null.  Byte-pointer
+// arithmetic (not an integer round trip) keeps the pointer's global address
+// space visible to the compiler, so that loads stay global_load.
+template <class T>
+__device__ __forceinline__ T *frame_at(T *p, size_t fs) {
+  using Byte = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return reinterpret_cast<T *>(reinterpret_cast<Byte *>(p) +
+                               (p ? (size_t)blockIdx.z * fs : (size_t)0));
 }
 
 __device__ __forceinline__ void shift_vector(uint32_t code, const float *box,

@@ -34,7 +34,8 @@ struct GridParams {
 constexpr int kSortCapMax = 1024;   // max atoms per column the sort handles
 
 // Sets the rebuild flag (a kernel, not a memset, so that graphs may hold it).
-__global__ void nb_request_build(int *flags) {
+__global__ void nb_request_build(int *flags, size_t fs) {
+  flags = frame_at(flags, fs);
   if (threadIdx.x == 0) flags[kFlagRebuild] = 1;
 }
 
@@ -50,7 +51,14 @@ __global__ void __launch_bounds__(256) nb_bin(int n, const double *__restrict__ 
                                              double *__restrict__ xref,
                                              int *__restrict__ col_count,
                                              int *__restrict__ bucket,
-                                             int *__restrict__ flags, GridParams g) {
+                                             int *__restrict__ flags, GridParams g,
+                                             size_t fs) {
+  x = frame_at(x, fs);
+  shift = frame_at(shift, fs);
+  xref = frame_at(xref, fs);
+  col_count = frame_at(col_count, fs);
+  bucket = frame_at(bucket, fs);
+  flags = frame_at(flags, fs);
   if (flags[kFlagRebuild] == 0) return;
   __shared__ int s_hist[kBinLdsColumns];
   const bool lds = g.ncol <= kBinLdsColumns;
@@ -106,7 +114,24 @@ __global__ void __launch_bounds__(256) nb_sort(
     float4 *__restrict__ bb_lo, float4 *__restrict__ bb_hi,
     int *__restrict__ cluster_col, uint32_t *__restrict__ real_mask,
     int *__restrict__ col_cl_start, int *__restrict__ col_cl_count,
-    int *__restrict__ counts_out, int *__restrict__ flags, GridParams g) {
+    int *__restrict__ counts_out, int *__restrict__ flags, GridParams g, size_t fs) {
+  x = frame_at(x, fs);
+  shift = frame_at(shift, fs);
+  col_count = frame_at(col_count, fs);
+  bucket = frame_at(bucket, fs);
+  atom_at = frame_at(atom_at, fs);
+  place_of = frame_at(place_of, fs);
+  xq = frame_at(xq, fs);
+  xpq = frame_at(xpq, fs);
+  type_at = frame_at(type_at, fs);
+  bb_lo = frame_at(bb_lo, fs);
+  bb_hi = frame_at(bb_hi, fs);
+  cluster_col = frame_at(cluster_col, fs);
+  real_mask = frame_at(real_mask, fs);
+  col_cl_start = frame_at(col_cl_start, fs);
+  col_cl_count = frame_at(col_cl_count, fs);
+  counts_out = frame_at(counts_out, fs);
+  flags = frame_at(flags, fs);
   if (flags[kFlagRebuild] == 0) return;
   __shared__ int s_red[256];
   __shared__ int s_atom[kSortCapMax];
@@ -234,7 +259,21 @@ __global__ void __launch_bounds__(256) nb_search(
     const int *__restrict__ counts_in, int *__restrict__ col_count,
     PairEntry *__restrict__ rows, int *__restrict__ row_count,
     uint64_t *__restrict__ ex_masks, int ex_cap, int *__restrict__ flags,
-    GridParams g) {
+    GridParams g, size_t fs) {
+  bb_lo = frame_at(bb_lo, fs);
+  bb_hi = frame_at(bb_hi, fs);
+  cluster_col = frame_at(cluster_col, fs);
+  real_mask = frame_at(real_mask, fs);
+  col_cl_start = frame_at(col_cl_start, fs);
+  col_cl_count = frame_at(col_cl_count, fs);
+  atom_at = frame_at(atom_at, fs);
+  place_of = frame_at(place_of, fs);
+  counts_in = frame_at(counts_in, fs);
+  col_count = frame_at(col_count, fs);
+  rows = frame_at(rows, fs);
+  row_count = frame_at(row_count, fs);
+  ex_masks = frame_at(ex_masks, fs);
+  flags = frame_at(flags, fs);
   if (flags[kFlagRebuild] == 0) return;
   // The next build starts from empty buckets (nb_sort has read them).
   if (blockIdx.x == 0)

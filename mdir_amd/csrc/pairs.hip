@@ -416,7 +416,21 @@ __global__ void __launch_bounds__(256) nb_pairs(
     const int *__restrict__ row_count, const uint64_t *__restrict__ ex_masks,
     int row_cap, int max_groups, const int *__restrict__ counts_in,
     const float2 *__restrict__ ljtab, NbParams p, long long *__restrict__ F,
-    double *__restrict__ energy_partial, int *__restrict__ flags, SpecialArgs sa) {
+    double *__restrict__ energy_partial, int *__restrict__ flags, SpecialArgs sa,
+    size_t fs) {
+  xq = frame_at(xq, fs);
+  type_at = frame_at(type_at, fs);
+  real_mask = frame_at(real_mask, fs);
+  rows = frame_at(rows, fs);
+  row_count = frame_at(row_count, fs);
+  ex_masks = frame_at(ex_masks, fs);
+  counts_in = frame_at(counts_in, fs);
+  F = frame_at(F, fs);
+  energy_partial = frame_at(energy_partial, fs);
+  flags = frame_at(flags, fs);
+  sa.x = frame_at(sa.x, fs);
+  sa.place_of = frame_at(sa.place_of, fs);
+  sa.energy_partial = frame_at(sa.energy_partial, fs);
   nb_pairs_body<MOD, COUL, ENERGY>(xq, type_at, real_mask, rows, row_count, ex_masks, row_cap,
                                    max_groups, counts_in, ljtab, p, F, energy_partial, flags,
                                    sa);
@@ -434,7 +448,8 @@ __global__ void stamp_clock(unsigned long long *__restrict__ slot) {
 #define MDHIP_PAIRS_ONE(MOD, C, E)                                                       \
   template __global__ void nb_pairs<MOD, C, E>(const float4 *, const int *,            \
       const uint32_t *, const PairEntry *, const int *, const uint64_t *, int, int,    \
-      const int *, const float2 *, NbParams, long long *, double *, int *, SpecialArgs);
+      const int *, const float2 *, NbParams, long long *, double *, int *, SpecialArgs,     \
+      size_t);
 #define MDHIP_PAIRS(MOD)                                                                 \
   MDHIP_PAIRS_ONE(MOD, true, false)                                                      \
   MDHIP_PAIRS_ONE(MOD, true, true)                                                       \

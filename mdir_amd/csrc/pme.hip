@@ -134,9 +134,14 @@ template <int N, int TL>
 __global__ void __launch_bounds__(256) pme_bin(const int *__restrict__ counts,
                                                const double4 *__restrict__ xpq,
                                                PmeParams pp, BrickParams bp,
-                                               int *__restrict__ flags) {
+                                               int *__restrict__ flags, size_t fs) {
   constexpr int BX = BrickShape<TL>::BX, BY = BrickShape<TL>::BY;
   __shared__ int s_count[kBinLdsBricks];
+  counts = frame_at(counts, fs);
+  xpq = frame_at(xpq, fs);
+  bp.count = frame_at(bp.count, fs);
+  bp.list = frame_at(bp.list, fs);
+  flags = frame_at(flags, fs);
   const int nbricks = bp.nbx * bp.nby;
   const bool lds = nbricks <= kBinLdsBricks;
   if (lds)
@@ -225,36 +230,18 @@ __device__ __forceinline__ void spread_atom(double4 pq, const PmeParams &pp, int
   }
 }
 
-template <int N, int TL>
-__global__ void __launch_bounds__(256) pme_spread_rows(
-    const double4 *__restrict__ xpq, PmeParams pp, BrickParams bp,
-    cpx<float> *__restrict__ out, FftAxis az) {
-  constexpr int R = BrickShape<TL>::R, BX = BrickShape<TL>::BX, BY = BrickShape<TL>::BY;
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+// The brick's R accumulated real rows (64-bit words in acc) to their half
+// spectra along z in `out`: rows 2t and 2t+1 as the real and imaginary
+// parts of one complex line t, one LDS transform, then the two spectra
+// split apart.  a, b hold TL K2 complex values each, tw the K2 twiddles.
+template <int TL>
+__device__ __forceinline__ void brick_forward_z(const unsigned long long *acc,
+                                                cpx<float> *a, cpx<float> *b,
+                                                const cpx<float> *tw, const FftAxis &az,
+                                                const PmeParams &pp, const BrickParams &bp,
+                                                int bx0, int by0, cpx<float> *__restrict__ out) {
+  constexpr int R = BrickShape<TL>::R, BY = BrickShape<TL>::BY;
   const int K0 = pp.K[0], K1 = pp.K[1], K2 = pp.K[2], H = K2 / 2 + 1;
-  unsigned long long *acc = reinterpret_cast<unsigned long long *>(smem);
-  cpx<float> *a = reinterpret_cast<cpx<float> *>(acc + (size_t)R * K2);
-  cpx<float> *b = a + TL * K2;
-  cpx<float> *tw = b + TL * K2;  // the z twiddles, read by every pass
-  const int brick = blockIdx.x;
-  const int bx0 = (brick % bp.nbx) * BX, by0 = (brick / bp.nbx) * BY;
-  for (int w = threadIdx.x; w < R * K2; w += blockDim.x) acc[w] = 0ull;
-  for (int w = threadIdx.x; w < K2; w += blockDim.x)
-    tw[w] = static_cast<const cpx<float> *>(az.tw)[w];
-  int nat = bp.count[brick];
-  nat = nat < bp.cap ? nat : bp.cap;
-  const int *lst = bp.list + (size_t)brick * bp.cap;
-  __syncthreads();
-  // two atoms per thread in flight: their loads are independent
-  for (int i0 = threadIdx.x; i0 < nat; i0 += 2 * blockDim.x) {
-    const int i1 = i0 + blockDim.x;
-    const double4 a0 = xpq[lst[i0]];
-    const double4 a1 = i1 < nat ? xpq[lst[i1]] : make_double4(0, 0, 0, 0);
-    spread_atom<N, BX, BY>(a0, pp, bx0, by0, acc);
-    if (i1 < nat) spread_atom<N, BX, BY>(a1, pp, bx0, by0, acc);
-  }
-  __syncthreads();
-  // rows 2t and 2t+1 of the brick as the real and imaginary parts of line t
   for (int w = threadIdx.x; w < R * K2; w += blockDim.x) {
     const int rr = w / K2, z = w % K2;
     const float v = (float)((double)(long long)acc[w] * kGridUnscale);
@@ -284,6 +271,42 @@ __global__ void __launch_bounds__(256) pme_spread_rows(
                                                             (zm.x - zk.x) * 0.5f};
     }
   }
+}
+
+template <int N, int TL>
+__global__ void __launch_bounds__(256) pme_spread_rows(
+    const double4 *__restrict__ xpq, PmeParams pp, BrickParams bp,
+    cpx<float> *__restrict__ out, FftAxis az, size_t fs) {
+  constexpr int R = BrickShape<TL>::R, BX = BrickShape<TL>::BX, BY = BrickShape<TL>::BY;
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  xpq = frame_at(xpq, fs);
+  bp.count = frame_at(bp.count, fs);
+  bp.list = frame_at(bp.list, fs);
+  out = frame_at(out, fs);
+  const int K2 = pp.K[2];
+  unsigned long long *acc = reinterpret_cast<unsigned long long *>(smem);
+  cpx<float> *a = reinterpret_cast<cpx<float> *>(acc + (size_t)R * K2);
+  cpx<float> *b = a + TL * K2;
+  cpx<float> *tw = b + TL * K2;  // the z twiddles, read by every pass
+  const int brick = blockIdx.x;
+  const int bx0 = (brick % bp.nbx) * BX, by0 = (brick / bp.nbx) * BY;
+  for (int w = threadIdx.x; w < R * K2; w += blockDim.x) acc[w] = 0ull;
+  for (int w = threadIdx.x; w < K2; w += blockDim.x)
+    tw[w] = static_cast<const cpx<float> *>(az.tw)[w];
+  int nat = bp.count[brick];
+  nat = nat < bp.cap ? nat : bp.cap;
+  const int *lst = bp.list + (size_t)brick * bp.cap;
+  __syncthreads();
+  // two atoms per thread in flight: their loads are independent
+  for (int i0 = threadIdx.x; i0 < nat; i0 += 2 * blockDim.x) {
+    const int i1 = i0 + blockDim.x;
+    const double4 a0 = xpq[lst[i0]];
+    const double4 a1 = i1 < nat ? xpq[lst[i1]] : make_double4(0, 0, 0, 0);
+    spread_atom<N, BX, BY>(a0, pp, bx0, by0, acc);
+    if (i1 < nat) spread_atom<N, BX, BY>(a1, pp, bx0, by0, acc);
+  }
+  __syncthreads();
+  brick_forward_z<TL>(acc, a, b, tw, az, pp, bp, bx0, by0, out);
 }
 
 // ---- backward z transform fused with the gather ---------------------------
@@ -361,26 +384,17 @@ __device__ __forceinline__ void gather_atom(double4 pq, int place, const PmePara
   atomicAdd(u + 2, (unsigned long long)to_fixed(-qf * (float)(K2 * pp.inv_box[2]) * g2));
 }
 
-template <int N, int TL>
-__global__ void __launch_bounds__(256) pme_gather_rows(
-    const double4 *__restrict__ xpq, PmeParams pp, BrickParams bp,
-    const cpx<float> *__restrict__ in, FftAxis az, long long *__restrict__ F) {
-  constexpr int BX = BrickShape<TL>::BX, BY = BrickShape<TL>::BY;
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+// The brick's half-spectrum rows of `in` back to real rows in LDS (one
+// complex line t per pair of rows 2t, 2t+1): returns the buffer (a or b)
+// holding the rows, phi of brick row r at z being phi[z TL + r/2], .x for
+// even r and .y for odd r.
+template <int TL>
+__device__ __forceinline__ const cpx<float> *brick_backward_z(
+    const cpx<float> *__restrict__ in, cpx<float> *a, cpx<float> *b, const cpx<float> *tw,
+    const FftAxis &az, const PmeParams &pp, const BrickParams &bp, int bx0, int by0) {
+  constexpr int BY = BrickShape<TL>::BY;
   const int K0 = pp.K[0], K1 = pp.K[1], K2 = pp.K[2], H = K2 / 2 + 1;
-  cpx<float> *a = reinterpret_cast<cpx<float> *>(smem);
-  cpx<float> *b = a + TL * K2;
-  cpx<float> *tw = b + TL * K2;  // the z twiddles, read by every pass
-  const int brick = blockIdx.x;
-  const int bx0 = (brick % bp.nbx) * BX, by0 = (brick / bp.nbx) * BY;
   const size_t rows = (size_t)K0 * K1;
-  for (int w = threadIdx.x; w < K2; w += blockDim.x)
-    tw[w] = static_cast<const cpx<float> *>(az.tw)[w];
-  int nat = bp.count[brick];
-  nat = nat < bp.cap ? nat : bp.cap;
-  const int *lst = bp.list + (size_t)brick * bp.cap;
-
-  // half spectra of rows 2t and 2t+1 as one complex line t
   for (int w = threadIdx.x; w < TL * K2; w += blockDim.x) {
     const int k = w % K2, t = w / K2;
     const int r1 = 2 * t, r2 = r1 + 1;
@@ -404,7 +418,32 @@ __global__ void __launch_bounds__(256) pme_gather_rows(
                           : cpx<float>{x1.x + x2.y, -x1.y + x2.x};
   }
   __syncthreads();
-  const cpx<float> *phi = lds_fft<float, TL>(a, b, az, +1, tw);
+  return lds_fft<float, TL>(a, b, az, +1, tw);
+}
+
+template <int N, int TL>
+__global__ void __launch_bounds__(256) pme_gather_rows(
+    const double4 *__restrict__ xpq, PmeParams pp, BrickParams bp,
+    const cpx<float> *__restrict__ in, FftAxis az, long long *__restrict__ F, size_t fs) {
+  constexpr int BX = BrickShape<TL>::BX, BY = BrickShape<TL>::BY;
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  xpq = frame_at(xpq, fs);
+  bp.count = frame_at(bp.count, fs);
+  bp.list = frame_at(bp.list, fs);
+  in = frame_at(in, fs);
+  F = frame_at(F, fs);
+  const int K2 = pp.K[2];
+  cpx<float> *a = reinterpret_cast<cpx<float> *>(smem);
+  cpx<float> *b = a + TL * K2;
+  cpx<float> *tw = b + TL * K2;  // the z twiddles, read by every pass
+  const int brick = blockIdx.x;
+  const int bx0 = (brick % bp.nbx) * BX, by0 = (brick / bp.nbx) * BY;
+  for (int w = threadIdx.x; w < K2; w += blockDim.x)
+    tw[w] = static_cast<const cpx<float> *>(az.tw)[w];
+  int nat = bp.count[brick];
+  nat = nat < bp.cap ? nat : bp.cap;
+  const int *lst = bp.list + (size_t)brick * bp.cap;
+  const cpx<float> *phi = brick_backward_z<TL>(in, a, b, tw, az, pp, bp, bx0, by0);
   for (int i0 = threadIdx.x; i0 < nat; i0 += 2 * blockDim.x) {
     const int i1 = i0 + blockDim.x;
     const int p0 = lst[i0];
@@ -622,9 +661,12 @@ constexpr int kPlaneAxisMax = 256;
 __global__ void __launch_bounds__(512) fft_plane_conv(cpx<float> *__restrict__ data, int K1,
                                                       int K2, FftAxis ax, FftAxis ay,
                                                       ConvTables ct,
-                                                      double *__restrict__ energy_partial) {
+                                                      double *__restrict__ energy_partial,
+                                                      size_t fs) {
   extern __shared__ unsigned char smem[];
   __shared__ double s_e[512];
+  data = frame_at(data, fs);
+  energy_partial = frame_at(energy_partial, fs);
   __shared__ cpx<float> s_twx[kPlaneAxisMax], s_twy[kPlaneAxisMax];
   __shared__ float s_t0[kPlaneAxisMax], s_m0[kPlaneAxisMax];
   __shared__ float s_t1[kPlaneAxisMax], s_m1[kPlaneAxisMax];
@@ -687,11 +729,11 @@ __global__ void __launch_bounds__(512) fft_plane_conv(cpx<float> *__restrict__ d
 template __global__ void fft_rows_r2c<float, true>(const void *, cpx<float> *, int, FftAxis);
 #define MDHIP_SPREAD(N, TL)                                                              \
   template __global__ void pme_bin<N, TL>(const int *, const double4 *, PmeParams,       \
-      BrickParams, int *);                                                               \
+      BrickParams, int *, size_t);                                                       \
   template __global__ void pme_spread_rows<N, TL>(const double4 *, PmeParams,            \
-      BrickParams, cpx<float> *, FftAxis);                                               \
+      BrickParams, cpx<float> *, FftAxis, size_t);                                       \
   template __global__ void pme_gather_rows<N, TL>(const double4 *, PmeParams,            \
-      BrickParams, const cpx<float> *, FftAxis, long long *);
+      BrickParams, const cpx<float> *, FftAxis, long long *, size_t);
 #define MDHIP_SPREAD_N(N) MDHIP_SPREAD(N, 8) MDHIP_SPREAD(N, 4) MDHIP_SPREAD(N, 2) MDHIP_SPREAD(N, 1)
 MDHIP_SPREAD_N(4)
 MDHIP_SPREAD_N(6)

@@ -1,0 +1,259 @@
+"""Frame evaluator: energies, forces and reverse products of many frames of
+one system on one GPU (libmdrt_hip.so, include/mdrt_hip.h group 3).
+
+Mirrors the reference's Python frame API (python/mdir/_frames.py:458
+FrameEvaluator, :715 evaluate, :272 FrameEnergies, :327 vjp, :220
+FrameGradient): evaluate a batch of frames, then take the product of a
+cotangent on the energies and on their position gradients with the
+derivatives in the tunables,
+
+    p = sum_n [ g_n dU_n/dtheta + sum_i G_ni . d2U_n/(dx_i dtheta) ],
+
+and, with gradients=("positions",), the position part
+g_n dU_n/dx + H_n G_n.  The tunables are "charges" (per atom), "sigma" and
+"epsilon" (per Lennard-Jones type, Lorentz-Berthelot pairs).  Unlike the
+reference (which refuses the charges under PME in the reverse product, and
+the reciprocal sum in the position Hessian), both include the PME
+reciprocal sum here.
+
+All frames of one call run in one launch per kernel (the frame index is the
+grid's z dimension); the force-matching call runs the forward pass, the
+residual and the adjoint as one captured HIP graph.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from collections.abc import Mapping
+
+import numpy as np
+
+from . import runtime as rt
+from .systems import System
+
+TUNABLES = ("charges", "sigma", "epsilon")
+
+
+def _frame_array(a, shape, name):
+    """(pointer, keep-alive, is_device) of a float64 frame array: a NumPy
+    array (host) or a CUDA torch tensor (device)."""
+    if hasattr(a, "data_ptr") and hasattr(a, "is_cuda"):
+        import torch
+        if a.dtype != torch.float64:
+            raise TypeError(f"{name}: expected float64, found {a.dtype}")
+        if tuple(a.shape) != tuple(shape):
+            raise ValueError(f"{name}: expected shape {tuple(shape)}, found {tuple(a.shape)}")
+        a = a.contiguous()
+        if a.is_cuda:  # the library's stream is not ordered after torch's
+            torch.cuda.synchronize(a.device)
+        return C.c_void_p(a.data_ptr()), a, a.is_cuda
+    a = np.ascontiguousarray(a, np.float64)
+    if a.shape != tuple(shape):
+        raise ValueError(f"{name}: expected shape {tuple(shape)}, found {a.shape}")
+    if not np.all(np.isfinite(a)):
+        raise ValueError(f"{name}: not finite")
+    return C.c_void_p(a.ctypes.data), a, False
+
+
+class FrameGradient(Mapping):
+    """The product in the tunables (read-only arrays by name) and, when
+    asked, `positions` (K, N, 3)."""
+
+    def __init__(self, values, positions=None):
+        self._values = values
+        self.positions = positions
+
+    def __getitem__(self, name):
+        return self._values[name]
+
+    def __iter__(self):
+        return iter(self._values)
+
+    def __len__(self):
+        return len(self._values)
+
+    def __repr__(self):
+        return f"FrameGradient({', '.join(self._values)})"
+
+
+class FrameEnergies:
+    """Energies of the evaluated frames: `energy` (K,), `terms`
+    (K, NUM_ENERGIES) in the order of runtime.ENERGY_NAMES, `forces`
+    (K, N, 3) and `gradients["positions"]` = -forces."""
+
+    def __init__(self, evaluator, count, terms, forces):
+        self._evaluator = evaluator
+        self.count = count
+        self.version = evaluator._version
+        self.terms = terms
+        self.energy = terms.sum(axis=1)
+        self.forces = forces
+        self.gradients = {"positions": -forces} if forces is not None else {}
+
+    def vjp(self, cotangent=None, *, positions=None, gradients=("tunables",)):
+        """sum_n cotangent[n] dU_n/dtheta + sum_n sum_i positions[n, i] .
+        d2U_n/dx_i dtheta (the reference's FrameEnergies.vjp, _frames.py:327);
+        `positions` is the cotangent on dU/dx (minus that on the forces).
+        With "positions" in `gradients`, also cotangent[n] dU_n/dx + H_n
+        positions[n] as the attribute `positions`."""
+        ev = self._evaluator
+        if isinstance(gradients, str):
+            raise ValueError("gradients is a tuple of names among 'tunables' and 'positions'")
+        names = tuple(dict.fromkeys(gradients))
+        for name in names:
+            if name not in ("tunables", "positions"):
+                raise ValueError(f"vjp: unknown gradient {name!r}")
+        if cotangent is None and positions is None:
+            raise ValueError("vjp: give a cotangent on the energies, on their position "
+                             "gradients (positions=), or both")
+        if ev._version != self.version:
+            raise RuntimeError("vjp: the evaluator has evaluated other frames or changed "
+                               "its tunables since")
+        K, n = self.count, ev.n
+        g = None
+        if cotangent is not None:
+            g = np.ascontiguousarray(cotangent, np.float64)
+            if g.shape != (K,) or not np.all(np.isfinite(g)):
+                raise ValueError(f"vjp: expected a finite cotangent of shape ({K},)")
+        values = {}
+        if "tunables" in names:
+            dq = np.zeros(n)
+            ds = np.zeros(ev.ntypes)
+            de = np.zeros(ev.ntypes)
+            if positions is not None:
+                pp, keep, _ = _frame_array(positions, (K, n, 3), "positions")
+            else:
+                pp, keep = None, None
+            rt.check(rt.lib().mdrtHipFramesVjp(
+                ev._h, K, None if g is None else rt.ptr(g, C.c_double), pp,
+                rt.ptr(dq, C.c_double), rt.ptr(ds, C.c_double), rt.ptr(de, C.c_double)))
+            del keep
+            values = {"charges": dq, "sigma": ds, "epsilon": de}
+        dx = None
+        if "positions" in names:
+            dx = np.zeros((K, n, 3))
+            if positions is not None:
+                dx += ev.hvp(positions)
+            if g is not None:
+                dx -= g[:, None, None] * self.forces
+        return FrameGradient(values, dx)
+
+
+class FrameEvaluator:
+    """Frames of one system (shared topology and box) on the GPU, up to
+    `max_frames` per call."""
+
+    def __init__(self, system: System, max_frames: int = 64):
+        self.system = system
+        self.n = system.n
+        self.ntypes = system.ntypes
+        self.max_frames = int(max_frames)
+        d, keep = rt.system_desc(system)
+        h = rt.lib().mdrtHipFramesCreate(C.byref(d), self.max_frames)
+        del keep
+        if not h:
+            raise RuntimeError("mdrt_hip: " + rt.lib().mdrtHipLastError().decode())
+        self._h = h
+        self._version = 0
+        self._loaded = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            rt.lib().mdrtHipFramesDestroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- tunables ----
+    @property
+    def tunables(self) -> dict:
+        q = np.zeros(self.n)
+        s = np.zeros(self.ntypes)
+        e = np.zeros(self.ntypes)
+        rt.check(rt.lib().mdrtHipFramesGetParameters(
+            self._h, rt.ptr(q, C.c_double), rt.ptr(s, C.c_double), rt.ptr(e, C.c_double)))
+        return {"charges": q, "sigma": s, "epsilon": e}
+
+    def update(self, charges=None, sigma=None, epsilon=None):
+        """Sets any of the tunables; the frames must be evaluated again."""
+        def arr(a, size, name):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, np.float64)
+            if a.shape != (size,):
+                raise ValueError(f"{name}: expected shape ({size},)")
+            return a
+        q = arr(charges, self.n, "charges")
+        s = arr(sigma, self.ntypes, "sigma")
+        e = arr(epsilon, self.ntypes, "epsilon")
+        rt.check(rt.lib().mdrtHipFramesSetParameters(
+            self._h, None if q is None else rt.ptr(q, C.c_double),
+            None if s is None else rt.ptr(s, C.c_double),
+            None if e is None else rt.ptr(e, C.c_double)))
+        self._version += 1
+
+    # ---- evaluation ----
+    def _count(self, positions):
+        K = int(positions.shape[0])
+        if not 1 <= K <= self.max_frames:
+            raise ValueError(f"frames: 1 to {self.max_frames} per call, found {K}")
+        return K
+
+    def evaluate(self, positions, forces_out=None) -> FrameEnergies:
+        """Energies and forces of the frames `positions` (K, N, 3); loads
+        them for vjp and hvp.  `forces_out`, a CUDA float64 tensor
+        (K, N, 3), receives the forces on the device instead."""
+        K = self._count(positions)
+        pp, keep, _ = _frame_array(positions, (K, self.n, 3), "positions")
+        terms = np.zeros((K, rt.NUM_ENERGIES))
+        if forces_out is not None:
+            fp, fkeep, _ = _frame_array(forces_out, (K, self.n, 3), "forces_out")
+            F = None
+        else:
+            F = np.zeros((K, self.n, 3))
+            fp, fkeep = C.c_void_p(F.ctypes.data), F
+        rt.check(rt.lib().mdrtHipFramesEvaluate(self._h, K, pp, rt.ptr(terms, C.c_double), fp))
+        del keep, fkeep
+        self._version += 1
+        self._loaded = K
+        return FrameEnergies(self, K, terms, F)
+
+    def hvp(self, v, out=None):
+        """H_n v_n (K, N, 3) for the loaded frames (the reference's
+        frame_hvp, torch.py:341).  `out`, a CUDA float64 tensor, receives
+        it on the device instead of a returned array."""
+        K = self._loaded
+        if K == 0:
+            raise RuntimeError("hvp: evaluate frames first")
+        vp, keep, _ = _frame_array(v, (K, self.n, 3), "v")
+        if out is not None:
+            op, okeep, _ = _frame_array(out, (K, self.n, 3), "out")
+            res = out
+        else:
+            res = np.zeros((K, self.n, 3))
+            op, okeep = C.c_void_p(res.ctypes.data), res
+        rt.check(rt.lib().mdrtHipFramesHvp(self._h, K, vp, op))
+        del keep, okeep
+        return res
+
+    def force_matching(self, positions, reference_forces, graph: bool = True):
+        """Loss per frame sum_i |F_ni - F_ref,ni|^2 (K,) and the gradient of
+        the summed loss in the tunables, in one call (forward, residual and
+        adjoint captured in one HIP graph per frame count)."""
+        K = self._count(positions)
+        pp, keep, _ = _frame_array(positions, (K, self.n, 3), "positions")
+        rp, rkeep, _ = _frame_array(reference_forces, (K, self.n, 3), "reference_forces")
+        loss = np.zeros(K)
+        dq = np.zeros(self.n)
+        ds = np.zeros(self.ntypes)
+        de = np.zeros(self.ntypes)
+        rt.check(rt.lib().mdrtHipFramesForceMatching(
+            self._h, K, pp, rp, int(bool(graph)), rt.ptr(loss, C.c_double),
+            rt.ptr(dq, C.c_double), rt.ptr(ds, C.c_double), rt.ptr(de, C.c_double)))
+        del keep, rkeep
+        self._version += 1
+        self._loaded = K
+        return loss, FrameGradient({"charges": dq, "sigma": ds, "epsilon": de})

@@ -120,6 +120,15 @@ SIGNATURES = [
     ("mdrtHipNeighborPairs", C.c_int, [C.c_void_p, D, I32, I64, PI32, PI64]),
     ("mdrtHipStats", C.c_int, [C.c_void_p, PI64]),
     ("mdrtHipProfileStep", C.c_int, [C.c_void_p, I32, PD]),
+    ("mdrtHipFramesCreate", C.c_void_p, [C.POINTER(SystemDesc), I32]),
+    ("mdrtHipFramesDestroy", None, [C.c_void_p]),
+    ("mdrtHipFramesSetParameters", C.c_int, [C.c_void_p, PD, PD, PD]),
+    ("mdrtHipFramesGetParameters", C.c_int, [C.c_void_p, PD, PD, PD]),
+    ("mdrtHipFramesEvaluate", C.c_int, [C.c_void_p, I32, C.c_void_p, PD, C.c_void_p]),
+    ("mdrtHipFramesVjp", C.c_int, [C.c_void_p, I32, PD, C.c_void_p, PD, PD, PD]),
+    ("mdrtHipFramesHvp", C.c_int, [C.c_void_p, I32, C.c_void_p, C.c_void_p]),
+    ("mdrtHipFramesForceMatching", C.c_int, [C.c_void_p, I32, C.c_void_p, C.c_void_p, I32,
+                                             PD, PD, PD, PD]),
 ]
 
 
@@ -147,3 +156,53 @@ def check(status: int) -> None:
 
 def ptr(a, ctype):
     return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def system_desc(s, skin: float = 0.1, prune_skin: float = 0.0):
+    """The mdrtHipSystemDesc of a System and the arrays it points into
+    (keep them alive until the create call returns)."""
+    keep = []
+
+    def k(a, dtype):
+        a = np.ascontiguousarray(a, dtype)
+        keep.append(a)
+        return a
+
+    x = k(s.positions, np.float64)
+    v = k(s.velocities if s.velocities is not None else np.zeros_like(x), np.float64)
+    d = SystemDesc()
+    d.n = s.n
+    d.positions = ptr(x, D)
+    d.velocities = ptr(v, D)
+    d.masses = ptr(k(s.masses, np.float64), D)
+    d.charges = ptr(k(s.charges, np.float64), D)
+    d.types = ptr(k(s.types, np.int32), I32)
+    d.ntypes = s.ntypes
+    d.sigma = ptr(k(s.sigma, np.float64), D)
+    d.epsilon = ptr(k(s.epsilon, np.float64), D)
+    for a in range(3):
+        d.box[a] = float(s.box[a])
+        d.pme_grid[a] = int(s.pme_grid[a])
+    d.cutoff = s.cutoff
+    d.switch_from = s.switch_from
+    d.lj_modifier = s.lj_modifier
+    d.coulomb_shift = int(s.coulomb_shift)
+    d.dispersion = int(s.dispersion)
+    d.beta = s.beta
+    d.pme_order = s.pme_order
+    ex = k(s.exclusions.reshape(-1, 2), np.int32)
+    d.nexclusions = ex.shape[0]
+    d.exclusions = ptr(ex, I32)
+    p14 = k(s.pairs14.reshape(-1, 2), np.int32)
+    d.npairs14 = p14.shape[0]
+    d.pairs14 = ptr(p14, I32)
+    d.pairs14_sigma = ptr(k(s.pairs14_sigma, np.float64), D)
+    d.pairs14_eps = ptr(k(s.pairs14_eps, np.float64), D)
+    d.pairs14_qq = ptr(k(s.pairs14_qq, np.float64), D)
+    w = k(s.waters, np.int32)
+    d.nwaters = w.shape[0]
+    d.water_first = ptr(w, I32)
+    d.d_oh, d.d_hh = s.water_geometry
+    d.skin = skin
+    d.prune_skin = prune_skin
+    return d, keep

@@ -20,50 +20,7 @@ class Simulation:
         self.system = system
         s = system
         L = rt.lib()
-        self._arrays = []
-
-        def keep(a):
-            self._arrays.append(a)
-            return a
-
-        x = keep(np.ascontiguousarray(s.positions, np.float64))
-        v = keep(np.ascontiguousarray(
-            s.velocities if s.velocities is not None else np.zeros_like(x), np.float64))
-        d = rt.SystemDesc()
-        d.n = s.n
-        d.positions = rt.ptr(x, C.c_double)
-        d.velocities = rt.ptr(v, C.c_double)
-        d.masses = rt.ptr(keep(np.ascontiguousarray(s.masses, np.float64)), C.c_double)
-        d.charges = rt.ptr(keep(np.ascontiguousarray(s.charges, np.float64)), C.c_double)
-        d.types = rt.ptr(keep(np.ascontiguousarray(s.types, np.int32)), C.c_int32)
-        d.ntypes = s.ntypes
-        d.sigma = rt.ptr(keep(np.ascontiguousarray(s.sigma, np.float64)), C.c_double)
-        d.epsilon = rt.ptr(keep(np.ascontiguousarray(s.epsilon, np.float64)), C.c_double)
-        for k in range(3):
-            d.box[k] = float(s.box[k])
-            d.pme_grid[k] = int(s.pme_grid[k])
-        d.cutoff = s.cutoff
-        d.switch_from = s.switch_from
-        d.lj_modifier = s.lj_modifier
-        d.coulomb_shift = int(s.coulomb_shift)
-        d.dispersion = int(s.dispersion)
-        d.beta = s.beta
-        d.pme_order = s.pme_order
-        ex = keep(np.ascontiguousarray(s.exclusions.reshape(-1, 2), np.int32))
-        d.nexclusions = ex.shape[0]
-        d.exclusions = rt.ptr(ex, C.c_int32)
-        p14 = keep(np.ascontiguousarray(s.pairs14.reshape(-1, 2), np.int32))
-        d.npairs14 = p14.shape[0]
-        d.pairs14 = rt.ptr(p14, C.c_int32)
-        d.pairs14_sigma = rt.ptr(keep(np.ascontiguousarray(s.pairs14_sigma, np.float64)), C.c_double)
-        d.pairs14_eps = rt.ptr(keep(np.ascontiguousarray(s.pairs14_eps, np.float64)), C.c_double)
-        d.pairs14_qq = rt.ptr(keep(np.ascontiguousarray(s.pairs14_qq, np.float64)), C.c_double)
-        w = keep(np.ascontiguousarray(s.waters, np.int32))
-        d.nwaters = w.shape[0]
-        d.water_first = rt.ptr(w, C.c_int32)
-        d.d_oh, d.d_hh = s.water_geometry
-        d.skin = skin
-        d.prune_skin = prune_skin
+        d, self._arrays = rt.system_desc(system, skin, prune_skin)
         self._desc = d
         h = L.mdrtHipSystemCreate(C.byref(d))
         if not h:

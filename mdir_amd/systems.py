@@ -325,3 +325,137 @@ def initial_velocities(system: System, seed: int, temperature: float) -> np.ndar
     if kin > 0:
         v *= np.sqrt(target / kin)
     return v
+
+
+# ---- parameters per type and the frames of the differentiation workload ----
+
+def lorentz_berthelot(sigma, epsilon, table_sigma=None):
+    """Type-pair tables from per-type sigma and epsilon:
+    sigma_ab = (sigma_a + sigma_b) / 2, eps_ab = sqrt(eps_a eps_b); where
+    eps_ab is 0 the sigma of `table_sigma` is kept (it does not act)."""
+    s = np.asarray(sigma, np.float64)
+    e = np.asarray(epsilon, np.float64)
+    eps = np.sqrt(np.outer(e, e))
+    sig = 0.5 * (s[:, None] + s[None, :])
+    if table_sigma is not None:
+        sig = np.where(eps != 0.0, sig, np.asarray(table_sigma, np.float64))
+    return sig, eps
+
+
+def type_parameters(system: System):
+    """Per-type sigma and epsilon: the diagonal of the tables."""
+    return np.diag(system.sigma).copy(), np.diag(system.epsilon).copy()
+
+
+def with_parameters(system: System, charges=None, sigma=None, epsilon=None) -> System:
+    """A copy with other charges and per-type sigma / epsilon (tables by the
+    Lorentz-Berthelot rules)."""
+    s0, e0 = type_parameters(system)
+    sig, eps = lorentz_berthelot(s0 if sigma is None else sigma,
+                                 e0 if epsilon is None else epsilon, system.sigma)
+    return dataclasses.replace(
+        system, positions=system.positions.copy(),
+        charges=system.charges.copy() if charges is None else np.asarray(charges, np.float64),
+        sigma=sig, epsilon=eps)
+
+
+def solvated_solutes(nwaters: int = 1000, nsolutes: int = 30, seed: int = 4,
+                     ntypes: int = 4, cutoff: float = 0.9, grid=(32, 32, 32),
+                     tolerance: float = 1e-5, jitter: float = 0.05) -> System:
+    """TIP3P water with five-site solutes (a centre and four sites at 0.1 nm
+    on a tetrahedron; all ten pairs of a solute excluded): solute atoms of
+    `ntypes` Lennard-Jones types with seeded sigma in [0.3, 0.4] nm, epsilon
+    in [0.2, 1.0] kJ/mol and charges in [-0.5, 0.5] e; Lorentz-Berthelot
+    tables.  The box holds the waters at 1.0 g/cm^3 plus 0.1 nm^3 per
+    solute; waters and solute centres sit on a jittered cubic lattice, with
+    no water on the six sites next to a solute.
+    The default (1,000 waters, 30 solutes: 3,150 atoms, PME 32^3, rc 0.9 nm)
+    is the frame-gradient workload."""
+    rng = np.random.default_rng(seed)
+    p = TIP3P
+    nsol_atoms = 5 * nsolutes
+    n = 3 * nwaters + nsol_atoms
+    m_water = (p["m_o"] + 2 * p["m_h"]) / 6.02214076e23  # g
+    vol_w = nwaters * m_water / 1.0 * 1e21                # nm^3 at 1.0 g/cm^3
+    edge = (vol_w + 0.1 * nsolutes) ** (1.0 / 3.0)
+    m = int(math.ceil((nwaters + 7 * nsolutes) ** (1.0 / 3.0) - 1e-9))
+    spacing = edge / m
+    ids = np.arange(m ** 3)
+    coords = np.stack([ids % m, (ids // m) % m, ids // (m * m)], 1)
+    # solute centres: spread out, then the six neighbours blocked
+    blocked = np.zeros(m ** 3, bool)
+    centres = []
+    for s in rng.permutation(m ** 3):
+        if blocked[s]:
+            continue
+        c = coords[s]
+        nb = [(c + d) % m for d in ([0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0],
+                                      [0, -1, 0], [0, 0, 1], [0, 0, -1])]
+        idx = [int(v[0] + m * v[1] + m * m * v[2]) for v in nb]
+        if any(blocked[i] for i in idx):
+            continue
+        centres.append(s)
+        for i in idx:
+            blocked[i] = True
+        if len(centres) == nsolutes:
+            break
+    if len(centres) < nsolutes:
+        raise ValueError("too many solutes for the lattice")
+    free = np.flatnonzero(~blocked)
+    wsites = np.sort(rng.choice(free, size=nwaters, replace=False))
+    half = 0.5 * p["d_hh"]
+    h = math.sqrt(p["d_oh"] ** 2 - half ** 2)
+    total = p["m_o"] + 2 * p["m_h"]
+    ra = 2 * p["m_h"] * h / total
+    body = np.array([[0.0, ra, 0.0], [-half, ra - h, 0.0], [half, ra - h, 0.0]])
+    wc = (coords[wsites] + 0.5) * spacing + (rng.random((nwaters, 3)) - 0.5) * 2 * jitter * spacing
+    rot = _random_rotations(rng, nwaters)
+    xw = (wc[:, None, :] + np.einsum("wij,aj->wai", rot, body)).reshape(-1, 3)
+    tet = 0.1 / math.sqrt(3.0) * np.array([[0.0, 0.0, 0.0], [1, 1, 1], [1, -1, -1],
+                                           [-1, 1, -1], [-1, -1, 1]])
+    sc = (coords[np.array(centres)] + 0.5) * spacing
+    rot_s = _random_rotations(rng, nsolutes)
+    xs = (sc[:, None, :] + np.einsum("wij,aj->wai", rot_s, tet)).reshape(-1, 3)
+    x = np.vstack([xw, xs])
+    sig_t = np.concatenate([[p["sigma_o"], 0.0], rng.uniform(0.3, 0.4, ntypes)])
+    eps_t = np.concatenate([[p["eps_o"], 0.0], rng.uniform(0.2, 1.0, ntypes)])
+    table0 = np.zeros((ntypes + 2, ntypes + 2))
+    sig, eps = lorentz_berthelot(sig_t, eps_t, table0)
+    types = np.concatenate([np.tile(np.array([0, 1, 1]), nwaters),
+                            rng.integers(2, 2 + ntypes, nsol_atoms)]).astype(np.int32)
+    q = np.concatenate([np.tile([p["q_o"], p["q_h"], p["q_h"]], nwaters),
+                        rng.uniform(-0.5, 0.5, nsol_atoms)])
+    masses = np.concatenate([np.tile([p["m_o"], p["m_h"], p["m_h"]], nwaters),
+                             np.full(nsol_atoms, 12.011)])
+    first = np.arange(0, 3 * nwaters, 3, dtype=np.int32)
+    excl = [np.stack([first, first + 1], 1), np.stack([first, first + 2], 1),
+            np.stack([first + 1, first + 2], 1)]
+    o = 3 * nwaters + 5 * np.arange(nsolutes)
+    for i in range(5):
+        for j in range(i + 1, 5):
+            excl.append(np.stack([o + i, o + j], 1))
+    excl = np.concatenate(excl).astype(np.int32)
+    excl = excl[np.lexsort((excl[:, 1], excl[:, 0]))]
+    box = np.full(3, float(edge))
+    return System(positions=x, masses=masses, charges=q, types=types, sigma=sig,
+                  epsilon=eps, box=box, cutoff=cutoff,
+                  beta=ewald_beta(cutoff, tolerance), pme_grid=tuple(grid), pme_order=4,
+                  exclusions=excl, waters=first, water_geometry=(p["d_oh"], p["d_hh"]))
+
+
+def perturbed_frames(system: System, nframes: int, seed: int = 9, scale: float = 0.02):
+    """nframes positions (nframes, n, 3): the system's positions with seeded
+    Gaussian displacements of `scale` nm on every atom."""
+    rng = np.random.default_rng(seed)
+    return system.positions[None] + rng.normal(0.0, scale, (nframes,) + system.positions.shape)
+
+
+def perturbed_parameters(system: System, seed: int = 13, rel: float = 0.02, dq: float = 0.02):
+    """Charges and per-type sigma, epsilon moved by seeded amounts (the
+    parameters of the reference forces of the force-matching workload)."""
+    rng = np.random.default_rng(seed)
+    s0, e0 = type_parameters(system)
+    q = system.charges + rng.uniform(-dq, dq, system.n)
+    s = s0 * (1.0 + rng.uniform(-rel, rel, s0.shape))
+    e = e0 * (1.0 + rng.uniform(-rel, rel, e0.shape))
+    return q, s, e
