@@ -161,7 +161,8 @@ __device__ __forceinline__ PairParts<T> pair_parts(T r2, const AdjParams &p) {
 // cot[place] = (cotangent / c, 0): exact scalings.
 constexpr double kForceMatchWeight = -2.0;
 
-__global__ void __launch_bounds__(256) cot_prepare(int mode, const int *__restrict__ counts,
+constexpr int kCotThreads = 1024;
+__global__ void __launch_bounds__(kCotThreads) cot_prepare(int mode, const int *__restrict__ counts,
                                                   const int *__restrict__ atom_at,
                                                   const long long *__restrict__ F,
                                                   const double *__restrict__ cin,
@@ -174,7 +175,7 @@ __global__ void __launch_bounds__(256) cot_prepare(int mode, const int *__restri
   cin = frame_at(cin, fs);
   meta = frame_at(meta, fs);
   cot = frame_at(cot, fs);
-  __shared__ double s_max[256], s_loss[256];
+  __shared__ double s_max[kCotThreads], s_loss[kCotThreads];
   const int np = counts[0];
   double mx = 0.0, loss = 0.0;
   for (int pl = threadIdx.x; pl < np; pl += blockDim.x) {
@@ -195,7 +196,7 @@ __global__ void __launch_bounds__(256) cot_prepare(int mode, const int *__restri
   s_max[threadIdx.x] = mx;
   s_loss[threadIdx.x] = loss;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
+  for (int s = kCotThreads / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s) {
       s_max[threadIdx.x] = fmax(s_max[threadIdx.x], s_max[threadIdx.x + s]);
       s_loss[threadIdx.x] += s_loss[threadIdx.x + s];
@@ -773,6 +774,39 @@ __global__ void __launch_bounds__(256) pme_gather_adj(
   }
   __syncthreads();  // every thread has read the count
   if (threadIdx.x == 0) bp.count[brick] = 0;
+}
+
+// ---- frame arrays in and out ---------------------------------------------
+
+// Between a contiguous array of `count` doubles per frame and the per-frame
+// buffer (fs bytes apart): to_frames != 0 copies contiguous -> frames.
+__global__ void frames_copy(const double *__restrict__ src, double *__restrict__ dst, int count,
+                            int to_frames, size_t fs) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  const size_t f = blockIdx.z;
+  if (to_frames)
+    frame_at(dst, fs)[j] = src[f * count + j];
+  else
+    dst[f * count + j] = frame_at(src, fs)[j];
+}
+
+// The small results of a batch gathered for one read by the host:
+// loss[f] = meta_f[2] (the loss of frame f) and flag_or[k] = the OR over the
+// frames of flag k (as a double).
+__global__ void frames_collect(int frames, const double *__restrict__ meta,
+                               const int *__restrict__ flags, double *__restrict__ loss,
+                               double *__restrict__ flag_or, size_t fs) {
+  const int t = threadIdx.x;
+  const char *mb = reinterpret_cast<const char *>(meta);
+  const char *fb = reinterpret_cast<const char *>(flags);
+  for (int f = t; f < frames; f += blockDim.x)
+    loss[f] = reinterpret_cast<const double *>(mb + f * fs)[2];
+  if (t < kNumFlags) {
+    int v = 0;
+    for (int f = 0; f < frames; ++f) v |= reinterpret_cast<const int *>(fb + f * fs)[t];
+    flag_or[t] = (double)v;
+  }
 }
 
 // ---- read-out -------------------------------------------------------------

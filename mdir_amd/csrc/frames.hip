@@ -38,7 +38,12 @@ struct FrameBatch {
   double *d_meta = nullptr;     // 4: scale c, g / c, loss, -
   // shared
   float4 *d_ljadj = nullptr;    // per type pair: C12, C6, 12 C12/sigma, 6 C6/sigma
-  double *d_sum = nullptr;      // 3n: products summed over the frames
+  // small results read by the host at once: loss [max_frames], flags OR'ed
+  // over the frames [kNumFlags], products summed over the frames [3n]
+  double *d_result = nullptr;
+  double *d_sum = nullptr;      // inside d_result
+  std::vector<double> h_result;
+  double *d_stage = nullptr;    // max_frames x 3n: frame arrays to and from the host
   AdjParams ap{};
   std::vector<double> sig_t, eps_t;   // per-type parameters
   // force-matching graph
@@ -48,7 +53,8 @@ struct FrameBatch {
   ~FrameBatch() {
     if (fm_graph) hipGraphExecDestroy(fm_graph);
     dfree(d_ljadj);
-    dfree(d_sum);
+    dfree(d_result);
+    dfree(d_stage);
   }
 
   void alloc_extra() {
@@ -93,7 +99,11 @@ struct FrameBatch {
       }
     set_adj_params();
     d_ljadj = dalloc<float4>((size_t)nt * nt);
-    d_sum = dalloc<double>(3 * (size_t)e.n);
+    const size_t nres = (size_t)frames + kNumFlags + 3 * (size_t)e.n;
+    d_result = dalloc<double>(nres);
+    d_sum = d_result + frames + kNumFlags;
+    h_result.assign(nres, 0.0);
+    d_stage = dalloc<double>((size_t)frames * 3 * e.n);
     upload_types();
     alloc_extra();
     e.batch(max_frames);
@@ -170,15 +180,46 @@ struct FrameBatch {
       throw HipError("frames must be in [1, " + std::to_string(max_frames) + "]");
   }
 
-  // The per-frame array `src` (frames x count doubles, host or device) into
-  // the per-frame buffer `dst` of every frame.
+  static bool device_pointer(const void *p) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+      (void)hipGetLastError();  // pageable host memory
+      return false;
+    }
+    return a.type == hipMemoryTypeDevice;
+  }
+
+  // The frame array `src` (frames x count doubles, host or device) into the
+  // per-frame buffer `dst` of every frame: one kernel (host arrays first go
+  // to a staging buffer in one copy).
   void scatter(double *dst, const double *src, size_t count, int frames) {
-    HIPCHK(hipMemcpy2DAsync(dst, e.fs, src, count * sizeof(double), count * sizeof(double),
-                            frames, hipMemcpyDefault, e.stream));
+    if (!device_pointer(src)) {
+      HIPCHK(hipMemcpyAsync(d_stage, src, (size_t)frames * count * sizeof(double),
+                            hipMemcpyHostToDevice, e.stream));
+      src = d_stage;
+    }
+    frames_copy<<<dim3(blocks_for((int)count), 1, frames), 256, 0, e.stream>>>(
+        src, dst, (int)count, 1, e.fs);
   }
   void gather(double *dst, const double *src, size_t count, int frames) {
-    HIPCHK(hipMemcpy2DAsync(dst, count * sizeof(double), src, e.fs, count * sizeof(double),
-                            frames, hipMemcpyDefault, e.stream));
+    const bool dev = device_pointer(dst);
+    frames_copy<<<dim3(blocks_for((int)count), 1, frames), 256, 0, e.stream>>>(
+        src, dev ? dst : d_stage, (int)count, 0, e.fs);
+    if (!dev)
+      HIPCHK(hipMemcpyAsync(dst, d_stage, (size_t)frames * count * sizeof(double),
+                            hipMemcpyDeviceToHost, e.stream));
+  }
+
+  // Loss per frame and the flags OR'ed over the frames into d_result.
+  void launch_collect(int frames) {
+    frames_collect<<<1, 256, 0, e.stream>>>(frames, d_meta, e.d_flags, d_result,
+                                            d_result + max_frames, e.fs);
+  }
+  // d_result to the host (waits for the stream).
+  void read_result() {
+    HIPCHK(hipMemcpyAsync(h_result.data(), d_result, h_result.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, e.stream));
+    HIPCHK(hipStreamSynchronize(e.stream));
   }
 
   // ---- forward ----
@@ -190,16 +231,11 @@ struct FrameBatch {
     e.forces(energy);
   }
 
-  // Flags of every frame; an overflow regrows the buffers (all frames) and
-  // returns true.
-  bool regrow_if_overflowed(int frames) {
-    std::vector<int> f((size_t)frames * kNumFlags);
-    HIPCHK(hipMemcpy2DAsync(f.data(), kNumFlags * sizeof(int), e.d_flags, e.fs,
-                            kNumFlags * sizeof(int), frames, hipMemcpyDeviceToHost, e.stream));
-    HIPCHK(hipStreamSynchronize(e.stream));
+  // The flags of the frames as read by read_result(); an overflow regrows
+  // the buffers (all frames) and returns true.
+  bool regrow_if_overflowed() {
     std::vector<int> m(kNumFlags, 0);
-    for (int fr = 0; fr < frames; ++fr)
-      for (int k = 0; k < kNumFlags; ++k) m[k] |= f[(size_t)fr * kNumFlags + k];
+    for (int k = 0; k < kNumFlags; ++k) m[k] = (int)h_result[max_frames + k];
     if (!m[kFlagBrickOverflow] && !m[kFlagOverflowColumn] && !m[kFlagOverflowRow]) return false;
     if (fm_graph) hipGraphExecDestroy(fm_graph);
     fm_graph = nullptr;
@@ -217,7 +253,9 @@ struct FrameBatch {
     for (int attempt = 0; attempt < 8; ++attempt) {
       scatter(e.d_x, positions, 3 * (size_t)e.n, frames);
       forward(frames, energies != nullptr);
-      if (!regrow_if_overflowed(frames)) {
+      launch_collect(frames);
+      read_result();
+      if (!regrow_if_overflowed()) {
         loaded = frames;
         if (forces) {
           forces_to_atoms<<<dim3(blocks_for(e.n), 1, frames), 256, 0, e.stream>>>(
@@ -352,7 +390,7 @@ struct FrameBatch {
   // engine's stream, the PME terms on the side stream, both adding words.
   template <int KIND>
   void adjoint(int mode) {
-    cot_prepare<<<dim3(1, 1, e.nz), 256, 0, e.stream>>>(mode, e.d_counts, e.d_atom_at, e.d_F,
+    cot_prepare<<<dim3(1, 1, e.nz), kCotThreads, 0, e.stream>>>(mode, e.d_counts, e.d_atom_at, e.d_F,
                                                         d_fin, d_meta, d_cot, e.max_places,
                                                         e.fs);
     clear_words<<<dim3(64, 1, e.nz), 256, 0, e.stream>>>(d_A, 3 * (size_t)e.max_places, e.fs);
@@ -368,6 +406,7 @@ struct FrameBatch {
                                                                       d_meta, d_fout, e.fs);
     if (KIND == kAdjTheta)
       frames_sum<<<blocks_for(3 * e.n), 256, 0, e.stream>>>(e.nz, 3 * e.n, d_fout, d_sum, e.fs);
+    launch_collect(e.nz);
   }
 
   // theta products from the per-atom sums (dq, S, E) and the energy
@@ -425,6 +464,11 @@ struct FrameBatch {
     if (deps) for (int a = 0; a < nt; ++a) deps[a] = de[a];
   }
 
+  std::vector<double> sums() const {
+    const size_t o = (size_t)max_frames + kNumFlags;
+    return std::vector<double>(h_result.begin() + o, h_result.begin() + o + 3 * (size_t)e.n);
+  }
+
   void require_loaded(int frames) const {
     if (loaded != frames)
       throw HipError("the products need the frames evaluated first (evaluate with the same "
@@ -449,11 +493,8 @@ struct FrameBatch {
                             e.stream));
     e.nz = frames;
     adjoint<kAdjTheta>(0);
-    std::vector<double> s(3 * (size_t)e.n);
-    HIPCHK(hipMemcpyAsync(s.data(), d_sum, s.size() * sizeof(double), hipMemcpyDeviceToHost,
-                          e.stream));
-    HIPCHK(hipStreamSynchronize(e.stream));
-    theta_from_sums(s, gsum, dq, dsig, deps);
+    read_result();
+    theta_from_sums(sums(), gsum, dq, dsig, deps);
   }
 
   void hvp(int frames, const double *v, double *out) {
@@ -501,17 +542,11 @@ struct FrameBatch {
       } else {
         fm_sequence(frames);
       }
-      if (!regrow_if_overflowed(frames)) {
+      read_result();
+      if (!regrow_if_overflowed()) {
         loaded = frames;
-        std::vector<double> s(3 * (size_t)e.n);
-        HIPCHK(hipMemcpyAsync(s.data(), d_sum, s.size() * sizeof(double),
-                              hipMemcpyDeviceToHost, e.stream));
-        if (loss)
-          HIPCHK(hipMemcpy2DAsync(loss, sizeof(double),
-                                  reinterpret_cast<char *>(d_meta) + 2 * sizeof(double), e.fs,
-                                  sizeof(double), frames, hipMemcpyDeviceToHost, e.stream));
-        HIPCHK(hipStreamSynchronize(e.stream));
-        theta_from_sums(s, 0.0, dq, dsig, deps);
+        if (loss) std::copy(h_result.begin(), h_result.begin() + frames, loss);
+        theta_from_sums(sums(), 0.0, dq, dsig, deps);
         return;
       }
     }
