@@ -234,95 +234,12 @@ def main():
         out["cpu_baseline"] = cpu_baseline(system, sim.positions, sim.velocities, dt,
                                            args.cpu_seconds)
     if args.frames > 0:
-        out["frame_gradients"] = frame_gradients(args.frames, args.frame_repeats,
-                                                 args.frame_loop_passes)
+        from mdir_amd.frames import force_matching_benchmark
+        out["frame_gradients"] = force_matching_benchmark(args.frames, args.frame_repeats,
+                                                          args.frame_loop_passes)
     print(json.dumps(out), flush=True)
     if args.calibrate_copy_mib:
         calibrate_copy(args.calibrate_copy_mib)
-
-
-def frame_gradients(nframes, repeats, loop_passes):
-    """Force-matching frame-gradients per second on the differentiation
-    workload (64 frames of 3,150 atoms: 1,000 TIP3P waters and 30 five-site
-    Lennard-Jones solutes of 4 types, PME 32^3, rc 0.9 nm): one call
-    evaluates the frames, the loss sum |F(theta) - F_ref|^2 against forces
-    at moved parameters, and its gradient in every charge and the per-type
-    sigma and epsilon.  Positions and reference forces are resident on the
-    device before the timed region; a call ends with the gradient on the
-    host.  Batched (all frames per call, one captured graph) against the
-    plain per-frame loop (one frame per call, same library and graphs)."""
-    import ctypes as C
-    from mdir_amd import runtime as rt
-    from mdir_amd import systems as S
-    from mdir_amd.frames import FrameEvaluator
-    L = rt.lib()
-    system = S.solvated_solutes()
-    n = system.n
-    X = S.perturbed_frames(system, nframes, seed=9)
-    q1, s1, e1 = S.perturbed_parameters(system)
-    ev = FrameEvaluator(system, max_frames=nframes)
-    theta = ev.tunables
-    ev.update(q1, s1, e1)
-    Fref = ev.evaluate(X).forces
-    ev.update(theta["charges"], theta["sigma"], theta["epsilon"])
-    stream = L.mgpuStreamCreate()
-    nbytes = X.nbytes
-    dx = L.mgpuMemAlloc(nbytes, stream, False)
-    df = L.mgpuMemAlloc(nbytes, stream, False)
-    Xc = np.ascontiguousarray(X)
-    Fc = np.ascontiguousarray(Fref)
-    L.mgpuMemcpy(dx, Xc.ctypes.data, nbytes, stream)
-    L.mgpuMemcpy(df, Fc.ctypes.data, nbytes, stream)
-    L.mgpuStreamSynchronize(stream)
-    loss = np.zeros(nframes)
-    dq = np.zeros(n)
-    ds = np.zeros(system.ntypes)
-    de = np.zeros(system.ntypes)
-    P = lambda a: rt.ptr(a, C.c_double)
-
-    def call(h, count, px, pf):
-        rt.check(L.mdrtHipFramesForceMatching(h, count, C.c_void_p(px), C.c_void_p(pf), 1,
-                                              P(loss), P(dq), P(ds), P(de)))
-    for _ in range(3):
-        call(ev._h, nframes, dx, df)
-    t0 = time.perf_counter()
-    for _ in range(repeats):
-        call(ev._h, nframes, dx, df)
-    batched_s = (time.perf_counter() - t0) / repeats
-    grad_batched = (dq.copy(), ds.copy(), de.copy())
-    one = FrameEvaluator(system, max_frames=1)
-    frame_bytes = n * 3 * 8
-    for k in range(3):
-        call(one._h, 1, dx + k * frame_bytes, df + k * frame_bytes)
-    t0 = time.perf_counter()
-    for _ in range(loop_passes):
-        for k in range(nframes):
-            call(one._h, 1, dx + k * frame_bytes, df + k * frame_bytes)
-    loop_s = (time.perf_counter() - t0) / loop_passes
-    L.mgpuMemFree(dx, stream)
-    L.mgpuMemFree(df, stream)
-    L.mgpuStreamSynchronize(stream)
-    L.mgpuStreamDestroy(stream)
-    ev.close()
-    one.close()
-    rate = nframes / batched_s
-    rate_loop = nframes / loop_s
-    print(f"frames: {nframes} x {n} atoms, batched {batched_s * 1e3:.3f} ms "
-          f"({rate:.0f} frame-gradients/s), per-frame loop {loop_s * 1e3:.3f} ms "
-          f"({rate_loop:.0f}/s), x{rate / rate_loop:.1f}", file=sys.stderr, flush=True)
-    return {"metric": "force-matching frame-gradients/sec", "value": round(rate, 1),
-            "unit": "frame-gradients/s", "higher_is_better": True,
-            "ms_per_batch": round(batched_s * 1e3, 4),
-            "per_frame_loop": {"value": round(rate_loop, 1), "ms_per_batch": round(loop_s * 1e3, 4)},
-            "speedup_vs_per_frame_loop": round(rate / rate_loop, 2),
-            "dtype": "f32", "data": "synthetic",
-            "config": {"workload": "tip3p_1000_waters_30_solutes_pme32", "frames": nframes,
-                       "atoms": n, "box_nm": round(float(system.box[0]), 4),
-                       "cutoff_nm": system.cutoff, "pme_grid": list(system.pme_grid),
-                       "tunables": {"charges": n, "sigma": system.ntypes,
-                                    "epsilon": system.ntypes},
-                       "repeats": repeats, "loop_passes": loop_passes},
-            "gradient_norm": float(np.sqrt(sum(float((g * g).sum()) for g in grad_batched)))}
 
 
 def calibrate_copy(mib):

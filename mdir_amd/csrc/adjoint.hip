@@ -794,19 +794,26 @@ __global__ void frames_copy(const double *__restrict__ src, double *__restrict__
 // The small results of a batch gathered for one read by the host:
 // loss[f] = meta_f[2] (the loss of frame f) and flag_or[k] = the OR over the
 // frames of flag k (as a double).
-__global__ void frames_collect(int frames, const double *__restrict__ meta,
-                               const int *__restrict__ flags, double *__restrict__ loss,
-                               double *__restrict__ flag_or, size_t fs) {
+// One block of 1024 threads: thread (f, k) reads flag k of frame f.
+__global__ void __launch_bounds__(1024) frames_collect(int frames, const double *__restrict__ meta,
+                                                       const int *__restrict__ flags,
+                                                       double *__restrict__ loss,
+                                                       double *__restrict__ flag_or, size_t fs) {
+  __shared__ int s_or[kNumFlags];
   const int t = threadIdx.x;
+  if (t < kNumFlags) s_or[t] = 0;
+  __syncthreads();
   const char *mb = reinterpret_cast<const char *>(meta);
   const char *fb = reinterpret_cast<const char *>(flags);
   for (int f = t; f < frames; f += blockDim.x)
     loss[f] = reinterpret_cast<const double *>(mb + f * fs)[2];
-  if (t < kNumFlags) {
-    int v = 0;
-    for (int f = 0; f < frames; ++f) v |= reinterpret_cast<const int *>(fb + f * fs)[t];
-    flag_or[t] = (double)v;
+  for (int w = t; w < frames * kNumFlags; w += blockDim.x) {
+    const int f = w / kNumFlags, k = w % kNumFlags;
+    const int v = reinterpret_cast<const int *>(fb + f * fs)[k];
+    if (v) atomicOr(&s_or[k], v);
   }
+  __syncthreads();
+  if (t < kNumFlags) flag_or[t] = (double)s_or[t];
 }
 
 // ---- read-out -------------------------------------------------------------
@@ -834,6 +841,8 @@ __global__ void frames_sum(int frames, int count, const double *__restrict__ in,
   if (j >= count) return;
   double s = 0.0;
   const char *base = reinterpret_cast<const char *>(in);
+  // the loads of 8 frames issued together, the adds in frame order
+#pragma unroll 8
   for (int f = 0; f < frames; ++f) s += reinterpret_cast<const double *>(base + f * fs)[j];
   sum[j] = s;
 }

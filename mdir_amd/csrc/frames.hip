@@ -212,7 +212,7 @@ struct FrameBatch {
 
   // Loss per frame and the flags OR'ed over the frames into d_result.
   void launch_collect(int frames) {
-    frames_collect<<<1, 256, 0, e.stream>>>(frames, d_meta, e.d_flags, d_result,
+    frames_collect<<<1, 1024, 0, e.stream>>>(frames, d_meta, e.d_flags, d_result,
                                             d_result + max_frames, e.fs);
   }
   // d_result to the host (waits for the stream).
