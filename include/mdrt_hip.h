@@ -1,7 +1,7 @@
 /* mdrt_hip.h — C ABI of libmdrt_hip.so, the MI355X (gfx950) runtime and
  * per-step force path.
  *
- * Two groups of entry points:
+ * Three groups of entry points:
  *
  *  1. The runtime ABI of the reference's runtime/mdrt_cuda.c, same names and
  *     signatures, on HIP (modules are gfx950 code objects, streams are HIP
@@ -12,6 +12,10 @@
  *     produced without its MLIR toolchain, so the path is exposed here with
  *     plain pointers and sizes): a system handle, forces and energies,
  *     velocity-Verlet steps with SETTLE waters, and the neighbor pair set.
+ *
+ *  3. The frame evaluator: energies, forces and reverse products (charges,
+ *     per-type sigma and epsilon; position Hessian products) of many frames
+ *     of one system per launch, and force matching in one call.
  *
  * Units are the reference's: nm, ps, amu, kJ/mol, e.  All pointers in
  * group 2 are host pointers unless the name says device.
@@ -222,6 +226,19 @@ int mdrtHipGetVelocities(void *system, double *v);
 /* Forces (n x 3, may be NULL) and energies (MDRT_NUM_ENERGIES, may be NULL)
    at the current positions; builds the list. */
 int mdrtHipComputeForces(void *system, double *forces, double *energies);
+
+/* Force terms of mdrtHipComputeForceTerms. */
+enum {
+  MDRT_TERM_PAIRS = 1,    /* nonbonded pairs within rc: LJ + direct Coulomb */
+  MDRT_TERM_SPECIAL = 2,  /* excluded-pair Ewald correction and 1-4 pairs */
+  MDRT_TERM_RECIP = 4     /* PME reciprocal sum */
+};
+
+/* As mdrtHipComputeForces, for the terms of the mask `terms` only (the
+   reference states its precision per force term, docs/paper/07-precision.md;
+   the per-term split of TermNonbonded.cpp's nonbonded, excluded and
+   reciprocal terms); the energies of the other terms are 0. */
+int mdrtHipComputeForceTerms(void *system, int32_t terms, double *forces, double *energies);
 
 /* `steps` velocity-Verlet steps of dt (ps) with SETTLE waters.  graph != 0
    replays the step sequence as a captured HIP graph of `graph` steps.

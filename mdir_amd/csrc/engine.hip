@@ -231,6 +231,12 @@ struct Engine {
   IntParams ip{};
   double dt_graph = -1;
 
+  // ---- force terms ----
+  // Which terms a force evaluation computes (bits of MDRT_TERM_*): the
+  // nonbonded pairs (LJ + direct Coulomb), the excluded and 1-4 pairs, the
+  // PME reciprocal sum.  Steps always take all of them.
+  int terms = 7;
+
   // ---- frame batches ----
   // The frame-path launches run nz frames (blockIdx.z); the per-frame
   // buffers of frame f lie f * fs bytes after frame 0's (one system: 1, 0).
@@ -815,7 +821,7 @@ struct Engine {
   template <int MOD, bool COUL, bool ENERGY>
   void launch_pairs_t() {
     SpecialArgs sa{};
-    sa.npairs = nspecial;
+    sa.npairs = (terms & 2) ? nspecial : 0;
     sa.pairs = d_special;
     sa.is14 = d_is14;
     sa.p14_sigma = d_p14_sigma;
@@ -826,7 +832,8 @@ struct Engine {
     sa.place_of = d_place_of;
     sa.energy_partial = ENERGY ? d_espec : nullptr;
     sa.p = spp;
-    const dim3 grid(std::max(blocks_for(max_groups, 4), blocks_for(nspecial)), nseg + 1, nz);
+    const dim3 grid(std::max(blocks_for(max_groups, 4), blocks_for(nspecial)),
+                    (terms & 1) ? nseg + 1 : 1, nz);
     const bool pruned = prune_skin > 0;
     nb_pairs<MOD, COUL, ENERGY><<<grid, 256, ntypes * ntypes * sizeof(float2), stream>>>(
         d_xq, d_type_at, d_real_mask, pruned ? d_rows_in : d_rows,
@@ -994,7 +1001,7 @@ struct Engine {
   // With timed >= 0, the pair kernel records its span in slot `timed` of
   // the timing buffer.
   void forces(bool energy, int timed = -1) {
-    if (use_pme) {
+    if (use_pme && (terms & 4)) {
       HIPCHK(hipEventRecord(ev_fork, stream));
       HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
       pme(energy);
@@ -1006,7 +1013,7 @@ struct Engine {
       stamp_clock<<<1, 64, 0, stream>>>(d_timing + 3 * timed + 1);
       stamp_clock<<<1, 64, 0, stream>>>(d_timing + 3 * timed + 2);
     }
-    if (use_pme) HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
+    if (use_pme && (terms & 4)) HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
   }
 
   void clear_forces() {
@@ -1129,6 +1136,20 @@ struct Engine {
     throw HipError("the neighbor list kept overflowing");
   }
 
+  // Forces and energies of the terms `mask` names (the others are 0).
+  void compute_terms(int mask, double *F_out, double *E_out) {
+    if (mask < 1 || mask > 7) throw HipError("terms: a nonzero mask of MDRT_TERM_* bits");
+    terms = mask;
+    try {
+      compute(F_out, E_out);
+    } catch (...) {
+      terms = 7;
+      throw;
+    }
+    terms = 7;
+    forces_valid = false;
+  }
+
   void compute(double *F_out, double *E_out) {
     if (list_reach != cutoff + skin) configure_list(cutoff + skin);
     evaluate(true);
@@ -1144,11 +1165,11 @@ struct Engine {
       // one partial per (segment, group) wave, summed in a fixed order
       std::vector<double> ep(2 * (size_t)max_groups * nseg);
       to_host(ep.data(), d_epair, ep.size() * sizeof(double));
-      for (size_t w = 0; w < ep.size() / 2; ++w) {
+      for (size_t w = 0; (terms & 1) && w < ep.size() / 2; ++w) {
         E_out[MDRT_E_LJ] += ep[2 * w];
         E_out[MDRT_E_COUL_DIRECT] += ep[2 * w + 1];
       }
-      if (nspecial) {
+      if (nspecial && (terms & 2)) {
         const int nb = blocks_for(nspecial);
         std::vector<double> es(3 * (size_t)nb);
         to_host(es.data(), d_espec, es.size() * sizeof(double));
@@ -1158,7 +1179,7 @@ struct Engine {
           E_out[MDRT_E_COUL14] += es[3 * b + 2];
         }
       }
-      if (use_pme) {
+      if (use_pme && (terms & 4)) {
         std::vector<double> er(recip_blocks());
         to_host(er.data(), d_erecip, er.size() * sizeof(double));
         for (double e : er) E_out[MDRT_E_COUL_RECIP] += e;
@@ -1606,6 +1627,11 @@ int mdrtHipGetVelocities(void *system, double *v) {
 int mdrtHipComputeForces(void *system, double *forces, double *energies) {
   Engine *e = static_cast<Engine *>(system);
   return guarded([&] { e->compute(forces, energies); });
+}
+
+int mdrtHipComputeForceTerms(void *system, int32_t terms, double *forces, double *energies) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] { e->compute_terms(terms, forces, energies); });
 }
 
 int mdrtHipRun(void *system, int64_t steps, double dt, int32_t graph, double *kinetic) {

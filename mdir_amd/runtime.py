@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libmdrt_hip.so")
 E_LJ, E_COUL_DIRECT, E_COUL_EXCL, E_COUL_RECIP, E_COUL_SELF, E_DISP_CORR, \
     E_LJ14, E_COUL14, E_KINETIC = range(9)
 NUM_ENERGIES = 9
+TERM_PAIRS, TERM_SPECIAL, TERM_RECIP = 1, 2, 4
 ENERGY_NAMES = ["lj", "coul_direct", "coul_excl", "coul_recip", "coul_self",
                 "disp_corr", "lj14", "coul14", "kinetic"]
 
@@ -114,6 +115,7 @@ SIGNATURES = [
     ("mdrtHipGetPositions", C.c_int, [C.c_void_p, PD]),
     ("mdrtHipGetVelocities", C.c_int, [C.c_void_p, PD]),
     ("mdrtHipComputeForces", C.c_int, [C.c_void_p, PD, PD]),
+    ("mdrtHipComputeForceTerms", C.c_int, [C.c_void_p, I32, PD, PD]),
     ("mdrtHipRun", C.c_int, [C.c_void_p, I64, D, I32, PD]),
     ("mdrtHipRunMeasured", C.c_int, [C.c_void_p, I64, D, I32, PD]),
     ("mdrtHipMinimize", C.c_int, [C.c_void_p, I64, D, PD]),

@@ -72,6 +72,15 @@ class Simulation:
                                                rt.ptr(e, C.c_double)))
         return F, e
 
+    def force_terms(self, terms: int):
+        """Forces and energies of the terms `terms` names (bits
+        runtime.TERM_PAIRS, TERM_SPECIAL, TERM_RECIP) only."""
+        F = np.zeros((self.n, 3))
+        e = np.zeros(rt.NUM_ENERGIES)
+        rt.check(rt.lib().mdrtHipComputeForceTerms(self._h, int(terms), rt.ptr(F, C.c_double),
+                                                   rt.ptr(e, C.c_double)))
+        return F, e
+
     def energies(self) -> dict:
         _, e = self.forces()
         return dict(zip(rt.ENERGY_NAMES, e))

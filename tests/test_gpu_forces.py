@@ -213,3 +213,38 @@ def test_determinism_bitwise():
     b.run(20, 0.002, graph=10)
     assert np.array_equal(a.positions, b.positions)
     assert np.array_equal(a.velocities, b.velocities)
+
+
+def _special_forces(system, x):
+    """Excluded-pair correction plus 1-4 pairs from the oracle: the total
+    less the nonbonded pairs and the reciprocal sum."""
+    o = O.Oracle(system)
+    F, _ = o.total(x)
+    Fn, _ = o.nonbonded(x)
+    Fr = o.pme(x)[0] if system.beta > 0 else 0.0
+    return F - Fn - Fr
+
+
+@pytest.mark.parametrize("case", ["tip3p", "chains"])
+def test_force_error_per_term(case):
+    """Each force term alone against its FP64 oracle term: the nonbonded
+    pairs (LJ + direct Coulomb), the excluded and 1-4 pairs, the PME
+    reciprocal sum; and the terms add up to the total bit for bit in
+    fixed point (to 2^-32 kJ/mol/nm per term)."""
+    from mdir_amd import runtime as rt
+    from fixtures import chain_box
+    say(f"case: per-term forces, {case}")
+    s = S.tip3p_box(2000, 3.9, seed=21) if case == "tip3p" else chain_box(nchains=120)
+    sm = sim(s)
+    o = O.Oracle(s)
+    want = {rt.TERM_PAIRS: o.nonbonded()[0], rt.TERM_RECIP: o.pme()[0],
+            rt.TERM_SPECIAL: _special_forces(s, s.positions)}
+    total, _ = sm.forces()
+    parts = 0.0
+    for term, Fo in want.items():
+        F, _ = sm.force_terms(term)
+        err = O.rel_rms(F, Fo)
+        say(f"  term {term}: rel rms force error {err:.2e} (rms force {np.sqrt((Fo * Fo).sum(1).mean()):.3e})")
+        assert err < 5e-6
+        parts = parts + F
+    assert np.abs(parts - total).max() <= 3 * 2.0 ** -32
