@@ -318,11 +318,16 @@ int mdrtHipFramesSetParameters(void *frames, const double *charges, const double
 int mdrtHipFramesGetParameters(void *frames, double *charges, double *sigma,
                                double *epsilon);
 
-/* Energies [count][MDRT_NUM_ENERGIES] (kinetic 0) and forces [count][n][3]
-   of `count` frames (either output may be NULL).  Loads the frames for
+/* Energies [count][MDRT_NUM_ENERGIES] (kinetic 0), forces [count][n][3]
+   and the gradient in the strain [count][3][3] of `count` frames (any
+   output may be NULL).  The strain gradient is S_ab = dU/d(eps_ab) for
+   x -> (1 + eps) x, h -> h (1 + eps)^T, row a, column b, kJ/mol: minus the
+   virial tensor, with the host's 1/V terms (net-charge background,
+   dispersion correction) on the diagonal (the reference's
+   gradients["strain"], docs/python-frames.md D252).  Loads the frames for
    mdrtHipFramesVjp and mdrtHipFramesHvp. */
 int mdrtHipFramesEvaluate(void *frames, int32_t count, const double *positions,
-                          double *energies, double *forces);
+                          double *energies, double *forces, double *strain);
 
 /* The product with the parameters, summed over the loaded frames,
      p = sum_n [ g_n dU_n/dtheta + sum_i G_ni . d2U_n/(dx_i dtheta) ],

@@ -35,7 +35,7 @@ __device__ __forceinline__ long long to_fixed_adj(float f) {
   return __double2ll_rn((double)f * kAdjScale);
 }
 
-enum AdjKind { kAdjTheta = 0, kAdjHessian = 1 };
+enum AdjKind { kAdjTheta = 0, kAdjHessian = 1, kAdjVirial = 2 };
 
 // Per-system constants of the adjoint pair terms (unscaled modifier
 // constants, unlike NbParams's).
@@ -289,6 +289,55 @@ __device__ void lj14_d12(double sigma, double eps, double r, const SpecialAdjArg
   d2 = u2;
 }
 
+// The virial sum_pairs (-u'/r) d (x) d of the excluded and 1-4 pairs, f64:
+// xx yy zz xy xz yz of this block into vspec[6 block + c] (s_v: 256
+// doubles of LDS; every thread of the block takes part).
+__device__ __forceinline__ void special_virial(int block, const SpecialAdjArgs &sa,
+                                              double *__restrict__ vspec, double *s_v) {
+  const int t = block * blockDim.x + threadIdx.x;
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  if (t < sa.npairs) {
+    const int2 ij = sa.pairs[t];
+    double d[3], r2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      double dd = sa.x[3 * ij.x + k] - sa.x[3 * ij.y + k];
+      dd -= sa.box[k] * rint(dd / sa.box[k]);
+      d[k] = dd;
+      r2 += dd * dd;
+    }
+    const double r = sqrt(r2);
+    double dudr = 0.0;
+    if (sa.beta > 0.0) {
+      const double qq = sa.coulomb * sa.q[ij.x] * sa.q[ij.y];
+      const double ef = erf(sa.beta * r), E = exp(-sa.beta * sa.beta * r2);
+      dudr += -qq * (1.1283791670955126 * sa.beta * E / r - ef / r2);
+    }
+    if (sa.is14 && sa.is14[t] >= 0) {
+      const int k14 = sa.is14[t];
+      double l1, l2;
+      lj14_d12(sa.p14_sigma[k14], sa.p14_eps[k14], r, sa, l1, l2);
+      dudr += l1 - sa.p14_qq[k14] / r2;
+    }
+    const double fsc = -dudr / r;
+    v[0] = fsc * d[0] * d[0];
+    v[1] = fsc * d[1] * d[1];
+    v[2] = fsc * d[2] * d[2];
+    v[3] = fsc * d[0] * d[1];
+    v[4] = fsc * d[0] * d[2];
+    v[5] = fsc * d[1] * d[2];
+  }
+  for (int c = 0; c < 6; ++c) {
+    __syncthreads();
+    s_v[threadIdx.x] = v[c];
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) s_v[threadIdx.x] += s_v[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) vspec[6 * block + c] = s_v[0];
+  }
+}
+
 template <int KIND>
 __device__ __forceinline__ void special_adjoint(int block, const SpecialAdjArgs &sa,
                                                 const float4 *__restrict__ cot, float gs,
@@ -361,14 +410,18 @@ __device__ __forceinline__ void special_adjoint(int block, const SpecialAdjArgs 
 //     dU/deps_a = sum_{i of type a} E_i / (2 eps_a) under Lorentz-Berthelot
 //   Hessian: with fs = -u'/r and D = (dfs/dr)/r = 2 dfs/d(r^2),
 //     H (v_i - v_j) = -fs dv - D (r_ij . dv) r_ij, added to i, taken from j.
+//   virial (KIND kAdjVirial): W = sum_pairs fs r (x) r, f64 per lane, one
+//     partial of 6 (xx yy zz xy xz yz) per (segment, group) wave into vpart
+//     and per block of the special pairs into vspec.
 template <int MOD, bool COUL, int KIND>
 __global__ void __launch_bounds__(256) nb_adjoint(
     const float4 *__restrict__ xq, const int *__restrict__ type_at,
     const uint32_t *__restrict__ real_mask, const PairEntry *__restrict__ rows,
     const int *__restrict__ row_count, const uint64_t *__restrict__ ex_masks, int row_cap,
-    const int *__restrict__ counts_in, const float4 *__restrict__ ljadj, AdjParams p,
-    const float4 *__restrict__ cot, const double *__restrict__ meta,
-    long long *__restrict__ A, SpecialAdjArgs sa, size_t fs) {
+    int max_groups, const int *__restrict__ counts_in, const float4 *__restrict__ ljadj,
+    AdjParams p, const float4 *__restrict__ cot, const double *__restrict__ meta,
+    long long *__restrict__ A, double *__restrict__ vpart, double *__restrict__ vspec,
+    SpecialAdjArgs sa, size_t fs) {
   xq = frame_at(xq, fs);
   type_at = frame_at(type_at, fs);
   real_mask = frame_at(real_mask, fs);
@@ -379,19 +432,29 @@ __global__ void __launch_bounds__(256) nb_adjoint(
   cot = frame_at(cot, fs);
   meta = frame_at(meta, fs);
   A = frame_at(A, fs);
+  vpart = frame_at(vpart, fs);
+  vspec = frame_at(vspec, fs);
   sa.x = frame_at(sa.x, fs);
   sa.place_of = frame_at(sa.place_of, fs);
   extern __shared__ float4 s_lj4[];  // ntypes^2 (launch size)
-  __shared__ struct {
-    float aj[4][kSegment * kCluster * 3];
-    int cj[4][kSegment];
+  __shared__ union {
+    struct {
+      float aj[4][kSegment * kCluster * 3];
+      int cj[4][kSegment];
+    } pr;
+    double v[256];
   } sm;
   const int nt2 = p.ntypes * p.ntypes;
   for (int k = threadIdx.x; k < nt2; k += blockDim.x) s_lj4[k] = ljadj[k];
   __syncthreads();
   const float gs = (float)meta[1];
   if (blockIdx.y == 0) {
-    if ((int)blockIdx.x * (int)blockDim.x < sa.npairs) special_adjoint<KIND>(blockIdx.x, sa, cot, gs, A);
+    if ((int)blockIdx.x * (int)blockDim.x < sa.npairs) {
+      if (KIND == kAdjVirial)
+        special_virial(blockIdx.x, sa, vspec, sm.v);
+      else
+        special_adjoint<KIND>(blockIdx.x, sa, cot, gs, A);
+    }
     return;
   }
   const int lane = threadIdx.x & 63;
@@ -402,9 +465,14 @@ __global__ void __launch_bounds__(256) nb_adjoint(
   const int ngroups = counts_in[1] / kGroup;
   int n = gi < ngroups ? row_count[gi] - e0 : 0;
   n = __builtin_amdgcn_readfirstlane(n < kSegment ? n : kSegment);
-  if (n <= 0) return;
+  const size_t slot = (size_t)seg * max_groups + gi;
+  if (n <= 0) {
+    if (KIND == kAdjVirial && gi < max_groups && lane < 6) vpart[6 * slot + lane] = 0.0;
+    return;
+  }
   const int a = lane & 7, b = lane >> 3;
   const int g0 = gi * kGroup;
+  double vir[6] = {0, 0, 0, 0, 0, 0};
   float4 xi[kGroup], ci[kGroup];
   int ti[kGroup];
   bool ireal[kGroup];
@@ -447,7 +515,18 @@ __global__ void __launch_bounds__(256) nb_adjoint(
       const float4 T4 = s_lj4[ti[m] + tj];
       const float dvx = ci[m].x - cj.x, dvy = ci[m].y - cj.y, dvz = ci[m].z - cj.z;
       const float s = dx * dvx + dy * dvy + dz * dvz;
-      if (KIND == kAdjTheta) {
+      if (KIND == kAdjVirial) {
+        const PairParts<float> P = pair_parts<MOD, COUL, float>(on ? r2 : 1.0f, p);
+        float fsv = T4.x * P.f12 - T4.y * P.f6;
+        if (COUL) fsv += xi[m].w * xj.w * P.hc;
+        const double f = on ? (double)fsv : 0.0;
+        vir[0] += f * (double)(dx * dx);
+        vir[1] += f * (double)(dy * dy);
+        vir[2] += f * (double)(dz * dz);
+        vir[3] += f * (double)(dx * dy);
+        vir[4] += f * (double)(dx * dz);
+        vir[5] += f * (double)(dy * dz);
+      } else if (KIND == kAdjTheta) {
         const PairParts<float> P = pair_parts<MOD, COUL, float>(on ? r2 : 1.0f, p);
         const float Bc = COUL ? gs * P.uc - P.hc * s : 0.0f;
         const float P12 = gs * P.u12 - P.f12 * s;
@@ -482,19 +561,31 @@ __global__ void __launch_bounds__(256) nb_adjoint(
         aj2 -= hz;
       }
     }
-    const float g0s = sum8(aj0), g1s = sum8(aj1), g2s = sum8(aj2);
-    if (a == 0) {
-      float *dst = &sm.aj[wave][(e * kCluster + b) * 3];
-      dst[0] = g0s;
-      dst[1] = g1s;
-      dst[2] = g2s;
+    if (KIND != kAdjVirial) {
+      const float g0s = sum8(aj0), g1s = sum8(aj1), g2s = sum8(aj2);
+      if (a == 0) {
+        float *dst = &sm.pr.aj[wave][(e * kCluster + b) * 3];
+        dst[0] = g0s;
+        dst[1] = g1s;
+        dst[2] = g2s;
+      }
+      if (lane == 0) sm.pr.cj[wave][e] = E.cj;
     }
-    if (lane == 0) sm.cj[wave][e] = E.cj;
     E = E1;
     E1 = E2;
     xj = xj1;
     cj = cj1;
     tj = tj1;
+  }
+  if (KIND == kAdjVirial) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+#pragma unroll
+      for (int s = 1; s < 64; s <<= 1) vir[c] += __shfl_xor(vir[c], s);
+    }
+    if (lane == 0)
+      for (int c = 0; c < 6; ++c) vpart[6 * slot + c] = vir[c];
+    return;
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -502,8 +593,8 @@ __global__ void __launch_bounds__(256) nb_adjoint(
   unsigned long long *u = reinterpret_cast<unsigned long long *>(A);
   for (int k = lane; k < n * kCluster * 3; k += 64) {
     const int e = k / (kCluster * 3), rem = k - e * (kCluster * 3);
-    const long long w = to_fixed_adj(sm.aj[wave][k]);
-    if (w) atomicAdd(u + 3 * (sm.cj[wave][e] * kCluster) + rem, (unsigned long long)w);
+    const long long w = to_fixed_adj(sm.pr.aj[wave][k]);
+    if (w) atomicAdd(u + 3 * (sm.pr.cj[wave][e] * kCluster) + rem, (unsigned long long)w);
   }
 #pragma unroll
   for (int m = 0; m < kGroup; ++m) {

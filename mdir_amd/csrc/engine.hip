@@ -225,6 +225,7 @@ struct Engine {
   float *d_tab = nullptr;
   ConvTables ct{};
   double *d_erecip = nullptr;
+  double *d_vrecip = nullptr;  // reciprocal virial per plane block, when asked (frames)
 
   // ---- integration ----
   int *d_unit_first = nullptr, *d_unit_size = nullptr;
@@ -779,6 +780,7 @@ struct Engine {
       ct.m[a] = d_tab + (3 + a) * Kmax;
     }
     ct.K3 = pme_grid[2];
+    ct.pb2 = (float)(M_PI * M_PI / (beta * beta));
   }
 
   // ---- launches ----
@@ -977,7 +979,8 @@ struct Engine {
     if (plane_fft()) {
       const size_t lds = (size_t)2 * pme_grid[0] * pme_grid[1] * sizeof(cpx<float>);
       fft_plane_conv<<<dim3(fft.H(), 1, nz), 512, lds, pstream>>>(
-          d_cplx, fft.K[0], fft.K[1], fft.ax[0].axis, fft.ax[1].axis, ct, ep, fs);
+          d_cplx, fft.K[0], fft.K[1], fft.ax[0].axis, fft.ax[1].axis, ct, ep,
+          energy ? d_vrecip : nullptr, fs);
       return;
     }
     const int TL = Fft3D<float>::TL;
@@ -1688,9 +1691,9 @@ int mdrtHipFramesGetParameters(void *frames, double *charges, double *sigma, dou
 }
 
 int mdrtHipFramesEvaluate(void *frames, int32_t count, const double *positions,
-                          double *energies, double *forces) {
+                          double *energies, double *forces, double *strain) {
   FrameBatch *f = static_cast<FrameBatch *>(frames);
-  return guarded([&] { f->evaluate(count, positions, energies, forces); });
+  return guarded([&] { f->evaluate(count, positions, energies, forces, strain); });
 }
 
 int mdrtHipFramesVjp(void *frames, int32_t count, const double *energy_cotangent,

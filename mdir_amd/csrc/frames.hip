@@ -36,6 +36,8 @@ struct FrameBatch {
   long long *d_A = nullptr;     // 3 max_places product words
   cpx<float> *d_cplx2 = nullptr;  // dipole grid spectrum
   double *d_meta = nullptr;     // 4: scale c, g / c, loss, -
+  double *d_vpart = nullptr;    // 6 per (segment, group) wave: the pair virial
+  double *d_vspec = nullptr;    // 6 per block of the excluded and 1-4 pairs
   // shared
   float4 *d_ljadj = nullptr;    // per type pair: C12, C6, 12 C12/sigma, 6 C6/sigma
   // small results read by the host at once: loss [max_frames], flags OR'ed
@@ -64,11 +66,15 @@ struct FrameBatch {
     d_cot = dalloc<float4>(e.max_places);
     d_A = dalloc<long long>(3 * (size_t)e.max_places);
     d_meta = dalloc<double>(4);
+    d_vpart = dalloc<double>(6 * (size_t)e.max_groups * e.nseg);
+    d_vspec = dalloc<double>(6 * (size_t)std::max(1, blocks_for(e.nspecial)));
     e.frame_extra = {(void **)&d_fin, (void **)&d_fout, (void **)&d_cot, (void **)&d_A,
-                     (void **)&d_meta};
+                     (void **)&d_meta, (void **)&d_vpart, (void **)&d_vspec};
     if (e.use_pme) {
       d_cplx2 = dalloc<cpx<float>>((size_t)e.pme_grid[0] * e.pme_grid[1] * (e.pme_grid[2] / 2 + 1));
+      e.d_vrecip = dalloc<double>(6 * (size_t)e.fft.H());
       e.frame_extra.push_back((void **)&d_cplx2);
+      e.frame_extra.push_back((void **)&e.d_vrecip);
     }
   }
 
@@ -248,11 +254,13 @@ struct FrameBatch {
     return true;
   }
 
-  void evaluate(int frames, const double *positions, double *energies, double *forces) {
+  void evaluate(int frames, const double *positions, double *energies, double *forces,
+                double *strain) {
     check_frames(frames);
     for (int attempt = 0; attempt < 8; ++attempt) {
       scatter(e.d_x, positions, 3 * (size_t)e.n, frames);
-      forward(frames, energies != nullptr);
+      forward(frames, energies != nullptr || strain != nullptr);
+      if (strain) launch_adjoint_k<kAdjVirial>();
       launch_collect(frames);
       read_result();
       if (!regrow_if_overflowed()) {
@@ -263,6 +271,7 @@ struct FrameBatch {
           gather(forces, d_fout, 3 * (size_t)e.n, frames);
         }
         if (energies) frame_energies(frames, energies);
+        if (strain) frame_strain(frames, strain);
         HIPCHK(hipStreamSynchronize(e.stream));
         return;
       }
@@ -306,6 +315,46 @@ struct FrameBatch {
     }
   }
 
+  // dU/d(strain) of every frame (9 values, row alpha, column beta) for
+  // x -> (1 + eps) x, h -> h (1 + eps)^T: minus the virial of the pairs,
+  // the excluded and 1-4 pairs and the reciprocal sum (partials in fixed
+  // order), minus the 1/V host terms (net-charge background, dispersion
+  // correction) on the diagonal.
+  void frame_strain(int frames, double *out) {
+    auto read = [&](const double *src, size_t count) {
+      std::vector<double> v((size_t)frames * count);
+      if (count)
+        HIPCHK(hipMemcpy2DAsync(v.data(), count * sizeof(double), src, e.fs,
+                                count * sizeof(double), frames, hipMemcpyDeviceToHost,
+                                e.stream));
+      return v;
+    };
+    const size_t np = 6 * (size_t)e.max_groups * e.nseg;
+    const size_t ns = e.nspecial ? 6 * (size_t)blocks_for(e.nspecial) : 0;
+    const size_t nr = e.use_pme ? 6 * (size_t)e.fft.H() : 0;
+    const std::vector<double> vp = read(d_vpart, np), vs = read(d_vspec, ns),
+                              vr = read(e.d_vrecip, nr);
+    HIPCHK(hipStreamSynchronize(e.stream));
+    double net = 0;
+    for (double q : e.h_q) net += q;
+    const double volume = e.box[0] * e.box[1] * e.box[2];
+    const double e_net =
+        e.beta > 0 ? -kCoulomb * M_PI * net * net / (2.0 * volume * e.beta * e.beta) : 0.0;
+    const int ij[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+    for (int fr = 0; fr < frames; ++fr) {
+      double w[6] = {0, 0, 0, 0, 0, 0};
+      for (size_t k = 0; k < np; ++k) w[k % 6] += vp[fr * np + k];
+      for (size_t k = 0; k < ns; ++k) w[k % 6] += vs[fr * ns + k];
+      for (size_t k = 0; k < nr; ++k) w[k % 6] += vr[fr * nr + k];
+      double *S = out + 9 * (size_t)fr;
+      for (int c = 0; c < 6; ++c) {
+        S[3 * ij[c][0] + ij[c][1]] = -w[c];
+        S[3 * ij[c][1] + ij[c][0]] = -w[c];
+      }
+      for (int a = 0; a < 3; ++a) S[4 * a] -= e_net + e.e_disp;
+    }
+  }
+
   // ---- adjoint ----
   template <int MOD, bool COUL, int KIND>
   void launch_adjoint_t() {
@@ -329,7 +378,8 @@ struct FrameBatch {
                     e.nz);
     nb_adjoint<MOD, COUL, KIND><<<grid, 256, e.ntypes * e.ntypes * sizeof(float4), e.stream>>>(
         e.d_xq, e.d_type_at, e.d_real_mask, e.d_rows, e.d_row_count, e.d_ex, e.g.row_cap,
-        e.d_counts, d_ljadj, ap, d_cot, d_meta, d_A, sa, e.fs);
+        e.max_groups, e.d_counts, d_ljadj, ap, d_cot, d_meta, d_A,
+        KIND == kAdjVirial ? d_vpart : nullptr, KIND == kAdjVirial ? d_vspec : nullptr, sa, e.fs);
   }
   template <int KIND>
   void launch_adjoint_k() {
@@ -363,7 +413,8 @@ struct FrameBatch {
         e.d_xpq, d_cot, e.pmp, e.bp, d_cplx2, e.fft.ax[2].axis, e.fs);
     const size_t plds = (size_t)2 * e.pme_grid[0] * e.pme_grid[1] * sizeof(cpx<float>);
     fft_plane_conv<<<dim3(e.fft.H(), 1, e.nz), 512, plds, e.pstream>>>(
-        d_cplx2, e.fft.K[0], e.fft.K[1], e.fft.ax[0].axis, e.fft.ax[1].axis, e.ct, nullptr, e.fs);
+        d_cplx2, e.fft.K[0], e.fft.K[1], e.fft.ax[0].axis, e.fft.ax[1].axis, e.ct, nullptr,
+        nullptr, e.fs);
     pme_gather_adj<N, TL, KIND><<<dim3(blocks, 1, e.nz), 256, lds, e.pstream>>>(
         e.d_xpq, d_cot, d_meta, e.pmp, e.bp, e.d_cplx, d_cplx2, e.fft.ax[2].axis, d_A, e.fs);
   }

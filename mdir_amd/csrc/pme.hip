@@ -583,6 +583,7 @@ struct ConvTables {
   const void *t[3];   // t_a(k) = exp(-pi^2 m_a^2 / beta^2) |b_a(k)|^2, t_0 * f/(pi V)
   const void *m[3];   // m_a(k) = signed k / L_a
   int K3;
+  float pb2;          // pi^2 / beta^2 (the reciprocal virial)
 };
 
 template <typename T>
@@ -662,11 +663,13 @@ __global__ void __launch_bounds__(512) fft_plane_conv(cpx<float> *__restrict__ d
                                                       int K2, FftAxis ax, FftAxis ay,
                                                       ConvTables ct,
                                                       double *__restrict__ energy_partial,
+                                                      double *__restrict__ virial_partial,
                                                       size_t fs) {
   extern __shared__ unsigned char smem[];
   __shared__ double s_e[512];
   data = frame_at(data, fs);
   energy_partial = frame_at(energy_partial, fs);
+  virial_partial = frame_at(virial_partial, fs);
   __shared__ cpx<float> s_twx[kPlaneAxisMax], s_twy[kPlaneAxisMax];
   __shared__ float s_t0[kPlaneAxisMax], s_m0[kPlaneAxisMax];
   __shared__ float s_t1[kPlaneAxisMax], s_m1[kPlaneAxisMax];
@@ -698,6 +701,9 @@ __global__ void __launch_bounds__(512) fft_plane_conv(cpx<float> *__restrict__ d
   const float m2 = static_cast<const float *>(ct.m[2])[kz];
   const double h = (kz == 0 || (ct.K3 % 2 == 0 && kz == ct.K3 / 2)) ? 0.5 : 1.0;
   double e = 0.0;
+  // the reciprocal virial sum_m E_m [delta_ab - 2 (1/m^2 + pi^2/beta^2) m_a m_b]
+  // (05-electrostatics.md:262-279): xx yy zz xy xz yz
+  double vir[6] = {0, 0, 0, 0, 0, 0};
   for (int w = threadIdx.x; w < np; w += blockDim.x) {
     const int kx = w / K2, ky = w % K2;
     const int i = plane_index(kx, ky, K2, swz);
@@ -706,7 +712,20 @@ __global__ void __launch_bounds__(512) fft_plane_conv(cpx<float> *__restrict__ d
     if ((kx | ky | kz) != 0) {
       const float msq = s_m0[kx] * s_m0[kx] + s_m1[ky] * s_m1[ky] + m2 * m2;
       bc = s_t0[kx] * s_t1[ky] * t2 / msq;
-      if (energy_partial) e += h * (double)bc * ((double)v.x * v.x + (double)v.y * v.y);
+      if (energy_partial || virial_partial) {
+        const double em = h * (double)bc * ((double)v.x * v.x + (double)v.y * v.y);
+        e += em;
+        if (virial_partial) {
+          const double c = 2.0 * (1.0 / (double)msq + (double)ct.pb2);
+          const double mx = s_m0[kx], my = s_m1[ky], mz = m2;
+          vir[0] += em * (1.0 - c * mx * mx);
+          vir[1] += em * (1.0 - c * my * my);
+          vir[2] += em * (1.0 - c * mz * mz);
+          vir[3] -= em * c * mx * my;
+          vir[4] -= em * c * mx * mz;
+          vir[5] -= em * c * my * mz;
+        }
+      }
     }
     r[i] = {v.x * bc, v.y * bc};
   }
@@ -723,6 +742,18 @@ __global__ void __launch_bounds__(512) fft_plane_conv(cpx<float> *__restrict__ d
       __syncthreads();
     }
     if (threadIdx.x == 0) energy_partial[blockIdx.x] = s_e[0];
+  }
+  if (virial_partial) {
+    for (int c = 0; c < 6; ++c) {
+      __syncthreads();
+      s_e[threadIdx.x] = vir[c];
+      __syncthreads();
+      for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) s_e[threadIdx.x] += s_e[threadIdx.x + s];
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) virial_partial[6 * blockIdx.x + c] = s_e[0];
+    }
   }
 }
 

@@ -78,9 +78,11 @@ class FrameGradient(Mapping):
 class FrameEnergies:
     """Energies of the evaluated frames: `energy` (K,), `terms`
     (K, NUM_ENERGIES) in the order of runtime.ENERGY_NAMES, `forces`
-    (K, N, 3) and `gradients["positions"]` = -forces."""
+    (K, N, 3), and `gradients`: "positions" = dU/dx = -forces (K, N, 3) and,
+    when asked, "strain" = dU/d(eps) (K, 3, 3), row alpha, column beta, for
+    x -> (1 + eps) x, h -> h (1 + eps)^T (the reference's D252)."""
 
-    def __init__(self, evaluator, count, terms, forces):
+    def __init__(self, evaluator, count, terms, forces, strain=None):
         self._evaluator = evaluator
         self.count = count
         self.version = evaluator._version
@@ -88,6 +90,8 @@ class FrameEnergies:
         self.energy = terms.sum(axis=1)
         self.forces = forces
         self.gradients = {"positions": -forces} if forces is not None else {}
+        if strain is not None:
+            self.gradients["strain"] = strain
 
     def vjp(self, cotangent=None, *, positions=None, gradients=("tunables",)):
         """sum_n cotangent[n] dU_n/dtheta + sum_n sum_i positions[n, i] .
@@ -202,11 +206,13 @@ class FrameEvaluator:
             raise ValueError(f"frames: 1 to {self.max_frames} per call, found {K}")
         return K
 
-    def evaluate(self, positions, forces_out=None) -> FrameEnergies:
+    def evaluate(self, positions, forces_out=None, gradients=("positions",)) -> FrameEnergies:
         """Energies and forces of the frames `positions` (K, N, 3); loads
         them for vjp and hvp.  `forces_out`, a CUDA float64 tensor
-        (K, N, 3), receives the forces on the device instead."""
+        (K, N, 3), receives the forces on the device instead.  "strain" in
+        `gradients` adds gradients["strain"] (K, 3, 3)."""
         K = self._count(positions)
+        strain = np.zeros((K, 3, 3)) if "strain" in gradients else None
         pp, keep, _ = _frame_array(positions, (K, self.n, 3), "positions")
         terms = np.zeros((K, rt.NUM_ENERGIES))
         if forces_out is not None:
@@ -215,11 +221,13 @@ class FrameEvaluator:
         else:
             F = np.zeros((K, self.n, 3))
             fp, fkeep = C.c_void_p(F.ctypes.data), F
-        rt.check(rt.lib().mdrtHipFramesEvaluate(self._h, K, pp, rt.ptr(terms, C.c_double), fp))
+        rt.check(rt.lib().mdrtHipFramesEvaluate(
+            self._h, K, pp, rt.ptr(terms, C.c_double), fp,
+            None if strain is None else rt.ptr(strain, C.c_double)))
         del keep, fkeep
         self._version += 1
         self._loaded = K
-        return FrameEnergies(self, K, terms, F)
+        return FrameEnergies(self, K, terms, F, strain)
 
     def hvp(self, v, out=None):
         """H_n v_n (K, N, 3) for the loaded frames (the reference's

@@ -126,7 +126,7 @@ SIGNATURES = [
     ("mdrtHipFramesDestroy", None, [C.c_void_p]),
     ("mdrtHipFramesSetParameters", C.c_int, [C.c_void_p, PD, PD, PD]),
     ("mdrtHipFramesGetParameters", C.c_int, [C.c_void_p, PD, PD, PD]),
-    ("mdrtHipFramesEvaluate", C.c_int, [C.c_void_p, I32, C.c_void_p, PD, C.c_void_p]),
+    ("mdrtHipFramesEvaluate", C.c_int, [C.c_void_p, I32, C.c_void_p, PD, C.c_void_p, PD]),
     ("mdrtHipFramesVjp", C.c_int, [C.c_void_p, I32, PD, C.c_void_p, PD, PD, PD]),
     ("mdrtHipFramesHvp", C.c_int, [C.c_void_p, I32, C.c_void_p, C.c_void_p]),
     ("mdrtHipFramesForceMatching", C.c_int, [C.c_void_p, I32, C.c_void_p, C.c_void_p, I32,
