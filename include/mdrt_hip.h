@@ -341,6 +341,15 @@ int mdrtHipFramesVjp(void *frames, int32_t count, const double *energy_cotangent
                      const double *position_cotangent, double *d_charges, double *d_sigma,
                      double *d_epsilon);
 
+/* The same product in individual type-pair table entries (a, b) (the
+   reference's "per pair of types" sigma and epsilon tunables, NBFIX
+   entries; sigma_ab = sigma_ba and eps_ab = eps_ba are one parameter each)
+   instead of the per-type parameters: 1 to 4 pairs type_pairs [npairs][2],
+   outputs d_sigma [npairs], d_epsilon [npairs]. */
+int mdrtHipFramesVjpPairs(void *frames, int32_t count, const double *energy_cotangent,
+                          const double *position_cotangent, int32_t npairs,
+                          const int32_t *type_pairs, double *d_sigma, double *d_epsilon);
+
 /* out[n] = H_n v[n], H_n the Hessian of U_n in the positions, for the
    loaded frames (the reference's frame_hvp, torch.py:341; with the
    reciprocal PME sum included). */

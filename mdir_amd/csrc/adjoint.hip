@@ -35,7 +35,22 @@ __device__ __forceinline__ long long to_fixed_adj(float f) {
   return __double2ll_rn((double)f * kAdjScale);
 }
 
-enum AdjKind { kAdjTheta = 0, kAdjHessian = 1, kAdjVirial = 2 };
+enum AdjKind { kAdjTheta = 0, kAdjHessian = 1, kAdjVirial = 2, kAdjPairTable = 3 };
+
+// Doubles per (segment, group) wave slot of the per-wave partials: the
+// virial (6) or the products in up to kMaxTablePairs type-pair entries
+// (sigma and epsilon of each).
+constexpr int kMaxTablePairs = 4;
+constexpr int kPartStride = 2 * kMaxTablePairs;
+
+// Per type pair (ti * ntypes + tj): lj = (C12, C6, 12 C12/sigma, 6 C6/sigma);
+// unit = (4 sigma^12, 4 sigma^6), the coefficients per unit epsilon; sel =
+// the selected entry 0 .. kMaxTablePairs-1 of the pair-table products, or -1.
+struct AdjTables {
+  const float4 *lj;
+  const float2 *unit;
+  const int *sel;
+};
 
 // Per-system constants of the adjoint pair terms (unscaled modifier
 // constants, unlike NbParams's).
@@ -413,12 +428,16 @@ __device__ __forceinline__ void special_adjoint(int block, const SpecialAdjArgs 
 //   virial (KIND kAdjVirial): W = sum_pairs fs r (x) r, f64 per lane, one
 //     partial of 6 (xx yy zz xy xz yz) per (segment, group) wave into vpart
 //     and per block of the special pairs into vspec.
+//   type-pair entries (KIND kAdjPairTable): for the selected table entries
+//     (a, b), sum over the pairs of those types of dl/dsigma_ab =
+//     12 C12/sigma P12 + 6 C6/sigma P6 and dl/deps_ab = 4 sigma^12 P12 +
+//     4 sigma^6 P6, one partial of 2 kMaxTablePairs per wave into vpart.
 template <int MOD, bool COUL, int KIND>
 __global__ void __launch_bounds__(256) nb_adjoint(
     const float4 *__restrict__ xq, const int *__restrict__ type_at,
     const uint32_t *__restrict__ real_mask, const PairEntry *__restrict__ rows,
     const int *__restrict__ row_count, const uint64_t *__restrict__ ex_masks, int row_cap,
-    int max_groups, const int *__restrict__ counts_in, const float4 *__restrict__ ljadj,
+    int max_groups, const int *__restrict__ counts_in, AdjTables tab,
     AdjParams p, const float4 *__restrict__ cot, const double *__restrict__ meta,
     long long *__restrict__ A, double *__restrict__ vpart, double *__restrict__ vspec,
     SpecialAdjArgs sa, size_t fs) {
@@ -436,7 +455,7 @@ __global__ void __launch_bounds__(256) nb_adjoint(
   vspec = frame_at(vspec, fs);
   sa.x = frame_at(sa.x, fs);
   sa.place_of = frame_at(sa.place_of, fs);
-  extern __shared__ float4 s_lj4[];  // ntypes^2 (launch size)
+  extern __shared__ float4 s_lj4[];  // ntypes^2 x (float4, float2, int) (launch size)
   __shared__ union {
     struct {
       float aj[4][kSegment * kCluster * 3];
@@ -445,14 +464,22 @@ __global__ void __launch_bounds__(256) nb_adjoint(
     double v[256];
   } sm;
   const int nt2 = p.ntypes * p.ntypes;
-  for (int k = threadIdx.x; k < nt2; k += blockDim.x) s_lj4[k] = ljadj[k];
+  float2 *s_unit = reinterpret_cast<float2 *>(s_lj4 + nt2);
+  int *s_sel = reinterpret_cast<int *>(s_unit + nt2);
+  for (int k = threadIdx.x; k < nt2; k += blockDim.x) {
+    s_lj4[k] = tab.lj[k];
+    if (KIND == kAdjPairTable) {
+      s_unit[k] = tab.unit[k];
+      s_sel[k] = tab.sel[k];
+    }
+  }
   __syncthreads();
   const float gs = (float)meta[1];
   if (blockIdx.y == 0) {
     if ((int)blockIdx.x * (int)blockDim.x < sa.npairs) {
       if (KIND == kAdjVirial)
         special_virial(blockIdx.x, sa, vspec, sm.v);
-      else
+      else if (KIND == kAdjTheta || KIND == kAdjHessian)
         special_adjoint<KIND>(blockIdx.x, sa, cot, gs, A);
     }
     return;
@@ -467,12 +494,14 @@ __global__ void __launch_bounds__(256) nb_adjoint(
   n = __builtin_amdgcn_readfirstlane(n < kSegment ? n : kSegment);
   const size_t slot = (size_t)seg * max_groups + gi;
   if (n <= 0) {
-    if (KIND == kAdjVirial && gi < max_groups && lane < 6) vpart[6 * slot + lane] = 0.0;
+    if ((KIND == kAdjVirial || KIND == kAdjPairTable) && gi < max_groups && lane < kPartStride)
+      vpart[kPartStride * slot + lane] = 0.0;
     return;
   }
   const int a = lane & 7, b = lane >> 3;
   const int g0 = gi * kGroup;
   double vir[6] = {0, 0, 0, 0, 0, 0};
+  float tps[kMaxTablePairs] = {0, 0, 0, 0}, tpe[kMaxTablePairs] = {0, 0, 0, 0};
   float4 xi[kGroup], ci[kGroup];
   int ti[kGroup];
   bool ireal[kGroup];
@@ -515,7 +544,23 @@ __global__ void __launch_bounds__(256) nb_adjoint(
       const float4 T4 = s_lj4[ti[m] + tj];
       const float dvx = ci[m].x - cj.x, dvy = ci[m].y - cj.y, dvz = ci[m].z - cj.z;
       const float s = dx * dvx + dy * dvy + dz * dvz;
-      if (KIND == kAdjVirial) {
+      if (KIND == kAdjPairTable) {
+        const int sel = s_sel[ti[m] + tj];
+        if (sel >= 0) {  // the pairs of a selected entry only (divergent)
+          const PairParts<float> P = pair_parts<MOD, COUL, float>(on ? r2 : 1.0f, p);
+          const float P12 = gs * P.u12 - P.f12 * s;
+          const float P6 = P.f6 * s - gs * P.u6;
+          const float2 U = s_unit[ti[m] + tj];
+          const float Ss = on ? T4.z * P12 + T4.w * P6 : 0.0f;
+          const float Eu = on ? U.x * P12 + U.y * P6 : 0.0f;
+#pragma unroll
+          for (int t = 0; t < kMaxTablePairs; ++t)
+            if (sel == t) {
+              tps[t] += Ss;
+              tpe[t] += Eu;
+            }
+        }
+      } else if (KIND == kAdjVirial) {
         const PairParts<float> P = pair_parts<MOD, COUL, float>(on ? r2 : 1.0f, p);
         float fsv = T4.x * P.f12 - T4.y * P.f6;
         if (COUL) fsv += xi[m].w * xj.w * P.hc;
@@ -561,7 +606,7 @@ __global__ void __launch_bounds__(256) nb_adjoint(
         aj2 -= hz;
       }
     }
-    if (KIND != kAdjVirial) {
+    if (KIND == kAdjTheta || KIND == kAdjHessian) {
       const float g0s = sum8(aj0), g1s = sum8(aj1), g2s = sum8(aj2);
       if (a == 0) {
         float *dst = &sm.pr.aj[wave][(e * kCluster + b) * 3];
@@ -584,7 +629,23 @@ __global__ void __launch_bounds__(256) nb_adjoint(
       for (int s = 1; s < 64; s <<= 1) vir[c] += __shfl_xor(vir[c], s);
     }
     if (lane == 0)
-      for (int c = 0; c < 6; ++c) vpart[6 * slot + c] = vir[c];
+      for (int c = 0; c < 6; ++c) vpart[kPartStride * slot + c] = vir[c];
+    return;
+  }
+  if (KIND == kAdjPairTable) {
+    double w[kPartStride];
+#pragma unroll
+    for (int t = 0; t < kMaxTablePairs; ++t) {
+      w[2 * t] = tps[t];
+      w[2 * t + 1] = tpe[t];
+    }
+#pragma unroll
+    for (int c = 0; c < kPartStride; ++c) {
+#pragma unroll
+      for (int s = 1; s < 64; s <<= 1) w[c] += __shfl_xor(w[c], s);
+    }
+    if (lane == 0)
+      for (int c = 0; c < kPartStride; ++c) vpart[kPartStride * slot + c] = w[c];
     return;
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

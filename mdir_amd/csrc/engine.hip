@@ -1705,6 +1705,16 @@ int mdrtHipFramesVjp(void *frames, int32_t count, const double *energy_cotangent
   });
 }
 
+int mdrtHipFramesVjpPairs(void *frames, int32_t count, const double *energy_cotangent,
+                          const double *position_cotangent, int32_t npairs,
+                          const int32_t *type_pairs, double *d_sigma, double *d_epsilon) {
+  FrameBatch *f = static_cast<FrameBatch *>(frames);
+  return guarded([&] {
+    f->vjp_pairs(count, energy_cotangent, position_cotangent, npairs, type_pairs, d_sigma,
+                 d_epsilon);
+  });
+}
+
 int mdrtHipFramesHvp(void *frames, int32_t count, const double *v, double *out) {
   FrameBatch *f = static_cast<FrameBatch *>(frames);
   return guarded([&] { f->hvp(count, v, out); });

@@ -40,6 +40,8 @@ struct FrameBatch {
   double *d_vspec = nullptr;    // 6 per block of the excluded and 1-4 pairs
   // shared
   float4 *d_ljadj = nullptr;    // per type pair: C12, C6, 12 C12/sigma, 6 C6/sigma
+  float2 *d_ljunit = nullptr;   // per type pair: 4 sigma^12, 4 sigma^6
+  int *d_sel = nullptr;         // per type pair: selected table entry, or -1
   // small results read by the host at once: loss [max_frames], flags OR'ed
   // over the frames [kNumFlags], products summed over the frames [3n]
   double *d_result = nullptr;
@@ -55,6 +57,8 @@ struct FrameBatch {
   ~FrameBatch() {
     if (fm_graph) hipGraphExecDestroy(fm_graph);
     dfree(d_ljadj);
+    dfree(d_ljunit);
+    dfree(d_sel);
     dfree(d_result);
     dfree(d_stage);
   }
@@ -66,7 +70,7 @@ struct FrameBatch {
     d_cot = dalloc<float4>(e.max_places);
     d_A = dalloc<long long>(3 * (size_t)e.max_places);
     d_meta = dalloc<double>(4);
-    d_vpart = dalloc<double>(6 * (size_t)e.max_groups * e.nseg);
+    d_vpart = dalloc<double>((size_t)kPartStride * e.max_groups * e.nseg);
     d_vspec = dalloc<double>(6 * (size_t)std::max(1, blocks_for(e.nspecial)));
     e.frame_extra = {(void **)&d_fin, (void **)&d_fout, (void **)&d_cot, (void **)&d_A,
                      (void **)&d_meta, (void **)&d_vpart, (void **)&d_vspec};
@@ -105,6 +109,8 @@ struct FrameBatch {
       }
     set_adj_params();
     d_ljadj = dalloc<float4>((size_t)nt * nt);
+    d_ljunit = dalloc<float2>((size_t)nt * nt);
+    d_sel = dalloc<int>((size_t)nt * nt);
     const size_t nres = (size_t)frames + kNumFlags + 3 * (size_t)e.n;
     d_result = dalloc<double>(nres);
     d_sum = d_result + frames + kNumFlags;
@@ -150,6 +156,7 @@ struct FrameBatch {
   void upload_types() {
     const int nt = e.ntypes;
     std::vector<float4> adj((size_t)nt * nt);
+    std::vector<float2> unit((size_t)nt * nt);
     for (int a = 0; a < nt; ++a)
       for (int b = 0; b < nt; ++b) {
         const double ee = sqrt(eps_t[a] * eps_t[b]);
@@ -160,8 +167,10 @@ struct FrameBatch {
         adj[(size_t)a * nt + b] =
             make_float4((float)c12, (float)c6, ss > 0 ? (float)(12.0 * c12 / ss) : 0.0f,
                         ss > 0 ? (float)(6.0 * c6 / ss) : 0.0f);
+        unit[(size_t)a * nt + b] = make_float2((float)(4.0 * s6 * s6), (float)(4.0 * s6));
       }
     e.to_device(d_ljadj, adj.data(), adj.size() * sizeof(float4));
+    e.to_device(d_ljunit, unit.data(), unit.size() * sizeof(float2));
     e.upload_lj();
     e.host_constants();
   }
@@ -329,7 +338,7 @@ struct FrameBatch {
                                 e.stream));
       return v;
     };
-    const size_t np = 6 * (size_t)e.max_groups * e.nseg;
+    const size_t np = (size_t)kPartStride * e.max_groups * e.nseg;
     const size_t ns = e.nspecial ? 6 * (size_t)blocks_for(e.nspecial) : 0;
     const size_t nr = e.use_pme ? 6 * (size_t)e.fft.H() : 0;
     const std::vector<double> vp = read(d_vpart, np), vs = read(d_vspec, ns),
@@ -343,7 +352,8 @@ struct FrameBatch {
     const int ij[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
     for (int fr = 0; fr < frames; ++fr) {
       double w[6] = {0, 0, 0, 0, 0, 0};
-      for (size_t k = 0; k < np; ++k) w[k % 6] += vp[fr * np + k];
+      for (size_t k = 0; k < np; ++k)
+        if (k % kPartStride < 6) w[k % kPartStride] += vp[fr * np + k];
       for (size_t k = 0; k < ns; ++k) w[k % 6] += vs[fr * ns + k];
       for (size_t k = 0; k < nr; ++k) w[k % 6] += vr[fr * nr + k];
       double *S = out + 9 * (size_t)fr;
@@ -376,10 +386,13 @@ struct FrameBatch {
     sa.lj_modifier = e.lj_modifier;
     const dim3 grid(std::max(blocks_for(e.max_groups, 4), blocks_for(e.nspecial)), e.nseg + 1,
                     e.nz);
-    nb_adjoint<MOD, COUL, KIND><<<grid, 256, e.ntypes * e.ntypes * sizeof(float4), e.stream>>>(
+    const AdjTables tab{d_ljadj, d_ljunit, d_sel};
+    const bool parts = KIND == kAdjVirial || KIND == kAdjPairTable;
+    const size_t lds = (size_t)e.ntypes * e.ntypes * (sizeof(float4) + sizeof(float2) + sizeof(int));
+    nb_adjoint<MOD, COUL, KIND><<<grid, 256, lds, e.stream>>>(
         e.d_xq, e.d_type_at, e.d_real_mask, e.d_rows, e.d_row_count, e.d_ex, e.g.row_cap,
-        e.max_groups, e.d_counts, d_ljadj, ap, d_cot, d_meta, d_A,
-        KIND == kAdjVirial ? d_vpart : nullptr, KIND == kAdjVirial ? d_vspec : nullptr, sa, e.fs);
+        e.max_groups, e.d_counts, tab, ap, d_cot, d_meta, d_A, parts ? d_vpart : nullptr,
+        KIND == kAdjVirial ? d_vspec : nullptr, sa, e.fs);
   }
   template <int KIND>
   void launch_adjoint_k() {
@@ -546,6 +559,89 @@ struct FrameBatch {
     adjoint<kAdjTheta>(0);
     read_result();
     theta_from_sums(sums(), gsum, dq, dsig, deps);
+  }
+
+  // The product with up to kMaxTablePairs type-pair table entries (a, b)
+  // (sigma_ab = sigma_ba and eps_ab = eps_ba each one parameter), summed
+  // over the loaded frames: the pair terms, and the energy cotangents'
+  // share of the dispersion correction.
+  void vjp_pairs(int frames, const double *g, const double *cot, int npairs,
+                 const int32_t *type_pairs, double *dsig, double *deps) {
+    check_frames(frames);
+    require_loaded(frames);
+    const int nt = e.ntypes;
+    if (npairs < 1 || npairs > kMaxTablePairs)
+      throw HipError("1 to " + std::to_string(kMaxTablePairs) + " type pairs per call");
+    std::vector<int> sel((size_t)nt * nt, -1);
+    for (int p = 0; p < npairs; ++p) {
+      const int a = type_pairs[2 * p], b = type_pairs[2 * p + 1];
+      if (a < 0 || b < 0 || a >= nt || b >= nt) throw HipError("type pair out of range");
+      if (sel[a * nt + b] >= 0) throw HipError("a type pair named twice");
+      sel[a * nt + b] = p;
+      sel[b * nt + a] = p;
+    }
+    e.to_device(d_sel, sel.data(), sel.size() * sizeof(int));
+    std::vector<double> gz(frames, 0.0);
+    if (g) gz.assign(g, g + frames);
+    if (cot)
+      scatter(d_fin, cot, 3 * (size_t)e.n, frames);
+    else
+      HIPCHK(hipMemset2DAsync(d_fin, e.fs, 0, 3 * (size_t)e.n * sizeof(double), frames,
+                              e.stream));
+    HIPCHK(hipMemcpy2DAsync(reinterpret_cast<char *>(d_meta) + sizeof(double), e.fs, gz.data(),
+                            sizeof(double), sizeof(double), frames, hipMemcpyHostToDevice,
+                            e.stream));
+    e.nz = frames;
+    cot_prepare<<<dim3(1, 1, e.nz), kCotThreads, 0, e.stream>>>(
+        0, e.d_counts, e.d_atom_at, e.d_F, d_fin, d_meta, d_cot, e.max_places, e.fs);
+    launch_adjoint_k<kAdjPairTable>();
+    const size_t np = (size_t)kPartStride * e.max_groups * e.nseg;
+    std::vector<double> vp((size_t)frames * np), meta((size_t)frames * 4);
+    HIPCHK(hipMemcpy2DAsync(vp.data(), np * sizeof(double), d_vpart, e.fs, np * sizeof(double),
+                            frames, hipMemcpyDeviceToHost, e.stream));
+    HIPCHK(hipMemcpy2DAsync(meta.data(), 4 * sizeof(double), d_meta, e.fs, 4 * sizeof(double),
+                            frames, hipMemcpyDeviceToHost, e.stream));
+    HIPCHK(hipStreamSynchronize(e.stream));
+    std::vector<double> ds(npairs, 0.0), de(npairs, 0.0);
+    for (int fr = 0; fr < frames; ++fr) {
+      std::vector<double> w(kPartStride, 0.0);
+      for (size_t k = 0; k < np; ++k) w[k % kPartStride] += vp[fr * np + k];
+      const double c = meta[4 * (size_t)fr];  // the frame's cotangent scale
+      for (int p = 0; p < npairs; ++p) {
+        ds[p] += c * w[2 * p];
+        de[p] += c * w[2 * p + 1];
+      }
+    }
+    double gsum = 0;
+    for (double v : gz) gsum += v;
+    if (e.dispersion && gsum != 0.0) {
+      // E_disp = k sum_ab w_ab C6_ab over ordered type pairs
+      std::vector<double> num(nt, 0.0);
+      for (int t : e.h_type) num[t] += 1.0;
+      const size_t nex = e.h_excl_pairs.size() / 2;
+      const double volume = e.box[0] * e.box[1] * e.box[2];
+      const double nn = e.n, pairs = nn * (nn - 1.0) - 2.0 * (double)nex;
+      const double k = pairs > 0 ? -2.0 * M_PI / (3.0 * volume) * nn * nn /
+                                       (e.cutoff * e.cutoff * e.cutoff) / pairs
+                                 : 0.0;
+      for (int p = 0; p < npairs; ++p) {
+        const int a = type_pairs[2 * p], b = type_pairs[2 * p + 1];
+        auto wab = [&](int x, int y) {
+          double v = num[x] * (num[y] - (x == y ? 1.0 : 0.0));
+          for (size_t q = 0; q < nex; ++q) {
+            const int tx = e.h_type[e.h_excl_pairs[2 * q]], ty = e.h_type[e.h_excl_pairs[2 * q + 1]];
+            if ((tx == x && ty == y) || (tx == y && ty == x)) v -= (x == y) ? 2.0 : 1.0;
+          }
+          return v;
+        };
+        const double wt = a == b ? wab(a, a) : wab(a, b) + wab(b, a);
+        const double ee = e.h_eps[a * nt + b], ss = e.h_sigma[a * nt + b];
+        ds[p] += gsum * k * wt * 24.0 * ee * pow(ss, 5.0);
+        de[p] += gsum * k * wt * 4.0 * pow(ss, 6.0);
+      }
+    }
+    if (dsig) std::copy(ds.begin(), ds.end(), dsig);
+    if (deps) std::copy(de.begin(), de.end(), deps);
   }
 
   void hvp(int frames, const double *v, double *out) {

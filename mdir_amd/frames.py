@@ -141,6 +141,29 @@ class FrameEnergies:
                 dx -= g[:, None, None] * self.forces
         return FrameGradient(values, dx)
 
+    def vjp_pairs(self, pairs, cotangent=None, *, positions=None):
+        """The product of vjp in type-pair table entries (the reference's
+        sigma and epsilon "by pairs of types", NBFIX entries) instead of the
+        per-type parameters: `pairs` names 1 to 4 pairs of types (a, b);
+        returns {"sigma": (len(pairs),), "epsilon": (len(pairs),)}."""
+        ev = self._evaluator
+        if ev._version != self.version:
+            raise RuntimeError("vjp_pairs: the evaluator has evaluated other frames since")
+        K, n = self.count, ev.n
+        tp = np.ascontiguousarray(np.asarray(pairs, np.int32).reshape(-1, 2))
+        g = None if cotangent is None else np.ascontiguousarray(cotangent, np.float64)
+        if g is not None and g.shape != (K,):
+            raise ValueError(f"vjp_pairs: expected a cotangent of shape ({K},)")
+        pp, keep = (None, None) if positions is None else \
+            _frame_array(positions, (K, n, 3), "positions")[:2]
+        ds = np.zeros(len(tp))
+        de = np.zeros(len(tp))
+        rt.check(rt.lib().mdrtHipFramesVjpPairs(
+            ev._h, K, None if g is None else rt.ptr(g, C.c_double), pp, len(tp),
+            rt.ptr(tp, C.c_int32), rt.ptr(ds, C.c_double), rt.ptr(de, C.c_double)))
+        del keep
+        return {"sigma": ds, "epsilon": de}
+
 
 class FrameEvaluator:
     """Frames of one system (shared topology and box) on the GPU, up to

@@ -128,6 +128,7 @@ SIGNATURES = [
     ("mdrtHipFramesGetParameters", C.c_int, [C.c_void_p, PD, PD, PD]),
     ("mdrtHipFramesEvaluate", C.c_int, [C.c_void_p, I32, C.c_void_p, PD, C.c_void_p, PD]),
     ("mdrtHipFramesVjp", C.c_int, [C.c_void_p, I32, PD, C.c_void_p, PD, PD, PD]),
+    ("mdrtHipFramesVjpPairs", C.c_int, [C.c_void_p, I32, PD, C.c_void_p, I32, PI32, PD, PD]),
     ("mdrtHipFramesHvp", C.c_int, [C.c_void_p, I32, C.c_void_p, C.c_void_p]),
     ("mdrtHipFramesForceMatching", C.c_int, [C.c_void_p, I32, C.c_void_p, C.c_void_p, I32,
                                              PD, PD, PD, PD]),

@@ -240,3 +240,34 @@ def test_chains_with_14_pairs(modifier):
     Hv = ev.hvp(V)
     for n in range(2):
         check_hvp(c, X[n], V[n], Hv[n], f"frame {n}")
+
+
+def test_vjp_type_pair_entries():
+    """The product in individual type-pair table entries (sigma_ab = sigma_ba,
+    eps_ab = eps_ba, the reference's tunables "by pairs of types"), with the
+    dispersion correction's share of the energy cotangent, against central
+    differences of the oracle with the two table entries moved together."""
+    from mdir_amd.frames import FrameEvaluator
+    say("case: vjp in type-pair entries (O-O, O-solute, solute pairs), dispersion correction")
+    s = small_solutes(seed=9)
+    s.dispersion = True
+    X = S.perturbed_frames(s, 2, seed=11)
+    rng = np.random.default_rng(12)
+    G = rng.normal(size=X.shape) * 0.5
+    g = rng.normal(size=2)
+    ev = FrameEvaluator(s, max_frames=2)
+    pairs = [(0, 0), (0, 2), (3, 2), (4, 4)]
+    p = ev.evaluate(X).vjp_pairs(pairs, g, positions=G)
+
+    def central_entry(which, a, b, h=H_THETA):
+        def at(delta):
+            t = s.copy()
+            t.sigma, t.epsilon = s.sigma.copy(), s.epsilon.copy()
+            tab = t.sigma if which == "sigma" else t.epsilon
+            tab[a, b] += delta
+            if a != b:
+                tab[b, a] += delta
+            return product_objective(t, X, g, G)
+        return (at(h) - at(-h)) / (2 * h)
+    want = {k: np.array([central_entry(k, a, b) for a, b in pairs]) for k in ("sigma", "epsilon")}
+    check_products(p, want, "type-pair entries")
