@@ -240,6 +240,13 @@ enum {
    reciprocal terms); the energies of the other terms are 0. */
 int mdrtHipComputeForceTerms(void *system, int32_t terms, double *forces, double *energies);
 
+/* The y and x transforms of the PME planes: 0 = the LDS Stockham FFT (the
+   default), 1 = DFT matrix products on the matrix cores in split BF16x3
+   (f32 operands as three bf16 pieces, six MFMA products per real product,
+   f32 accumulation; plane edges of 32 or 64).  Replaces the reference's
+   cuFFT plan choice (mdrt_cuda.c:2646-2991) with this library's own. */
+int mdrtHipSetPmeTransform(void *system, int32_t mode);
+
 /* `steps` velocity-Verlet steps of dt (ps) with SETTLE waters.  graph != 0
    replays the step sequence as a captured HIP graph of `graph` steps.
    kinetic (may be NULL) receives the kinetic energy after the last step. */
@@ -349,6 +356,9 @@ int mdrtHipFramesVjp(void *frames, int32_t count, const double *energy_cotangent
 int mdrtHipFramesVjpPairs(void *frames, int32_t count, const double *energy_cotangent,
                           const double *position_cotangent, int32_t npairs,
                           const int32_t *type_pairs, double *d_sigma, double *d_epsilon);
+
+/* The PME plane transform of the frame evaluator (as mdrtHipSetPmeTransform). */
+int mdrtHipFramesSetPmeTransform(void *frames, int32_t mode);
 
 /* out[n] = H_n v[n], H_n the Hessian of U_n in the positions, for the
    loaded frames (the reference's frame_hvp, torch.py:341; with the

@@ -24,6 +24,7 @@
 #include "nbnxm.hip"
 #include "pairs.hip"
 #include "pme.hip"
+#include "pme_mfma.hip"
 
 namespace mdhip {
 
@@ -226,6 +227,11 @@ struct Engine {
   ConvTables ct{};
   double *d_erecip = nullptr;
   double *d_vrecip = nullptr;  // reciprocal virial per plane block, when asked (frames)
+  // The y and x transforms of the planes: 0 = LDS Stockham FFT
+  // (fft_plane_conv), 1 = DFT matrix products on MFMA in split BF16x3
+  // (fft_plane_conv_mfma; plane edges 32 or 64).
+  int pme_transform = 0;
+  __bf16 *d_dft1 = nullptr, *d_dft2 = nullptr;
 
   // ---- integration ----
   int *d_unit_first = nullptr, *d_unit_size = nullptr;
@@ -365,6 +371,7 @@ struct Engine {
     dfree(d_p14_eps); dfree(d_p14_qq); dfree(d_espec);
     fft.release();
     dfree(d_cplx); dfree(d_tab); dfree(d_erecip);
+    dfree(d_dft1); dfree(d_dft2);
     dfree(bp.count); dfree(bp.list);
     dfree(d_unit_first); dfree(d_unit_size);
     if (ev_fork) hipEventDestroy(ev_fork);
@@ -974,13 +981,94 @@ struct Engine {
   }
   static constexpr int kPlaneMax = 4096;
   int recip_blocks() const { return plane_fft() ? fft.H() : fft.x_blocks(); }
+  static bool mfma_edge(int k) { return k == 32 || k == 64; }
+  bool mfma_dft() const {
+    return pme_transform == 1 && plane_fft() && mfma_edge(pme_grid[0]) && mfma_edge(pme_grid[1]);
+  }
+
+  // The DFT matrix W_K[n][k] = exp(-2 pi i n k / K) in three bf16 pieces
+  // per part (h + m + l = W to 2^-24), laid out as fft_plane_conv_mfma reads
+  // it: ((piece * 2 + part) * K + n) * K + k.
+  static uint16_t bf16_bits(float v) {  // round to nearest even
+    uint32_t u;
+    memcpy(&u, &v, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+  }
+  static float bf16_value(uint16_t b) {
+    const uint32_t u = (uint32_t)b << 16;
+    float v;
+    memcpy(&v, &u, 4);
+    return v;
+  }
+  static __bf16 *dft_table(int K) {
+    std::vector<uint16_t> t((size_t)6 * K * K);
+    for (int n = 0; n < K; ++n)
+      for (int k = 0; k < K; ++k) {
+        const long nk = ((long)n * k) % K;
+        const double ang = -2.0 * M_PI * (double)nk / (double)K;
+        const float part[2] = {(float)cos(ang), (float)sin(ang)};
+        for (int c = 0; c < 2; ++c) {
+          float r = part[c];
+          for (int p = 0; p < 3; ++p) {
+            const uint16_t b = bf16_bits(r);
+            t[((size_t)(p * 2 + c) * K + n) * K + k] = b;
+            r -= bf16_value(b);
+          }
+        }
+      }
+    __bf16 *d = dalloc<__bf16>(t.size());
+    HIPCHK(hipMemcpy(d, t.data(), t.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    return d;
+  }
+
+  void set_pme_transform(int mode) {
+    if (mode != 0 && mode != 1) throw HipError("PME transform: 0 (FFT) or 1 (MFMA DFT)");
+    if (mode == 1 && !(use_pme && plane_fft() && mfma_edge(pme_grid[0]) && mfma_edge(pme_grid[1])))
+      throw HipError("the MFMA DFT takes PME grids with x and y edges of 32 or 64");
+    if (mode == 1 && !d_dft1) {
+      d_dft1 = dft_table(pme_grid[0]);
+      d_dft2 = pme_grid[1] == pme_grid[0] ? nullptr : dft_table(pme_grid[1]);
+    }
+    pme_transform = mode;
+    invalidate_graph();
+  }
+
+  template <int K1, int K2>
+  void plane_conv_mfma_t(cpx<float> *data, double *ep, double *vp) {
+    const size_t lds = plane_mfma_lds(K1, K2);
+    static bool attr = false;
+    if (!attr) {
+      HIPCHK(hipFuncSetAttribute((const void *)fft_plane_conv_mfma<K1, K2>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      attr = true;
+    }
+    const DftTables dt{d_dft1, d_dft2 ? d_dft2 : d_dft1};
+    fft_plane_conv_mfma<K1, K2><<<dim3(fft.H(), 1, nz), 512, lds, pstream>>>(data, dt, ct, ep,
+                                                                              vp, fs);
+  }
+
+  // y and x transforms with the influence product of every kz plane of
+  // `data` (energy and virial partials when not null), by the chosen
+  // transform.
+  void plane_conv(cpx<float> *data, double *ep, double *vp) {
+    if (mfma_dft()) {
+      const int a = pme_grid[0], b = pme_grid[1];
+      if (a == 64 && b == 64) plane_conv_mfma_t<64, 64>(data, ep, vp);
+      else if (a == 32 && b == 32) plane_conv_mfma_t<32, 32>(data, ep, vp);
+      else if (a == 64) plane_conv_mfma_t<64, 32>(data, ep, vp);
+      else plane_conv_mfma_t<32, 64>(data, ep, vp);
+      return;
+    }
+    const size_t lds = (size_t)2 * pme_grid[0] * pme_grid[1] * sizeof(cpx<float>);
+    fft_plane_conv<<<dim3(fft.H(), 1, nz), 512, lds, pstream>>>(
+        data, fft.K[0], fft.K[1], fft.ax[0].axis, fft.ax[1].axis, ct, ep, vp, fs);
+  }
+
   void pme_yx(bool energy) {
     double *ep = energy ? d_erecip : nullptr;
     if (plane_fft()) {
-      const size_t lds = (size_t)2 * pme_grid[0] * pme_grid[1] * sizeof(cpx<float>);
-      fft_plane_conv<<<dim3(fft.H(), 1, nz), 512, lds, pstream>>>(
-          d_cplx, fft.K[0], fft.K[1], fft.ax[0].axis, fft.ax[1].axis, ct, ep,
-          energy ? d_vrecip : nullptr, fs);
+      plane_conv(d_cplx, ep, energy ? d_vrecip : nullptr);
       return;
     }
     const int TL = Fft3D<float>::TL;
@@ -1637,6 +1725,11 @@ int mdrtHipComputeForceTerms(void *system, int32_t terms, double *forces, double
   return guarded([&] { e->compute_terms(terms, forces, energies); });
 }
 
+int mdrtHipSetPmeTransform(void *system, int32_t mode) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] { e->set_pme_transform(mode); });
+}
+
 int mdrtHipRun(void *system, int64_t steps, double dt, int32_t graph, double *kinetic) {
   Engine *e = static_cast<Engine *>(system);
   return guarded([&] { e->run(steps, dt, graph, kinetic); });
@@ -1712,6 +1805,15 @@ int mdrtHipFramesVjpPairs(void *frames, int32_t count, const double *energy_cota
   return guarded([&] {
     f->vjp_pairs(count, energy_cotangent, position_cotangent, npairs, type_pairs, d_sigma,
                  d_epsilon);
+  });
+}
+
+int mdrtHipFramesSetPmeTransform(void *frames, int32_t mode) {
+  FrameBatch *f = static_cast<FrameBatch *>(frames);
+  return guarded([&] {
+    f->e.set_pme_transform(mode);
+    if (f->fm_graph) hipGraphExecDestroy(f->fm_graph);
+    f->fm_graph = nullptr;
   });
 }
 

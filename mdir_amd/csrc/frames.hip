@@ -424,10 +424,7 @@ struct FrameBatch {
     const size_t lds = (size_t)(32 * TL + 8) * e.pme_grid[2];
     pme_spread_dipole<N, TL><<<dim3(blocks, 1, e.nz), 256, lds, e.pstream>>>(
         e.d_xpq, d_cot, e.pmp, e.bp, d_cplx2, e.fft.ax[2].axis, e.fs);
-    const size_t plds = (size_t)2 * e.pme_grid[0] * e.pme_grid[1] * sizeof(cpx<float>);
-    fft_plane_conv<<<dim3(e.fft.H(), 1, e.nz), 512, plds, e.pstream>>>(
-        d_cplx2, e.fft.K[0], e.fft.K[1], e.fft.ax[0].axis, e.fft.ax[1].axis, e.ct, nullptr,
-        nullptr, e.fs);
+    e.plane_conv(d_cplx2, nullptr, nullptr);
     pme_gather_adj<N, TL, KIND><<<dim3(blocks, 1, e.nz), 256, lds, e.pstream>>>(
         e.d_xpq, d_cot, d_meta, e.pmp, e.bp, e.d_cplx, d_cplx2, e.fft.ax[2].axis, d_A, e.fs);
   }

@@ -116,6 +116,8 @@ SIGNATURES = [
     ("mdrtHipGetVelocities", C.c_int, [C.c_void_p, PD]),
     ("mdrtHipComputeForces", C.c_int, [C.c_void_p, PD, PD]),
     ("mdrtHipComputeForceTerms", C.c_int, [C.c_void_p, I32, PD, PD]),
+    ("mdrtHipSetPmeTransform", C.c_int, [C.c_void_p, I32]),
+    ("mdrtHipFramesSetPmeTransform", C.c_int, [C.c_void_p, I32]),
     ("mdrtHipRun", C.c_int, [C.c_void_p, I64, D, I32, PD]),
     ("mdrtHipRunMeasured", C.c_int, [C.c_void_p, I64, D, I32, PD]),
     ("mdrtHipMinimize", C.c_int, [C.c_void_p, I64, D, PD]),

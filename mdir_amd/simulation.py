@@ -81,6 +81,11 @@ class Simulation:
                                                    rt.ptr(e, C.c_double)))
         return F, e
 
+    def set_pme_transform(self, mode: int):
+        """0: the LDS Stockham FFT; 1: DFT matrix products on MFMA in split
+        BF16x3 (PME plane edges of 32 or 64)."""
+        rt.check(rt.lib().mdrtHipSetPmeTransform(self._h, int(mode)))
+
     def energies(self) -> dict:
         _, e = self.forces()
         return dict(zip(rt.ENERGY_NAMES, e))

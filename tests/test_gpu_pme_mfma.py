@@ -1,0 +1,67 @@
+"""The MFMA plane transform (DFT matrix products in split BF16x3 on the
+matrix cores) against the FP32 Stockham FFT of the same library and the
+FP64 oracle.
+
+Bars: the reciprocal forces of the two transforms within 3e-5 of the
+largest (the reference's mixed-precision bar for its axis transforms,
+test/Integration/pme-axis-fft.py:45-75); total forces within the force
+path's 5e-6 relative RMS of FP64; reciprocal energy within 2e-6."""
+import numpy as np
+import pytest
+
+from mdir_amd import runtime as rt
+from mdir_amd import systems as S
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("case", [("tip3p 512, 32^3", 512, 2.5, 32),
+                                  ("tip3p 2000, 64^3", 2000, 3.9, 64)])
+def test_mfma_transform_forces(case):
+    from mdir_amd.simulation import Simulation
+    name, nw, edge, K = case
+    say(f"case: MFMA plane transform, {name}")
+    w = S.tip3p_box(nw, edge, seed=31, grid=(K, K, K))
+    sim = Simulation(w)
+    Ff, ef = sim.force_terms(rt.TERM_RECIP)
+    sim.set_pme_transform(1)
+    Fm, em = sim.force_terms(rt.TERM_RECIP)
+    d = np.abs(Fm - Ff).max() / np.abs(Ff).max()
+    say(f"  reciprocal forces, MFMA vs FFT: max difference / largest {d:.2e}; "
+        f"energy {em[O.E_COUL_RECIP]:.6f} vs {ef[O.E_COUL_RECIP]:.6f}")
+    assert d < 3e-5
+    F, e = sim.forces()
+    Fo, eo = O.Oracle(w).total()
+    err = O.rel_rms(F, Fo)
+    Fr, er = O.Oracle(w).pme()
+    errr = O.rel_rms(Fm, Fr)
+    say(f"  total rel rms force error {err:.2e}; reciprocal term {errr:.2e}")
+    assert err < 5e-6
+    assert abs(e[O.E_COUL_RECIP] - er) <= 2e-6 * abs(er)
+
+
+def test_mfma_transform_frames():
+    """The frame evaluator's forward and adjoint with the MFMA transform:
+    the same forces and charge products as with the FFT, to the
+    transforms' difference."""
+    from mdir_amd.frames import FrameEvaluator
+    say("case: frames with the MFMA transform, 960 atoms, 32^3")
+    s = S.solvated_solutes(nwaters=300, nsolutes=12, seed=6, grid=(32, 32, 32))
+    X = S.perturbed_frames(s, 3, seed=2)
+    G = np.random.default_rng(5).normal(size=X.shape)
+    ev = FrameEvaluator(s, max_frames=3)
+    a = ev.evaluate(X)
+    pa = a.vjp(positions=G)
+    rt.check(rt.lib().mdrtHipFramesSetPmeTransform(ev._h, 1))
+    b = ev.evaluate(X)
+    pb = b.vjp(positions=G)
+    for n in range(3):
+        assert O.rel_rms(b.forces[n], O.Oracle(s).total(X[n])[0]) < 5e-6
+    d = np.abs(pb["charges"] - pa["charges"]).max() / np.abs(pa["charges"]).max()
+    say(f"  charge products, MFMA vs FFT: {d:.2e}")
+    assert d < 3e-5
