@@ -187,6 +187,14 @@ struct Engine {
   float4 *d_xq = nullptr;
   double4 *d_xpq = nullptr;   // f64 positions and charges in place order (PME)
   long long *d_F = nullptr;
+  // PME in atom order (pme_atoms): the reciprocal part reads positions and
+  // charges the integrator writes by atom and adds its force words to d_Fr
+  // by atom, so it depends on no list build and runs beside it.  Frame
+  // batches keep PME in place order.
+  bool pme_atoms = true;
+  double4 *d_xpa = nullptr;   // f64 positions and charges in atom order
+  long long *d_Fr = nullptr;  // reciprocal force words in atom order
+  int *d_pme_n = nullptr;     // [0] = n, the element count of pme_bin
   int *d_col_count = nullptr, *d_bucket = nullptr;
   float4 *d_bb_lo = nullptr, *d_bb_hi = nullptr;
   int *d_cluster_col = nullptr;
@@ -364,7 +372,7 @@ struct Engine {
     timing_slots = 0;
     dfree(d_x); dfree(d_v); dfree(d_xref); dfree(d_shift); dfree(d_q);
     dfree(d_inv_mass); dfree(d_type); dfree(d_place_of); dfree(d_save_x); dfree(d_save_v);
-    dfree(d_xprune);
+    dfree(d_xprune); dfree(d_xpa); dfree(d_Fr); dfree(d_pme_n);
     release_list();
     dfree(d_excl_start); dfree(d_excl_atoms); dfree(d_flags); dfree(d_lj);
     dfree(d_special); dfree(d_is14); dfree(d_isexcl); dfree(d_p14_sigma);
@@ -638,6 +646,14 @@ struct Engine {
 
     // PME
     use_pme = beta > 0;
+    if (use_pme && pme_atoms) {
+      d_xpa = dalloc<double4>(n);
+      d_Fr = dalloc<long long>(3 * (size_t)n);
+      HIPCHK(hipMemset(d_Fr, 0, 3 * (size_t)n * sizeof(long long)));
+      const int cnt[4] = {n, 0, 0, 0};
+      d_pme_n = dalloc<int>(4);
+      HIPCHK(hipMemcpy(d_pme_n, cnt, sizeof(cnt), hipMemcpyHostToDevice));
+    }
     if (use_pme) {
       for (int k = 0; k < 3; ++k) {
         if (pme_grid[k] < 2 * pme_order) throw HipError("PME grid smaller than twice the order");
@@ -901,11 +917,13 @@ struct Engine {
     bp.plane_major = plane_fft() ? 1 : 0;
   }
 
+  const double4 *pme_positions() const { return pme_atoms ? d_xpa : d_xpq; }
+
   // Lists the atoms of every brick their stencils touch.
   template <int N, int TL>
   void bin_t() {
-    pme_bin<N, TL><<<dim3(blocks_for(max_places), 1, nz), 256, 0, pstream>>>(
-        d_counts, d_xpq, pmp, bp, d_flags, fs);
+    pme_bin<N, TL><<<dim3(blocks_for(pme_atoms ? n : max_places), 1, nz), 256, 0, pstream>>>(
+        pme_atoms ? d_pme_n : d_counts, pme_positions(), pmp, bp, d_flags, fs);
   }
   template <int N>
   void bin_n() {
@@ -929,7 +947,7 @@ struct Engine {
     const int blocks = ((pme_grid[0] + BX - 1) / BX) * ((pme_grid[1] + BY - 1) / BY);
     const size_t lds = (size_t)(32 * TL + 8) * pme_grid[2];
     pme_spread_rows<N, TL><<<dim3(blocks, 1, nz), 256, lds, pstream>>>(
-        d_xpq, pmp, bp, d_cplx, fft.ax[2].axis, fs);
+        pme_positions(), pmp, bp, d_cplx, fft.ax[2].axis, fs);
   }
   template <int N>
   void spread_rows_n() {
@@ -946,7 +964,7 @@ struct Engine {
     const int blocks = ((pme_grid[0] + BX - 1) / BX) * ((pme_grid[1] + BY - 1) / BY);
     const size_t lds = (size_t)(16 * TL + 8) * pme_grid[2];
     pme_gather_rows<N, TL><<<dim3(blocks, 1, nz), 256, lds, pstream>>>(
-        d_xpq, pmp, bp, d_cplx, fft.ax[2].axis, d_F, fs);
+        pme_positions(), pmp, bp, d_cplx, fft.ax[2].axis, pme_atoms ? d_Fr : d_F, fs);
   }
   template <int N>
   void gather_rows_n() {
@@ -1089,15 +1107,21 @@ struct Engine {
   }
 
   // Pair loop on the engine's stream, PME on the side stream, joined.
-  // With timed >= 0, the pair kernel records its span in slot `timed` of
-  // the timing buffer.
-  void forces(bool energy, int timed = -1) {
-    if (use_pme && (terms & 4)) {
+  // With with_build, the list build (a no-op unless a rebuild was asked
+  // for) comes first on the engine's stream: before the fork when PME reads
+  // the place order the build writes, after it when PME runs in atom order
+  // (pme_atoms), beside the build.  With timed >= 0, the pair kernel
+  // records its span in slot `timed` of the timing buffer.
+  void forces(bool energy, int timed = -1, bool with_build = false) {
+    const bool recip = use_pme && (terms & 4);
+    if (with_build && !(recip && pme_atoms)) build();
+    if (recip) {
       HIPCHK(hipEventRecord(ev_fork, stream));
       HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
       pme(energy);
       HIPCHK(hipEventRecord(ev_join, side));
     }
+    if (with_build && recip && pme_atoms) build();
     if (timed >= 0) stamp_clock<<<1, 64, 0, stream>>>(d_timing + 3 * timed);
     pairs(energy);
     if (timed >= 0) {
@@ -1109,14 +1133,27 @@ struct Engine {
 
   void clear_forces() {
     clear_words<<<dim3(256, 1, nz), 256, 0, stream>>>(d_F, 3 * (size_t)max_places, fs);
+    if (d_Fr) clear_words<<<dim3(64, 1, nz), 256, 0, stream>>>(d_Fr, 3 * (size_t)n, 0);
+  }
+
+  // Outside the step sequence (evaluation, minimisation): PME in atom order
+  // reads the positions the integrator would have written, and its words
+  // are folded into d_F afterwards, so that every reader of d_F sees the
+  // whole force.
+  void pack_pme_atoms() {
+    if (d_xpa) pme_pack_atoms<<<blocks_for(n), 256, 0, stream>>>(n, d_x, d_q, d_xpa);
+  }
+  void fold_pme_atoms() {
+    if (d_Fr)
+      pme_fold_atoms<<<blocks_for(n), 256, 0, stream>>>(n, d_place_of, d_Fr, d_F);
   }
 
   template <int PART>
   void integrate(double dt) {
     ip.dt = dt;
     integrate_part<PART><<<blocks_for(nunits, 64), 64, 0, stream>>>(
-        nunits, d_unit_first, d_unit_size, d_x, d_v, d_inv_mass, d_F, d_place_of, d_shift,
-        d_xref, d_q, d_xq, d_xpq, ip, d_flags);
+        nunits, d_unit_first, d_unit_size, d_x, d_v, d_inv_mass, d_F, d_Fr, d_place_of,
+        d_shift, d_xref, d_q, d_xq, d_xpq, d_xpa, ip, d_flags);
   }
   void step_first(double dt) { integrate<kFirstHalf>(dt); }
   void step_second(double dt) { integrate<kSecondHalf>(dt); }
@@ -1128,8 +1165,7 @@ struct Engine {
     if (count <= 0) return;
     step_first(dt);
     for (int64_t s = 0; s < count; ++s) {
-      build();
-      forces(false, measure ? (int)s : -1);
+      forces(false, measure ? (int)s : -1, true);
       if (s + 1 < count) step_between(dt);
       else step_second(dt);
     }
@@ -1216,8 +1252,9 @@ struct Engine {
     for (int attempt = 0; attempt < 8; ++attempt) {
       clear_forces();
       request_build();
-      build();
-      forces(energy);
+      pack_pme_atoms();
+      forces(energy, -1, true);
+      fold_pme_atoms();
       const std::vector<int> f = read_flags();
       if (!grow_if_overflowed(f)) {
         forces_valid = true;

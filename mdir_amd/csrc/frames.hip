@@ -88,6 +88,7 @@ struct FrameBatch {
     dd.skin = 0.02;       // the list is built at every evaluation
     dd.prune_skin = 0.0;
     dd.nwaters = 0;       // no integration
+    e.pme_atoms = false;  // the adjoint reads PME inputs in place order
     e.create(&dd);
     max_frames = frames;
     // per-type parameters: the diagonal of the tables, which must follow

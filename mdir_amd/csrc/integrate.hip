@@ -150,12 +150,15 @@ struct UnitState {
 };
 
 // Loads the unit at `first`; with clear, the force words are zeroed for the
-// next evaluation once read.
+// next evaluation once read.  Fr (or null): reciprocal force words by atom,
+// added to the place-ordered words as integers before the conversion, so
+// the force is the one a single word would hold.
 template <int S>
 __device__ __forceinline__ void load_unit(int first, const double *__restrict__ x,
                                           const double *__restrict__ v,
                                           const double *__restrict__ inv_mass,
                                           long long *__restrict__ F,
+                                          long long *__restrict__ Fr,
                                           const int *__restrict__ place_of, bool clear,
                                           UnitState<S> &s) {
   s.first = first;
@@ -166,7 +169,12 @@ __device__ __forceinline__ void load_unit(int first, const double *__restrict__ 
     s.im[m] = inv_mass[at];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      s.f[m][k] = (double)F[3 * s.pl[m] + k] * kForceUnscale;
+      long long w = F[3 * s.pl[m] + k];
+      if (Fr) {
+        w += Fr[3 * at + k];
+        if (clear) Fr[3 * at + k] = 0;
+      }
+      s.f[m][k] = (double)w * kForceUnscale;
       if (clear) F[3 * s.pl[m] + k] = 0;
       s.v[m][k] = v[3 * at + k];
       s.x[m][k] = x[3 * at + k];
@@ -198,7 +206,8 @@ __device__ __forceinline__ int drift_store(UnitState<S> &s, double *__restrict__
                                            const double *__restrict__ xref,
                                            const double *__restrict__ q,
                                            float4 *__restrict__ xq,
-                                           double4 *__restrict__ xpq, const IntParams &p) {
+                                           double4 *__restrict__ xpq,
+                                           double4 *__restrict__ xpa, const IntParams &p) {
   double xn[S][3];
 #pragma unroll
   for (int m = 0; m < S; ++m)
@@ -240,6 +249,7 @@ __device__ __forceinline__ int drift_store(UnitState<S> &s, double *__restrict__
                               (float)(xn[m][2] - shift[3 * at + 2] - p.half_box[2]),
                               (float)q[at]);
     xpq[s.pl[m]] = make_double4(xn[m][0], xn[m][1], xn[m][2], q[at]);
+    if (xpa) xpa[at] = make_double4(xn[m][0], xn[m][1], xn[m][2], q[at]);
   }
   return moved;
 }
@@ -252,23 +262,25 @@ __device__ __forceinline__ void step_unit(int first, double *__restrict__ x,
                                           double *__restrict__ v,
                                           const double *__restrict__ inv_mass,
                                           long long *__restrict__ F,
+                                          long long *__restrict__ Fr,
                                           const int *__restrict__ place_of,
                                           const double *__restrict__ shift,
                                           const double *__restrict__ xref,
                                           const double *__restrict__ q,
                                           float4 *__restrict__ xq,
-                                          double4 *__restrict__ xpq, const IntParams &p,
+                                          double4 *__restrict__ xpq,
+                                          double4 *__restrict__ xpa, const IntParams &p,
                                           int *__restrict__ flags) {
   UnitState<S> s;
   // the force words are cleared by the launch that reads them last
-  load_unit<S>(first, x, v, inv_mass, F, place_of, PART != kSecondHalf, s);
+  load_unit<S>(first, x, v, inv_mass, F, Fr, place_of, PART != kSecondHalf, s);
   if (PART & kSecondHalf) {  // end of a step: half kick, velocity constraint
     half_kick<S>(s, p.dt);
     constrain_velocities<S>(s, p);
   }
   if (PART & kFirstHalf) {   // start of a step: half kick, drift, SETTLE
     half_kick<S>(s, p.dt);
-    const int moved = drift_store<S>(s, x, v, shift, xref, q, xq, xpq, p);
+    const int moved = drift_store<S>(s, x, v, shift, xref, q, xq, xpq, xpa, p);
     if (moved & 1) atomicOr(&flags[kFlagRebuild], 1);
     if (moved & 2) atomicOr(&flags[kFlagPrune], 1);
   } else {
@@ -289,17 +301,40 @@ template <int PART>
 __global__ void __launch_bounds__(64) integrate_part(
     int nunits, const int *__restrict__ unit_first, const int *__restrict__ unit_size,
     double *__restrict__ x, double *__restrict__ v, const double *__restrict__ inv_mass,
-    long long *__restrict__ F, const int *__restrict__ place_of,
+    long long *__restrict__ F, long long *__restrict__ Fr, const int *__restrict__ place_of,
     const double *__restrict__ shift, const double *__restrict__ xref,
     const double *__restrict__ q, float4 *__restrict__ xq, double4 *__restrict__ xpq,
-    IntParams p, int *__restrict__ flags) {
+    double4 *__restrict__ xpa, IntParams p, int *__restrict__ flags) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= nunits) return;
   const int first = unit_first[u];
   if (unit_size[u] == 3)
-    step_unit<PART, 3>(first, x, v, inv_mass, F, place_of, shift, xref, q, xq, xpq, p, flags);
+    step_unit<PART, 3>(first, x, v, inv_mass, F, Fr, place_of, shift, xref, q, xq, xpq, xpa,
+                       p, flags);
   else
-    step_unit<PART, 1>(first, x, v, inv_mass, F, place_of, shift, xref, q, xq, xpq, p, flags);
+    step_unit<PART, 1>(first, x, v, inv_mass, F, Fr, place_of, shift, xref, q, xq, xpq, xpa,
+                       p, flags);
+}
+
+// Positions and charges by atom for PME in atom order (evaluations outside
+// the step sequence, where no integration writes them).
+__global__ void pme_pack_atoms(int n, const double *__restrict__ x, const double *__restrict__ q,
+                               double4 *__restrict__ xpa) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) xpa[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], q[i]);
+}
+
+// Reciprocal force words by atom added into the place-ordered words (and
+// cleared): integer sums, so the result is the word a single buffer holds.
+__global__ void pme_fold_atoms(int n, const int *__restrict__ place_of,
+                               long long *__restrict__ Fr, long long *__restrict__ F) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int pl = place_of[i];
+  for (int k = 0; k < 3; ++k) {
+    F[3 * pl + k] += Fr[3 * i + k];
+    Fr[3 * i + k] = 0;
+  }
 }
 
 // Force words (place order) -> f64 forces (atom order).
