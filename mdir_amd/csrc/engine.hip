@@ -1153,7 +1153,7 @@ struct Engine {
     ip.dt = dt;
     integrate_part<PART><<<blocks_for(nunits, 64), 64, 0, stream>>>(
         nunits, d_unit_first, d_unit_size, d_x, d_v, d_inv_mass, d_F, d_Fr, d_place_of,
-        d_shift, d_xref, d_q, d_xq, d_xpq, d_xpa, ip, d_flags);
+        d_shift, d_xref, d_q, d_xq, d_xpa ? nullptr : d_xpq, d_xpa, ip, d_flags);
   }
   void step_first(double dt) { integrate<kFirstHalf>(dt); }
   void step_second(double dt) { integrate<kSecondHalf>(dt); }

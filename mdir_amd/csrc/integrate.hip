@@ -248,7 +248,7 @@ __device__ __forceinline__ int drift_store(UnitState<S> &s, double *__restrict__
                               (float)(xn[m][1] - shift[3 * at + 1] - p.half_box[1]),
                               (float)(xn[m][2] - shift[3 * at + 2] - p.half_box[2]),
                               (float)q[at]);
-    xpq[s.pl[m]] = make_double4(xn[m][0], xn[m][1], xn[m][2], q[at]);
+    if (xpq) xpq[s.pl[m]] = make_double4(xn[m][0], xn[m][1], xn[m][2], q[at]);
     if (xpa) xpa[at] = make_double4(xn[m][0], xn[m][1], xn[m][2], q[at]);
   }
   return moved;
