@@ -172,6 +172,7 @@ struct Engine {
   // fork it after the list build and join it before the integration.
   hipStream_t side = nullptr;
   hipStream_t pstream = nullptr;   // where the PME launches go
+  hipStream_t lstream = nullptr;   // where the list build and the pair loop go
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
   // ---- state (atom order) ----
@@ -512,6 +513,7 @@ struct Engine {
     HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     pstream = side;
+    lstream = stream;
 
     h_q.assign(d->charges, d->charges + n);
     h_mass.assign(d->masses, d->masses + n);
@@ -811,17 +813,17 @@ struct Engine {
   void request_build() { nb_request_build<<<dim3(1, 1, nz), 64, 0, stream>>>(d_flags, fs); }
 
   void launch_bin() {
-    nb_bin<<<dim3(blocks_for(n), 1, nz), 256, 0, stream>>>(n, d_x, d_shift, d_xref,
+    nb_bin<<<dim3(blocks_for(n), 1, nz), 256, 0, lstream>>>(n, d_x, d_shift, d_xref,
                                                           d_col_count, d_bucket, d_flags, g, fs);
   }
   void launch_sort() {
-    nb_sort<<<dim3(g.ncol, 1, nz), 256, 0, stream>>>(
+    nb_sort<<<dim3(g.ncol, 1, nz), 256, 0, lstream>>>(
         d_x, d_shift, d_q, d_type, d_col_count, d_bucket, d_atom_at, d_place_of, d_xq, d_xpq,
         d_type_at, d_bb_lo, d_bb_hi, d_cluster_col, d_real_mask, d_col_cl_start,
         d_col_cl_count, d_counts, d_flags, g, fs);
   }
   void launch_search() {
-    nb_search<<<dim3(blocks_for(max_groups, 4), 1, nz), 256, 0, stream>>>(
+    nb_search<<<dim3(blocks_for(max_groups, 4), 1, nz), 256, 0, lstream>>>(
         d_bb_lo, d_bb_hi, d_cluster_col, d_real_mask, d_col_cl_start, d_col_cl_count,
         d_atom_at, d_place_of, d_excl_start, d_excl_atoms, d_counts, d_col_count, d_rows,
         d_row_count, d_ex, ex_cap, d_flags, g, fs);
@@ -829,7 +831,7 @@ struct Engine {
   void launch_prune() {
     if (prune_skin <= 0) return;
     const double reach = cutoff + prune_skin + 1e-4;
-    nb_prune<<<blocks_for(max_groups, 4), 256, 0, stream>>>(
+    nb_prune<<<blocks_for(max_groups, 4), 256, 0, lstream>>>(
         d_xq, d_rows, d_row_count, d_rows_in, d_row_count_in, d_counts, d_atom_at, d_x,
         d_xprune, g.row_cap, (float)(reach * reach), (float)box[0], (float)box[1],
         (float)box[2], d_flags);
@@ -860,7 +862,7 @@ struct Engine {
     const dim3 grid(std::max(blocks_for(max_groups, 4), blocks_for(nspecial)),
                     (terms & 1) ? nseg + 1 : 1, nz);
     const bool pruned = prune_skin > 0;
-    nb_pairs<MOD, COUL, ENERGY><<<grid, 256, ntypes * ntypes * sizeof(float2), stream>>>(
+    nb_pairs<MOD, COUL, ENERGY><<<grid, 256, ntypes * ntypes * sizeof(float2), lstream>>>(
         d_xq, d_type_at, d_real_mask, pruned ? d_rows_in : d_rows,
         pruned ? d_row_count_in : d_row_count, d_ex, g.row_cap, max_groups, d_counts, d_lj,
         nbp, d_F, d_epair, d_flags, sa, fs);
@@ -1114,21 +1116,37 @@ struct Engine {
   // records its span in slot `timed` of the timing buffer.
   void forces(bool energy, int timed = -1, bool with_build = false) {
     const bool recip = use_pme && (terms & 4);
-    if (with_build && !(recip && pme_atoms)) build();
-    if (recip) {
+    const bool split = recip && pme_atoms;
+    if (with_build && !split) build();
+    if (split) {
+      // the list branch on the side stream, PME on the engine's stream
+      // (launched first: the graph keeps the branch launched first on the
+      // queue of the integration, which starts it sooner)
+      HIPCHK(hipEventRecord(ev_fork, stream));
+      HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
+      const hipStream_t ps = pstream;
+      pstream = stream;
+      pme(energy);
+      pstream = ps;
+      lstream = side;
+      if (with_build) build();
+    } else if (recip) {
       HIPCHK(hipEventRecord(ev_fork, stream));
       HIPCHK(hipStreamWaitEvent(side, ev_fork, 0));
       pme(energy);
       HIPCHK(hipEventRecord(ev_join, side));
     }
-    if (with_build && recip && pme_atoms) build();
-    if (timed >= 0) stamp_clock<<<1, 64, 0, stream>>>(d_timing + 3 * timed);
+    if (timed >= 0) stamp_clock<<<1, 64, 0, lstream>>>(d_timing + 3 * timed);
     pairs(energy);
     if (timed >= 0) {
-      stamp_clock<<<1, 64, 0, stream>>>(d_timing + 3 * timed + 1);
-      stamp_clock<<<1, 64, 0, stream>>>(d_timing + 3 * timed + 2);
+      stamp_clock<<<1, 64, 0, lstream>>>(d_timing + 3 * timed + 1);
+      stamp_clock<<<1, 64, 0, lstream>>>(d_timing + 3 * timed + 2);
     }
-    if (use_pme && (terms & 4)) HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
+    if (split) {
+      HIPCHK(hipEventRecord(ev_join, side));
+      lstream = stream;
+    }
+    if (recip) HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
   }
 
   void clear_forces() {
