@@ -823,7 +823,7 @@ struct Engine {
         d_col_cl_count, d_counts, d_flags, g, fs);
   }
   void launch_search() {
-    nb_search<<<dim3(blocks_for(max_groups, 4), 1, nz), 256, 0, lstream>>>(
+    nb_search<<<dim3(max_groups, 1, nz), 256, 0, lstream>>>(
         d_bb_lo, d_bb_hi, d_cluster_col, d_real_mask, d_col_cl_start, d_col_cl_count,
         d_atom_at, d_place_of, d_excl_start, d_excl_atoms, d_counts, d_col_count, d_rows,
         d_row_count, d_ex, ex_cap, d_flags, g, fs);

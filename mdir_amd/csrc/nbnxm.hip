@@ -237,19 +237,21 @@ __device__ __forceinline__ float axis_gap(float lo_i, float hi_i, float lo_j,
   return g > 0.0f ? g : 0.0f;
 }
 
-// Wave per group of kGroup consecutive i-clusters (one column's).  The
+// Block per group of kGroup consecutive i-clusters (one column's).  The
 // candidates are the clusters of the columns within the reach of the group's
 // box, at z shifts -1, 0, +1, whose z range meets the group's z range
 // widened by the reach (a contiguous range of each column's z-sorted
-// clusters, found by binary search).  Lanes test 64 column slots at a time,
-// a wave prefix sum lays out their candidates, and lanes then test 64
-// candidates at a time, every load of an iteration independent of the
-// others.  Bit m of an
-// entry's imask is set when j is within the reach of member m and the tile
-// (member, j, shift) is ordered, so that each unordered cluster pair occurs
-// once.  Tiles of a cluster with itself, and tiles holding excluded pairs,
-// get four explicit 64-bit masks (MDExecKernels.cpp:1748-1823 removes the
-// same partners from a matrix row).
+// clusters, found by binary search).  Wave 1 tests 64 column slots at a
+// time and lays out their candidates with a wave prefix sum, while wave 0
+// builds the group's exclusion table; the four waves then test 64
+// candidates each per round (wave w takes the round's iteration w), and a
+// block prefix over the waves places their entries, so the row holds the
+// candidates in their order, as one wave walking them would write it.  Bit
+// m of an entry's imask is set when j is within the reach of member m and
+// the tile (member, j, shift) is ordered, so that each unordered cluster
+// pair occurs once.  Tiles of a cluster with itself, and tiles holding
+// excluded pairs, get four explicit 64-bit masks (MDExecKernels.cpp:1748-1823
+// removes the same partners from a matrix row).
 __global__ void __launch_bounds__(256) nb_search(
     const float4 *__restrict__ bb_lo, const float4 *__restrict__ bb_hi,
     const int *__restrict__ cluster_col, const uint32_t *__restrict__ real_mask,
@@ -278,60 +280,31 @@ __global__ void __launch_bounds__(256) nb_search(
   // The next build starts from empty buckets (nb_sort has read them).
   if (blockIdx.x == 0)
     for (int c = threadIdx.x; c < g.ncol; c += blockDim.x) col_count[c] = 0;
+  constexpr int kWaves = 4;
   constexpr int kExMax = kGroup * kCluster * 16;  // 16 partners per atom at most
-  __shared__ int s_ex_cluster[4][kExMax];
-  __shared__ int s_ex_bit[4][kExMax];             // member << 8 | mask bit
-  __shared__ int s_ex_n[4];
+  __shared__ int s_ex_cluster[kExMax];
+  __shared__ int s_ex_bit[kExMax];                // member << 8 | mask bit
+  __shared__ int s_ex_n;
   // partner clusters of the group: open-addressing set, key -1 = empty,
   // value = the members that have a partner there
   constexpr int kPartnerSlots = kExMax;  // a power of two
   static_assert((kPartnerSlots & (kPartnerSlots - 1)) == 0, "partner set size");
-  __shared__ int s_pkey[4][kPartnerSlots];
-  __shared__ int s_pmem[4][kPartnerSlots];
-  __shared__ int s_cstart[4][64], s_cbase[4][64], s_ccode[4][64];
-  __shared__ int s_k0[4][3][64], s_n[4][2][64];  // per z shift: first cluster, count
+  __shared__ int s_pkey[kPartnerSlots];
+  __shared__ int s_pmem[kPartnerSlots];
+  __shared__ int s_cstart[64], s_cbase[64], s_ccode[64];
+  __shared__ int s_k0[3][64], s_n[2][64];  // per z shift: first cluster, count
+  __shared__ int s_total;                  // candidates of the column chunk
+  __shared__ int s_wcount[kWaves];         // entries per wave in a round
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int gi = blockIdx.x * 4 + wave;
+  const int gi = blockIdx.x;
   const int ngroups = counts_in[1] / kGroup;
-  const bool active = gi < ngroups;
+  if (gi >= ngroups) return;  // the whole block: no barrier skipped
   const int g0 = gi * kGroup;
-  if (lane == 0) s_ex_n[wave] = 0;
-  for (int k = lane; k < kPartnerSlots; k += 64) {
-    s_pkey[wave][k] = -1;
-    s_pmem[wave][k] = 0;
+  if (threadIdx.x == 0) s_ex_n = 0;
+  for (int k = threadIdx.x; k < kPartnerSlots; k += blockDim.x) {
+    s_pkey[k] = -1;
+    s_pmem[k] = 0;
   }
-  __syncthreads();
-
-  // exclusion table of the group: (partner cluster, member << 8 | mask bit)
-  if (active && lane < kGroup * kCluster) {
-    const int m = lane >> 3, a = lane & 7;
-    const int atom = atom_at[g0 * kCluster + lane];
-    if (atom >= 0) {
-      const int b0 = excl_start[atom], b1 = excl_start[atom + 1];
-      for (int k = b0; k < b1; ++k) {
-        const int pp = place_of[excl_atoms[k]];
-        const int slot = atomicAdd(&s_ex_n[wave], 1);
-        if (slot < kExMax) {
-          const int pc = pp / kCluster;
-          s_ex_cluster[wave][slot] = pc;
-          s_ex_bit[wave][slot] = (m << 8) | (a + kCluster * (pp % kCluster));
-          // at most kExMax distinct keys: the probe always finds a slot
-          for (int h = pc & (kPartnerSlots - 1);; h = (h + 1) & (kPartnerSlots - 1)) {
-            const int prev = atomicCAS(&s_pkey[wave][h], -1, pc);
-            if (prev == -1 || prev == pc) {
-              atomicOr(&s_pmem[wave][h], 1 << m);
-              break;
-            }
-          }
-        } else {
-          atomicOr(&flags[kFlagOverflowRow], 2);
-        }
-      }
-    }
-  }
-  __syncthreads();
-  if (!active) return;
-  const int nex = s_ex_n[wave] < kExMax ? s_ex_n[wave] : kExMax;
 
   uint32_t mreal[kGroup];
   float4 mlo[kGroup], mhi[kGroup];
@@ -355,74 +328,108 @@ __global__ void __launch_bounds__(256) nb_search(
   const float Lx = (float)g.box[0], Ly = (float)g.box[1], Lz = (float)g.box[2];
   const float wxf = (float)g.wx, wyf = (float)g.wy;
   const float zreach = sqrtf(g.rlist2) * 1.0001f + 1.0e-5f;
-  int count = 0;
-  PairEntry *row = rows + (size_t)gi * g.row_cap;
   const int ncolx = 2 * g.rx + 1, ncols = any_real ? ncolx * (2 * g.ry + 1) : 0;
-  for (int cbase = 0; cbase < ncols; cbase += 64) {
-    const int c = cbase + lane;
-    int ncand = 0, cl0 = 0, code = 0;
-    int k0[3] = {0, 0, 0}, nz[3] = {0, 0, 0};
-    if (c < ncols) {
-      const int dy = c / ncolx - g.ry, dx = c % ncolx - g.rx;
-      int jy = cy + dy, sy = 0, jx = cx + dx, sx = 0;
-      while (jy < 0) { jy += g.ncy; --sy; }
-      while (jy >= g.ncy) { jy -= g.ncy; ++sy; }
-      while (jx < 0) { jx += g.ncx; --sx; }
-      while (jx >= g.ncx) { jx -= g.ncx; ++sx; }
-      const float ylo = jy * wyf - 0.5f * Ly + sy * Ly, yhi = ylo + wyf;
-      const float xlo = jx * wxf - 0.5f * Lx + sx * Lx, xhi = xlo + wxf;
-      const float gy = axis_gap(glo.y, ghi.y, ylo, yhi);
-      const float gx = axis_gap(glo.x, ghi.x, xlo, xhi);
-      if (gx * gx + gy * gy < g.rlist2 && sx >= -1 && sx <= 1 && sy >= -1 && sy <= 1) {
-        const int jcol = jy * g.ncx + jx;
-        cl0 = col_cl_start[jcol];
-        const int nreal = col_cl_count[jcol];
-        // a tile is ordered only when cj >= ci: clusters before the group
-        // never pair with it
-        const int kmin = g0 - cl0 > 0 ? g0 - cl0 : 0;
-        code = (sx + 1) | ((sy + 1) << 2);
-#pragma unroll
-        for (int s = 0; s < 3; ++s) {
-          // the z window in j's own frame: bb_hi.z >= zlo and bb_lo.z <= zhi
-          const float oz = (float)(s - 1) * Lz;
-          const float zlo = glo.z - zreach - oz, zhi = ghi.z + zreach - oz;
-          int lo = kmin < nreal ? kmin : nreal, hi = nreal;
-          while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (bb_hi[cl0 + mid].z < zlo) lo = mid + 1; else hi = mid;
+  __syncthreads();
+
+  // exclusion table of the group: (partner cluster, member << 8 | mask bit)
+  if (wave == 0 && lane < kGroup * kCluster) {
+    const int m = lane >> 3, a = lane & 7;
+    const int atom = atom_at[g0 * kCluster + lane];
+    if (atom >= 0) {
+      const int b0 = excl_start[atom], b1 = excl_start[atom + 1];
+      for (int k = b0; k < b1; ++k) {
+        const int pp = place_of[excl_atoms[k]];
+        const int slot = atomicAdd(&s_ex_n, 1);
+        if (slot < kExMax) {
+          const int pc = pp / kCluster;
+          s_ex_cluster[slot] = pc;
+          s_ex_bit[slot] = (m << 8) | (a + kCluster * (pp % kCluster));
+          // at most kExMax distinct keys: the probe always finds a slot
+          for (int h = pc & (kPartnerSlots - 1);; h = (h + 1) & (kPartnerSlots - 1)) {
+            const int prev = atomicCAS(&s_pkey[h], -1, pc);
+            if (prev == -1 || prev == pc) {
+              atomicOr(&s_pmem[h], 1 << m);
+              break;
+            }
           }
-          const int first = lo;
-          hi = nreal;
-          while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (bb_lo[cl0 + mid].z <= zhi) lo = mid + 1; else hi = mid;
-          }
-          k0[s] = first;
-          nz[s] = lo - first;
+        } else {
+          atomicOr(&flags[kFlagOverflowRow], 2);
         }
-        ncand = nz[0] + nz[1] + nz[2];
       }
     }
-    // inclusive wave scan of ncand
-    int scan = ncand;
+  }
+
+  int count = 0;  // entries of the row so far (the same in every thread)
+  PairEntry *row = rows + (size_t)gi * g.row_cap;
+  for (int cbase = 0; cbase < ncols; cbase += 64) {
+    // column slots of the chunk (wave 1; wave 0 with a single wave)
+    if (wave == 1) {
+      const int c = cbase + lane;
+      int ncand = 0, cl0 = 0, code = 0;
+      int k0[3] = {0, 0, 0}, nz[3] = {0, 0, 0};
+      if (c < ncols) {
+        const int dy = c / ncolx - g.ry, dx = c % ncolx - g.rx;
+        int jy = cy + dy, sy = 0, jx = cx + dx, sx = 0;
+        while (jy < 0) { jy += g.ncy; --sy; }
+        while (jy >= g.ncy) { jy -= g.ncy; ++sy; }
+        while (jx < 0) { jx += g.ncx; --sx; }
+        while (jx >= g.ncx) { jx -= g.ncx; ++sx; }
+        const float ylo = jy * wyf - 0.5f * Ly + sy * Ly, yhi = ylo + wyf;
+        const float xlo = jx * wxf - 0.5f * Lx + sx * Lx, xhi = xlo + wxf;
+        const float gy = axis_gap(glo.y, ghi.y, ylo, yhi);
+        const float gx = axis_gap(glo.x, ghi.x, xlo, xhi);
+        if (gx * gx + gy * gy < g.rlist2 && sx >= -1 && sx <= 1 && sy >= -1 && sy <= 1) {
+          const int jcol = jy * g.ncx + jx;
+          cl0 = col_cl_start[jcol];
+          const int nreal = col_cl_count[jcol];
+          // a tile is ordered only when cj >= ci: clusters before the group
+          // never pair with it
+          const int kmin = g0 - cl0 > 0 ? g0 - cl0 : 0;
+          code = (sx + 1) | ((sy + 1) << 2);
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int up = __shfl_up(scan, off);
-      if (lane >= off) scan += up;
+          for (int s = 0; s < 3; ++s) {
+            // the z window in j's own frame: bb_hi.z >= zlo and bb_lo.z <= zhi
+            const float oz = (float)(s - 1) * Lz;
+            const float zlo = glo.z - zreach - oz, zhi = ghi.z + zreach - oz;
+            int lo = kmin < nreal ? kmin : nreal, hi = nreal;
+            while (lo < hi) {
+              const int mid = (lo + hi) >> 1;
+              if (bb_hi[cl0 + mid].z < zlo) lo = mid + 1; else hi = mid;
+            }
+            const int first = lo;
+            hi = nreal;
+            while (lo < hi) {
+              const int mid = (lo + hi) >> 1;
+              if (bb_lo[cl0 + mid].z <= zhi) lo = mid + 1; else hi = mid;
+            }
+            k0[s] = first;
+            nz[s] = lo - first;
+          }
+          ncand = nz[0] + nz[1] + nz[2];
+        }
+      }
+      // inclusive wave scan of ncand
+      int scan = ncand;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int up = __shfl_up(scan, off);
+        if (lane >= off) scan += up;
+      }
+      s_cstart[lane] = cl0;
+      s_cbase[lane] = scan - ncand;  // exclusive
+      s_ccode[lane] = code;
+#pragma unroll
+      for (int s = 0; s < 3; ++s) s_k0[s][lane] = k0[s];
+      s_n[0][lane] = nz[0];
+      s_n[1][lane] = nz[1];
+      if (lane == 63) s_total = scan;
     }
-    const int total = __shfl(scan, 63);
-    s_cstart[wave][lane] = cl0;
-    s_cbase[wave][lane] = scan - ncand;  // exclusive
-    s_ccode[wave][lane] = code;
-#pragma unroll
-    for (int s = 0; s < 3; ++s) s_k0[wave][s][lane] = k0[s];
-    s_n[wave][0][lane] = nz[0];
-    s_n[wave][1][lane] = nz[1];
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int base = 0; base < total; base += 64) {
-      const int t = base + lane;
+    __syncthreads();  // column slots and (first chunk) exclusion table
+    const int total = s_total;
+    const int nex = s_ex_n < kExMax ? s_ex_n : kExMax;
+    const int niter = (total + 63) / 64;
+    for (int round = 0; round * kWaves < niter; ++round) {
+      const int t = (round * kWaves + wave) * 64 + lane;
       uint32_t imask = 0, jreal = 0, scode = 0;
       int cj = 0, ex = -1;
       if (t < total) {
@@ -430,10 +437,10 @@ __global__ void __launch_bounds__(256) nb_search(
         int lo = 0, hi = 63;
         while (lo < hi) {
           const int mid = (lo + hi + 1) >> 1;
-          if (s_cbase[wave][mid] <= t) lo = mid; else hi = mid - 1;
+          if (s_cbase[mid] <= t) lo = mid; else hi = mid - 1;
         }
-        int within = t - s_cbase[wave][lo];
-        const int n0 = s_n[wave][0][lo], n1 = s_n[wave][1][lo];
+        int within = t - s_cbase[lo];
+        const int n0 = s_n[0][lo], n1 = s_n[1][lo];
         int sz = -1;
         if (within >= n0) {
           within -= n0;
@@ -443,8 +450,8 @@ __global__ void __launch_bounds__(256) nb_search(
             sz = 1;
           }
         }
-        cj = s_cstart[wave][lo] + s_k0[wave][sz + 1][lo] + within;
-        const int ccode = s_ccode[wave][lo];
+        cj = s_cstart[lo] + s_k0[sz + 1][lo] + within;
+        const int ccode = s_ccode[lo];
         const int sx = (ccode & 3) - 1, sy = ((ccode >> 2) & 3) - 1;
         const int lex = sz != 0 ? sz : (sy != 0 ? sy : sx);
         jreal = real_mask[cj];
@@ -470,9 +477,9 @@ __global__ void __launch_bounds__(256) nb_search(
           if (!special && nex > 0) {
             int h = cj & (kPartnerSlots - 1);
             for (int probe = 0; probe < kPartnerSlots; ++probe, h = (h + 1) & (kPartnerSlots - 1)) {
-              const int key = s_pkey[wave][h];
+              const int key = s_pkey[h];
               if (key == cj) {
-                special = (s_pmem[wave][h] & (int)imask) != 0;
+                special = (s_pmem[h] & (int)imask) != 0;
                 break;
               }
               if (key == -1) break;
@@ -500,9 +507,9 @@ __global__ void __launch_bounds__(256) nb_search(
               }
             }
             for (int k = 0; k < nex; ++k)
-              if (s_ex_cluster[wave][k] == cj) {
-                const int mm = s_ex_bit[wave][k] >> 8;
-                const uint64_t bit = 1ull << (s_ex_bit[wave][k] & 0xff);
+              if (s_ex_cluster[k] == cj) {
+                const int mm = s_ex_bit[k] >> 8;
+                const uint64_t bit = 1ull << (s_ex_bit[k] & 0xff);
 #pragma unroll
                 for (int m = 0; m < kGroup; ++m)
                   if (m == mm) mk[m] &= ~bit;
@@ -526,9 +533,13 @@ __global__ void __launch_bounds__(256) nb_search(
       }
       const bool accept = imask != 0u;
       const uint64_t ballot = __ballot(accept);
-      const int before = __popcll(ballot & ((1ull << lane) - 1ull));
+      if (lane == 0) s_wcount[wave] = __popcll(ballot);
+      __syncthreads();
+      // the entries of the waves before this one in the round come first
+      int base = count;
+      for (int w = 0; w < wave; ++w) base += s_wcount[w];
       if (accept) {
-        const int slot = count + before;
+        const int slot = base + __popcll(ballot & ((1ull << lane) - 1ull));
         if (slot < g.row_cap) {
           PairEntry e;
           e.cj = cj;
@@ -540,11 +551,13 @@ __global__ void __launch_bounds__(256) nb_search(
           atomicOr(&flags[kFlagOverflowRow], 1);
         }
       }
-      count += __popcll(ballot);
+      for (int w = 0; w < kWaves; ++w) count += s_wcount[w];
+      __syncthreads();  // s_wcount is rewritten next round
     }
-    __builtin_amdgcn_wave_barrier();
+    // the column arrays are rewritten by the next chunk
+    __syncthreads();
   }
-  if (lane == 0) row_count[gi] = count < g.row_cap ? count : g.row_cap;
+  if (threadIdx.x == 0) row_count[gi] = count < g.row_cap ? count : g.row_cap;
 }
 
 // ---- the pruned list ------------------------------------------------------
