@@ -247,6 +247,20 @@ int mdrtHipComputeForceTerms(void *system, int32_t terms, double *forces, double
    cuFFT plan choice (mdrt_cuda.c:2646-2991) with this library's own. */
 int mdrtHipSetPmeTransform(void *system, int32_t mode);
 
+/* Launch-shape trial: times the PME chain's launch shapes on this system
+   and keeps the fastest, replacing the reference's TOWERS launch-shape
+   trial and its step-graph trial (tools/mdir/TowersTrial.h:13-33,
+   Run.cpp:913-926). Bit 0 of `what`: bricks of 8, 4, 2 or 1 complex z
+   lines (those that fit LDS). Bit 1: the plane transform as well (0 or 1
+   of mdrtHipSetPmeTransform, where the grid allows 1). Each candidate runs
+   one untimed captured graph of `steps` steps of dt (ps), then `graphs`
+   timed ones, from the same positions and velocities, which are restored
+   at the end. ms_per_step[8]: [4 * transform + k] for TL = 8 >> k, NaN for
+   shapes not run. The shapes give the same forces up to the f32 rounding
+   of the gather's per-brick partial sums (about 1e-8 relative). */
+int mdrtHipAutotune(void *system, int32_t what, int32_t steps, int32_t graphs, double dt,
+                    double *ms_per_step);
+
 /* `steps` velocity-Verlet steps of dt (ps) with SETTLE waters.  graph != 0
    replays the step sequence as a captured HIP graph of `graph` steps.
    kinetic (may be NULL) receives the kinetic energy after the last step. */

@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -890,6 +891,7 @@ struct Engine {
   // ---- bricks of the spread and gather ----
   BrickParams bp{};
   int brick_tl = 8;            // complex lines of a brick's z transform
+  int brick_tl_max = 8;        // the widest the shape may take (autotune)
   int nbricks = 0;
 
   // Brick shape from the z length (LDS of 2 TL K3 64-bit accumulators, two
@@ -899,7 +901,7 @@ struct Engine {
   // overflow).
   void make_bricks(int cap_min = 0) {
     const int K2 = pme_grid[2];
-    brick_tl = 8;
+    brick_tl = brick_tl_max;
     while (brick_tl > 1 && (size_t)(32 * brick_tl + 8) * K2 > 60 * 1024) brick_tl /= 2;
     if ((size_t)(32 * brick_tl + 8) * K2 > 60 * 1024) throw HipError("PME grid too long along z");
     const int R = 2 * brick_tl, BY = R >= 8 ? 4 : 2, BX = R / BY;
@@ -1405,6 +1407,70 @@ struct Engine {
     if (kinetic) *kinetic = kinetic_energy();
   }
 
+  // ---- launch-shape trial ----
+  // Times the launch shapes the PME chain can take on this system and keeps
+  // the fastest (the counterpart of the reference's launch-shape trial,
+  // tools/mdir/TowersTrial.h:13-33, Run.cpp:913-926). Candidates: bricks of
+  // 8, 4, 2 or 1 complex z lines (2 TL grid rows; those whose LDS fits) and,
+  // with `what` & 2, the plane transform (Stockham passes or the split-BF16x3
+  // MFMA products, where the grid allows them). Each candidate runs from the
+  // same state: one untimed graph of `steps` steps (list build, capture),
+  // then `graphs` graphs timed on the host clock around run(), overflow
+  // checks included. Positions and velocities are restored at the end.
+  // ms_out[4 * transform + k] = ms per step of TL = 8 >> k (NaN: not run).
+  // The shapes give the same forces up to the f32 rounding of the gather's
+  // per-brick partial sums (about 1e-8 relative); a fixed choice is
+  // bit-exact run to run.
+  static constexpr int kTrialShapes = 8;
+  void autotune(int what, int steps, int graphs, double dt, double *ms_out) {
+    for (int k = 0; k < kTrialShapes; ++k) ms_out[k] = NAN;
+    if (!use_pme || what <= 0 || steps <= 0 || graphs <= 0) return;
+    const size_t bytes = 3 * (size_t)n * sizeof(double);
+    std::vector<double> x0(3 * (size_t)n), v0(3 * (size_t)n);
+    to_host(x0.data(), d_x, bytes);
+    to_host(v0.data(), d_v, bytes);
+    auto restore = [&]() {
+      HIPCHK(hipMemcpyAsync(d_x, x0.data(), bytes, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(d_v, v0.data(), bytes, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      forces_valid = false;
+    };
+    const int K2 = pme_grid[2];
+    const bool mfma_ok = plane_fft() && mfma_edge(pme_grid[0]) && mfma_edge(pme_grid[1]);
+    const int tl0 = brick_tl_max, tr0 = pme_transform;
+    double best = INFINITY;
+    int best_tl = tl0, best_tr = tr0;
+    for (int tr = 0; tr < 2; ++tr) {
+      if (!(what & 2) && tr != tr0) continue;
+      if (tr == 1 && !mfma_ok) continue;
+      for (int k = 0; k < 4; ++k) {
+        const int tl = 8 >> k;
+        if (!(what & 1) && tl != brick_tl) continue;
+        if ((size_t)(32 * tl + 8) * K2 > 60 * 1024) continue;
+        brick_tl_max = tl;
+        make_bricks();
+        set_pme_transform(tr);
+        restore();
+        run(steps, dt, steps, nullptr);
+        const auto t0 = std::chrono::steady_clock::now();
+        run((int64_t)steps * graphs, dt, steps, nullptr);
+        const auto t1 = std::chrono::steady_clock::now();
+        const double ms =
+            std::chrono::duration<double, std::milli>(t1 - t0).count() / ((double)steps * graphs);
+        ms_out[4 * tr + k] = ms;
+        if (ms < best) {
+          best = ms;
+          best_tl = tl;
+          best_tr = tr;
+        }
+      }
+    }
+    brick_tl_max = best_tl;
+    make_bricks();
+    set_pme_transform(best_tr);
+    restore();
+  }
+
   void minimize(int64_t steps, double max_step, double *energy_out) {
     double E[MDRT_NUM_ENERGIES];
     std::vector<double> F(3 * (size_t)n);
@@ -1783,6 +1849,12 @@ int mdrtHipComputeForceTerms(void *system, int32_t terms, double *forces, double
 int mdrtHipSetPmeTransform(void *system, int32_t mode) {
   Engine *e = static_cast<Engine *>(system);
   return guarded([&] { e->set_pme_transform(mode); });
+}
+
+int mdrtHipAutotune(void *system, int32_t what, int32_t steps, int32_t graphs, double dt,
+                    double *ms_per_step) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] { e->autotune(what, steps, graphs, dt, ms_per_step); });
 }
 
 int mdrtHipRun(void *system, int64_t steps, double dt, int32_t graph, double *kinetic) {

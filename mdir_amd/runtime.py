@@ -117,6 +117,7 @@ SIGNATURES = [
     ("mdrtHipComputeForces", C.c_int, [C.c_void_p, PD, PD]),
     ("mdrtHipComputeForceTerms", C.c_int, [C.c_void_p, I32, PD, PD]),
     ("mdrtHipSetPmeTransform", C.c_int, [C.c_void_p, I32]),
+    ("mdrtHipAutotune", C.c_int, [C.c_void_p, I32, I32, I32, D, PD]),
     ("mdrtHipFramesSetPmeTransform", C.c_int, [C.c_void_p, I32]),
     ("mdrtHipRun", C.c_int, [C.c_void_p, I64, D, I32, PD]),
     ("mdrtHipRunMeasured", C.c_int, [C.c_void_p, I64, D, I32, PD]),

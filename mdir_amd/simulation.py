@@ -86,6 +86,20 @@ class Simulation:
         BF16x3 (PME plane edges of 32 or 64)."""
         rt.check(rt.lib().mdrtHipSetPmeTransform(self._h, int(mode)))
 
+    TUNE_BRICKS = 1
+    TUNE_TRANSFORM = 2
+
+    def autotune(self, what: int = TUNE_BRICKS, steps: int = 50, graphs: int = 4,
+                 dt: float = 0.002) -> dict:
+        """Times the PME launch shapes on this system and keeps the fastest
+        (the launch-shape trial of tools/mdir/TowersTrial.h); positions and
+        velocities are restored. Returns {(brick z lines, transform): ms per
+        step} for the shapes run."""
+        ms = np.full(8, np.nan)
+        rt.check(rt.lib().mdrtHipAutotune(self._h, int(what), int(steps), int(graphs),
+                                          float(dt), rt.ptr(ms, C.c_double)))
+        return {(8 >> (k % 4), k // 4): float(v) for k, v in enumerate(ms) if np.isfinite(v)}
+
     def energies(self) -> dict:
         _, e = self.forces()
         return dict(zip(rt.ENERGY_NAMES, e))

@@ -1,0 +1,61 @@
+"""The launch-shape trial (mdrtHipAutotune): every PME brick shape and plane
+transform it times gives the forces of the FP64 oracle, the state is
+restored, and the shape it keeps runs the dynamics."""
+import numpy as np
+import pytest
+
+from mdir_amd import systems as S
+from mdir_amd.simulation import Simulation
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def water(grid):
+    w = S.tip3p_box(512, 2.5, seed=21, grid=grid)
+    base = Simulation(w, skin=0.1)
+    base.minimize(50, 0.01)
+    w = w.copy()
+    w.positions = base.positions
+    base.close()
+    return w
+
+
+@pytest.mark.parametrize("grid,what", [((24, 24, 24), Simulation.TUNE_BRICKS),
+                                       ((32, 32, 32), Simulation.TUNE_BRICKS
+                                        | Simulation.TUNE_TRANSFORM)])
+def test_autotune_keeps_state_and_forces(grid, what):
+    say(f"case: autotune TIP3P 512 waters, grid {grid}, what {what}")
+    w = water(grid)
+    sim = Simulation(w, skin=0.1)
+    sim.velocities = S.initial_velocities(w, 5, 300.0)
+    x0, v0 = sim.positions, sim.velocities
+    F0, _ = sim.forces()
+    times = sim.autotune(what, steps=20, graphs=2)
+    say(f"  ms per step: {times}")
+    # every brick shape that fits a z edge of 24 or 32 (8 z lines at most)
+    assert {tl for tl, _ in times} == {8, 4, 2, 1}
+    if what & Simulation.TUNE_TRANSFORM:
+        assert {tr for _, tr in times} == {0, 1}
+    assert all(t > 0 for t in times.values())
+    # positions and velocities restored bit for bit
+    assert np.array_equal(sim.positions, x0)
+    assert np.array_equal(sim.velocities, v0)
+    # the kept shape: the oracle's forces, and the untuned forces up to the
+    # f32 rounding of the gather's per-brick partial sums (9e-9 measured at
+    # 24^3), or, with the MFMA transform kept, its split-BF16x3 error
+    F1, _ = sim.forces()
+    Fo, _ = O.Oracle(w).total()
+    assert O.rel_rms(F1, Fo) < 5e-6
+    assert O.rel_rms(F1, F0) < (2e-6 if what & Simulation.TUNE_TRANSFORM else 1e-7)
+    # and it runs: 200 NVE steps conserve energy to the usual bar
+    e0 = sum(sim.energies().values())
+    sim.run(200, 0.002, graph=50)
+    e1 = sum(sim.energies().values())
+    say(f"  total energy {e0:.3f} -> {e1:.3f}")
+    assert abs(e1 - e0) < 5e-4 * abs(e0) + 5.0
+    sim.close()
