@@ -48,6 +48,9 @@ def parse():
                         "atom moved half of it")
     p.add_argument("--prune-skin", type=float, default=0.0,
                    help="buffer of a pruned inner list (nm, below --skin); 0: none")
+    p.add_argument("--autotune", type=int, default=3,
+                   help="launch-shape trial before the warm-up: 0 none, 1 PME brick "
+                        "shapes, 3 brick shapes and plane transforms")
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -141,6 +144,13 @@ def main():
     print(f"setup: {system.n} atoms minimised, velocities at 300 K", file=sys.stderr,
           flush=True)
 
+    tuned = {}
+    if args.autotune:
+        tuned = sim.autotune(args.autotune, steps=args.graph, graphs=4, dt=dt)
+        best = min(tuned, key=tuned.get)
+        print(f"autotune: kept bricks of {best[0]} z lines, transform {best[1]} "
+              f"({tuned[best] * 1e3:.1f} us/step)", file=sys.stderr, flush=True)
+
     sim.run(args.warmup, dt, graph=args.graph)
     builds0 = sim.stats()["builds"]
     t0 = time.perf_counter()
@@ -230,13 +240,17 @@ def main():
         "nve_drift_ps": float(times[-1]),
         "energy_terms_kj_mol": [round(float(t), 4) for t in terms],
     }
+    if tuned:
+        out["autotune_us_per_step"] = {f"bricks{tl}_transform{tr}": round(v * 1e3, 2)
+                                       for (tl, tr), v in sorted(tuned.items())}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(system, sim.positions, sim.velocities, dt,
                                            args.cpu_seconds)
     if args.frames > 0:
         from mdir_amd.frames import force_matching_benchmark
         out["frame_gradients"] = force_matching_benchmark(args.frames, args.frame_repeats,
-                                                          args.frame_loop_passes)
+                                                          args.frame_loop_passes,
+                                                          autotune=args.autotune)
     print(json.dumps(out), flush=True)
     if args.calibrate_copy_mib:
         calibrate_copy(args.calibrate_copy_mib)

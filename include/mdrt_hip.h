@@ -255,7 +255,8 @@ int mdrtHipSetPmeTransform(void *system, int32_t mode);
    of mdrtHipSetPmeTransform, where the grid allows 1). Each candidate runs
    one untimed captured graph of `steps` steps of dt (ps), then `graphs`
    timed ones, from the same positions and velocities, which are restored
-   at the end. ms_per_step[8]: [4 * transform + k] for TL = 8 >> k, NaN for
+   at the end; two passes over the candidates, the faster time of each
+   counting, and the current shape stays unless another is 2 % faster. ms_per_step[8]: [4 * transform + k] for TL = 8 >> k, NaN for
    shapes not run. The shapes give the same forces up to the f32 rounding
    of the gather's per-brick partial sums (about 1e-8 relative). */
 int mdrtHipAutotune(void *system, int32_t what, int32_t steps, int32_t graphs, double dt,
@@ -387,6 +388,17 @@ int mdrtHipFramesHvp(void *frames, int32_t count, const double *v, double *out);
 int mdrtHipFramesForceMatching(void *frames, int32_t count, const double *positions,
                                const double *reference_forces, int32_t graph, double *loss,
                                double *d_charges, double *d_sigma, double *d_epsilon);
+
+/* Launch-shape trial of the frame evaluator (as mdrtHipAutotune, for the
+   frame batch): times the graph-replayed mdrtHipFramesForceMatching on
+   these `count` frames for each PME brick shape that fits LDS (what & 1)
+   and plane transform (what & 2), one untimed call and `repeats` timed ones
+   each in two passes, and keeps the fastest (the current shape unless
+   another is 2 % faster). ms_per_call[8]: [4 * transform + k] for
+   TL = 8 >> k, NaN for shapes not run. The evaluator holds no evaluated
+   frames afterwards. */
+int mdrtHipFramesAutotune(void *frames, int32_t what, int32_t count, const double *positions,
+                          const double *reference_forces, int32_t repeats, double *ms_per_call);
 
 #ifdef __cplusplus
 }

@@ -1416,12 +1416,52 @@ struct Engine {
   // MFMA products, where the grid allows them). Each candidate runs from the
   // same state: one untimed graph of `steps` steps (list build, capture),
   // then `graphs` graphs timed on the host clock around run(), overflow
-  // checks included. Positions and velocities are restored at the end.
-  // ms_out[4 * transform + k] = ms per step of TL = 8 >> k (NaN: not run).
-  // The shapes give the same forces up to the f32 rounding of the gather's
-  // per-brick partial sums (about 1e-8 relative); a fixed choice is
-  // bit-exact run to run.
+  // checks included; two passes over the candidates, the faster of each
+  // candidate's two times counting. The current shape is kept unless
+  // another is at least kTrialMargin faster. Positions and velocities are
+  // restored at the end. ms_out[4 * transform + k] = ms per step of
+  // TL = 8 >> k (NaN: not run). The shapes give the same forces up to the
+  // f32 rounding of the gather's per-brick partial sums (about 1e-8
+  // relative); a fixed choice is bit-exact run to run.
   static constexpr int kTrialShapes = 8;
+  static constexpr double kTrialMargin = 0.02;
+  static std::vector<std::pair<int, int>> trial_shapes(int what, int K2, bool mfma_ok, int tl0,
+                                                       int tr0) {
+    std::vector<std::pair<int, int>> c;  // (k: TL = 8 >> k, transform)
+    for (int tr = 0; tr < 2; ++tr) {
+      if (!(what & 2) && tr != tr0) continue;
+      if (tr == 1 && !mfma_ok) continue;
+      for (int k = 0; k < 4; ++k) {
+        const int tl = 8 >> k;
+        if (!(what & 1) && tl != tl0) continue;
+        if ((size_t)(32 * tl + 8) * K2 > 60 * 1024) continue;
+        c.push_back({k, tr});
+      }
+    }
+    return c;
+  }
+  // Runs measure(k, transform) for every candidate in two passes, fills
+  // ms_out with the faster time of each, and returns the candidate to keep.
+  template <class Measure>
+  static std::pair<int, int> trial(const std::vector<std::pair<int, int>> &cands, int tl0,
+                                   int tr0, Measure measure, double *ms_out) {
+    for (int pass = 0; pass < 2; ++pass)
+      for (const auto &c : cands) {
+        const double ms = measure(8 >> c.first, c.second);
+        double &slot = ms_out[4 * c.second + c.first];
+        slot = std::isnan(slot) ? ms : std::min(slot, ms);
+      }
+    std::pair<int, int> best{-1, -1}, cur{-1, -1};
+    for (const auto &c : cands) {
+      const double ms = ms_out[4 * c.second + c.first];
+      if (best.first < 0 || ms < ms_out[4 * best.second + best.first]) best = c;
+      if ((8 >> c.first) == tl0 && c.second == tr0) cur = c;
+    }
+    if (cur.first >= 0 && ms_out[4 * best.second + best.first] >
+                              (1.0 - kTrialMargin) * ms_out[4 * cur.second + cur.first])
+      best = cur;
+    return best;
+  }
   void autotune(int what, int steps, int graphs, double dt, double *ms_out) {
     for (int k = 0; k < kTrialShapes; ++k) ms_out[k] = NAN;
     if (!use_pme || what <= 0 || steps <= 0 || graphs <= 0) return;
@@ -1435,39 +1475,27 @@ struct Engine {
       HIPCHK(hipStreamSynchronize(stream));
       forces_valid = false;
     };
-    const int K2 = pme_grid[2];
     const bool mfma_ok = plane_fft() && mfma_edge(pme_grid[0]) && mfma_edge(pme_grid[1]);
-    const int tl0 = brick_tl_max, tr0 = pme_transform;
-    double best = INFINITY;
-    int best_tl = tl0, best_tr = tr0;
-    for (int tr = 0; tr < 2; ++tr) {
-      if (!(what & 2) && tr != tr0) continue;
-      if (tr == 1 && !mfma_ok) continue;
-      for (int k = 0; k < 4; ++k) {
-        const int tl = 8 >> k;
-        if (!(what & 1) && tl != brick_tl) continue;
-        if ((size_t)(32 * tl + 8) * K2 > 60 * 1024) continue;
-        brick_tl_max = tl;
-        make_bricks();
-        set_pme_transform(tr);
-        restore();
-        run(steps, dt, steps, nullptr);
-        const auto t0 = std::chrono::steady_clock::now();
-        run((int64_t)steps * graphs, dt, steps, nullptr);
-        const auto t1 = std::chrono::steady_clock::now();
-        const double ms =
-            std::chrono::duration<double, std::milli>(t1 - t0).count() / ((double)steps * graphs);
-        ms_out[4 * tr + k] = ms;
-        if (ms < best) {
-          best = ms;
-          best_tl = tl;
-          best_tr = tr;
-        }
-      }
-    }
-    brick_tl_max = best_tl;
-    make_bricks();
-    set_pme_transform(best_tr);
+    const int tl0 = brick_tl, tr0 = pme_transform;
+    auto shape = [&](int tl, int tr) {
+      brick_tl_max = tl;
+      make_bricks();
+      set_pme_transform(tr);
+    };
+    const auto keep = trial(
+        trial_shapes(what, pme_grid[2], mfma_ok, tl0, tr0), tl0, tr0,
+        [&](int tl, int tr) {
+          shape(tl, tr);
+          restore();
+          run(steps, dt, steps, nullptr);
+          const auto t0 = std::chrono::steady_clock::now();
+          run((int64_t)steps * graphs, dt, steps, nullptr);
+          const auto t1 = std::chrono::steady_clock::now();
+          return std::chrono::duration<double, std::milli>(t1 - t0).count() /
+                 ((double)steps * graphs);
+        },
+        ms_out);
+    shape(keep.first >= 0 ? 8 >> keep.first : tl0, keep.first >= 0 ? keep.second : tr0);
     restore();
   }
 
@@ -1942,6 +1970,13 @@ int mdrtHipFramesSetPmeTransform(void *frames, int32_t mode) {
     if (f->fm_graph) hipGraphExecDestroy(f->fm_graph);
     f->fm_graph = nullptr;
   });
+}
+
+int mdrtHipFramesAutotune(void *frames, int32_t what, int32_t count, const double *positions,
+                          const double *reference_forces, int32_t repeats, double *ms_per_call) {
+  FrameBatch *f = static_cast<FrameBatch *>(frames);
+  return guarded(
+      [&] { f->autotune(what, count, positions, reference_forces, repeats, ms_per_call); });
 }
 
 int mdrtHipFramesHvp(void *frames, int32_t count, const double *v, double *out) {

@@ -656,6 +656,58 @@ struct FrameBatch {
     HIPCHK(hipStreamSynchronize(e.stream));
   }
 
+  // A PME launch shape for every frame: the brick width (TL complex z
+  // lines, at most) and the plane transform. The per-frame brick lists live
+  // in the frame arena, so it is laid out again around the change. The
+  // evaluator holds no evaluated frames afterwards.
+  void set_shape(int tl, int transform) {
+    if (fm_graph) hipGraphExecDestroy(fm_graph);
+    fm_graph = nullptr;
+    e.unbatch();
+    for (void **p : e.frame_extra) dfree(*p);
+    e.frame_extra.clear();
+    e.brick_tl_max = tl;
+    e.make_bricks();
+    alloc_extra();
+    e.batch(max_frames);
+    e.set_pme_transform(transform);
+    loaded = 0;
+  }
+
+  // Launch-shape trial of the frame evaluator (the engine's autotune, for
+  // the frames): times the graph-replayed force-matching call on the
+  // caller's `frames` frames for every PME brick shape that fits LDS (what &
+  // 1) and plane transform (what & 2), one untimed call (list build,
+  // capture) and `repeats` timed ones each, in two passes (the faster
+  // counts), and keeps the fastest unless the current shape is within the
+  // trial margin. ms_out[4 * transform + k] = ms per call of TL = 8 >> k
+  // (NaN: not run).
+  void autotune(int what, int frames, const double *positions, const double *f_ref,
+                int repeats, double *ms_out) {
+    for (int k = 0; k < 8; ++k) ms_out[k] = NAN;
+    check_frames(frames);
+    if (!e.use_pme || what <= 0 || repeats <= 0) return;
+    const bool mfma_ok =
+        e.plane_fft() && Engine::mfma_edge(e.pme_grid[0]) && Engine::mfma_edge(e.pme_grid[1]);
+    const int tl0 = e.brick_tl, tr0 = e.pme_transform;
+    std::vector<double> loss(frames), dq(e.n), ds(e.ntypes), de(e.ntypes);
+    auto call = [&]() {
+      force_matching(frames, positions, f_ref, loss.data(), dq.data(), ds.data(), de.data(), 1);
+    };
+    const auto keep = Engine::trial(
+        Engine::trial_shapes(what, e.pme_grid[2], mfma_ok, tl0, tr0), tl0, tr0,
+        [&](int tl, int tr) {
+          set_shape(tl, tr);
+          call();
+          const auto t0 = std::chrono::steady_clock::now();
+          for (int r = 0; r < repeats; ++r) call();
+          const auto t1 = std::chrono::steady_clock::now();
+          return std::chrono::duration<double, std::milli>(t1 - t0).count() / repeats;
+        },
+        ms_out);
+    set_shape(keep.first >= 0 ? 8 >> keep.first : tl0, keep.first >= 0 ? keep.second : tr0);
+  }
+
   // Forward, residual, adjoint and the sum over frames, as one graph per
   // frame count (captured at the first call, replayed after).
   void fm_sequence(int frames) {

@@ -289,8 +289,25 @@ class FrameEvaluator:
         self._loaded = K
         return loss, FrameGradient({"charges": dq, "sigma": ds, "epsilon": de})
 
+    def autotune(self, positions, reference_forces, what: int = 1, repeats: int = 5) -> dict:
+        """Times force_matching on these frames for each PME launch shape
+        (bit 0 of what: brick widths, bit 1: plane transforms) and keeps the
+        fastest; returns {(brick z lines, transform): ms per call}. The
+        evaluator holds no evaluated frames afterwards."""
+        K = self._count(positions)
+        pp, keep, _ = _frame_array(positions, (K, self.n, 3), "positions")
+        rp, rkeep, _ = _frame_array(reference_forces, (K, self.n, 3), "reference_forces")
+        ms = np.full(8, np.nan)
+        rt.check(rt.lib().mdrtHipFramesAutotune(self._h, int(what), K, pp, rp, int(repeats),
+                                                rt.ptr(ms, C.c_double)))
+        del keep, rkeep
+        self._version += 1
+        self._loaded = 0
+        return {(8 >> (k % 4), k // 4): float(v) for k, v in enumerate(ms) if np.isfinite(v)}
 
-def force_matching_benchmark(nframes: int = 64, repeats: int = 20, loop_passes: int = 3):
+
+def force_matching_benchmark(nframes: int = 64, repeats: int = 20, loop_passes: int = 3,
+                             autotune: int = 3):
     """Force-matching frame-gradients per second on the differentiation
     workload (64 frames of 3,150 atoms: 1,000 TIP3P waters and 30 five-site
     Lennard-Jones solutes of 4 types, PME 32^3, rc 0.9 nm): one call
@@ -299,7 +316,9 @@ def force_matching_benchmark(nframes: int = 64, repeats: int = 20, loop_passes: 
     sigma and epsilon.  Positions and reference forces are resident on the
     device before the timed region; a call ends with the gradient on the
     host.  Batched (all frames per call, one captured graph) against the
-    plain per-frame loop (one frame per call, same library and graphs)."""
+    plain per-frame loop (one frame per call, same library and graphs).
+    With autotune, both evaluators first keep their fastest PME launch shape
+    (FrameEvaluator.autotune on the same frames)."""
     import sys
     import time
     from . import systems as S
@@ -313,6 +332,7 @@ def force_matching_benchmark(nframes: int = 64, repeats: int = 20, loop_passes: 
     ev.update(q1, s1, e1)
     Fref = ev.evaluate(X).forces
     ev.update(theta["charges"], theta["sigma"], theta["epsilon"])
+    tuned = ev.autotune(X, Fref, what=autotune, repeats=5) if autotune else {}
     stream = L.mgpuStreamCreate()
     nbytes = X.nbytes
     dx = L.mgpuMemAlloc(nbytes, stream, False)
@@ -339,6 +359,8 @@ def force_matching_benchmark(nframes: int = 64, repeats: int = 20, loop_passes: 
     batched_s = (time.perf_counter() - t0) / repeats
     grad_batched = (dq.copy(), ds.copy(), de.copy())
     one = FrameEvaluator(system, max_frames=1) if loop_passes > 0 else None
+    if one is not None and autotune:
+        one.autotune(X[:1], Fref[:1], what=autotune, repeats=5)
     frame_bytes = n * 3 * 8
     loop_s = float("nan")
     if one is not None:
@@ -372,7 +394,9 @@ def force_matching_benchmark(nframes: int = 64, repeats: int = 20, loop_passes: 
                        "tunables": {"charges": n, "sigma": system.ntypes,
                                     "epsilon": system.ntypes},
                        "repeats": repeats, "loop_passes": loop_passes},
-            "gradient_norm": float(np.sqrt(sum(float((g * g).sum()) for g in grad_batched)))}
+            "gradient_norm": float(np.sqrt(sum(float((g * g).sum()) for g in grad_batched))),
+            "autotune_ms_per_call": {f"bricks{tl}_transform{tr}": round(v, 4)
+                                     for (tl, tr), v in sorted(tuned.items())}}
 
 
 if __name__ == "__main__":
@@ -382,5 +406,8 @@ if __name__ == "__main__":
     p.add_argument("--frames", type=int, default=64)
     p.add_argument("--repeats", type=int, default=20)
     p.add_argument("--loop-passes", type=int, default=3)
+    p.add_argument("--autotune", type=int, default=3,
+                   help="0 none, 1 PME brick shapes, 3 brick shapes and plane transforms")
     a = p.parse_args()
-    print(json.dumps(force_matching_benchmark(a.frames, a.repeats, a.loop_passes)), flush=True)
+    print(json.dumps(force_matching_benchmark(a.frames, a.repeats, a.loop_passes, a.autotune)),
+          flush=True)

@@ -133,6 +133,8 @@ SIGNATURES = [
     ("mdrtHipFramesVjp", C.c_int, [C.c_void_p, I32, PD, C.c_void_p, PD, PD, PD]),
     ("mdrtHipFramesVjpPairs", C.c_int, [C.c_void_p, I32, PD, C.c_void_p, I32, PI32, PD, PD]),
     ("mdrtHipFramesHvp", C.c_int, [C.c_void_p, I32, C.c_void_p, C.c_void_p]),
+    ("mdrtHipFramesAutotune", C.c_int, [C.c_void_p, I32, I32, C.c_void_p, C.c_void_p, I32,
+                                        PD]),
     ("mdrtHipFramesForceMatching", C.c_int, [C.c_void_p, I32, C.c_void_p, C.c_void_p, I32,
                                              PD, PD, PD, PD]),
 ]

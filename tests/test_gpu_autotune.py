@@ -59,3 +59,31 @@ def test_autotune_keeps_state_and_forces(grid, what):
     say(f"  total energy {e0:.3f} -> {e1:.3f}")
     assert abs(e1 - e0) < 5e-4 * abs(e0) + 5.0
     sim.close()
+
+
+def test_frames_autotune():
+    """The frame evaluator's trial: every shape it times, then the kept one,
+    gives the force-matching loss and gradient of the untuned evaluator."""
+    from mdir_amd.frames import FrameEvaluator
+    say("case: frames autotune, 300 waters + 12 solutes, 4 frames, grid 32")
+    system = S.solvated_solutes(nwaters=300, nsolutes=12, grid=(32, 32, 32))
+    X = S.perturbed_frames(system, 4, seed=3)
+    q1, s1, e1 = S.perturbed_parameters(system)
+    ev = FrameEvaluator(system, max_frames=4)
+    theta = ev.tunables
+    ev.update(q1, s1, e1)
+    Fref = ev.evaluate(X).forces
+    ev.update(theta["charges"], theta["sigma"], theta["epsilon"])
+    loss0, g0 = ev.force_matching(X, Fref)
+    times = ev.autotune(X, Fref, what=3, repeats=2)
+    say(f"  ms per call: {times}")
+    assert {tl for tl, _ in times} == {8, 4, 2, 1}
+    assert {tr for _, tr in times} == {0, 1}
+    loss1, g1 = ev.force_matching(X, Fref)
+    # brick shapes differ by the f32 rounding of per-brick gather sums, the
+    # MFMA transform (if kept) by its split-BF16x3 error
+    assert np.allclose(loss1, loss0, rtol=1e-4)
+    for name in ("charges", "sigma", "epsilon"):
+        a, b = g1[name], g0[name]
+        assert np.linalg.norm(a - b) <= 1e-3 * np.linalg.norm(b), name
+    ev.close()
