@@ -48,9 +48,9 @@ def parse():
                         "atom moved half of it")
     p.add_argument("--prune-skin", type=float, default=0.0,
                    help="buffer of a pruned inner list (nm, below --skin); 0: none")
-    p.add_argument("--autotune", type=int, default=3,
-                   help="launch-shape trial before the warm-up: 0 none, 1 PME brick "
-                        "shapes, 3 brick shapes and plane transforms")
+    p.add_argument("--autotune", type=int, default=7,
+                   help="launch-shape trial before the warm-up: bit 0 PME brick shapes, "
+                        "bit 1 plane transforms, bit 2 pair-loop segment lengths (0: none)")
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -148,8 +148,8 @@ def main():
     if args.autotune:
         tuned = sim.autotune(args.autotune, steps=args.graph, graphs=4, dt=dt)
         best = min(tuned, key=tuned.get)
-        print(f"autotune: kept bricks of {best[0]} z lines, transform {best[1]} "
-              f"({tuned[best] * 1e3:.1f} us/step)", file=sys.stderr, flush=True)
+        print(f"autotune: best bricks of {best[0]} z lines, transform {best[1]}, segments "
+              f"of {best[2]} ({tuned[best] * 1e3:.1f} us/step)", file=sys.stderr, flush=True)
 
     sim.run(args.warmup, dt, graph=args.graph)
     builds0 = sim.stats()["builds"]
@@ -241,8 +241,8 @@ def main():
         "energy_terms_kj_mol": [round(float(t), 4) for t in terms],
     }
     if tuned:
-        out["autotune_us_per_step"] = {f"bricks{tl}_transform{tr}": round(v * 1e3, 2)
-                                       for (tl, tr), v in sorted(tuned.items())}
+        out["autotune_us_per_step"] = {f"seg{sg}_bricks{tl}_transform{tr}": round(v * 1e3, 2)
+                                       for (tl, tr, sg), v in sorted(tuned.items())}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(system, sim.positions, sim.velocities, dt,
                                            args.cpu_seconds)

@@ -247,18 +247,21 @@ int mdrtHipComputeForceTerms(void *system, int32_t terms, double *forces, double
    cuFFT plan choice (mdrt_cuda.c:2646-2991) with this library's own. */
 int mdrtHipSetPmeTransform(void *system, int32_t mode);
 
-/* Launch-shape trial: times the PME chain's launch shapes on this system
+/* Launch-shape trial: times the launch shapes of this system's kernels
    and keeps the fastest, replacing the reference's TOWERS launch-shape
    trial and its step-graph trial (tools/mdir/TowersTrial.h:13-33,
-   Run.cpp:913-926). Bit 0 of `what`: bricks of 8, 4, 2 or 1 complex z
-   lines (those that fit LDS). Bit 1: the plane transform as well (0 or 1
-   of mdrtHipSetPmeTransform, where the grid allows 1). Each candidate runs
-   one untimed captured graph of `steps` steps of dt (ps), then `graphs`
-   timed ones, from the same positions and velocities, which are restored
-   at the end; two passes over the candidates, the faster time of each
-   counting, and the current shape stays unless another is 2 % faster. ms_per_step[8]: [4 * transform + k] for TL = 8 >> k, NaN for
-   shapes not run. The shapes give the same forces up to the f32 rounding
-   of the gather's per-brick partial sums (about 1e-8 relative). */
+   Run.cpp:913-926). Bit 0 of `what`: PME bricks of 8, 4, 2 or 1 complex z
+   lines (those that fit LDS). Bit 1: the plane transform (0 or 1 of
+   mdrtHipSetPmeTransform, where the grid allows 1). Bit 2: pair-loop
+   segments of 16, 24 or 48 row entries per wave. Each candidate runs one
+   untimed captured graph of `steps` steps of dt (ps), then `graphs` timed
+   ones, from the same positions and velocities, which are restored at the
+   end; two passes over the candidates, the faster time of each counting,
+   and the current shape stays unless another is 2 % faster.
+   ms_per_step[24]: [8 * s + 4 * transform + k] for segments of 16, 24, 48
+   (s = 0, 1, 2) and bricks of TL = 8 >> k, NaN for shapes not run. The
+   shapes give the same forces up to the f32 rounding of the gather's
+   per-brick partial sums (about 1e-8 relative). */
 int mdrtHipAutotune(void *system, int32_t what, int32_t steps, int32_t graphs, double dt,
                     double *ms_per_step);
 
@@ -391,11 +394,10 @@ int mdrtHipFramesForceMatching(void *frames, int32_t count, const double *positi
 
 /* Launch-shape trial of the frame evaluator (as mdrtHipAutotune, for the
    frame batch): times the graph-replayed mdrtHipFramesForceMatching on
-   these `count` frames for each PME brick shape that fits LDS (what & 1)
-   and plane transform (what & 2), one untimed call and `repeats` timed ones
-   each in two passes, and keeps the fastest (the current shape unless
-   another is 2 % faster). ms_per_call[8]: [4 * transform + k] for
-   TL = 8 >> k, NaN for shapes not run. The evaluator holds no evaluated
+   these `count` frames for each shape `what` names (as mdrtHipAutotune),
+   one untimed call and `repeats` timed ones each in two passes, and keeps
+   the fastest (the current shape unless another is 2 % faster).
+   ms_per_call[24] as ms_per_step there. The evaluator holds no evaluated
    frames afterwards. */
 int mdrtHipFramesAutotune(void *frames, int32_t what, int32_t count, const double *positions,
                           const double *reference_forces, int32_t repeats, double *ms_per_call);

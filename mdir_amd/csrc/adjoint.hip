@@ -437,7 +437,7 @@ __global__ void __launch_bounds__(256) nb_adjoint(
     const float4 *__restrict__ xq, const int *__restrict__ type_at,
     const uint32_t *__restrict__ real_mask, const PairEntry *__restrict__ rows,
     const int *__restrict__ row_count, const uint64_t *__restrict__ ex_masks, int row_cap,
-    int max_groups, const int *__restrict__ counts_in, AdjTables tab,
+    int seg_len, int max_groups, const int *__restrict__ counts_in, AdjTables tab,
     AdjParams p, const float4 *__restrict__ cot, const double *__restrict__ meta,
     long long *__restrict__ A, double *__restrict__ vpart, double *__restrict__ vspec,
     SpecialAdjArgs sa, size_t fs) {
@@ -458,8 +458,8 @@ __global__ void __launch_bounds__(256) nb_adjoint(
   extern __shared__ float4 s_lj4[];  // ntypes^2 x (float4, float2, int) (launch size)
   __shared__ union {
     struct {
-      float aj[4][kSegment * kCluster * 3];
-      int cj[4][kSegment];
+      float aj[4][kSegmentMax * kCluster * 3];
+      int cj[4][kSegmentMax];
     } pr;
     double v[256];
   } sm;
@@ -488,10 +488,10 @@ __global__ void __launch_bounds__(256) nb_adjoint(
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int gi = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
   const int seg = blockIdx.y - 1;
-  const int e0 = seg * kSegment;
+  const int e0 = seg * seg_len;
   const int ngroups = counts_in[1] / kGroup;
   int n = gi < ngroups ? row_count[gi] - e0 : 0;
-  n = __builtin_amdgcn_readfirstlane(n < kSegment ? n : kSegment);
+  n = __builtin_amdgcn_readfirstlane(n < seg_len ? n : seg_len);
   const size_t slot = (size_t)seg * max_groups + gi;
   if (n <= 0) {
     if ((KIND == kAdjVirial || KIND == kAdjPairTable) && gi < max_groups && lane < kPartStride)

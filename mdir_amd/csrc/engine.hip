@@ -211,6 +211,10 @@ struct Engine {
   double *d_xprune = nullptr;      // positions at the last prune
   int ex_cap = 0;
   int max_groups = 0, nseg = 0;    // group rows; pair-loop segments per row
+  // entries per pair-loop segment (16, 24 or 48): short segments give one
+  // system more waves to hide latency with; a batch of frames fills the
+  // GPU with long ones (a launch shape, see autotune)
+  int seg_len = 16, seg_len_cfg = 0;
   int *d_excl_start = nullptr, *d_excl_atoms = nullptr;
   int *d_flags = nullptr;
   double list_reach = 0;  // rlist of the current list configuration
@@ -441,17 +445,18 @@ struct Engine {
     est = std::min(est, (double)ng.max_clusters * 3.0);
     int rc_ = (int)(1.25 * est + 64);
     rc_ = std::max(rc_, row_cap_min);
-    ng.row_cap = (rc_ + kSegment - 1) / kSegment * kSegment;
+    ng.row_cap = (rc_ + kSegmentMax - 1) / kSegmentMax * kSegmentMax;
     ng.max_excl = 16;
     const int nex = std::max(ex_cap_min, ngroups_max * 16 + 1024);
     const bool same = d_rows && ng.ncol == g.ncol && ng.col_cap == g.col_cap &&
                       ng.row_cap == g.row_cap && ng.max_clusters == g.max_clusters &&
-                      nex == ex_cap;
+                      nex == ex_cap && seg_len == seg_len_cfg;
     g = ng;
     list_reach = reach;
     if (same) return;
     max_groups = ngroups_max;
-    nseg = g.row_cap / kSegment;
+    nseg = g.row_cap / seg_len;
+    seg_len_cfg = seg_len;
     ex_cap = nex;
     release_list();
     max_places = g.max_clusters * kCluster;
@@ -865,7 +870,8 @@ struct Engine {
     const bool pruned = prune_skin > 0;
     nb_pairs<MOD, COUL, ENERGY><<<grid, 256, ntypes * ntypes * sizeof(float2), lstream>>>(
         d_xq, d_type_at, d_real_mask, pruned ? d_rows_in : d_rows,
-        pruned ? d_row_count_in : d_row_count, d_ex, g.row_cap, max_groups, d_counts, d_lj,
+        pruned ? d_row_count_in : d_row_count, d_ex, g.row_cap, seg_len, max_groups, d_counts,
+        d_lj,
         nbp, d_F, d_epair, d_flags, sa, fs);
   }
 
@@ -1408,63 +1414,82 @@ struct Engine {
   }
 
   // ---- launch-shape trial ----
-  // Times the launch shapes the PME chain can take on this system and keeps
-  // the fastest (the counterpart of the reference's launch-shape trial,
-  // tools/mdir/TowersTrial.h:13-33, Run.cpp:913-926). Candidates: bricks of
-  // 8, 4, 2 or 1 complex z lines (2 TL grid rows; those whose LDS fits) and,
-  // with `what` & 2, the plane transform (Stockham passes or the split-BF16x3
-  // MFMA products, where the grid allows them). Each candidate runs from the
-  // same state: one untimed graph of `steps` steps (list build, capture),
-  // then `graphs` graphs timed on the host clock around run(), overflow
-  // checks included; two passes over the candidates, the faster of each
-  // candidate's two times counting. The current shape is kept unless
+  // Times the launch shapes of this system's kernels and keeps the fastest
+  // (the counterpart of the reference's launch-shape trial,
+  // tools/mdir/TowersTrial.h:13-33, Run.cpp:913-926). Candidates: with
+  // `what` & 1, PME bricks of 8, 4, 2 or 1 complex z lines (2 TL grid rows;
+  // those whose LDS fits); with & 2, the plane transform (Stockham passes or
+  // the split-BF16x3 MFMA products, where the grid allows them); with & 4,
+  // pair-loop segments of 16, 24 or 48 entries. Each candidate runs from
+  // the same state: one untimed graph of `steps` steps (list build,
+  // capture), then `graphs` graphs timed on the host clock around run(),
+  // overflow checks included; two passes over the candidates, the faster of
+  // each candidate's two times counting. The current shape is kept unless
   // another is at least kTrialMargin faster. Positions and velocities are
-  // restored at the end. ms_out[4 * transform + k] = ms per step of
-  // TL = 8 >> k (NaN: not run). The shapes give the same forces up to the
-  // f32 rounding of the gather's per-brick partial sums (about 1e-8
-  // relative); a fixed choice is bit-exact run to run.
-  static constexpr int kTrialShapes = 8;
+  // restored at the end. ms_out[(8 * s + 4 * transform) + k] = ms per step
+  // with segments of kSegLens[s] and bricks of TL = 8 >> k (NaN: not run).
+  // The shapes give the same forces up to the f32 rounding of the gather's
+  // per-brick partial sums (about 1e-8 relative) and the order of the
+  // pair loop's per-wave energy partials; a fixed choice is bit-exact run
+  // to run.
+  static constexpr int kTrialShapes = 24;
+  static constexpr int kSegLens[3] = {16, 24, 48};
   static constexpr double kTrialMargin = 0.02;
-  static std::vector<std::pair<int, int>> trial_shapes(int what, int K2, bool mfma_ok, int tl0,
-                                                       int tr0) {
-    std::vector<std::pair<int, int>> c;  // (k: TL = 8 >> k, transform)
-    for (int tr = 0; tr < 2; ++tr) {
-      if (!(what & 2) && tr != tr0) continue;
-      if (tr == 1 && !mfma_ok) continue;
-      for (int k = 0; k < 4; ++k) {
-        const int tl = 8 >> k;
-        if (!(what & 1) && tl != tl0) continue;
-        if ((size_t)(32 * tl + 8) * K2 > 60 * 1024) continue;
-        c.push_back({k, tr});
+  struct Shape {
+    int s, tr, k;  // segment index, transform, brick index (TL = 8 >> k)
+    int slot() const { return 8 * s + 4 * tr + k; }
+  };
+  static int seg_index(int len) { return len == 16 ? 0 : (len == 24 ? 1 : 2); }
+  static std::vector<Shape> trial_shapes(int what, int K2, bool mfma_ok, int tl0, int tr0,
+                                         int seg0) {
+    std::vector<Shape> c;
+    for (int si = 0; si < 3; ++si) {
+      if (!(what & 4) && kSegLens[si] != seg0) continue;
+      for (int tr = 0; tr < 2; ++tr) {
+        if (!(what & 2) && tr != tr0) continue;
+        if (tr == 1 && !mfma_ok) continue;
+        for (int k = 0; k < 4; ++k) {
+          const int tl = 8 >> k;
+          if (!(what & 1) && tl != tl0) continue;
+          if (K2 > 0 && (size_t)(32 * tl + 8) * K2 > 60 * 1024) continue;
+          c.push_back({si, tr, k});
+        }
       }
     }
     return c;
   }
-  // Runs measure(k, transform) for every candidate in two passes, fills
-  // ms_out with the faster time of each, and returns the candidate to keep.
+  // Runs measure(shape) for every candidate in two passes, fills ms_out
+  // with the faster time of each, and returns the shape to keep.
   template <class Measure>
-  static std::pair<int, int> trial(const std::vector<std::pair<int, int>> &cands, int tl0,
-                                   int tr0, Measure measure, double *ms_out) {
+  static Shape trial(const std::vector<Shape> &cands, Shape cur, Measure measure,
+                     double *ms_out) {
     for (int pass = 0; pass < 2; ++pass)
-      for (const auto &c : cands) {
-        const double ms = measure(8 >> c.first, c.second);
-        double &slot = ms_out[4 * c.second + c.first];
+      for (const Shape &c : cands) {
+        const double ms = measure(c);
+        double &slot = ms_out[c.slot()];
         slot = std::isnan(slot) ? ms : std::min(slot, ms);
       }
-    std::pair<int, int> best{-1, -1}, cur{-1, -1};
-    for (const auto &c : cands) {
-      const double ms = ms_out[4 * c.second + c.first];
-      if (best.first < 0 || ms < ms_out[4 * best.second + best.first]) best = c;
-      if ((8 >> c.first) == tl0 && c.second == tr0) cur = c;
+    Shape best = cur;
+    bool have = false, cur_in = false;
+    for (const Shape &c : cands) {
+      if (!have || ms_out[c.slot()] < ms_out[best.slot()]) best = c;
+      have = true;
+      if (c.slot() == cur.slot()) cur_in = true;
     }
-    if (cur.first >= 0 && ms_out[4 * best.second + best.first] >
-                              (1.0 - kTrialMargin) * ms_out[4 * cur.second + cur.first])
-      best = cur;
+    if (cur_in && ms_out[best.slot()] > (1.0 - kTrialMargin) * ms_out[cur.slot()]) best = cur;
     return best;
+  }
+  // A segment length: the list layout follows (reallocated, rebuilt at the
+  // next evaluation).
+  void set_segment(int len) {
+    if (len != 16 && len != 24 && len != 48)
+      throw HipError("pair-loop segments: 16, 24 or 48 entries");
+    seg_len = len;
+    if (d_rows && seg_len_cfg != len) configure_list(list_reach, g.col_cap, g.row_cap, ex_cap);
   }
   void autotune(int what, int steps, int graphs, double dt, double *ms_out) {
     for (int k = 0; k < kTrialShapes; ++k) ms_out[k] = NAN;
-    if (!use_pme || what <= 0 || steps <= 0 || graphs <= 0) return;
+    if (what <= 0 || steps <= 0 || graphs <= 0) return;
     const size_t bytes = 3 * (size_t)n * sizeof(double);
     std::vector<double> x0(3 * (size_t)n), v0(3 * (size_t)n);
     to_host(x0.data(), d_x, bytes);
@@ -1475,17 +1500,23 @@ struct Engine {
       HIPCHK(hipStreamSynchronize(stream));
       forces_valid = false;
     };
-    const bool mfma_ok = plane_fft() && mfma_edge(pme_grid[0]) && mfma_edge(pme_grid[1]);
-    const int tl0 = brick_tl, tr0 = pme_transform;
-    auto shape = [&](int tl, int tr) {
-      brick_tl_max = tl;
-      make_bricks();
-      set_pme_transform(tr);
+    if (!use_pme) what &= 4;  // no PME shapes to try
+    const bool mfma_ok =
+        use_pme && plane_fft() && mfma_edge(pme_grid[0]) && mfma_edge(pme_grid[1]);
+    const int tl0 = use_pme ? brick_tl : 8, tr0 = pme_transform;
+    const Shape cur{seg_index(seg_len), tr0, tl0 == 8 ? 0 : (tl0 == 4 ? 1 : (tl0 == 2 ? 2 : 3))};
+    auto apply = [&](const Shape &c) {
+      set_segment(kSegLens[c.s]);
+      if (use_pme) {
+        brick_tl_max = 8 >> c.k;
+        make_bricks();
+        set_pme_transform(c.tr);
+      }
     };
-    const auto keep = trial(
-        trial_shapes(what, pme_grid[2], mfma_ok, tl0, tr0), tl0, tr0,
-        [&](int tl, int tr) {
-          shape(tl, tr);
+    const Shape keep = trial(
+        trial_shapes(what, use_pme ? pme_grid[2] : 0, mfma_ok, tl0, tr0, seg_len), cur,
+        [&](const Shape &c) {
+          apply(c);
           restore();
           run(steps, dt, steps, nullptr);
           const auto t0 = std::chrono::steady_clock::now();
@@ -1495,7 +1526,7 @@ struct Engine {
                  ((double)steps * graphs);
         },
         ms_out);
-    shape(keep.first >= 0 ? 8 >> keep.first : tl0, keep.first >= 0 ? keep.second : tr0);
+    apply(keep);
     restore();
   }
 
@@ -1610,7 +1641,7 @@ struct Engine {
     int64_t tiles = 0, largest = 0;
     for (int gi = 0; gi < counts[1] / kGroup; ++gi) {
       for (int e = 0; e < rc[gi]; ++e)
-        tiles += __builtin_popcount((rows[(size_t)gi * g.row_cap + e].info >> 8) & 0xFu);
+        tiles += __builtin_popcount((rows[(size_t)gi * g.row_cap + e].info >> 8) & ((1u << kGroup) - 1u));
       largest = std::max<int64_t>(largest, rc[gi]);
     }
     s[0] = counts[0];
@@ -1632,7 +1663,7 @@ struct Engine {
       int64_t t = 0;
       for (int gi = 0; gi < counts[1] / kGroup; ++gi)
         for (int e = 0; e < rci[gi]; ++e)
-          t += __builtin_popcount((rin[(size_t)gi * g.row_cap + e].info >> 8) & 0xFu);
+          t += __builtin_popcount((rin[(size_t)gi * g.row_cap + e].info >> 8) & ((1u << kGroup) - 1u));
       s[8] = t;
     }
   }

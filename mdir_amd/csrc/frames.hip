@@ -89,6 +89,7 @@ struct FrameBatch {
     dd.prune_skin = 0.0;
     dd.nwaters = 0;       // no integration
     e.pme_atoms = false;  // the adjoint reads PME inputs in place order
+    e.seg_len = 48;       // many frames fill the GPU: long segments
     e.create(&dd);
     max_frames = frames;
     // per-type parameters: the diagonal of the tables, which must follow
@@ -392,7 +393,7 @@ struct FrameBatch {
     const size_t lds = (size_t)e.ntypes * e.ntypes * (sizeof(float4) + sizeof(float2) + sizeof(int));
     nb_adjoint<MOD, COUL, KIND><<<grid, 256, lds, e.stream>>>(
         e.d_xq, e.d_type_at, e.d_real_mask, e.d_rows, e.d_row_count, e.d_ex, e.g.row_cap,
-        e.max_groups, e.d_counts, tab, ap, d_cot, d_meta, d_A, parts ? d_vpart : nullptr,
+        e.seg_len, e.max_groups, e.d_counts, tab, ap, d_cot, d_meta, d_A, parts ? d_vpart : nullptr,
         KIND == kAdjVirial ? d_vspec : nullptr, sa, e.fs);
   }
   template <int KIND>
@@ -656,48 +657,56 @@ struct FrameBatch {
     HIPCHK(hipStreamSynchronize(e.stream));
   }
 
-  // A PME launch shape for every frame: the brick width (TL complex z
-  // lines, at most) and the plane transform. The per-frame brick lists live
-  // in the frame arena, so it is laid out again around the change. The
-  // evaluator holds no evaluated frames afterwards.
-  void set_shape(int tl, int transform) {
+  // A launch shape for every frame: the PME brick width (TL complex z
+  // lines, at most), the plane transform and the pair-loop segment length.
+  // The per-frame list and brick buffers live in the frame arena, so it is
+  // laid out again around the change. The evaluator holds no evaluated
+  // frames afterwards.
+  void set_shape(int tl, int transform, int seg) {
     if (fm_graph) hipGraphExecDestroy(fm_graph);
     fm_graph = nullptr;
     e.unbatch();
     for (void **p : e.frame_extra) dfree(*p);
     e.frame_extra.clear();
-    e.brick_tl_max = tl;
-    e.make_bricks();
+    e.set_segment(seg);
+    if (e.use_pme) {
+      e.brick_tl_max = tl;
+      e.make_bricks();
+    }
     alloc_extra();
     e.batch(max_frames);
-    e.set_pme_transform(transform);
+    if (e.use_pme) e.set_pme_transform(transform);
     loaded = 0;
   }
 
   // Launch-shape trial of the frame evaluator (the engine's autotune, for
   // the frames): times the graph-replayed force-matching call on the
-  // caller's `frames` frames for every PME brick shape that fits LDS (what &
-  // 1) and plane transform (what & 2), one untimed call (list build,
-  // capture) and `repeats` timed ones each, in two passes (the faster
-  // counts), and keeps the fastest unless the current shape is within the
-  // trial margin. ms_out[4 * transform + k] = ms per call of TL = 8 >> k
-  // (NaN: not run).
+  // caller's `frames` frames for every shape `what` names (bit 0: PME brick
+  // widths that fit LDS, bit 1: plane transforms, bit 2: pair-loop segment
+  // lengths), one untimed call (list build, capture) and `repeats` timed
+  // ones each, in two passes (the faster counts), and keeps the fastest
+  // unless the current shape is within the trial margin. ms_out as the
+  // engine's autotune, in ms per call.
   void autotune(int what, int frames, const double *positions, const double *f_ref,
                 int repeats, double *ms_out) {
-    for (int k = 0; k < 8; ++k) ms_out[k] = NAN;
+    for (int k = 0; k < Engine::kTrialShapes; ++k) ms_out[k] = NAN;
     check_frames(frames);
-    if (!e.use_pme || what <= 0 || repeats <= 0) return;
-    const bool mfma_ok =
-        e.plane_fft() && Engine::mfma_edge(e.pme_grid[0]) && Engine::mfma_edge(e.pme_grid[1]);
-    const int tl0 = e.brick_tl, tr0 = e.pme_transform;
+    if (what <= 0 || repeats <= 0) return;
+    if (!e.use_pme) what &= 4;
+    const bool mfma_ok = e.use_pme && e.plane_fft() && Engine::mfma_edge(e.pme_grid[0]) &&
+                         Engine::mfma_edge(e.pme_grid[1]);
+    const int tl0 = e.use_pme ? e.brick_tl : 8, tr0 = e.pme_transform;
+    const Engine::Shape cur{Engine::seg_index(e.seg_len), tr0,
+                            tl0 == 8 ? 0 : (tl0 == 4 ? 1 : (tl0 == 2 ? 2 : 3))};
     std::vector<double> loss(frames), dq(e.n), ds(e.ntypes), de(e.ntypes);
     auto call = [&]() {
       force_matching(frames, positions, f_ref, loss.data(), dq.data(), ds.data(), de.data(), 1);
     };
-    const auto keep = Engine::trial(
-        Engine::trial_shapes(what, e.pme_grid[2], mfma_ok, tl0, tr0), tl0, tr0,
-        [&](int tl, int tr) {
-          set_shape(tl, tr);
+    const Engine::Shape keep = Engine::trial(
+        Engine::trial_shapes(what, e.use_pme ? e.pme_grid[2] : 0, mfma_ok, tl0, tr0, e.seg_len),
+        cur,
+        [&](const Engine::Shape &c) {
+          set_shape(8 >> c.k, c.tr, Engine::kSegLens[c.s]);
           call();
           const auto t0 = std::chrono::steady_clock::now();
           for (int r = 0; r < repeats; ++r) call();
@@ -705,7 +714,7 @@ struct FrameBatch {
           return std::chrono::duration<double, std::milli>(t1 - t0).count() / repeats;
         },
         ms_out);
-    set_shape(keep.first >= 0 ? 8 >> keep.first : tl0, keep.first >= 0 ? keep.second : tr0);
+    set_shape(8 >> keep.k, keep.tr, Engine::kSegLens[keep.s]);
   }
 
   // Forward, residual, adjoint and the sum over frames, as one graph per

@@ -41,7 +41,9 @@ enum Flag { kFlagRebuild = 0, kFlagOverflowColumn = 1, kFlagOverflowRow = 2,
             kFlagMaskCount = 6, kFlagPrune = 7, kFlagPrunes = 8, kNumFlags = 16 };
 
 constexpr int kGroup = 4;            // i-clusters of a group (one row)
-constexpr int kSegment = 48;         // row entries one wave of the pair loop takes
+// Row entries one wave of the pair loop takes: a launch shape of 16, 24 or
+// 48 entries (row capacities are multiples of the largest, LDS sized for it).
+constexpr int kSegmentMax = 48;
 
 // One entry of a group row: j cluster cj against the i clusters of the
 // group whose bits are set in imask.  The interaction mask of a tile is

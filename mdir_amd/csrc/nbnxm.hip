@@ -322,7 +322,10 @@ __global__ void __launch_bounds__(256) nb_search(
       ghi.z = fmaxf(ghi.z, mhi[m].z);
     }
   }
-  const bool any_real = (mreal[0] | mreal[1] | mreal[2] | mreal[3]) != 0u;
+  uint32_t group_real = 0u;
+#pragma unroll
+  for (int m = 0; m < kGroup; ++m) group_real |= mreal[m];
+  const bool any_real = group_real != 0u;
   const int icol = cluster_col[g0];
   const int cx = icol % g.ncx, cy = icol / g.ncx;
   const float Lx = (float)g.box[0], Ly = (float)g.box[1], Lz = (float)g.box[2];
@@ -637,7 +640,7 @@ __global__ void __launch_bounds__(256) nb_prune(
     const uint64_t ballot = __ballot(keep);
     if (keep) {
       PairEntry o = E;
-      o.info = (E.info & ~0xF00u) | (inner << 8);
+      o.info = (E.info & ~(((1u << kGroup) - 1u) << 8)) | (inner << 8);
       out[count + __popcll(ballot & ((1ull << lane) - 1ull))] = o;
     }
     count += __popcll(ballot);

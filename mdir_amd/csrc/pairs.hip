@@ -8,7 +8,7 @@
 // (TermPairs14.cpp:11-17).
 //
 // Pair kernel: one wave per segment of a group row (kGroup i-clusters against
-// up to kSegment j-clusters).  Lane l holds i atom a = l & 7 and j atom
+// up to a segment's j-clusters).  Lane l holds i atom a = l & 7 and j atom
 // b = l >> 3 of each 8 x 8 tile, so a tile is one instruction stream with one
 // pair per lane.  The force on j is accumulated over the members an entry
 // pairs with, summed over the 8 lanes of a column by three DPP row operations
@@ -255,14 +255,14 @@ __device__ __forceinline__ void nb_pairs_body(
     const float4 *__restrict__ xq, const int *__restrict__ type_at,
     const uint32_t *__restrict__ real_mask, const PairEntry *__restrict__ rows,
     const int *__restrict__ row_count, const uint64_t *__restrict__ ex_masks,
-    int row_cap, int max_groups, const int *__restrict__ counts_in,
+    int row_cap, int seg_len, int max_groups, const int *__restrict__ counts_in,
     const float2 *__restrict__ ljtab, const NbParams &p, long long *__restrict__ F,
     double *__restrict__ energy_partial, int *__restrict__ flags, const SpecialArgs &sa) {
   extern __shared__ float2 s_lj[];                   // ntypes^2 (launch size)
   __shared__ union {
     struct {
-      float fj[4][kSegment * kCluster * 3];          // j forces of the entries
-      int cj[4][kSegment];
+      float fj[4][kSegmentMax * kCluster * 3];       // j forces of the entries
+      int cj[4][kSegmentMax];
     } pr;
     double e[3][256];                                // special-pair energies
   } sm;
@@ -282,10 +282,10 @@ __device__ __forceinline__ void nb_pairs_body(
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int gi = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wave);
   const int seg = blockIdx.y - 1;
-  const int e0 = seg * kSegment;
+  const int e0 = seg * seg_len;
   const int ngroups = counts_in[1] / kGroup;
   int n = gi < ngroups ? row_count[gi] - e0 : 0;
-  n = __builtin_amdgcn_readfirstlane(n < kSegment ? n : kSegment);
+  n = __builtin_amdgcn_readfirstlane(n < seg_len ? n : seg_len);
   const size_t slot = (size_t)seg * max_groups + gi;
   if (n <= 0) {
     if (ENERGY && gi < max_groups && lane == 0) {
@@ -414,7 +414,7 @@ __global__ void __launch_bounds__(256) nb_pairs(
     const float4 *__restrict__ xq, const int *__restrict__ type_at,
     const uint32_t *__restrict__ real_mask, const PairEntry *__restrict__ rows,
     const int *__restrict__ row_count, const uint64_t *__restrict__ ex_masks,
-    int row_cap, int max_groups, const int *__restrict__ counts_in,
+    int row_cap, int seg_len, int max_groups, const int *__restrict__ counts_in,
     const float2 *__restrict__ ljtab, NbParams p, long long *__restrict__ F,
     double *__restrict__ energy_partial, int *__restrict__ flags, SpecialArgs sa,
     size_t fs) {
@@ -432,7 +432,7 @@ __global__ void __launch_bounds__(256) nb_pairs(
   sa.place_of = frame_at(sa.place_of, fs);
   sa.energy_partial = frame_at(sa.energy_partial, fs);
   nb_pairs_body<MOD, COUL, ENERGY>(xq, type_at, real_mask, rows, row_count, ex_masks, row_cap,
-                                   max_groups, counts_in, ljtab, p, F, energy_partial, flags,
+                                   seg_len, max_groups, counts_in, ljtab, p, F, energy_partial, flags,
                                    sa);
 }
 
@@ -447,7 +447,7 @@ __global__ void stamp_clock(unsigned long long *__restrict__ slot) {
 // Instantiations used by the engine.
 #define MDHIP_PAIRS_ONE(MOD, C, E)                                                       \
   template __global__ void nb_pairs<MOD, C, E>(const float4 *, const int *,            \
-      const uint32_t *, const PairEntry *, const int *, const uint64_t *, int, int,    \
+      const uint32_t *, const PairEntry *, const int *, const uint64_t *, int, int, int, \
       const int *, const float2 *, NbParams, long long *, double *, int *, SpecialArgs,     \
       size_t);
 #define MDHIP_PAIRS(MOD)                                                                 \

@@ -289,25 +289,27 @@ class FrameEvaluator:
         self._loaded = K
         return loss, FrameGradient({"charges": dq, "sigma": ds, "epsilon": de})
 
-    def autotune(self, positions, reference_forces, what: int = 1, repeats: int = 5) -> dict:
-        """Times force_matching on these frames for each PME launch shape
-        (bit 0 of what: brick widths, bit 1: plane transforms) and keeps the
-        fastest; returns {(brick z lines, transform): ms per call}. The
-        evaluator holds no evaluated frames afterwards."""
+    def autotune(self, positions, reference_forces, what: int = 7, repeats: int = 5) -> dict:
+        """Times force_matching on these frames for each launch shape (bit 0
+        of what: PME brick widths, bit 1: plane transforms, bit 2: pair-loop
+        segment lengths) and keeps the fastest; returns {(brick z lines,
+        transform, segment entries): ms per call}. The evaluator holds no
+        evaluated frames afterwards."""
         K = self._count(positions)
         pp, keep, _ = _frame_array(positions, (K, self.n, 3), "positions")
         rp, rkeep, _ = _frame_array(reference_forces, (K, self.n, 3), "reference_forces")
-        ms = np.full(8, np.nan)
+        ms = np.full(24, np.nan)
         rt.check(rt.lib().mdrtHipFramesAutotune(self._h, int(what), K, pp, rp, int(repeats),
                                                 rt.ptr(ms, C.c_double)))
         del keep, rkeep
         self._version += 1
         self._loaded = 0
-        return {(8 >> (k % 4), k // 4): float(v) for k, v in enumerate(ms) if np.isfinite(v)}
+        from .simulation import shape_table
+        return shape_table(ms)
 
 
 def force_matching_benchmark(nframes: int = 64, repeats: int = 20, loop_passes: int = 3,
-                             autotune: int = 3):
+                             autotune: int = 7):
     """Force-matching frame-gradients per second on the differentiation
     workload (64 frames of 3,150 atoms: 1,000 TIP3P waters and 30 five-site
     Lennard-Jones solutes of 4 types, PME 32^3, rc 0.9 nm): one call
@@ -395,8 +397,8 @@ def force_matching_benchmark(nframes: int = 64, repeats: int = 20, loop_passes: 
                                     "epsilon": system.ntypes},
                        "repeats": repeats, "loop_passes": loop_passes},
             "gradient_norm": float(np.sqrt(sum(float((g * g).sum()) for g in grad_batched))),
-            "autotune_ms_per_call": {f"bricks{tl}_transform{tr}": round(v, 4)
-                                     for (tl, tr), v in sorted(tuned.items())}}
+            "autotune_ms_per_call": {f"seg{sg}_bricks{tl}_transform{tr}": round(v, 4)
+                                     for (tl, tr, sg), v in sorted(tuned.items())}}
 
 
 if __name__ == "__main__":
@@ -406,8 +408,9 @@ if __name__ == "__main__":
     p.add_argument("--frames", type=int, default=64)
     p.add_argument("--repeats", type=int, default=20)
     p.add_argument("--loop-passes", type=int, default=3)
-    p.add_argument("--autotune", type=int, default=3,
-                   help="0 none, 1 PME brick shapes, 3 brick shapes and plane transforms")
+    p.add_argument("--autotune", type=int, default=7,
+                   help="launch-shape trial: bit 0 PME brick shapes, bit 1 plane transforms, "
+                        "bit 2 pair-loop segment lengths (0: none)")
     a = p.parse_args()
     print(json.dumps(force_matching_benchmark(a.frames, a.repeats, a.loop_passes, a.autotune)),
           flush=True)

@@ -15,6 +15,13 @@ from . import runtime as rt
 from .systems import System
 
 
+def shape_table(ms) -> dict:
+    """{(brick z lines, transform, segment entries): time} from the trial's
+    [segment][transform][brick] array (NaN: not run)."""
+    return {(8 >> (k % 4), (k // 4) % 2, Simulation.SEGMENTS[k // 8]): float(v)
+            for k, v in enumerate(ms) if np.isfinite(v)}
+
+
 class Simulation:
     def __init__(self, system: System, skin: float = 0.1, prune_skin: float = 0.0):
         self.system = system
@@ -88,17 +95,19 @@ class Simulation:
 
     TUNE_BRICKS = 1
     TUNE_TRANSFORM = 2
+    TUNE_SEGMENTS = 4
+    SEGMENTS = (16, 24, 48)
 
-    def autotune(self, what: int = TUNE_BRICKS, steps: int = 50, graphs: int = 4,
-                 dt: float = 0.002) -> dict:
-        """Times the PME launch shapes on this system and keeps the fastest
-        (the launch-shape trial of tools/mdir/TowersTrial.h); positions and
-        velocities are restored. Returns {(brick z lines, transform): ms per
-        step} for the shapes run."""
-        ms = np.full(8, np.nan)
+    def autotune(self, what: int = TUNE_BRICKS | TUNE_TRANSFORM | TUNE_SEGMENTS,
+                 steps: int = 50, graphs: int = 4, dt: float = 0.002) -> dict:
+        """Times the launch shapes on this system and keeps the fastest (the
+        launch-shape trial of tools/mdir/TowersTrial.h); positions and
+        velocities are restored. Returns {(brick z lines, transform, segment
+        entries): ms per step} for the shapes run."""
+        ms = np.full(24, np.nan)
         rt.check(rt.lib().mdrtHipAutotune(self._h, int(what), int(steps), int(graphs),
                                           float(dt), rt.ptr(ms, C.c_double)))
-        return {(8 >> (k % 4), k // 4): float(v) for k, v in enumerate(ms) if np.isfinite(v)}
+        return shape_table(ms)
 
     def energies(self) -> dict:
         _, e = self.forces()

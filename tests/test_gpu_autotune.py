@@ -27,7 +27,8 @@ def water(grid):
 
 @pytest.mark.parametrize("grid,what", [((24, 24, 24), Simulation.TUNE_BRICKS),
                                        ((32, 32, 32), Simulation.TUNE_BRICKS
-                                        | Simulation.TUNE_TRANSFORM)])
+                                        | Simulation.TUNE_TRANSFORM
+                                        | Simulation.TUNE_SEGMENTS)])
 def test_autotune_keeps_state_and_forces(grid, what):
     say(f"case: autotune TIP3P 512 waters, grid {grid}, what {what}")
     w = water(grid)
@@ -38,9 +39,14 @@ def test_autotune_keeps_state_and_forces(grid, what):
     times = sim.autotune(what, steps=20, graphs=2)
     say(f"  ms per step: {times}")
     # every brick shape that fits a z edge of 24 or 32 (8 z lines at most)
-    assert {tl for tl, _ in times} == {8, 4, 2, 1}
+    assert {tl for tl, _, _ in times} == {8, 4, 2, 1}
     if what & Simulation.TUNE_TRANSFORM:
-        assert {tr for _, tr in times} == {0, 1}
+        assert {tr for _, tr, _ in times} == {0, 1}
+    if what & Simulation.TUNE_SEGMENTS:
+        assert {sg for _, _, sg in times} == {16, 24, 48}
+        assert len(times) == 24
+    else:
+        assert {sg for _, _, sg in times} == {16}
     assert all(t > 0 for t in times.values())
     # positions and velocities restored bit for bit
     assert np.array_equal(sim.positions, x0)
@@ -75,13 +81,15 @@ def test_frames_autotune():
     Fref = ev.evaluate(X).forces
     ev.update(theta["charges"], theta["sigma"], theta["epsilon"])
     loss0, g0 = ev.force_matching(X, Fref)
-    times = ev.autotune(X, Fref, what=3, repeats=2)
+    times = ev.autotune(X, Fref, what=7, repeats=2)
     say(f"  ms per call: {times}")
-    assert {tl for tl, _ in times} == {8, 4, 2, 1}
-    assert {tr for _, tr in times} == {0, 1}
+    assert {tl for tl, _, _ in times} == {8, 4, 2, 1}
+    assert {tr for _, tr, _ in times} == {0, 1}
+    assert {sg for _, _, sg in times} == {16, 24, 48}
     loss1, g1 = ev.force_matching(X, Fref)
     # brick shapes differ by the f32 rounding of per-brick gather sums, the
-    # MFMA transform (if kept) by its split-BF16x3 error
+    # MFMA transform (if kept) by its split-BF16x3 error, segment lengths by
+    # the order of the per-wave partial sums
     assert np.allclose(loss1, loss0, rtol=1e-4)
     for name in ("charges", "sigma", "epsilon"):
         a, b = g1[name], g0[name]
