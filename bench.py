@@ -126,8 +126,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
-              "(every box here has one MI355X; the path shards as described in "
-              "DESIGN.md)", file=sys.stderr)
+              "(the force path runs on one MI355X; several GPUs would shard it by "
+              "slabs of xy columns, which is not built)", file=sys.stderr)
         sys.exit(2)
     from mdir_amd import systems as S
     from mdir_amd.simulation import Simulation

@@ -1,7 +1,7 @@
 // engine.hip — the per-step force path on one MI355X: buffers, launch order,
 // step graphs and the C entry points of include/mdrt_hip.h (group 2).
 //
-// Step order (the reference's, 08-gpu.md:112-121 and SURVEY call stack):
+// Step order (the reference's, 08-gpu.md:112-121):
 //   integrate_first (half kick, drift, SETTLE, f32 positions, validity)
 //   nb_bin / nb_sort / nb_search   (no-ops unless the validity test failed)
 //   nb_pairs                       (LJ + erfc Ewald, 64-bit force words)

@@ -1,6 +1,6 @@
 """Parity of the HIP force path with the FP64 oracle (through the C ABI).
 
-Bar (north star): relative RMS force error <= 5e-6 against FP64 for the
+Bar: relative RMS force error <= 5e-6 against FP64 for the
 mixed-precision path, energies per term to the stated tolerances below,
 pair sets bit-exact."""
 import numpy as np
