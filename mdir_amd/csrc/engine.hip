@@ -2,14 +2,20 @@
 // step graphs and the C entry points of include/mdrt_hip.h (group 2).
 //
 // Step order (the reference's, 08-gpu.md:112-121):
-//   integrate_first (half kick, drift, SETTLE, f32 positions, validity)
-//   nb_bin / nb_sort / nb_search   (no-ops unless the validity test failed)
-//   nb_pairs                       (LJ + erfc Ewald, 64-bit force words)
-//   nb_special                     (excluded-pair correction, 1-4 pairs)
-//   pme_spread, fft z/y, fft x + influence + inverse x, fft y/z, pme_gather
-//   integrate_second               (half kick, velocity constraint)
-// All kernels are launched on one stream, so a run of steps captures into
-// one HIP graph and replays without host decisions.
+//   integrate (second half of a step and first half of the next: half
+//   kicks, drift, SETTLE, f32 positions by place, f64 positions by atom for
+//   PME, the validity test), then two branches joined before the next one:
+//   - the list branch: nb_bin / nb_sort / nb_search (no-ops unless the
+//     validity test failed), nb_pairs (LJ + erfc Ewald, 64-bit force words;
+//     its first plane takes the excluded-pair correction and 1-4 pairs);
+//   - the PME branch, in atom order: pme_bin, pme_spread_rows (spread + z
+//     transform), fft_plane_conv (y and x transforms, influence, inverse),
+//     pme_gather_rows (inverse z transform + gather into its own words).
+// The branches run on two streams with default priorities, forked and
+// joined by events, so a run of steps captures into one HIP graph and
+// replays without host decisions. Launch shapes (segment length, PME
+// bricks, plane transform) are runtime choices the launch-shape trial
+// (autotune) times on the system.
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
