@@ -247,6 +247,18 @@ int mdrtHipComputeForceTerms(void *system, int32_t terms, double *forces, double
    cuFFT plan choice (mdrt_cuda.c:2646-2991) with this library's own. */
 int mdrtHipSetPmeTransform(void *system, int32_t mode);
 
+/* One launch shape, set directly (as the trial below would keep it):
+   pair-loop segments of 16, 24 or 48 row entries, PME bricks of 8, 4, 2 or
+   1 complex z lines (those that fit LDS at this grid), plane transform 0 or
+   1 (as mdrtHipSetPmeTransform). The list is rebuilt at the next
+   evaluation. The counterpart of a launch shape chosen in the reference's
+   TOWERS set-up (tools/mdir/TowersShape.h). */
+int mdrtHipSetLaunchShape(void *system, int32_t segment, int32_t bricks, int32_t transform);
+
+/* The current shape: shape[3] = {segment entries, brick z lines (0 without
+   PME), transform}. */
+int mdrtHipGetLaunchShape(void *system, int32_t *shape);
+
 /* Launch-shape trial: times the launch shapes of this system's kernels
    and keeps the fastest, replacing the reference's TOWERS launch-shape
    trial and its step-graph trial (tools/mdir/TowersTrial.h:13-33,
@@ -261,7 +273,7 @@ int mdrtHipSetPmeTransform(void *system, int32_t mode);
    ms_per_step[24]: [8 * s + 4 * transform + k] for segments of 16, 24, 48
    (s = 0, 1, 2) and bricks of TL = 8 >> k, NaN for shapes not run. The
    shapes give the same forces up to the f32 rounding of the gather's
-   per-brick partial sums (about 1e-8 relative). */
+   per-brick partial sums (up to about 1e-7 relative). */
 int mdrtHipAutotune(void *system, int32_t what, int32_t steps, int32_t graphs, double dt,
                     double *ms_per_step);
 

@@ -1435,7 +1435,7 @@ struct Engine {
   // restored at the end. ms_out[(8 * s + 4 * transform) + k] = ms per step
   // with segments of kSegLens[s] and bricks of TL = 8 >> k (NaN: not run).
   // The shapes give the same forces up to the f32 rounding of the gather's
-  // per-brick partial sums (about 1e-8 relative) and the order of the
+  // per-brick partial sums (up to about 1e-7 relative) and the order of the
   // pair loop's per-wave energy partials; a fixed choice is bit-exact run
   // to run.
   static constexpr int kTrialShapes = 24;
@@ -1493,6 +1493,30 @@ struct Engine {
     seg_len = len;
     if (d_rows && seg_len_cfg != len) configure_list(list_reach, g.col_cap, g.row_cap, ex_cap);
   }
+  // One launch shape, set directly (what the trial keeps can be set again
+  // without timing): segment entries 16, 24 or 48; PME bricks of 8, 4, 2 or
+  // 1 z lines that fit LDS; plane transform 0 or 1 where the grid allows.
+  void set_launch_shape(int seg, int tl, int tr) {
+    if (tl != 8 && tl != 4 && tl != 2 && tl != 1) throw HipError("PME bricks: 8, 4, 2 or 1 z lines");
+    if (use_pme && (size_t)(32 * tl + 8) * pme_grid[2] > 60 * 1024)
+      throw HipError("PME bricks of that many z lines exceed LDS at this grid");
+    set_segment(seg);
+    if (use_pme) {
+      brick_tl_max = tl;
+      make_bricks();
+      set_pme_transform(tr);
+    } else if (tr != 0) {
+      throw HipError("no PME: the plane transform is 0");
+    }
+    forces_valid = false;
+    invalidate_graph();
+  }
+  void get_launch_shape(int32_t *out) const {
+    out[0] = seg_len;
+    out[1] = use_pme ? brick_tl : 0;
+    out[2] = pme_transform;
+  }
+
   void autotune(int what, int steps, int graphs, double dt, double *ms_out) {
     for (int k = 0; k < kTrialShapes; ++k) ms_out[k] = NAN;
     if (what <= 0 || steps <= 0 || graphs <= 0) return;
@@ -1914,6 +1938,16 @@ int mdrtHipComputeForceTerms(void *system, int32_t terms, double *forces, double
 int mdrtHipSetPmeTransform(void *system, int32_t mode) {
   Engine *e = static_cast<Engine *>(system);
   return guarded([&] { e->set_pme_transform(mode); });
+}
+
+int mdrtHipSetLaunchShape(void *system, int32_t segment, int32_t bricks, int32_t transform) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] { e->set_launch_shape(segment, bricks, transform); });
+}
+
+int mdrtHipGetLaunchShape(void *system, int32_t *shape) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] { e->get_launch_shape(shape); });
 }
 
 int mdrtHipAutotune(void *system, int32_t what, int32_t steps, int32_t graphs, double dt,

@@ -118,6 +118,8 @@ SIGNATURES = [
     ("mdrtHipComputeForceTerms", C.c_int, [C.c_void_p, I32, PD, PD]),
     ("mdrtHipSetPmeTransform", C.c_int, [C.c_void_p, I32]),
     ("mdrtHipAutotune", C.c_int, [C.c_void_p, I32, I32, I32, D, PD]),
+    ("mdrtHipSetLaunchShape", C.c_int, [C.c_void_p, I32, I32, I32]),
+    ("mdrtHipGetLaunchShape", C.c_int, [C.c_void_p, PI32]),
     ("mdrtHipFramesSetPmeTransform", C.c_int, [C.c_void_p, I32]),
     ("mdrtHipRun", C.c_int, [C.c_void_p, I64, D, I32, PD]),
     ("mdrtHipRunMeasured", C.c_int, [C.c_void_p, I64, D, I32, PD]),

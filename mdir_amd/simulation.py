@@ -98,6 +98,18 @@ class Simulation:
     TUNE_SEGMENTS = 4
     SEGMENTS = (16, 24, 48)
 
+    @property
+    def launch_shape(self) -> tuple:
+        """(segment entries, PME brick z lines, plane transform)."""
+        out = np.zeros(3, np.int32)
+        rt.check(rt.lib().mdrtHipGetLaunchShape(self._h, rt.ptr(out, C.c_int32)))
+        return tuple(int(v) for v in out)
+
+    @launch_shape.setter
+    def launch_shape(self, shape):
+        seg, bricks, transform = shape
+        rt.check(rt.lib().mdrtHipSetLaunchShape(self._h, int(seg), int(bricks), int(transform)))
+
     def autotune(self, what: int = TUNE_BRICKS | TUNE_TRANSFORM | TUNE_SEGMENTS,
                  steps: int = 50, graphs: int = 4, dt: float = 0.002) -> dict:
         """Times the launch shapes on this system and keeps the fastest (the
