@@ -1700,7 +1700,7 @@ struct Engine {
 
   // Device-event timing of every kernel of one step, on the stream the
   // kernels run on: ms per launch, averaged over `repeats` steps.
-  //  0 integrate_first  1 nb_bin  2 nb_sort  3 nb_search
+  //  0 integrate_first  1 nb_bin  2 nb_sort  3 nb_search (+ nb_prune with a pruned list)
   //  4 nb_pairs (with the excluded and 1-4 pairs)  5 pme_bin
   //  6 pme_spread_rows (spread + forward z transform)
   //  7 pme_yx (y and x transforms and the influence product)
@@ -1722,6 +1722,7 @@ struct Engine {
       launch_sort();
       HIPCHK(hipEventRecord(ev[k++], stream));
       launch_search();
+      launch_prune();  // the pruned rows the pair kernel reads (if any)
       HIPCHK(hipEventRecord(ev[k++], stream));
       pairs(false);
       HIPCHK(hipEventRecord(ev[k++], stream));

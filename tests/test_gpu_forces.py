@@ -121,6 +121,27 @@ def test_chain_box_scaled_pairs(modifier):
         assert abs(e[k] - eo[k]) <= bar, (k, e[k], eo[k])
 
 
+def test_profile_step_follows_the_dynamics():
+    """Kernel-by-kernel profiling (mdrtHipProfileStep) advances the state as
+    the step sequence does, with the pruned list too: its build prunes the
+    rows the pair kernel then reads."""
+    from mdir_amd.simulation import Simulation
+    say("case: profiled steps against run steps, TIP3P 512 waters, skin 0.2, pruned 0.05")
+    w = S.tip3p_box(512, 2.5, seed=11, grid=(24, 24, 24))
+    v = O.Oracle(w).initial_velocities(42, 300.0)
+    a = Simulation(w, skin=0.2, prune_skin=0.05)
+    b = Simulation(w, skin=0.2, prune_skin=0.05)
+    a.velocities = v
+    b.velocities = v
+    a.profile_step(30)
+    b.run(30, 0.002, graph=0)
+    d = np.abs(a.positions - b.positions).max()
+    say(f"  largest position difference {d:.3e} nm")
+    assert d < 1e-9
+    a.close()
+    b.close()
+
+
 def test_pruned_list():
     """The dual list: the pair loop over the pruned rows gives the oracle's
     forces, keeps fewer tiles than the list, re-prunes as atoms move, and
