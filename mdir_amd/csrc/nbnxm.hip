@@ -571,9 +571,13 @@ __global__ void __launch_bounds__(256) nb_search(
 // reference, 04-neighbors.md).  Pruning runs when the list was rebuilt this
 // step or an atom moved more than half the pruned buffer since the last
 // prune (the flags the integration sets); it also records the positions the
-// next test measures from.  Wave per group; a lane per entry tests its
-// j-cluster against every member atom, so the 64 entries' loads are in
-// flight together.  Row order is kept, so the pruned rows are deterministic.
+// next test measures from.  Block per group: the four waves take four
+// contiguous quarters of the row and test one entry at a time as the pair
+// loop does (lane l: i atom l & 7, j atom l >> 3 of each member tile, one
+// ballot per member); each wave notes its kept entries (row index and new
+// member mask) in LDS, and a block prefix over the quarters places them, so
+// the pruned row keeps the list's order (deterministic).  Dynamic LDS: one
+// int per row entry (row_cap).
 __global__ void __launch_bounds__(256) nb_prune(
     const float4 *__restrict__ xq, const PairEntry *__restrict__ rows,
     const int *__restrict__ row_count, PairEntry *__restrict__ rows_in,
@@ -583,69 +587,71 @@ __global__ void __launch_bounds__(256) nb_prune(
     float Lz, int *__restrict__ flags) {
   if (flags[kFlagRebuild] == 0 && flags[kFlagPrune] == 0) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&flags[kFlagPrunes], 1);
-  __shared__ float4 s_xi[4][kGroup * kCluster];
+  extern __shared__ int s_kept[];  // row_cap: (entry << kGroup) | members
+  __shared__ int s_n[4];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int gi = blockIdx.x * 4 + wave;
+  const int gi = blockIdx.x;
   const int ngroups = counts_in[1] / kGroup;
-  if (gi >= ngroups) return;  // whole waves: no block barrier below
+  if (gi >= ngroups) return;  // the whole block
   const int g0 = gi * kGroup;
-  if (lane < kGroup * kCluster) {
-    s_xi[wave][lane] = xq[g0 * kCluster + lane];
+  if (wave == 0 && lane < kGroup * kCluster) {
     const int atom = atom_at[g0 * kCluster + lane];
     if (atom >= 0)
       for (int k = 0; k < 3; ++k) x_prune[3 * atom + k] = x[3 * atom + k];
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   const int n = row_count[gi];
+  const int quarter = (n + 3) / 4;
+  const int lo = wave * quarter < n ? wave * quarter : n;
+  const int hi = lo + quarter < n ? lo + quarter : n;
+  const int a = lane & 7, b = lane >> 3;
+  float4 xi[kGroup];
+#pragma unroll
+  for (int m = 0; m < kGroup; ++m) xi[m] = xq[(g0 + m) * kCluster + a];
   const PairEntry *row = rows + (size_t)gi * row_cap;
-  PairEntry *out = rows_in + (size_t)gi * row_cap;
-  int count = 0;
-  for (int base = 0; base < n; base += 64) {
-    const int e = base + lane;
-    PairEntry E = {0, 0u, -1, 0};
-    uint32_t inner = 0;
-    if (e < n) {
-      E = row[e];
-      const uint32_t info = E.info;
-      float4 xj[kCluster];
-#pragma unroll
-      for (int b = 0; b < kCluster; ++b) xj[b] = xq[E.cj * kCluster + b];
-      const float sx = (float)((int)(info & 3u) - 1) * Lx;
-      const float sy = (float)((int)((info >> 2) & 3u) - 1) * Ly;
-      const float sz = (float)((int)((info >> 4) & 3u) - 1) * Lz;
-#pragma unroll
-      for (int b = 0; b < kCluster; ++b) {
-        xj[b].x += sx;
-        xj[b].y += sy;
-        xj[b].z += sz;
-      }
+  const int4 *row4 = reinterpret_cast<const int4 *>(row);
+  int kept = 0;
+  // 64 entries at a time, one per lane (cj, info, ex, pad), taken by
+  // readlane; the j atoms of the next entry load while this one is tested
+  for (int base = lo; base < hi; base += 64) {
+    const int cnt = hi - base < 64 ? hi - base : 64;
+    const int4 ent = row4[base + (lane < cnt ? lane : 0)];
+    float4 xj = xq[__builtin_amdgcn_readlane(ent.x, 0) * kCluster + b];
+    for (int k = 0; k < cnt; ++k) {
+      const int cj1 = __builtin_amdgcn_readlane(ent.x, k + 1 < cnt ? k + 1 : k);
+      const float4 xj1 = xq[cj1 * kCluster + b];
+      const uint32_t info = (uint32_t)__builtin_amdgcn_readlane(ent.y, k);
+      const float xjx = xj.x + (float)((int)(info & 3u) - 1) * Lx;
+      const float xjy = xj.y + (float)((int)((info >> 2) & 3u) - 1) * Ly;
+      const float xjz = xj.z + (float)((int)((info >> 4) & 3u) - 1) * Lz;
+      uint32_t members = 0;
 #pragma unroll
       for (int m = 0; m < kGroup; ++m) {
-        if (!((info >> (8 + m)) & 1u)) continue;
-        bool near = false;
-        for (int a = 0; a < kCluster && !near; ++a) {
-          const float4 xi = s_xi[wave][m * kCluster + a];
-#pragma unroll
-          for (int b = 0; b < kCluster; ++b) {
-            const float dx = xi.x - xj[b].x, dy = xi.y - xj[b].y, dz = xi.z - xj[b].z;
-            near |= dx * dx + dy * dy + dz * dz < reach2;
-          }
-        }
-        if (near) inner |= 1u << m;
+        if (!((info >> (8 + m)) & 1u)) continue;  // uniform
+        const float dx = xi[m].x - xjx, dy = xi[m].y - xjy, dz = xi[m].z - xjz;
+        if (__ballot(dx * dx + dy * dy + dz * dz < reach2)) members |= 1u << m;
       }
+      if (members) {
+        if (lane == 0) s_kept[lo + kept] = ((base + k) << kGroup) | (int)members;
+        ++kept;
+      }
+      xj = xj1;
     }
-    const bool keep = inner != 0u;
-    const uint64_t ballot = __ballot(keep);
-    if (keep) {
-      PairEntry o = E;
-      o.info = (E.info & ~(((1u << kGroup) - 1u) << 8)) | (inner << 8);
-      out[count + __popcll(ballot & ((1ull << lane) - 1ull))] = o;
-    }
-    count += __popcll(ballot);
   }
-  if (lane == 0) row_count_in[gi] = count;
+  if (lane == 0) s_n[wave] = kept;
+  __syncthreads();
+  int base = 0, total = 0;
+  for (int w = 0; w < 4; ++w) {
+    if (w < wave) base += s_n[w];
+    total += s_n[w];
+  }
+  PairEntry *out = rows_in + (size_t)gi * row_cap;
+  for (int k = lane; k < kept; k += 64) {
+    const int packed = s_kept[lo + k];
+    PairEntry o = row[packed >> kGroup];
+    o.info = (o.info & ~(((1u << kGroup) - 1u) << 8)) | ((uint32_t)(packed & ((1 << kGroup) - 1)) << 8);
+    out[base + k] = o;
+  }
+  if (threadIdx.x == 0) row_count_in[gi] = total;
 }
 
 // ---- exports for parity tests -------------------------------------------
