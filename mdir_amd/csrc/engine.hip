@@ -845,7 +845,7 @@ struct Engine {
     if ((size_t)g.row_cap * sizeof(int) > 64 * 1024)
       throw HipError("pruned list: rows longer than 16,384 entries");
     const double reach = cutoff + prune_skin + 1e-4;
-    nb_prune<<<max_groups, 256, (size_t)g.row_cap * sizeof(int), lstream>>>(
+    nb_prune<<<max_groups, 512, (size_t)g.row_cap * sizeof(int), lstream>>>(
         d_xq, d_rows, d_row_count, d_rows_in, d_row_count_in, d_counts, d_atom_at, d_x,
         d_xprune, g.row_cap, (float)(reach * reach), (float)box[0], (float)box[1],
         (float)box[2], d_flags);

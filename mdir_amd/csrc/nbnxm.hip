@@ -571,14 +571,14 @@ __global__ void __launch_bounds__(256) nb_search(
 // reference, 04-neighbors.md).  Pruning runs when the list was rebuilt this
 // step or an atom moved more than half the pruned buffer since the last
 // prune (the flags the integration sets); it also records the positions the
-// next test measures from.  Block per group: the four waves take four
-// contiguous quarters of the row and test one entry at a time as the pair
+// next test measures from.  Block per group: the eight waves take eight
+// contiguous parts of the row and test one entry at a time as the pair
 // loop does (lane l: i atom l & 7, j atom l >> 3 of each member tile, one
 // ballot per member); each wave notes its kept entries (row index and new
-// member mask) in LDS, and a block prefix over the quarters places them, so
+// member mask) in LDS, and a block prefix over the parts places them, so
 // the pruned row keeps the list's order (deterministic).  Dynamic LDS: one
 // int per row entry (row_cap).
-__global__ void __launch_bounds__(256) nb_prune(
+__global__ void __launch_bounds__(512) nb_prune(
     const float4 *__restrict__ xq, const PairEntry *__restrict__ rows,
     const int *__restrict__ row_count, PairEntry *__restrict__ rows_in,
     int *__restrict__ row_count_in, const int *__restrict__ counts_in,
@@ -588,7 +588,8 @@ __global__ void __launch_bounds__(256) nb_prune(
   if (flags[kFlagRebuild] == 0 && flags[kFlagPrune] == 0) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&flags[kFlagPrunes], 1);
   extern __shared__ int s_kept[];  // row_cap: (entry << kGroup) | members
-  __shared__ int s_n[4];
+  constexpr int kParts = 8;        // waves of a block, parts of a row
+  __shared__ int s_n[kParts];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int gi = blockIdx.x;
   const int ngroups = counts_in[1] / kGroup;
@@ -600,9 +601,9 @@ __global__ void __launch_bounds__(256) nb_prune(
       for (int k = 0; k < 3; ++k) x_prune[3 * atom + k] = x[3 * atom + k];
   }
   const int n = row_count[gi];
-  const int quarter = (n + 3) / 4;
-  const int lo = wave * quarter < n ? wave * quarter : n;
-  const int hi = lo + quarter < n ? lo + quarter : n;
+  const int part = (n + kParts - 1) / kParts;
+  const int lo = wave * part < n ? wave * part : n;
+  const int hi = lo + part < n ? lo + part : n;
   const int a = lane & 7, b = lane >> 3;
   float4 xi[kGroup];
 #pragma unroll
@@ -640,7 +641,7 @@ __global__ void __launch_bounds__(256) nb_prune(
   if (lane == 0) s_n[wave] = kept;
   __syncthreads();
   int base = 0, total = 0;
-  for (int w = 0; w < 4; ++w) {
+  for (int w = 0; w < kParts; ++w) {
     if (w < wave) base += s_n[w];
     total += s_n[w];
   }
