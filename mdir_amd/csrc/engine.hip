@@ -193,6 +193,16 @@ struct Engine {
   int max_places = 0;
   int *d_atom_at = nullptr, *d_type_at = nullptr;
   float4 *d_xq = nullptr;
+  float4 *d_gorig = nullptr;  // origin of each group of 32 places (xq is relative to it)
+  // Group origins for boxes with a half edge from kOriginHalfEdge up: there
+  // the f32 spacing about the box centre (4.8e-7 nm from 4 nm) outweighs the
+  // rounding of each pair's origin offset (about 1.2e-7 nm).  Frame batches
+  // keep positions about the box centre (the adjoint pair loop reads them).
+  static constexpr double kOriginHalfEdge = 4.0;
+  bool origins_allowed = true;
+  bool use_origins() const {
+    return origins_allowed && 0.5 * std::max(box[0], std::max(box[1], box[2])) >= kOriginHalfEdge;
+  }
   double4 *d_xpq = nullptr;   // f64 positions and charges in place order (PME)
   long long *d_F = nullptr;
   // PME in atom order (pme_atoms): the reciprocal part reads positions and
@@ -277,7 +287,8 @@ struct Engine {
   std::vector<void **> frame_buffers() {
     std::vector<void **> v = {
         (void **)&d_x, (void **)&d_xref, (void **)&d_shift, (void **)&d_place_of,
-        (void **)&d_atom_at, (void **)&d_type_at, (void **)&d_xq, (void **)&d_xpq,
+        (void **)&d_atom_at, (void **)&d_type_at, (void **)&d_xq, (void **)&d_gorig,
+        (void **)&d_xpq,
         (void **)&d_F, (void **)&d_col_count, (void **)&d_bucket, (void **)&d_bb_lo,
         (void **)&d_bb_hi, (void **)&d_cluster_col, (void **)&d_real_mask,
         (void **)&d_col_cl_start, (void **)&d_col_cl_count, (void **)&d_counts,
@@ -404,7 +415,7 @@ struct Engine {
   }
 
   void release_list() {
-    dfree(d_atom_at); dfree(d_type_at); dfree(d_xq); dfree(d_xpq); dfree(d_F); dfree(d_col_count);
+    dfree(d_atom_at); dfree(d_type_at); dfree(d_xq); dfree(d_gorig); dfree(d_xpq); dfree(d_F); dfree(d_col_count);
     dfree(d_bucket); dfree(d_bb_lo); dfree(d_bb_hi); dfree(d_cluster_col);
     dfree(d_real_mask); dfree(d_col_cl_start); dfree(d_col_cl_count); dfree(d_counts);
     dfree(d_rows); dfree(d_row_count); dfree(d_epair); dfree(d_ex);
@@ -471,6 +482,7 @@ struct Engine {
     HIPCHK(hipDeviceSynchronize());
     d_type_at = dalloc<int>(max_places);
     d_xq = dalloc<float4>(max_places);
+    d_gorig = dalloc<float4>(max_places / (kCluster * kGroup) + 1);
     d_xpq = dalloc<double4>(max_places);
     d_F = dalloc<long long>(3 * (size_t)max_places);
     d_col_count = dalloc<int>(g.ncol);
@@ -579,6 +591,7 @@ struct Engine {
     // nonbonded constants
     for (int k = 0; k < 3; ++k) {
       nbp.box[k] = (float)box[k];
+      nbp.box_lo[k] = (float)(box[k] - (double)nbp.box[k]);
       nbp.half_box[k] = (float)(0.5 * box[k]);
     }
     nbp.rc = (float)cutoff;
@@ -830,9 +843,9 @@ struct Engine {
   }
   void launch_sort() {
     nb_sort<<<dim3(g.ncol, 1, nz), 256, 0, lstream>>>(
-        d_x, d_shift, d_q, d_type, d_col_count, d_bucket, d_atom_at, d_place_of, d_xq, d_xpq,
-        d_type_at, d_bb_lo, d_bb_hi, d_cluster_col, d_real_mask, d_col_cl_start,
-        d_col_cl_count, d_counts, d_flags, g, fs);
+        d_x, d_shift, d_q, d_type, d_col_count, d_bucket, d_atom_at, d_place_of, d_xq, d_gorig,
+        d_xpq, d_type_at, d_bb_lo, d_bb_hi, d_cluster_col, d_real_mask, d_col_cl_start,
+        d_col_cl_count, d_counts, d_flags, g, use_origins() ? 1 : 0, fs);
   }
   void launch_search() {
     nb_search<<<dim3(max_groups, 1, nz), 256, 0, lstream>>>(
@@ -846,9 +859,9 @@ struct Engine {
       throw HipError("pruned list: rows longer than 16,384 entries");
     const double reach = cutoff + prune_skin + 1e-4;
     nb_prune<<<max_groups, 512, (size_t)g.row_cap * sizeof(int), lstream>>>(
-        d_xq, d_rows, d_row_count, d_rows_in, d_row_count_in, d_counts, d_atom_at, d_x,
-        d_xprune, g.row_cap, (float)(reach * reach), (float)box[0], (float)box[1],
-        (float)box[2], d_flags);
+        d_xq, d_gorig, d_rows, d_row_count, d_rows_in, d_row_count_in, d_counts, d_atom_at,
+        d_x, d_xprune, g.row_cap, (float)(reach * reach), nbp.box[0], nbp.box[1], nbp.box[2],
+        nbp.box_lo[0], nbp.box_lo[1], nbp.box_lo[2], d_flags);
   }
   void build() {
     launch_bin();
@@ -859,7 +872,7 @@ struct Engine {
 
   // One launch: plane y = 0 of the grid takes the excluded and 1-4 pairs,
   // the planes above the segments of the group rows.
-  template <int MOD, bool COUL, bool ENERGY>
+  template <int MOD, bool COUL, bool ENERGY, bool ORG>
   void launch_pairs_t() {
     SpecialArgs sa{};
     sa.npairs = (terms & 2) ? nspecial : 0;
@@ -876,20 +889,25 @@ struct Engine {
     const dim3 grid(std::max(blocks_for(max_groups, 4), blocks_for(nspecial)),
                     (terms & 1) ? nseg + 1 : 1, nz);
     const bool pruned = prune_skin > 0;
-    nb_pairs<MOD, COUL, ENERGY><<<grid, 256, ntypes * ntypes * sizeof(float2), lstream>>>(
-        d_xq, d_type_at, d_real_mask, pruned ? d_rows_in : d_rows,
+    nb_pairs<MOD, COUL, ENERGY, ORG><<<grid, 256, ntypes * ntypes * sizeof(float2), lstream>>>(
+        d_xq, d_gorig, d_type_at, d_real_mask, pruned ? d_rows_in : d_rows,
         pruned ? d_row_count_in : d_row_count, d_ex, g.row_cap, seg_len, max_groups, d_counts,
         d_lj,
         nbp, d_F, d_epair, d_flags, sa, fs);
   }
 
+  template <int MOD, bool ORG>
+  void launch_pairs_o(bool energy) {
+    const bool coul = beta > 0;
+    if (coul && energy) launch_pairs_t<MOD, true, true, ORG>();
+    else if (coul) launch_pairs_t<MOD, true, false, ORG>();
+    else if (energy) launch_pairs_t<MOD, false, true, ORG>();
+    else launch_pairs_t<MOD, false, false, ORG>();
+  }
   template <int MOD>
   void launch_pairs(bool energy) {
-    const bool coul = beta > 0;
-    if (coul && energy) launch_pairs_t<MOD, true, true>();
-    else if (coul) launch_pairs_t<MOD, true, false>();
-    else if (energy) launch_pairs_t<MOD, false, true>();
-    else launch_pairs_t<MOD, false, false>();
+    if (use_origins()) launch_pairs_o<MOD, true>(energy);
+    else launch_pairs_o<MOD, false>(energy);
   }
 
   void pairs(bool energy) {
@@ -1187,7 +1205,8 @@ struct Engine {
     ip.dt = dt;
     integrate_part<PART><<<blocks_for(nunits, 64), 64, 0, stream>>>(
         nunits, d_unit_first, d_unit_size, d_x, d_v, d_inv_mass, d_F, d_Fr, d_place_of,
-        d_shift, d_xref, d_q, d_xq, d_xpa ? nullptr : d_xpq, d_xpa, ip, d_flags);
+        d_shift, d_xref, d_q, d_xq, use_origins() ? d_gorig : nullptr,
+        d_xpa ? nullptr : d_xpq, d_xpa, ip, d_flags);
   }
   void step_first(double dt) { integrate<kFirstHalf>(dt); }
   void step_second(double dt) { integrate<kSecondHalf>(dt); }

@@ -90,6 +90,7 @@ struct FrameBatch {
     dd.nwaters = 0;       // no integration
     e.pme_atoms = false;  // the adjoint reads PME inputs in place order
     e.seg_len = 48;       // many frames fill the GPU: long segments
+    e.origins_allowed = false;  // the adjoint reads positions about the box centre
     e.create(&dd);
     max_frames = frames;
     // per-type parameters: the diagonal of the tables, which must follow

@@ -206,6 +206,7 @@ __device__ __forceinline__ int drift_store(UnitState<S> &s, double *__restrict__
                                            const double *__restrict__ xref,
                                            const double *__restrict__ q,
                                            float4 *__restrict__ xq,
+                                           const float4 *__restrict__ gorig,
                                            double4 *__restrict__ xpq,
                                            double4 *__restrict__ xpa, const IntParams &p) {
   double xn[S][3];
@@ -244,10 +245,12 @@ __device__ __forceinline__ int drift_store(UnitState<S> &s, double *__restrict__
     }
     if (d2 > p.skin_half2) moved |= 1;
     if (d2p > p.prune_half2) moved |= 2;
-    xq[s.pl[m]] = make_float4((float)(xn[m][0] - shift[3 * at + 0] - p.half_box[0]),
-                              (float)(xn[m][1] - shift[3 * at + 1] - p.half_box[1]),
-                              (float)(xn[m][2] - shift[3 * at + 2] - p.half_box[2]),
-                              (float)q[at]);
+    const float4 o = gorig ? gorig[s.pl[m] / (kCluster * kGroup)]
+                           : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    xq[s.pl[m]] = make_float4(
+        (float)(xn[m][0] - shift[3 * at + 0] - p.half_box[0] - (double)o.x),
+        (float)(xn[m][1] - shift[3 * at + 1] - p.half_box[1] - (double)o.y),
+        (float)(xn[m][2] - shift[3 * at + 2] - p.half_box[2] - (double)o.z), (float)q[at]);
     if (xpq) xpq[s.pl[m]] = make_double4(xn[m][0], xn[m][1], xn[m][2], q[at]);
     if (xpa) xpa[at] = make_double4(xn[m][0], xn[m][1], xn[m][2], q[at]);
   }
@@ -268,6 +271,7 @@ __device__ __forceinline__ void step_unit(int first, double *__restrict__ x,
                                           const double *__restrict__ xref,
                                           const double *__restrict__ q,
                                           float4 *__restrict__ xq,
+                                          const float4 *__restrict__ gorig,
                                           double4 *__restrict__ xpq,
                                           double4 *__restrict__ xpa, const IntParams &p,
                                           int *__restrict__ flags) {
@@ -280,7 +284,7 @@ __device__ __forceinline__ void step_unit(int first, double *__restrict__ x,
   }
   if (PART & kFirstHalf) {   // start of a step: half kick, drift, SETTLE
     half_kick<S>(s, p.dt);
-    const int moved = drift_store<S>(s, x, v, shift, xref, q, xq, xpq, xpa, p);
+    const int moved = drift_store<S>(s, x, v, shift, xref, q, xq, gorig, xpq, xpa, p);
     if (moved & 1) atomicOr(&flags[kFlagRebuild], 1);
     if (moved & 2) atomicOr(&flags[kFlagPrune], 1);
   } else {
@@ -303,17 +307,18 @@ __global__ void __launch_bounds__(64) integrate_part(
     double *__restrict__ x, double *__restrict__ v, const double *__restrict__ inv_mass,
     long long *__restrict__ F, long long *__restrict__ Fr, const int *__restrict__ place_of,
     const double *__restrict__ shift, const double *__restrict__ xref,
-    const double *__restrict__ q, float4 *__restrict__ xq, double4 *__restrict__ xpq,
-    double4 *__restrict__ xpa, IntParams p, int *__restrict__ flags) {
+    const double *__restrict__ q, float4 *__restrict__ xq, const float4 *__restrict__ gorig,
+    double4 *__restrict__ xpq, double4 *__restrict__ xpa, IntParams p,
+    int *__restrict__ flags) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= nunits) return;
   const int first = unit_first[u];
   if (unit_size[u] == 3)
-    step_unit<PART, 3>(first, x, v, inv_mass, F, Fr, place_of, shift, xref, q, xq, xpq, xpa,
-                       p, flags);
+    step_unit<PART, 3>(first, x, v, inv_mass, F, Fr, place_of, shift, xref, q, xq, gorig, xpq,
+                       xpa, p, flags);
   else
-    step_unit<PART, 1>(first, x, v, inv_mass, F, Fr, place_of, shift, xref, q, xq, xpq, xpa,
-                       p, flags);
+    step_unit<PART, 1>(first, x, v, inv_mass, F, Fr, place_of, shift, xref, q, xq, gorig, xpq,
+                       xpa, p, flags);
 }
 
 // Positions and charges by atom for PME in atom order (evaluations outside

@@ -3,8 +3,11 @@
 // Layout in HBM (one system):
 //   atom order   x, v, x_ref, wrap shift: double[3n]; mass, inverse mass,
 //                charge: double[n]; place_of: int[n]
-//   place order  xq: float4 (x, y, z, q) relative to the box centre, type_at,
-//                atom_at (-1 for padding), force words: int64[3 np]
+//   place order  xq: float4 (x, y, z, q) relative to the origin of the place's
+//                group, type_at, atom_at (-1 for padding), force words:
+//                int64[3 np]
+//   groups       gorig: float4 origin of each group of 32 places, relative to
+//                the box centre, on a grid of 1/64 nm
 //   clusters     8 consecutive places; bbox lo/hi float4; column of cluster
 //   pair list    rows of capacity row_cap entries per group of 4 i-clusters:
 //                {cj, shift | member mask | j real mask, explicit-mask slot};
@@ -60,6 +63,7 @@ struct __attribute__((aligned(16))) PairEntry {
 // Per-system constants of the nonbonded and PME kernels (passed by value).
 struct NbParams {
   float box[3];
+  float box_lo[3];     // box edge minus its f32 value
   float half_box[3];
   float rc2;
   float rc;
@@ -97,6 +101,21 @@ __device__ __forceinline__ void shift_vector(uint32_t code, const float *box,
   sx = (float)((int)(code & 3u) - 1) * box[0];
   sy = (float)((int)((code >> 2) & 3u) - 1) * box[1];
   sz = (float)((int)((code >> 4) & 3u) - 1) * box[2];
+}
+
+// The offset added to a j group's positions to put them in the i group's
+// frame: the difference of the two group origins plus the periodic shift
+// k L (k in -1, 0, 1), with the box edge L split as its f32 value box plus
+// box_lo.  The origins are multiples of 1/64 nm and box a multiple of its
+// own ulp u <= 1/64 nm (edges below 2^18 nm), so ogj - ogi and
+// (ogj - ogi) + k box are multiples of u; the latter is the distance of the
+// two origins across the boundary, below L for every list entry, hence
+// exact in f32.  Adding k box_lo rounds once, on an offset of the order of
+// the cut-off.
+__device__ __forceinline__ float group_offset(uint32_t code, int d, float ogj, float ogi,
+                                              float box, float box_lo) {
+  const float k = (float)((int)((code >> (2 * d)) & 3u) - 1);
+  return ((ogj - ogi) + k * box) + k * box_lo;
 }
 
 }  // namespace mdhip

@@ -104,17 +104,20 @@ __global__ void __launch_bounds__(256) nb_bin(int n, const double *__restrict__ 
 // Block per column: place offset by a prefix over the preceding columns,
 // rank sort of the column's atoms by (z, atom index) — a total order, so
 // the result does not depend on the order the bucket was filled in — then
-// places, f32 positions about the box centre, padding, and cluster boxes.
+// places, padding, cluster boxes (about the box centre), group origins and
+// the f32 positions relative to them (origins zero unless `origins`).
 __global__ void __launch_bounds__(256) nb_sort(
     const double *__restrict__ x, const double *__restrict__ shift,
     const double *__restrict__ q, const int *__restrict__ type,
     const int *__restrict__ col_count, const int *__restrict__ bucket,
     int *__restrict__ atom_at, int *__restrict__ place_of,
-    float4 *__restrict__ xq, double4 *__restrict__ xpq, int *__restrict__ type_at,
+    float4 *__restrict__ xq, float4 *__restrict__ gorig, double4 *__restrict__ xpq,
+    int *__restrict__ type_at,
     float4 *__restrict__ bb_lo, float4 *__restrict__ bb_hi,
     int *__restrict__ cluster_col, uint32_t *__restrict__ real_mask,
     int *__restrict__ col_cl_start, int *__restrict__ col_cl_count,
-    int *__restrict__ counts_out, int *__restrict__ flags, GridParams g, size_t fs) {
+    int *__restrict__ counts_out, int *__restrict__ flags, GridParams g, int origins,
+    size_t fs) {
   x = frame_at(x, fs);
   shift = frame_at(shift, fs);
   col_count = frame_at(col_count, fs);
@@ -122,6 +125,7 @@ __global__ void __launch_bounds__(256) nb_sort(
   atom_at = frame_at(atom_at, fs);
   place_of = frame_at(place_of, fs);
   xq = frame_at(xq, fs);
+  gorig = frame_at(gorig, fs);
   xpq = frame_at(xpq, fs);
   type_at = frame_at(type_at, fs);
   bb_lo = frame_at(bb_lo, fs);
@@ -137,6 +141,8 @@ __global__ void __launch_bounds__(256) nb_sort(
   __shared__ int s_atom[kSortCapMax];
   __shared__ double s_z[kSortCapMax];
   __shared__ float4 s_pos[kSortCapMax];
+  __shared__ int s_ratom[kSortCapMax];                       // atom by rank
+  __shared__ float4 s_gor[kSortCapMax / (kCluster * kGroup)];  // group origins
   const int col = blockIdx.x, tid = threadIdx.x;
   const int cap = g.col_cap < kSortCapMax ? g.col_cap : kSortCapMax;
 
@@ -183,10 +189,10 @@ __global__ void __launch_bounds__(256) nb_sort(
     p.y = (float)(x[3 * ia + 1] - shift[3 * ia + 1] - g.half_box[1]);
     p.z = (float)(x[3 * ia + 2] - shift[3 * ia + 2] - g.half_box[2]);
     p.w = (float)q[ia];
-    xq[place] = p;
     xpq[place] = make_double4(x[3 * ia + 0], x[3 * ia + 1], x[3 * ia + 2], q[ia]);
     type_at[place] = type[ia];
     s_pos[rank] = p;
+    s_ratom[rank] = ia;
   }
   for (int t = count + tid; t < padded; t += blockDim.x) {
     int place = start + t;
@@ -227,6 +233,37 @@ __global__ void __launch_bounds__(256) nb_sort(
     bb_hi[cl_start + c] = hi;
     cluster_col[cl_start + c] = col;
     real_mask[cl_start + c] = m;
+  }
+  // Group origins (large boxes): the lower corner of each group's atoms on
+  // a grid of 1/64 nm (exact in f32, as are their differences), and the
+  // pair loop's f32 positions relative to their group's origin, so that
+  // their spacing does not grow with the box.  The pair loop adds the exact
+  // difference of two origins and the periodic shift to the j atoms.
+  // Without origins they are zero: positions about the box centre.
+  const int ngr = ncl / kGroup;
+  for (int gl = tid; gl < ngr; gl += blockDim.x) {
+    const int r0 = gl * kPad, r1 = r0 + kPad < count ? r0 + kPad : count;
+    float lx = 3.0e38f, ly = 3.0e38f, lz = 3.0e38f;
+    for (int r = r0; r < r1; ++r) {
+      lx = fminf(lx, s_pos[r].x);
+      ly = fminf(ly, s_pos[r].y);
+      lz = fminf(lz, s_pos[r].z);
+    }
+    const float4 o = origins && r1 > r0 ? make_float4(floorf(lx * 64.0f) * 0.015625f,
+                                           floorf(ly * 64.0f) * 0.015625f,
+                                           floorf(lz * 64.0f) * 0.015625f, 0.0f)
+                             : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    s_gor[gl] = o;
+    gorig[cl_start / kGroup + gl] = o;
+  }
+  __syncthreads();
+  for (int r = tid; r < count; r += blockDim.x) {
+    const int ia = s_ratom[r];
+    const float4 o = s_gor[r / kPad];
+    xq[start + r] = make_float4(
+        (float)(x[3 * ia + 0] - shift[3 * ia + 0] - g.half_box[0] - (double)o.x),
+        (float)(x[3 * ia + 1] - shift[3 * ia + 1] - g.half_box[1] - (double)o.y),
+        (float)(x[3 * ia + 2] - shift[3 * ia + 2] - g.half_box[2] - (double)o.z), (float)q[ia]);
   }
 }
 
@@ -579,12 +616,13 @@ __global__ void __launch_bounds__(256) nb_search(
 // the pruned row keeps the list's order (deterministic).  Dynamic LDS: one
 // int per row entry (row_cap).
 __global__ void __launch_bounds__(512) nb_prune(
-    const float4 *__restrict__ xq, const PairEntry *__restrict__ rows,
+    const float4 *__restrict__ xq, const float4 *__restrict__ gorig,
+    const PairEntry *__restrict__ rows,
     const int *__restrict__ row_count, PairEntry *__restrict__ rows_in,
     int *__restrict__ row_count_in, const int *__restrict__ counts_in,
     const int *__restrict__ atom_at, const double *__restrict__ x,
     double *__restrict__ x_prune, int row_cap, float reach2, float Lx, float Ly,
-    float Lz, int *__restrict__ flags) {
+    float Lz, float Lx_lo, float Ly_lo, float Lz_lo, int *__restrict__ flags) {
   if (flags[kFlagRebuild] == 0 && flags[kFlagPrune] == 0) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&flags[kFlagPrunes], 1);
   extern __shared__ int s_kept[];  // row_cap: (entry << kGroup) | members
@@ -608,6 +646,7 @@ __global__ void __launch_bounds__(512) nb_prune(
   float4 xi[kGroup];
 #pragma unroll
   for (int m = 0; m < kGroup; ++m) xi[m] = xq[(g0 + m) * kCluster + a];
+  const float4 oi = gorig[gi];
   const PairEntry *row = rows + (size_t)gi * row_cap;
   const int4 *row4 = reinterpret_cast<const int4 *>(row);
   int kept = 0;
@@ -616,14 +655,17 @@ __global__ void __launch_bounds__(512) nb_prune(
   for (int base = lo; base < hi; base += 64) {
     const int cnt = hi - base < 64 ? hi - base : 64;
     const int4 ent = row4[base + (lane < cnt ? lane : 0)];
-    float4 xj = xq[__builtin_amdgcn_readlane(ent.x, 0) * kCluster + b];
+    const int cj0 = __builtin_amdgcn_readlane(ent.x, 0);
+    float4 xj = xq[cj0 * kCluster + b];
+    float4 oj = gorig[cj0 / kGroup];
     for (int k = 0; k < cnt; ++k) {
       const int cj1 = __builtin_amdgcn_readlane(ent.x, k + 1 < cnt ? k + 1 : k);
       const float4 xj1 = xq[cj1 * kCluster + b];
+      const float4 oj1 = gorig[cj1 / kGroup];
       const uint32_t info = (uint32_t)__builtin_amdgcn_readlane(ent.y, k);
-      const float xjx = xj.x + (float)((int)(info & 3u) - 1) * Lx;
-      const float xjy = xj.y + (float)((int)((info >> 2) & 3u) - 1) * Ly;
-      const float xjz = xj.z + (float)((int)((info >> 4) & 3u) - 1) * Lz;
+      const float xjx = xj.x + group_offset(info, 0, oj.x, oi.x, Lx, Lx_lo);
+      const float xjy = xj.y + group_offset(info, 1, oj.y, oi.y, Ly, Ly_lo);
+      const float xjz = xj.z + group_offset(info, 2, oj.z, oi.z, Lz, Lz_lo);
       uint32_t members = 0;
 #pragma unroll
       for (int m = 0; m < kGroup; ++m) {
@@ -636,6 +678,7 @@ __global__ void __launch_bounds__(512) nb_prune(
         ++kept;
       }
       xj = xj1;
+      oj = oj1;
     }
   }
   if (lane == 0) s_n[wave] = kept;

@@ -250,9 +250,10 @@ __device__ __forceinline__ void special_block(int block, const SpecialArgs &sa,
 
 // Grid: x over groups (4 waves per block), y = 0 the excluded and 1-4
 // pairs, y = 1 + segment the pair segments.
-template <int MOD, bool COUL, bool ENERGY>
+template <int MOD, bool COUL, bool ENERGY, bool ORG>
 __device__ __forceinline__ void nb_pairs_body(
-    const float4 *__restrict__ xq, const int *__restrict__ type_at,
+    const float4 *__restrict__ xq, const float4 *__restrict__ gorig,
+    const int *__restrict__ type_at,
     const uint32_t *__restrict__ real_mask, const PairEntry *__restrict__ rows,
     const int *__restrict__ row_count, const uint64_t *__restrict__ ex_masks,
     int row_cap, int seg_len, int max_groups, const int *__restrict__ counts_in,
@@ -308,6 +309,9 @@ __device__ __forceinline__ void nb_pairs_body(
     ireal[m] = (real_mask[g0 + m] >> a) & 1u;
     fi[m][0] = fi[m][1] = fi[m][2] = 0.0f;
   }
+  // ORG: positions relative to group origins (large boxes), else relative
+  // to the box centre with zero origins
+  const float4 oi = ORG ? gorig[gi] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   double e_lj = 0.0, e_c = 0.0;  // energy steps only: f64 partial sums
   const PairEntry *row = rows + (size_t)gi * row_cap + e0;
   // Software pipeline: the entry two ahead and the j atoms one ahead are
@@ -316,19 +320,31 @@ __device__ __forceinline__ void nb_pairs_body(
   PairEntry E = row[0];
   PairEntry E1 = row[n > 1 ? 1 : 0];
   float4 xj = xq[E.cj * kCluster + b];
+  float4 oj = ORG ? gorig[E.cj / kGroup] : oi;
   int tj = type_at[E.cj * kCluster + b];
   for (int e = 0; e < n; ++e) {
     const PairEntry E2 = row[e + 2 < n ? e + 2 : n - 1];
     const float4 xj1 = xq[E1.cj * kCluster + b];
+    const float4 oj1 = ORG ? gorig[E1.cj / kGroup] : oi;
     const int tj1 = type_at[E1.cj * kCluster + b];
 
     const uint32_t info = E.info;
     float2 ljm[kGroup];  // issued early: LDS latency overlaps the set-up
 #pragma unroll
     for (int m = 0; m < kGroup; ++m) ljm[m] = s_lj[ti[m] + tj];
-    float sx, sy, sz;
-    shift_vector(info & 63u, p.box, sx, sy, sz);
-    const float xjx = xj.x + sx, xjy = xj.y + sy, xjz = xj.z + sz;
+    // j positions in the i group's frame
+    float xjx, xjy, xjz;
+    if constexpr (ORG) {
+      xjx = xj.x + group_offset(info, 0, oj.x, oi.x, p.box[0], p.box_lo[0]);
+      xjy = xj.y + group_offset(info, 1, oj.y, oi.y, p.box[1], p.box_lo[1]);
+      xjz = xj.z + group_offset(info, 2, oj.z, oi.z, p.box[2], p.box_lo[2]);
+    } else {
+      float sx, sy, sz;
+      shift_vector(info & 63u, p.box, sx, sy, sz);
+      xjx = xj.x + sx;
+      xjy = xj.y + sy;
+      xjz = xj.z + sz;
+    }
     const bool jreal = (info >> (16 + b)) & 1u;
     float fjx = 0.0f, fjy = 0.0f, fjz = 0.0f;
 #pragma unroll
@@ -369,6 +385,7 @@ __device__ __forceinline__ void nb_pairs_body(
     E = E1;
     E1 = E2;
     xj = xj1;
+    oj = oj1;
     tj = tj1;
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -409,9 +426,10 @@ __device__ __forceinline__ void nb_pairs_body(
 }
 
 // The pair kernel.
-template <int MOD, bool COUL, bool ENERGY>
+template <int MOD, bool COUL, bool ENERGY, bool ORG>
 __global__ void __launch_bounds__(256) nb_pairs(
-    const float4 *__restrict__ xq, const int *__restrict__ type_at,
+    const float4 *__restrict__ xq, const float4 *__restrict__ gorig,
+    const int *__restrict__ type_at,
     const uint32_t *__restrict__ real_mask, const PairEntry *__restrict__ rows,
     const int *__restrict__ row_count, const uint64_t *__restrict__ ex_masks,
     int row_cap, int seg_len, int max_groups, const int *__restrict__ counts_in,
@@ -419,6 +437,7 @@ __global__ void __launch_bounds__(256) nb_pairs(
     double *__restrict__ energy_partial, int *__restrict__ flags, SpecialArgs sa,
     size_t fs) {
   xq = frame_at(xq, fs);
+  gorig = frame_at(gorig, fs);
   type_at = frame_at(type_at, fs);
   real_mask = frame_at(real_mask, fs);
   rows = frame_at(rows, fs);
@@ -431,7 +450,7 @@ __global__ void __launch_bounds__(256) nb_pairs(
   sa.x = frame_at(sa.x, fs);
   sa.place_of = frame_at(sa.place_of, fs);
   sa.energy_partial = frame_at(sa.energy_partial, fs);
-  nb_pairs_body<MOD, COUL, ENERGY>(xq, type_at, real_mask, rows, row_count, ex_masks, row_cap,
+  nb_pairs_body<MOD, COUL, ENERGY, ORG>(xq, gorig, type_at, real_mask, rows, row_count, ex_masks, row_cap,
                                    seg_len, max_groups, counts_in, ljtab, p, F, energy_partial, flags,
                                    sa);
 }
@@ -446,7 +465,10 @@ __global__ void stamp_clock(unsigned long long *__restrict__ slot) {
 
 // Instantiations used by the engine.
 #define MDHIP_PAIRS_ONE(MOD, C, E)                                                       \
-  template __global__ void nb_pairs<MOD, C, E>(const float4 *, const int *,            \
+  MDHIP_PAIRS_ORG(MOD, C, E, false)                                                      \
+  MDHIP_PAIRS_ORG(MOD, C, E, true)
+#define MDHIP_PAIRS_ORG(MOD, C, E, O)                                                    \
+  template __global__ void nb_pairs<MOD, C, E, O>(const float4 *, const float4 *, const int *, \
       const uint32_t *, const PairEntry *, const int *, const uint64_t *, int, int, int, \
       const int *, const float2 *, NbParams, long long *, double *, int *, SpecialArgs,     \
       size_t);
