@@ -7,7 +7,8 @@
 //                group, type_at, atom_at (-1 for padding), force words:
 //                int64[3 np]
 //   groups       gorig: float4 origin of each group of 32 places, relative to
-//                the box centre, on a grid of 1/64 nm
+//                the box centre, on a grid of 1/64 nm (boxes with a half edge
+//                from 4 nm; zero otherwise, and xq is about the box centre)
 //   clusters     8 consecutive places; bbox lo/hi float4; column of cluster
 //   pair list    rows of capacity row_cap entries per group of 4 i-clusters:
 //                {cj, shift | member mask | j real mask, explicit-mask slot};

@@ -189,6 +189,7 @@ __global__ void __launch_bounds__(256) nb_sort(
     p.y = (float)(x[3 * ia + 1] - shift[3 * ia + 1] - g.half_box[1]);
     p.z = (float)(x[3 * ia + 2] - shift[3 * ia + 2] - g.half_box[2]);
     p.w = (float)q[ia];
+    if (!origins) xq[place] = p;  // about the box centre
     xpq[place] = make_double4(x[3 * ia + 0], x[3 * ia + 1], x[3 * ia + 2], q[ia]);
     type_at[place] = type[ia];
     s_pos[rank] = p;
@@ -241,6 +242,11 @@ __global__ void __launch_bounds__(256) nb_sort(
   // difference of two origins and the periodic shift to the j atoms.
   // Without origins they are zero: positions about the box centre.
   const int ngr = ncl / kGroup;
+  if (!origins) {
+    for (int gl = tid; gl < ngr; gl += blockDim.x)
+      gorig[cl_start / kGroup + gl] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return;
+  }
   for (int gl = tid; gl < ngr; gl += blockDim.x) {
     const int r0 = gl * kPad, r1 = r0 + kPad < count ? r0 + kPad : count;
     float lx = 3.0e38f, ly = 3.0e38f, lz = 3.0e38f;
@@ -249,7 +255,7 @@ __global__ void __launch_bounds__(256) nb_sort(
       ly = fminf(ly, s_pos[r].y);
       lz = fminf(lz, s_pos[r].z);
     }
-    const float4 o = origins && r1 > r0 ? make_float4(floorf(lx * 64.0f) * 0.015625f,
+    const float4 o = r1 > r0 ? make_float4(floorf(lx * 64.0f) * 0.015625f,
                                            floorf(ly * 64.0f) * 0.015625f,
                                            floorf(lz * 64.0f) * 0.015625f, 0.0f)
                              : make_float4(0.0f, 0.0f, 0.0f, 0.0f);

@@ -1,6 +1,7 @@
 """The other workloads of the force path on the GPU: the small argon NVE run,
-a PME grid too wide for the plane transform (the tiled y / x passes), and
-the 999,000-atom water box at full size through size-independent checks."""
+a PME grid too wide for the plane transform (the tiled y / x passes), the
+999,000-atom water box at full size through size-independent checks, and
+an 8.2 nm box, the size from which the pair loop works on group origins."""
 import numpy as np
 import pytest
 
