@@ -36,10 +36,23 @@ PEAK_HBM_GBS = 8000.0            # MI355X HBM3E
 # profiles/nb_pairs_tip3p_7900_pme64.md
 NB_PAIRS_TRAFFIC_BYTES = 37.50e6
 
+# The measured workloads: the 23,700-atom box (the default bench line) and
+# the 999,000-atom box at full size on one GPU (same recipe, NVE).
+WORKLOADS = {
+    "tip3p_7900_waters_pme64": {
+        "waters": 7900, "edge": 6.2, "grid": 64, "traffic": NB_PAIRS_TRAFFIC_BYTES,
+        "metric": "ns/day on ~24k-atom PME water box (1 GPU)"},
+    "tip3p_333000_waters_pme216": {
+        "waters": 333000, "edge": 21.5, "grid": 216, "traffic": None,
+        "metric": "ns/day on 999k-atom PME water box (1 GPU)"},
+}
+
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--workload", choices=sorted(WORKLOADS), default="tip3p_7900_waters_pme64",
+                   help="TIP3P box: the 23,700-atom bench line or the 999,000-atom box")
     p.add_argument("--steps", type=int, default=5000)
     p.add_argument("--warmup", type=int, default=500)
     p.add_argument("--graph", type=int, default=50, help="steps per captured graph")
@@ -133,7 +146,12 @@ def main():
     from mdir_amd.simulation import Simulation
 
     dt = 0.002
-    system = S.tip3p_config2(seed=args.seed)
+    wl = WORKLOADS[args.workload]
+    if args.workload == "tip3p_7900_waters_pme64":
+        system = S.tip3p_config2(seed=args.seed)
+    else:
+        system = S.tip3p_box(wl["waters"], wl["edge"], seed=args.seed, cutoff=0.9,
+                             grid=(wl["grid"],) * 3)
     sim = Simulation(system, skin=args.skin, prune_skin=args.prune_skin)
     sim.minimize(200, 0.01)
     x = sim.positions
@@ -206,13 +224,13 @@ def main():
                 "duration_us": round(pair_ms * 1e3, 2),
                 "achieved": round(pair_tflops, 3), "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(pair_tflops / PEAK_FP32_TFLOPS, 4),
-                "traffic": NB_PAIRS_TRAFFIC_BYTES,
+                "traffic": wl["traffic"],
                 "work": {"pairs_within_rc": n_pairs, "lj_pairs": n_lj,
                          "flop_per_pair": [FLOP_PER_COULOMB_PAIR, FLOP_PER_LJ_COULOMB_PAIR],
                          "flop_per_launch": flops}}
 
     out = {
-        "metric": "ns/day on ~24k-atom PME water box (1 GPU)",
+        "metric": wl["metric"],
         "value": round(ns_day, 2),
         "unit": "ns/day",
         "n_gpus": 1,
@@ -224,8 +242,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "tip3p_7900_waters_pme64", "atoms": system.n,
-                   "box_nm": 6.2, "cutoff_nm": system.cutoff, "pme_grid": list(system.pme_grid),
+        "config": {"workload": args.workload, "atoms": system.n,
+                   "box_nm": wl["edge"], "cutoff_nm": system.cutoff, "pme_grid": list(system.pme_grid),
                    "pme_order": system.pme_order, "ewald_beta": system.beta,
                    "dt_fs": 2.0, "ensemble": "NVE", "constraints": "SETTLE",
                    "list_skin_nm": args.skin, "prune_skin_nm": args.prune_skin,
