@@ -43,7 +43,10 @@ WORKLOADS = {
         "waters": 7900, "edge": 6.2, "grid": 64, "traffic": NB_PAIRS_TRAFFIC_BYTES,
         "metric": "ns/day on ~24k-atom PME water box (1 GPU)"},
     "tip3p_333000_waters_pme216": {
-        "waters": 333000, "edge": 21.5, "grid": 216, "traffic": None,
+        "waters": 333000, "edge": 21.5, "grid": 216,
+        # the same two --pmc runs with --workload tip3p_333000_waters_pme216:
+        # profiles/md_step_tip3p_333000_pme216.md
+        "traffic": 1.5954e9,
         "metric": "ns/day on 999k-atom PME water box (1 GPU)"},
 }
 
