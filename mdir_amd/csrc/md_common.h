@@ -108,11 +108,14 @@ __device__ __forceinline__ void shift_vector(uint32_t code, const float *box,
 // frame: the difference of the two group origins plus the periodic shift
 // k L (k in -1, 0, 1), with the box edge L split as its f32 value box plus
 // box_lo.  The origins are multiples of 1/64 nm and box a multiple of its
-// own ulp u <= 1/64 nm (edges below 2^18 nm), so ogj - ogi and
-// (ogj - ogi) + k box are multiples of u; the latter is the distance of the
-// two origins across the boundary, below L for every list entry, hence
-// exact in f32.  Adding k box_lo rounds once, on an offset of the order of
-// the cut-off.
+// own ulp u <= 1/64 nm (edges below 2^18 nm), so ogj - ogi (exact) and
+// (ogj - ogi) + k box are multiples of u.  The latter, the distance of the
+// two origins across the boundary, is exact while it stays below L, which
+// holds in dense systems, where a group spans little more than a cluster
+// (the pair's origins lie within the cut-off, the list buffer and the two
+// groups' extents); a group stretched over a sparse column rounds it once,
+// as positions about the box centre would.  Adding k box_lo rounds once, on
+// an offset of the order of the cut-off.
 __device__ __forceinline__ float group_offset(uint32_t code, int d, float ogj, float ogi,
                                               float box, float box_lo) {
   const float k = (float)((int)((code >> (2 * d)) & 3u) - 1);
