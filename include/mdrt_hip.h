@@ -278,10 +278,32 @@ int mdrtHipAutotune(void *system, int32_t what, int32_t steps, int32_t graphs, d
                     double *ms_per_step);
 
 /* `steps` velocity-Verlet steps of dt (ps) with SETTLE waters.  graph != 0
-   replays the step sequence as a captured HIP graph of `graph` steps.
+   replays the step sequence as captured HIP graphs of `graph` steps (and
+   one of the remainder steps mod graph).
    kinetic (may be NULL) receives the kinetic energy after the last step. */
 int mdrtHipRun(void *system, int64_t steps, double dt, int32_t graph,
                double *kinetic);
+
+/* The check of the list between rebuilds, the counterpart of the
+   reference's MDIR_CHECK_PRUNING (runtime/mdrt.c:48-79, mdrtCheckPruned):
+   after mdrtHipRun, the forces (n x 3, may be NULL) and energies (may be
+   NULL) the next step would compute at the current positions, through the
+   list the steps hold (no forced rebuild: the build runs only if the last
+   drift asked for one, as in a step).  counts[3] (may be NULL): the pairs
+   within rc (+ 3e-6 (Lx+Ly+Lz), the pair-set margin) found in the list and
+   in the rows the pair loop runs over (the pruned rows with a dual list;
+   the two are equal unless the pruned list misses a pair), and the list
+   builds the check itself did.  Fails when the positions changed since
+   the last step. */
+int mdrtHipCheckList(void *system, double *forces, double *energies, int64_t *counts);
+
+/* Captures, without running them, the step graphs a later
+   mdrtHipRun(system, steps, dt, graph) replays: the graph of `graph` steps
+   and the graph of the remainder steps mod graph (each run replays only
+   captured graphs when graph != 0; this keeps the capture out of a timed
+   run).  No counterpart in the reference, whose step graph is captured by
+   mdrtStepCapture (runtime/mdrt_step_graph.cu:158-464) inside the run. */
+int mdrtHipPrepareRun(void *system, int64_t steps, double dt, int32_t graph);
 
 /* mdrtHipRun without the kinetic energy, timing the pair loop live: device
    clock (wall_clock64) readings by one-wave kernels just before and after

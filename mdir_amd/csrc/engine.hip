@@ -266,7 +266,6 @@ struct Engine {
   // ---- integration ----
   int *d_unit_first = nullptr, *d_unit_size = nullptr;
   IntParams ip{};
-  double dt_graph = -1;
 
   // ---- force terms ----
   // Which terms a force evaluation computes (bits of MDRT_TERM_*): the
@@ -357,14 +356,28 @@ struct Engine {
     invalidate_graph();
   }
 
+  // Captured step sequences, one per (steps, dt, measured): a run of K
+  // steps in chunks of G replays the G-step graph and, for a remainder, a
+  // graph of K mod G steps, so no run falls back to launch-by-launch steps.
+  struct StepGraph {
+    int steps;
+    double dt;
+    bool measured;
+    hipGraphExec_t exec;
+  };
+  static constexpr size_t kMaxStepGraphs = 4;
+  std::vector<StepGraph> step_graphs;
   void invalidate_graph() {
-    if (graph_exec) hipGraphExecDestroy(graph_exec);
-    graph_exec = nullptr;
+    for (StepGraph &g_ : step_graphs) hipGraphExecDestroy(g_.exec);
+    step_graphs.clear();
+  }
+  hipGraphExec_t find_graph(int steps, double dt, bool measured) const {
+    for (const StepGraph &g_ : step_graphs)
+      if (g_.steps == steps && g_.dt == dt && g_.measured == measured) return g_.exec;
+    return nullptr;
   }
 
   bool forces_valid = false;
-  hipGraphExec_t graph_exec = nullptr;
-  int graph_steps = 0;
   long long builds_seen = 0;
 
   ~Engine() { release(); }
@@ -389,8 +402,7 @@ struct Engine {
       nz = 1;
       fs = 0;
     }
-    if (graph_exec) hipGraphExecDestroy(graph_exec);
-    graph_exec = nullptr;
+    invalidate_graph();
     dfree(d_timing);
     timing_slots = 0;
     dfree(d_x); dfree(d_v); dfree(d_xref); dfree(d_shift); dfree(d_q);
@@ -503,10 +515,7 @@ struct Engine {
     }
     d_epair = dalloc<double>(2 * (size_t)max_groups * nseg);
     forces_valid = false;
-    if (graph_exec) {
-      hipGraphExecDestroy(graph_exec);
-      graph_exec = nullptr;
-    }
+    invalidate_graph();
   }
 
   void create(const mdrtHipSystemDesc *d) {
@@ -950,6 +959,7 @@ struct Engine {
     int cap = (int)(1.5 * mean + 128.0);
     cap = std::max(cap, cap_min);
     cap = (cap + 63) & ~63;
+    invalidate_graph();  // captured graphs point at the old lists
     dfree(bp.count);
     dfree(bp.list);
     bp.cap = cap;
@@ -1246,10 +1256,7 @@ struct Engine {
       dfree(d_timing);
       d_timing = dalloc<unsigned long long>(3 * (size_t)count);
       timing_slots = count;
-      if (graph_exec) {  // a captured graph points at the old buffer
-        hipGraphExecDestroy(graph_exec);
-        graph_exec = nullptr;
-      }
+      invalidate_graph();  // captured graphs point at the old buffer
     }
     if (clock_khz == 0.0) {
       int dev = 0, khz = 0;
@@ -1339,6 +1346,11 @@ struct Engine {
   void compute(double *F_out, double *E_out) {
     if (list_reach != cutoff + skin) configure_list(cutoff + skin);
     evaluate(true);
+    outputs(F_out, E_out);
+  }
+
+  // Forces (atom order, f64) and energy terms of the last evaluation.
+  void outputs(double *F_out, double *E_out) {
     if (F_out) {
       double *d_out = dalloc<double>(3 * (size_t)n);
       forces_to_atoms<<<blocks_for(n), 256, 0, stream>>>(n, d_place_of, d_F, d_out, 0);
@@ -1376,6 +1388,71 @@ struct Engine {
     }
   }
 
+  // The unordered atom pairs (i < j) of `rows` whose f32 minimum-image
+  // distance at the current positions is within `far`, as sorted unique
+  // keys (i << 32 | j); explicit masks drop the excluded pairs.
+  std::vector<long long> export_pairs(const PairEntry *rows, const int *row_count, double far,
+                                      int64_t capacity_hint) {
+    const float limit2 = (float)(far * far);
+    const unsigned long long cap2 =
+        (unsigned long long)std::max<int64_t>(capacity_hint, 1) * 4 + 1024;
+    int2 *d_pairs = dalloc<int2>(cap2);
+    unsigned long long *d_count = dalloc<unsigned long long>(1);
+    nb_export_pairs<<<blocks_for(max_groups, 4), 256, 0, stream>>>(
+        d_x, d_atom_at, d_real_mask, rows, row_count, d_ex, d_counts, g.row_cap, (float)box[0],
+        (float)box[1], (float)box[2], box[0], box[1], box[2], limit2, d_pairs, d_count, cap2);
+    unsigned long long cnt = 0;
+    HIPCHK(hipMemcpyAsync(&cnt, d_count, sizeof(cnt), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    if (cnt > cap2) {
+      dfree(d_pairs);
+      dfree(d_count);
+      return export_pairs(rows, row_count, far, (int64_t)cnt);
+    }
+    std::vector<int2> h(cnt);
+    if (cnt) to_host(h.data(), d_pairs, cnt * sizeof(int2));
+    dfree(d_pairs);
+    dfree(d_count);
+    std::vector<long long> keys(cnt);
+    for (size_t i = 0; i < cnt; ++i) keys[i] = ((long long)h[i].x << 32) | (unsigned)h[i].y;
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    return keys;
+  }
+
+  // The check of the list between rebuilds (the counterpart of the
+  // reference's MDIR_CHECK_PRUNING, runtime/mdrt.c:48-79): the forces the
+  // next step would compute at the current positions, through the list the
+  // steps hold (the build kernels run only if the last drift asked for a
+  // rebuild, as in a step), and the pairs within rc found in that list and
+  // in the rows the pair loop runs over (the pruned rows with a dual list).
+  // counts[0], counts[1]: those pairs; counts[2]: list builds the check
+  // itself did (0 unless the last drift asked for one).
+  void check_list(double *F_out, double *E_out, int64_t *counts) {
+    if (!forces_valid || list_reach != cutoff + skin)
+      throw HipError("no list of a run to check: the positions changed since the last step");
+    const int builds0 = read_flags()[kFlagBuilds];
+    clear_forces();
+    pack_pme_atoms();
+    forces(true, -1, true);
+    fold_pme_atoms();
+    const std::vector<int> f = read_flags();
+    if (grow_if_overflowed(f)) {
+      forces_valid = false;
+      throw HipError("the list overflowed at the check");
+    }
+    outputs(F_out, E_out);
+    if (counts) {
+      const double far = cutoff + 3.0e-6 * ((box[0] + box[1]) + box[2]);
+      const int64_t hint = 8 * (int64_t)n;
+      counts[0] = (int64_t)export_pairs(d_rows, d_row_count, far, hint).size();
+      counts[1] = prune_skin > 0
+                      ? (int64_t)export_pairs(d_rows_in, d_row_count_in, far, hint).size()
+                      : counts[0];
+      counts[2] = f[kFlagBuilds] - builds0;
+    }
+  }
+
   double kinetic_energy() {
     std::vector<double> v(3 * (size_t)n);
     to_host(v.data(), d_v, v.size() * sizeof(double));
@@ -1385,26 +1462,41 @@ struct Engine {
     return ke;
   }
 
-  void capture(int steps, double dt) {
-    if (graph_exec) {
-      hipGraphExecDestroy(graph_exec);
-      graph_exec = nullptr;
+  // The captured sequence of `steps` steps of dt (with the pair-loop clock
+  // readings when `measure` is set), from the cache or captured now.
+  hipGraphExec_t capture(int steps, double dt) {
+    if (hipGraphExec_t e = find_graph(steps, dt, measure)) return e;
+    if (step_graphs.size() >= kMaxStepGraphs) {
+      hipGraphExecDestroy(step_graphs.front().exec);
+      step_graphs.erase(step_graphs.begin());
     }
     hipGraph_t graph;
     HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     steps_seq(steps, dt);
     HIPCHK(hipStreamEndCapture(stream, &graph));
-    HIPCHK(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t st = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     HIPCHK(hipGraphDestroy(graph));
-    graph_steps = steps;
-    dt_graph = dt;
-    graph_measure = measure;
+    HIPCHK(st);
+    step_graphs.push_back({steps, dt, measure, exec});
+    return exec;
   }
 
-  // Steps in chunks (one graph launch, or up to 64 single steps); after each
-  // chunk the flags are read, and a chunk whose list overflowed is redone
-  // from its saved state with the grown list.
-  bool graph_measure = false;
+  // Captures, without running them, the graphs a run of `steps` steps in
+  // chunks of `graph` replays: the graph of `graph` steps and that of the
+  // remainder.  Capture time then stays out of a timed run.
+  void prepare(int64_t steps, double dt, int graph) {
+    if (graph <= 0 || steps <= 0) return;
+    if (list_reach != cutoff + skin) configure_list(cutoff + skin);
+    if (!forces_valid) evaluate(false);
+    if (steps >= graph) capture(graph, dt);
+    if (steps % graph) capture((int)(steps % graph), dt);
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+
+  // Steps in chunks (one graph launch, or up to 64 single steps without
+  // graphs); after each chunk the flags are read, and a chunk whose list
+  // overflowed is redone from its saved state with the grown list.
   void run(int64_t steps, double dt, int graph, double *kinetic, double *pair_ms = nullptr) {
     if (list_reach != cutoff + skin) configure_list(cutoff + skin);
     if (!forces_valid) evaluate(false);
@@ -1420,10 +1512,8 @@ struct Engine {
         if (measure) reset_timing(todo);
         on_device(d_save_x, d_x, 3 * n * sizeof(double));
         on_device(d_save_v, d_v, 3 * n * sizeof(double));
-        if (graph > 0 && todo == graph) {
-          if (!graph_exec || graph_steps != graph || dt_graph != dt || graph_measure != measure)
-            capture(graph, dt);
-          HIPCHK(hipGraphLaunch(graph_exec, stream));
+        if (graph > 0) {
+          HIPCHK(hipGraphLaunch(capture((int)todo, dt), stream));
         } else {
           steps_seq(todo, dt);
         }
@@ -1657,26 +1747,8 @@ struct Engine {
     }
     d_excl_start = keep_start;
     dfree(empty);
-    const double far = reach + margin;
-    const float limit2 = (float)(far * far);
-    const unsigned long long cap2 = (unsigned long long)std::max<int64_t>(capacity, 1) * 4 + 1024;
-    int2 *d_pairs = dalloc<int2>(cap2);
-    unsigned long long *d_count = dalloc<unsigned long long>(1);
-    nb_export_pairs<<<blocks_for(max_groups, 4), 256, 0, stream>>>(
-        d_x, d_atom_at, d_real_mask, d_rows, d_row_count, d_ex, d_counts, g.row_cap, (float)box[0],
-        (float)box[1], (float)box[2], box[0], box[1], box[2], limit2, d_pairs, d_count, cap2);
-    unsigned long long cnt = 0;
-    HIPCHK(hipMemcpyAsync(&cnt, d_count, sizeof(cnt), hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipStreamSynchronize(stream));
-    if (cnt > cap2) cnt = cap2;
-    std::vector<int2> h(cnt);
-    if (cnt) to_host(h.data(), d_pairs, cnt * sizeof(int2));
-    dfree(d_pairs);
-    dfree(d_count);
-    std::vector<long long> keys(cnt);
-    for (size_t i = 0; i < cnt; ++i) keys[i] = ((long long)h[i].x << 32) | (unsigned)h[i].y;
-    std::sort(keys.begin(), keys.end());
-    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    const std::vector<long long> keys =
+        export_pairs(d_rows, d_row_count, reach + margin, capacity);
     const int64_t total = (int64_t)keys.size();
     for (int64_t i = 0; i < std::min(total, capacity); ++i) {
       out[2 * i] = (int32_t)(keys[i] >> 32);
@@ -1986,6 +2058,16 @@ int mdrtHipAutotune(void *system, int32_t what, int32_t steps, int32_t graphs, d
 int mdrtHipRun(void *system, int64_t steps, double dt, int32_t graph, double *kinetic) {
   Engine *e = static_cast<Engine *>(system);
   return guarded([&] { e->run(steps, dt, graph, kinetic); });
+}
+
+int mdrtHipCheckList(void *system, double *forces, double *energies, int64_t *counts) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] { e->check_list(forces, energies, counts); });
+}
+
+int mdrtHipPrepareRun(void *system, int64_t steps, double dt, int32_t graph) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] { e->prepare(steps, dt, graph); });
 }
 
 int mdrtHipRunMeasured(void *system, int64_t steps, double dt, int32_t graph,

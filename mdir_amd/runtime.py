@@ -122,6 +122,8 @@ SIGNATURES = [
     ("mdrtHipGetLaunchShape", C.c_int, [C.c_void_p, PI32]),
     ("mdrtHipFramesSetPmeTransform", C.c_int, [C.c_void_p, I32]),
     ("mdrtHipRun", C.c_int, [C.c_void_p, I64, D, I32, PD]),
+    ("mdrtHipPrepareRun", C.c_int, [C.c_void_p, I64, D, I32]),
+    ("mdrtHipCheckList", C.c_int, [C.c_void_p, PD, PD, PI64]),
     ("mdrtHipRunMeasured", C.c_int, [C.c_void_p, I64, D, I32, PD]),
     ("mdrtHipMinimize", C.c_int, [C.c_void_p, I64, D, PD]),
     ("mdrtHipNeighborPairs", C.c_int, [C.c_void_p, D, I32, I64, PI32, PI64]),

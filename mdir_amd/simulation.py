@@ -132,6 +132,24 @@ class Simulation:
                                      C.byref(ke)))
         return ke.value
 
+    def check_list(self):
+        """The check of the list between rebuilds (the reference's
+        MDIR_CHECK_PRUNING): after run(), the forces and energies the next
+        step computes through the list the steps hold, and a dict of the
+        pairs within rc found in the list ("list") and in the rows the pair
+        loop runs over ("pair_loop"), and the builds the check did."""
+        F = np.zeros((self.n, 3))
+        e = np.zeros(rt.NUM_ENERGIES)
+        c = np.zeros(3, np.int64)
+        rt.check(rt.lib().mdrtHipCheckList(self._h, rt.ptr(F, C.c_double), rt.ptr(e, C.c_double),
+                                           rt.ptr(c, C.c_int64)))
+        return F, e, {"list": int(c[0]), "pair_loop": int(c[1]), "builds": int(c[2])}
+
+    def prepare(self, steps: int, dt: float = 0.002, graph: int = 50):
+        """Captures the step graphs a later run(steps, dt, graph) replays,
+        without running them, so that the capture stays out of a timed run."""
+        rt.check(rt.lib().mdrtHipPrepareRun(self._h, int(steps), float(dt), int(graph)))
+
     def run_measured(self, steps: int, dt: float, graph: int = 50) -> float:
         """Runs like run() and returns the mean duration (ms) of the pair
         kernel, timed live on the device clock in every step."""

@@ -443,11 +443,94 @@ def solvated_solutes(nwaters: int = 1000, nsolutes: int = 30, seed: int = 4,
                   exclusions=excl, waters=first, water_geometry=(p["d_oh"], p["d_hh"]))
 
 
-def perturbed_frames(system: System, nframes: int, seed: int = 9, scale: float = 0.02):
+def lj_contacts(system: System, x: np.ndarray, fraction: float = 0.8) -> np.ndarray:
+    """The non-excluded pairs (i < j) with Lennard-Jones (epsilon > 0)
+    closer than `fraction` of their sigma at positions x (periodic box),
+    as an (m, 2) array."""
+    from scipy.spatial import cKDTree
+    L = np.asarray(system.box, np.float64)
+    lj = system.epsilon > 0
+    none = np.zeros((0, 2), np.int64)
+    if not lj.any():
+        return none
+    reach = fraction * float(system.sigma[lj].max())
+    tree = cKDTree(np.mod(x, L), boxsize=L)
+    pairs = tree.query_pairs(reach, output_type="ndarray")
+    if len(pairs) == 0:
+        return none
+    i, j = np.minimum(pairs[:, 0], pairs[:, 1]), np.maximum(pairs[:, 0], pairs[:, 1])
+    ti, tj = system.types[i], system.types[j]
+    d = x[j] - x[i]
+    d -= L * np.rint(d / L)
+    r = np.linalg.norm(d, axis=1)
+    close = lj[ti, tj] & (r < fraction * system.sigma[ti, tj])
+    ex = system.exclusions
+    if len(ex) and close.any():
+        key = lambda a, b: a.astype(np.int64) * system.n + b
+        close &= ~np.isin(key(i, j), key(ex[:, 0], ex[:, 1]))
+    return np.stack([i[close], j[close]], 1)
+
+
+def perturbed_frames(system: System, nframes: int, seed: int = 9, scale: float = 0.02,
+                     angle: float = 0.1, contact: float = 0.8):
     """nframes positions (nframes, n, 3): the system's positions with seeded
-    Gaussian displacements of `scale` nm on every atom."""
+    moves: every rigid water turned by a random rotation of Gaussian angle
+    (`angle` rad) about its oxygen and moved as a unit, every other atom
+    moved on its own, by Gaussian displacements of `scale` nm per
+    component.  Molecules in a non-excluded Lennard-Jones pair closer than
+    `contact` sigma draw their moves again (from the same seeded stream)
+    until none is left, so no frame carries overlaps that would dominate
+    its forces (pairs already that close in the system's own positions
+    are left as they are)."""
     rng = np.random.default_rng(seed)
-    return system.positions[None] + rng.normal(0.0, scale, (nframes,) + system.positions.shape)
+    x0 = system.positions
+    n = system.n
+    w = np.asarray(system.waters, np.int64)
+    # molecule of each atom: a water (its index) or the atom on its own
+    mol = np.arange(n) + len(w)
+    for k in range(3):
+        mol[w + k] = np.arange(len(w))
+    is_free = np.ones(n, bool)
+    for k in range(3):
+        is_free[w + k] = False
+    free_atoms = np.flatnonzero(is_free)
+
+    def move(x, waters, atoms):
+        x[atoms] = x0[atoms] + rng.normal(0.0, scale, (len(atoms), 3))
+        if len(waters) == 0:
+            return
+        o = w[waters]
+        axis = rng.normal(size=(len(o), 3))
+        axis /= np.linalg.norm(axis, axis=1)[:, None]
+        phi = rng.normal(0.0, angle, len(o))
+        shift = rng.normal(0.0, scale, (len(o), 3))
+        c, s_ = np.cos(phi)[:, None], np.sin(phi)[:, None]
+        x[o] = x0[o] + shift
+        for k in (1, 2):   # Rodrigues' rotation of the O-H arms
+            r = x0[o + k] - x0[o]
+            kr = np.cross(axis, r)
+            kd = np.sum(axis * r, axis=1)[:, None]
+            x[o + k] = x[o] + r * c + kr * s_ + axis * kd * (1.0 - c)
+
+    # contacts of the system's own positions are its own: only those the
+    # moves create are drawn again
+    key = lambda p: p[:, 0].astype(np.int64) * n + p[:, 1]
+    base = key(lj_contacts(system, x0, contact))
+    out = np.empty((nframes,) + x0.shape)
+    for f in range(nframes):
+        x = x0.copy()
+        move(x, np.arange(len(w)), free_atoms)
+        for _round in range(200):
+            bad = lj_contacts(system, x, contact)
+            bad = bad[~np.isin(key(bad), base)]
+            if len(bad) == 0:
+                break
+            m = np.unique(mol[bad.ravel()])
+            move(x, m[m < len(w)], m[m >= len(w)] - len(w))
+        else:
+            raise ValueError("Lennard-Jones contacts left after 200 rounds of moves")
+        out[f] = x
+    return out
 
 
 def perturbed_parameters(system: System, seed: int = 13, rel: float = 0.02, dq: float = 0.02):

@@ -27,7 +27,7 @@
 /* ======================================================================
  * Neighbor matrix — lib/Runtime/Templates/NeighborsMatrix.mlir:36-460.
  * Pinned by the golden counts of test/Runtime/neighbors-matrix.mlir:111-160
- * and neighbors-matrix-gpu.mlir:148-186 (tests/test_oracle_neighbors.py)
+ * and neighbors-matrix-gpu.mlir:148-186 (tests/test_oracle_golden.py)
  * and by brute force over all pairs.
  * ====================================================================== */
 
@@ -1263,24 +1263,31 @@ uint64_t orc_xoshiro256ss(uint64_t state[4]) {
   return r;
 }
 
+/* The pre-constraint normals: three per particle, particle i drawing the
+   six uniforms at counters 6 i .. 6 i + 5 (InitialVelocityDraws.h:30-65;
+   System.cpp:1298-1311).  Particles are visited in order, so the counter
+   6 i is reached by drawing sequentially (the reference seeks otherwise). */
+void orc_initial_normals(uint64_t seed, int64_t n, double *out) {
+  xoshiro g;
+  uint64_t st = seed;
+  for (int k = 0; k < 4; ++k) g.s[k] = orc_splitmix64(&st);
+  for (int64_t i = 0; i < 3 * n; ++i) {
+    double u1 = ((double)(xoshiro_next(&g) >> 11) + 1.0) * 0x1.0p-53;
+    double u2 = ((double)(xoshiro_next(&g) >> 11) + 1.0) * 0x1.0p-53;
+    double radius = sqrt(-2.0 * log(u1));
+    out[i] = radius * cos(6.283185307179586 * u2);
+  }
+}
+
 void orc_initial_velocities(uint64_t seed, int64_t n, const double *mass,
                             double temperature, const double *x,
                             int64_t nwaters, const int32_t *water_first,
                             int64_t dof, double *v) {
   const double boltzmann = 0.0083144626181532395; /* kJ/(mol K), SI-derived */
-  xoshiro g;
-  uint64_t st = seed;
-  for (int k = 0; k < 4; ++k) g.s[k] = orc_splitmix64(&st);
-  /* Particles are visited in order, so the counter 6 i is reached by
-     drawing sequentially (InitialVelocityDraws.h:30-40 seeks otherwise). */
+  orc_initial_normals(seed, n, v);
   for (int64_t i = 0; i < n; ++i) {
     double width = mass[i] > 0.0 ? sqrt(boltzmann * temperature / mass[i]) : 0.0;
-    for (int c = 0; c < 3; ++c) {
-      double u1 = ((double)(xoshiro_next(&g) >> 11) + 1.0) * 0x1.0p-53;
-      double u2 = ((double)(xoshiro_next(&g) >> 11) + 1.0) * 0x1.0p-53;
-      double radius = sqrt(-2.0 * log(u1));
-      v[3 * i + c] = width * radius * cos(6.283185307179586 * u2);
-    }
+    for (int c = 0; c < 3; ++c) v[3 * i + c] *= width;
   }
   if (n < 2) return;
   /* Velocities that the constraints allow (System.cpp:1316-1321). */

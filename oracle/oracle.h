@@ -162,6 +162,7 @@ int orc_threads(void);
 /* ---- initial velocities (System.cpp:1298-1350; InitialVelocityDraws.h) ---- */
 uint64_t orc_splitmix64(uint64_t *state);
 uint64_t orc_xoshiro256ss(uint64_t state[4]);
+void orc_initial_normals(uint64_t seed, int64_t n, double *out);
 void orc_initial_velocities(uint64_t seed, int64_t n, const double *mass,
                             double temperature, const double *x,
                             int64_t nwaters, const int32_t *water_first,

@@ -14,6 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "liboracle.so")
 REF_FFT = os.path.join(HERE, "_ref", "libmdrt_fft_ref.so")
+REF_DRAWS = os.path.join(HERE, "_ref", "libref_draws.so")
 
 E_LJ, E_COUL_DIRECT, E_COUL_EXCL, E_COUL_RECIP, E_COUL_SELF, E_DISP_CORR, \
     E_LJ14, E_COUL14 = range(8)
@@ -108,6 +109,7 @@ def _declare(L):
     L.orc_splitmix64.argtypes = [C.POINTER(C.c_uint64)]
     L.orc_xoshiro256ss.restype = C.c_uint64
     L.orc_xoshiro256ss.argtypes = [C.POINTER(C.c_uint64)]
+    L.orc_initial_normals.argtypes = [C.c_uint64, I64, PD]
     L.orc_initial_velocities.argtypes = [C.c_uint64, I64, PD, D, PD, I64, PI32,
                                          I64, PD]
 
@@ -289,6 +291,13 @@ class Oracle:
         return v
 
 
+def initial_normals(seed, n):
+    """The pre-constraint normals of the initial velocities, (n, 3)."""
+    out = np.zeros((n, 3))
+    lib().orc_initial_normals(seed, n, _p(out, D))
+    return out
+
+
 # ---- FFT helpers ----
 
 def fft3d_r2c(real):
@@ -369,6 +378,23 @@ def ref_fft3d_c2r(cplx, k3):
     rb = _Buffer(out.ctypes.data, out.ctypes.data, 0, out.size, 1)
     L._mlir_ciface_mdrtFFT64Backward3D(C.byref(cb), C.byref(rb), I64(K[0]),
                                        I64(K[1]), I64(K[2]))
+    return out
+
+
+# ---- the reference's own initial-velocity draws (oracle/_ref) ----
+
+def ref_draws_available():
+    return os.path.exists(REF_DRAWS)
+
+
+def ref_initial_normals(seed, order):
+    """lib/Driver/InitialVelocityDraws.h (built into _ref): the three normals
+    of each particle of `order`, visited in that order; (n, 3) by particle."""
+    L = C.CDLL(REF_DRAWS)
+    L.ref_initial_normals.argtypes = [C.c_uint64, I64, PI64, PD]
+    order = np.ascontiguousarray(order, np.int64)
+    out = np.full((len(order), 3), np.nan)
+    L.ref_initial_normals(seed, len(order), _p(order, I64), _p(out, D))
     return out
 
 
