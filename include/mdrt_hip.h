@@ -79,9 +79,15 @@ void mgpuEventRecord(mgpuEvent event, mgpuStream stream);
 /* mdrt_cuda.c:2147, :2238 */
 void *mgpuMemAlloc(uint64_t size, mgpuStream stream, bool isHostShared);
 void mgpuMemFree(void *pointer, mgpuStream stream);
-/* mdrt_cuda.c:2431 */
+/* mdrt_cuda.c:2431 — finished on return when either end is host memory */
 void mgpuMemcpy(void *destination, void *source, size_t size,
                 mgpuStream stream);
+/* mdrt_cuda.c:2471, :2490 — a copy on the current stream, finished on
+   return without waiting for work on the side streams; and the same wait
+   without a copy */
+void mdrtCudaMemcpyScoped(uintptr_t destination, uintptr_t source,
+                          uint64_t size);
+void mdrtCudaWaitScoped(void);
 /* mdrt_cuda.c:2494, :2510 */
 void mgpuMemset32(void *destination, unsigned int value, size_t count,
                   mgpuStream stream);
@@ -118,6 +124,9 @@ void _mlir_ciface_mdrtHostResults(mdrtBuffer1 *result, int64_t place,
 void mdrtCountHostRead(size_t bytes, int waits, int flags);
 /* mdrt_cuda.c:2566 */
 void mdrtCudaSetStopHandler(void (*handler)(const char *));
+/* mdrt_cuda.c:3817 — called before the runtime aborts on a failed device
+   call or a misuse of the ABI */
+void mdrtSetFailureHook(void (*hook)(void));
 /* mdrt_cuda.c:2824, :2836 */
 uint64_t mdrtDeviceTrim(void);
 void mdrtDeviceMemoryCounts(int64_t counts[8]);

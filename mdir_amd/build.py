@@ -42,7 +42,21 @@ def build(verbose: bool = True) -> str:
     run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", obj_engine, obj_rt,
          "-o", tmp, f"-L{ROCM}/lib", "-lamdhip64"])
     os.replace(tmp, LIB)
+    build_abi_module()
     return LIB
+
+
+ABI_MODULE = os.path.join(LIBDIR, "abi_module.co")
+
+
+def build_abi_module() -> str:
+    """The small code object the ABI tests load through mgpuModuleLoad
+    (tests/abi_module.hip), as a compiled program hands its kernels over."""
+    src = os.path.join(ROOT, "tests", "abi_module.hip")
+    hipcc = os.path.join(ROCM, "bin", "hipcc")
+    run([hipcc, "--genco", f"--offload-arch={ARCH}", "-O3", "-Xclang", "-target-feature",
+         "-Xclang", "-packed-fp32-ops", src, "-o", ABI_MODULE])
+    return ABI_MODULE
 
 
 if __name__ == "__main__":

@@ -77,6 +77,8 @@ SIGNATURES = [
     ("mgpuMemAlloc", C.c_void_p, [C.c_uint64, C.c_void_p, C.c_bool]),
     ("mgpuMemFree", None, [C.c_void_p, C.c_void_p]),
     ("mgpuMemcpy", None, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    ("mdrtCudaMemcpyScoped", None, [C.c_size_t, C.c_size_t, C.c_uint64]),
+    ("mdrtCudaWaitScoped", None, []),
     ("mgpuMemset32", None, [C.c_void_p, C.c_uint, C.c_size_t, C.c_void_p]),
     ("mgpuMemset16", None, [C.c_void_p, C.c_ushort, C.c_size_t, C.c_void_p]),
     ("mdrtSideBegin", None, []),
@@ -93,6 +95,7 @@ SIGNATURES = [
     ("_mlir_ciface_mdrtHostResults", None, [C.POINTER(Buffer1), I64, I64]),
     ("mdrtCountHostRead", None, [C.c_size_t, C.c_int, C.c_int]),
     ("mdrtCudaSetStopHandler", None, [C.c_void_p]),
+    ("mdrtSetFailureHook", None, [C.c_void_p]),
     ("mdrtDeviceTrim", C.c_uint64, []),
     ("mdrtDeviceMemoryCounts", None, [PI64]),
     ("_mlir_ciface_mdrtCudaFFTForward3D", None,

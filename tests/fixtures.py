@@ -4,6 +4,7 @@ from __future__ import annotations
 
 import math
 import os
+import re
 
 import numpy as np
 
@@ -13,65 +14,82 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 KCAL = 4.184
 
 
+# Amber stores charges in units that make q_i q_j / r come out in kcal/mol
+# with r in Angstrom: e times sqrt(332.0522...) = 18.2223 (the Amber
+# topology format's fixed factor).
+AMBER_CHARGE_UNIT = 18.2223
+
+_FORTRAN_FIELD = re.compile(r"(\d*)([AaIiEeFf])(\d+)(?:\.\d+)?")
+
+
 def read_prmtop(path):
-    """Sections of an Amber prmtop, as the reference's
-    test/Driver/Inputs/ewald_reference.py:24-39 reads them."""
-    sections, flag, fmt = {}, None, None
-    for line in open(path):
-        if line.startswith("%FLAG"):
-            flag = line.split()[1]
-            sections[flag] = []
-        elif line.startswith("%FORMAT"):
-            fmt = line[line.index("(") + 1: line.index(")")]
-        elif line.startswith("%"):
-            continue
-        elif flag:
-            spec = fmt.split("a")[-1].split("I")[-1].split("E")[-1].split(".")[0]
-            width = int("".join(c for c in spec if c.isdigit()))
-            body = line.rstrip("\n")
-            for i in range(0, len(body), width):
-                field = body[i: i + width]
-                if field.strip():
-                    sections[flag].append(field.strip())
+    """An Amber topology (prmtop) as {FLAG: list of typed values}.
+
+    The file is a series of sections, each "%FLAG NAME", then
+    "%FORMAT(<count><kind><width>[.<decimals>])" (a Fortran edit
+    descriptor: kind a = text, I = integer, E/F = real), then lines holding
+    up to <count> fields of <width> columns each."""
+    sections = {}
+    name, kind, width = None, None, None
+    with open(path) as fh:
+        for raw in fh:
+            line = raw.rstrip("\n")
+            if line.startswith("%FLAG"):
+                name = line[5:].strip()
+                sections[name] = []
+            elif line.startswith("%FORMAT"):
+                m = _FORTRAN_FIELD.search(line)
+                kind, width = m.group(2).lower(), int(m.group(3))
+            elif line.startswith("%") or name is None:
+                continue
+            else:
+                cast = {"a": str.strip, "i": int}.get(kind, float)
+                fields = (line[k: k + width] for k in range(0, len(line), width))
+                sections[name].extend(cast(f) for f in fields if f.strip())
     return sections
+
+
+def read_inpcrd(path, n):
+    """Coordinates (nm) and box edges (nm) of an Amber restart file: a title,
+    a line with the atom count, then reals in fields of 12 columns: 3n
+    coordinates in Angstrom followed by the box lengths and angles."""
+    with open(path) as fh:
+        body = fh.read().splitlines()[2:]
+    vals = [float(line[k: k + 12]) for line in body for k in range(0, len(line), 12)
+            if line[k: k + 12].strip()]
+    xyz = 0.1 * np.asarray(vals[: 3 * n]).reshape(n, 3)
+    return xyz, 0.1 * np.asarray(vals[3 * n: 3 * n + 3])
 
 
 def load_opc3(cutoff=0.9, beta=3.47045919371208278, grid=(64, 64, 64), order=8):
     """The three OPC waters of test/Driver/Inputs/opc/three.{prmtop,inpcrd}
-    (data files of the reference's tests, copied as fixtures), extra points
-    placed along the bisector at their bond length as
-    ewald_reference.py:56-72 places them; every pair of a residue excluded."""
+    (data files of the reference's tests, kept as fixtures).  Each residue
+    is O, H, H and a massless charge site EP; the restart file gives EP no
+    meaningful position, so it is put where the OPC model has it: on the
+    bisector of the H-O-H angle, at the O-EP bond length of the topology's
+    bond table from the oxygen.  Every pair within a residue is excluded."""
     top = read_prmtop(os.path.join(GOLDEN, "opc3", "three.prmtop"))
-    charges = np.array([float(q) / 18.2223 for q in top["CHARGE"]])
-    n = len(charges)
-    starts = [int(s) - 1 for s in top["RESIDUE_POINTER"]] + [n]
-    types = top["AMBER_ATOM_TYPE"]
-    lines = open(os.path.join(GOLDEN, "opc3", "three.inpcrd")).read().splitlines()
-    numbers = []
-    for line in lines[2:]:
-        numbers += [float(line[i: i + 12]) for i in range(0, len(line), 12)
-                    if line[i: i + 12].strip()]
-    x = np.array([[0.1 * numbers[3 * i + k] for k in range(3)] for i in range(n)])
-    box = np.array([0.1 * numbers[3 * n + k] for k in range(3)])
-    bonds = [int(v) for v in top["BONDS_WITHOUT_HYDROGEN"]]
-    lengths = [0.1 * float(v) for v in top["BOND_EQUIL_VALUE"]]
-    for b in range(0, len(bonds), 3):
-        i, j, t = bonds[b] // 3, bonds[b + 1] // 3, bonds[b + 2] - 1
-        site, owner = (i, j) if types[i] == "EP" else (j, i)
-        if types[site] != "EP":
-            continue
-        units = []
-        for a in (owner + 1, owner + 2):
-            d = x[a] - x[owner]
-            units.append(d / np.linalg.norm(d))
-        s = units[0] + units[1]
-        x[site] = x[owner] + lengths[t] * s / np.linalg.norm(s)
+    q = np.asarray(top["CHARGE"]) / AMBER_CHARGE_UNIT
+    n = q.size
+    x, box = read_inpcrd(os.path.join(GOLDEN, "opc3", "three.inpcrd"), n)
+    kinds = top["AMBER_ATOM_TYPE"]
+    # bond table entries: (3 i, 3 j, type + 1) triplets, lengths in Angstrom
+    bonds = np.asarray(top["BONDS_WITHOUT_HYDROGEN"] + top["BONDS_INC_HYDROGEN"]).reshape(-1, 3)
+    length = {}
+    for a3, b3, t in bonds:
+        length[frozenset((a3 // 3, b3 // 3))] = 0.1 * top["BOND_EQUIL_VALUE"][t - 1]
+    first = [p - 1 for p in top["RESIDUE_POINTER"]] + [n]
     excl = []
-    for r in range(len(starts) - 1):
-        for i in range(starts[r], starts[r + 1]):
-            for j in range(i + 1, starts[r + 1]):
-                excl.append((i, j))
-    return System(positions=x, masses=np.ones(n), charges=charges,
+    for lo, hi in zip(first[:-1], first[1:]):
+        members = list(range(lo, hi))
+        excl += [(i, j) for k, i in enumerate(members) for j in members[k + 1:]]
+        oxygen = [i for i in members if kinds[i] == "OW"]
+        hydrogens = [i for i in members if kinds[i] == "HW"]
+        for site in (i for i in members if kinds[i] == "EP"):
+            o = oxygen[0]
+            u = sum((x[h] - x[o]) / np.linalg.norm(x[h] - x[o]) for h in hydrogens)
+            x[site] = x[o] + length[frozenset((site, o))] * u / np.linalg.norm(u)
+    return System(positions=x, masses=np.ones(n), charges=q,
                   types=np.zeros(n, np.int32), sigma=np.zeros((1, 1)),
                   epsilon=np.zeros((1, 1)), box=box, cutoff=cutoff, beta=beta,
                   pme_grid=tuple(grid), pme_order=order,
