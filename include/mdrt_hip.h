@@ -286,6 +286,52 @@ int mdrtHipGetLaunchShape(void *system, int32_t *shape);
 int mdrtHipAutotune(void *system, int32_t what, int32_t steps, int32_t graphs, double dt,
                     double *ms_per_step);
 
+/* Langevin dynamics for the steps of mdrtHipRun: friction gamma (1/ps) > 0
+   makes each step the g-BAOAB step (kick, half drift, O, half drift, kick;
+   SETTLE and the velocity constraint after each part) that the reference's
+   Langevin integrator with constraints takes (lib/Driver/DynamicsIR.cpp,
+   geodesic branch; docs/dialects/dyn.md:441-460 for the O step):
+   v <- exp(-gamma dt) v + sqrt((1 - exp(-2 gamma dt)) kB T / m) R, R three
+   normals per atom from one Philox 4x32-10 block of (seed, step number,
+   atom index, stream 2), as runtime/mdrt.c:289-312 and
+   lib/Driver/NoiseIR.cpp draw them. friction 0 restores velocity Verlet. */
+int mdrtHipSetLangevin(void *system, double temperature, double friction, uint64_t seed);
+
+/* The box edges (nm). */
+int mdrtHipGetBox(void *system, double *box /* 3 */);
+
+/* Scales every unit's centre of mass (a SETTLE water, or a free atom) by
+   scale[a] along axis a and the box with it; units keep their shape. */
+int mdrtHipScaleBox(void *system, const double *scale /* 3 */);
+
+/* One isotropic Monte Carlo volume move at pressure_bar and temperature (K)
+   (the NPT coupling of the reference's Amber comparison runs,
+   scripts/benchmarks/amber/README.md:56, with the reference's pressure
+   units, bar): a volume change uniform in [-dv, dv] (dv adapted towards
+   25-75 % acceptance), units' centres scaled, the energy evaluated at the
+   new volume, and the move kept with probability
+   min(1, exp(-W / kT)), W = U1 - U0 + P dV - N_units kT ln(V1 / V0).
+   Random numbers: Philox 4x32-10 of (seed, step number, 0, stream 1).
+   *accepted (may be NULL) 1 if kept; info[4] (may be NULL): U0, U1, dV
+   (nm^3), W (kJ/mol). */
+int mdrtHipBarostatMove(void *system, double pressure_bar, double temperature, uint64_t seed,
+                        int32_t *accepted, double *info);
+
+/* `steps` steps of mdrtHipRun (Langevin when set) with a volume move after
+   every `every` steps. */
+int mdrtHipRunNPT(void *system, int64_t steps, double dt, int32_t graph, int32_t every,
+                  double pressure_bar, double temperature, uint64_t seed, double *kinetic);
+
+/* Volume moves tried and kept (counts[2]) and the current largest volume
+   change (nm^3). */
+int mdrtHipBarostatStats(void *system, int64_t *counts, double *max_dv);
+
+/* The step number of the next step (the Langevin noise's step counter):
+   0 at creation, advanced by every step mdrtHipRun takes (a restart sets it
+   to the checkpoint's count). */
+int mdrtHipSetStepCount(void *system, int64_t step);
+int mdrtHipGetStepCount(void *system, int64_t *step);
+
 /* `steps` velocity-Verlet steps of dt (ps) with SETTLE waters.  graph != 0
    replays the step sequence as captured HIP graphs of `graph` steps (and
    one of the remainder steps mod graph).

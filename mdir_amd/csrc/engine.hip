@@ -267,6 +267,149 @@ struct Engine {
   int *d_unit_first = nullptr, *d_unit_size = nullptr;
   IntParams ip{};
 
+  // ---- Langevin thermostat ----
+  // friction > 0: the steps are g-BAOAB Langevin steps at `temperature`
+  // (K) with the noise of (seed, step number, atom); 0: velocity Verlet.
+  double lv_friction = 0.0, lv_temperature = 0.0;
+  unsigned long long lv_seed = 0;
+  long long *d_lstep = nullptr;   // [0] step number of the next step, [1] blocks done
+  int64_t step_count = 0;         // steps taken: the step number of the next step
+  void set_langevin(double temperature, double friction, unsigned long long seed) {
+    if (friction < 0.0 || !(temperature >= 0.0)) throw HipError("Langevin: friction and temperature must not be negative");
+    lv_friction = friction;
+    lv_temperature = temperature;
+    lv_seed = seed;
+    if (friction > 0.0 && !d_lstep) d_lstep = dalloc<long long>(2);
+    invalidate_graph();  // captured graphs hold the integration parameters
+  }
+  // ---- Monte Carlo barostat ----
+  // Isotropic volume moves: the units' centres of mass scale with the box
+  // (scale_units), the energy is evaluated at the new volume, and the move
+  // is kept with the Metropolis probability of
+  //   W = U1 - U0 + P (V1 - V0) - N_units kT ln(V1 / V0)
+  // (the isothermal-isobaric ensemble of molecular centres; Eppenga and
+  // Frenkel, Mol. Phys. 52, 1303 (1984); Chow and Ferguson, Comput. Phys.
+  // Commun. 91, 283 (1995)).  The volume change is uniform in
+  // [-dv, dv]; dv starts at 1 % of the volume and is adapted every 10
+  // moves towards an acceptance between 25 % and 75 %.  The random numbers
+  // are Philox words of (seed, step number, 0, stream 1 << 24).
+  double mc_dv = 0.0;
+  int64_t mc_tried = 0, mc_kept = 0, mc_window_tried = 0, mc_window_kept = 0;
+
+  void set_box(const double *nb) {
+    const double rl = cutoff + skin;
+    for (int k = 0; k < 3; ++k)
+      if (!(nb[k] > 2.0 * rl)) throw HipError("every box edge must exceed 2 (cutoff + skin)");
+    for (int k = 0; k < 3; ++k) {
+      box[k] = nb[k];
+      nbp.box[k] = (float)box[k];
+      nbp.box_lo[k] = (float)(box[k] - (double)nbp.box[k]);
+      nbp.half_box[k] = (float)(0.5 * box[k]);
+      spp.box[k] = box[k];
+      ip.half_box[k] = 0.5 * box[k];
+      pmp.box[k] = box[k];
+      pmp.inv_box[k] = 1.0 / box[k];
+    }
+    host_constants();
+    if (use_pme) {
+      dfree(d_tab);
+      make_influence_tables();
+    }
+    configure_list(list_reach > 0 ? list_reach : cutoff + skin);
+    forces_valid = false;
+    invalidate_graph();
+  }
+
+  // Units' centres of mass scaled by s (per axis) and the box with them.
+  void scale_box(const double *s) {
+    scale_units<<<blocks_for(nunits, 64), 64, 0, stream>>>(nunits, d_unit_first, d_unit_size,
+                                                           d_inv_mass, d_x, s[0], s[1], s[2]);
+    HIPCHK(hipGetLastError());
+    const double nb[3] = {box[0] * s[0], box[1] * s[1], box[2] * s[2]};
+    set_box(nb);
+  }
+
+  double potential_energy() {
+    double E[MDRT_NUM_ENERGIES];
+    compute(nullptr, E);
+    double u = 0.0;
+    for (int k = 0; k < MDRT_E_KINETIC; ++k) u += E[k];
+    return u;
+  }
+
+  // One volume move; info (or null): U0, U1, V1 - V0, W (kJ/mol, nm^3).
+  bool barostat_move(double pressure_bar, double temperature, unsigned long long seed,
+                     double *info) {
+    const double kB = 0x1.1072c483af26dp-7;                 // kJ mol^-1 K^-1
+    const double bar = 1e5 * 1e-27 * 6.02214076e23 * 1e-3;  // kJ mol^-1 nm^-3 per bar
+    const double kT = kB * temperature, P = pressure_bar * bar;
+    if (!(kT > 0.0)) throw HipError("the barostat needs a temperature above 0 K");
+    const double v0 = box[0] * box[1] * box[2];
+    if (!(mc_dv > 0.0)) mc_dv = 0.01 * v0;
+    uint32_t w[4] = {(uint32_t)(uint64_t)step_count, (uint32_t)((uint64_t)step_count >> 32),
+                     0u, 1u << 24};
+    philox4x32_10(w, (uint32_t)seed, (uint32_t)(seed >> 32));
+    auto uniform = [](uint32_t hi, uint32_t lo) {  // (0, 1), 53 bits
+      return ((double)((((uint64_t)hi << 32) | lo) >> 11) + 0.5) * 0x1.0p-53;
+    };
+    const double dv = mc_dv * (2.0 * uniform(w[0], w[1]) - 1.0);
+    const double v1 = v0 + dv;
+    const double u0 = potential_energy();
+    const double old_box[3] = {box[0], box[1], box[2]};
+    on_device(d_save_x, d_x, 3 * (size_t)n * sizeof(double));
+    const double sc = std::cbrt(v1 / v0);
+    const double s3[3] = {sc, sc, sc};
+    scale_box(s3);
+    const double u1 = potential_energy();
+    const double work = (u1 - u0) + P * dv - (double)nunits * kT * log(v1 / v0);
+    const bool keep = work <= 0.0 || uniform(w[2], w[3]) < exp(-work / kT);
+    if (!keep) {
+      on_device(d_x, d_save_x, 3 * (size_t)n * sizeof(double));
+      set_box(old_box);
+    }
+    if (info) {
+      info[0] = u0;
+      info[1] = u1;
+      info[2] = dv;
+      info[3] = work;
+    }
+    ++mc_tried;
+    ++mc_window_tried;
+    mc_kept += keep;
+    mc_window_kept += keep;
+    if (mc_window_tried == 10) {
+      const double rate = (double)mc_window_kept / (double)mc_window_tried;
+      if (rate < 0.25) mc_dv *= 0.9;
+      else if (rate > 0.75) mc_dv *= 1.1;
+      mc_dv = std::min(mc_dv, 0.1 * box[0] * box[1] * box[2]);
+      mc_window_tried = mc_window_kept = 0;
+    }
+    forces_valid = false;
+    return keep;
+  }
+
+  // steps in runs of `every` steps, each followed by a volume move.
+  void run_npt(int64_t steps, double dt, int graph, int every, double pressure_bar,
+               double temperature, unsigned long long seed, double *kinetic) {
+    if (every <= 0) throw HipError("the barostat interval must be at least one step");
+    int64_t done = 0;
+    while (done < steps) {
+      const int64_t todo = std::min<int64_t>(every, steps - done);
+      run(todo, dt, graph, nullptr);
+      done += todo;
+      if (todo == every) barostat_move(pressure_bar, temperature, seed, nullptr);
+    }
+    if (kinetic) *kinetic = kinetic_energy();
+  }
+
+  // The device's step number (read by the launches that take a first half)
+  // set to the host's count before a chunk of steps.
+  void put_step_count() {
+    if (!(lv_friction > 0.0)) return;
+    const long long h[2] = {(long long)step_count, 0};
+    to_device(d_lstep, h, sizeof(h));
+  }
+
   // ---- force terms ----
   // Which terms a force evaluation computes (bits of MDRT_TERM_*): the
   // nonbonded pairs (LJ + direct Coulomb), the excluded and 1-4 pairs, the
@@ -416,7 +559,7 @@ struct Engine {
     dfree(d_cplx); dfree(d_tab); dfree(d_erecip);
     dfree(d_dft1); dfree(d_dft2);
     dfree(bp.count); dfree(bp.list);
-    dfree(d_unit_first); dfree(d_unit_size);
+    dfree(d_unit_first); dfree(d_unit_size); dfree(d_lstep);
     if (ev_fork) hipEventDestroy(ev_fork);
     if (ev_join) hipEventDestroy(ev_join);
     ev_fork = ev_join = nullptr;
@@ -1218,6 +1361,16 @@ struct Engine {
   template <int PART>
   void integrate(double dt) {
     ip.dt = dt;
+    ip.langevin = lv_friction > 0.0;
+    if (ip.langevin) {
+      const double kB = 0x1.1072c483af26dp-7;  // kJ mol^-1 K^-1 (SI k_B N_A)
+      ip.decay = exp(-lv_friction * dt);
+      ip.spread = sqrt((1.0 - ip.decay * ip.decay) * kB * lv_temperature);
+      ip.seed = lv_seed;
+      ip.step = d_lstep;
+    } else {
+      ip.step = nullptr;
+    }
     integrate_part<PART><<<blocks_for(nunits, 64), 64, 0, stream>>>(
         nunits, d_unit_first, d_unit_size, d_x, d_v, d_inv_mass, d_F, d_Fr, d_place_of,
         d_shift, d_xref, d_q, d_xq, use_origins() ? d_gorig : nullptr,
@@ -1510,6 +1663,7 @@ struct Engine {
       const int64_t todo = std::min(chunk, steps - done);
       for (int attempt = 0;; ++attempt) {
         if (measure) reset_timing(todo);
+        put_step_count();
         on_device(d_save_x, d_x, 3 * n * sizeof(double));
         on_device(d_save_v, d_v, 3 * n * sizeof(double));
         if (graph > 0) {
@@ -1528,6 +1682,7 @@ struct Engine {
         evaluate(false);
       }
       done += todo;
+      step_count += todo;
     }
     HIPCHK(hipStreamSynchronize(stream));
     measure = false;
@@ -1640,7 +1795,9 @@ struct Engine {
     std::vector<double> x0(3 * (size_t)n), v0(3 * (size_t)n);
     to_host(x0.data(), d_x, bytes);
     to_host(v0.data(), d_v, bytes);
+    const int64_t step0 = step_count;
     auto restore = [&]() {
+      step_count = step0;
       HIPCHK(hipMemcpyAsync(d_x, x0.data(), bytes, hipMemcpyHostToDevice, stream));
       HIPCHK(hipMemcpyAsync(d_v, v0.data(), bytes, hipMemcpyHostToDevice, stream));
       HIPCHK(hipStreamSynchronize(stream));
@@ -2053,6 +2210,64 @@ int mdrtHipAutotune(void *system, int32_t what, int32_t steps, int32_t graphs, d
                     double *ms_per_step) {
   Engine *e = static_cast<Engine *>(system);
   return guarded([&] { e->autotune(what, steps, graphs, dt, ms_per_step); });
+}
+
+int mdrtHipSetLangevin(void *system, double temperature, double friction, uint64_t seed) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] { e->set_langevin(temperature, friction, seed); });
+}
+
+int mdrtHipGetBox(void *system, double *box) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] {
+    for (int k = 0; k < 3; ++k) box[k] = e->box[k];
+  });
+}
+
+int mdrtHipScaleBox(void *system, const double *scale) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] { e->scale_box(scale); });
+}
+
+int mdrtHipBarostatMove(void *system, double pressure_bar, double temperature, uint64_t seed,
+                        int32_t *accepted, double *info) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] {
+    const bool keep = e->barostat_move(pressure_bar, temperature, seed, info);
+    if (accepted) *accepted = keep;
+  });
+}
+
+int mdrtHipRunNPT(void *system, int64_t steps, double dt, int32_t graph, int32_t every,
+                  double pressure_bar, double temperature, uint64_t seed, double *kinetic) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] {
+    e->run_npt(steps, dt, graph, every, pressure_bar, temperature, seed, kinetic);
+  });
+}
+
+int mdrtHipBarostatStats(void *system, int64_t *counts, double *max_dv) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] {
+    if (counts) {
+      counts[0] = e->mc_tried;
+      counts[1] = e->mc_kept;
+    }
+    if (max_dv) *max_dv = e->mc_dv;
+  });
+}
+
+int mdrtHipSetStepCount(void *system, int64_t step) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] {
+    if (step < 0) throw mdhip::HipError("the step count must not be negative");
+    e->step_count = step;
+  });
+}
+
+int mdrtHipGetStepCount(void *system, int64_t *step) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] { *step = e->step_count; });
 }
 
 int mdrtHipRun(void *system, int64_t steps, double dt, int32_t graph, double *kinetic) {

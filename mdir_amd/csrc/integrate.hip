@@ -23,6 +23,15 @@ struct IntParams {
   double m_o, m_h;
   double ra, rb, rc;   // SETTLE canonical triangle
   double d_oh, d_hh;
+  // Langevin dynamics (langevin != 0): the g-BAOAB step with the O step
+  // v <- decay v + spread / sqrt(m) R, decay = exp(-gamma dt) and
+  // spread = sqrt((1 - decay^2) kB T); R from langevin_normals(seed, step,
+  // atom).  step[0] is the step number the next first half takes (the
+  // launch's last block advances it), step[1] counts finished blocks.
+  int langevin;
+  double decay, spread;
+  unsigned long long seed;
+  long long *step;
 };
 
 __device__ __forceinline__ void cross(const double *a, const double *b, double *c) {
@@ -195,38 +204,73 @@ __device__ __forceinline__ void constrain_velocities(UnitState<S> &s, const IntP
   if constexpr (S == 3) settle_velocities(s.x[0], s.x[1], s.x[2], s.v[0], s.v[1], s.v[2], p);
 }
 
-// Drift, SETTLE, v += dx_c/dt; stores x, v and the f32 positions of the
-// pair loop; returns bit 0 when an atom moved beyond half the list buffer
-// since the build, bit 1 beyond half the pruned list's buffer since the
-// last prune.
+// A drift of length h from positions x0 (constrained): xn = x0 + h v,
+// SETTLE from x0, and v += (xn_constrained - xn) / h.
 template <int S>
-__device__ __forceinline__ int drift_store(UnitState<S> &s, double *__restrict__ x,
-                                           double *__restrict__ v,
-                                           const double *__restrict__ shift,
-                                           const double *__restrict__ xref,
-                                           const double *__restrict__ q,
-                                           float4 *__restrict__ xq,
-                                           const float4 *__restrict__ gorig,
-                                           double4 *__restrict__ xpq,
-                                           double4 *__restrict__ xpa, const IntParams &p) {
-  double xn[S][3];
+__device__ __forceinline__ void drift_settle(const double (&x0)[S][3], double (&v)[S][3],
+                                             double (&xn)[S][3], double h, const IntParams &p) {
 #pragma unroll
   for (int m = 0; m < S; ++m)
 #pragma unroll
-    for (int k = 0; k < 3; ++k) xn[m][k] = s.x[m][k] + p.dt * s.v[m][k];
+    for (int k = 0; k < 3; ++k) xn[m][k] = x0[m][k] + h * v[m][k];
   if constexpr (S == 3) {
     double xu[3][3];
 #pragma unroll
     for (int m = 0; m < 3; ++m)
 #pragma unroll
       for (int k = 0; k < 3; ++k) xu[m][k] = xn[m][k];
-    settle_positions(s.x[0], s.x[1], s.x[2], xn[0], xn[1], xn[2], p);
-    const double idt = 1.0 / p.dt;
+    settle_positions(x0[0], x0[1], x0[2], xn[0], xn[1], xn[2], p);
+    const double ih = 1.0 / h;
 #pragma unroll
     for (int m = 0; m < 3; ++m)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) s.v[m][k] += (xn[m][k] - xu[m][k]) * idt;
+      for (int k = 0; k < 3; ++k) v[m][k] += (xn[m][k] - xu[m][k]) * ih;
   }
+}
+
+// The rest of a g-BAOAB step after its first half kick (B): the geodesic
+// integrator of Leimkuhler and Matthews (Proc. R. Soc. A 472, 20160138,
+// 2016), as the reference's Langevin step with constraints takes it
+// (lib/Driver/DynamicsIR.cpp, the geodesic branch): project v onto the
+// constraints; A: half drift, SETTLE, v += dx_c / (dt/2); O: v <- decay v +
+// spread / sqrt(m) R, projected again at the new positions; A: half drift
+// from there, SETTLE, v += dx_c / (dt/2).  The closing B (kick and
+// projection) is the second half of the step, as in velocity Verlet.
+template <int S>
+__device__ __forceinline__ void baoab_rest(UnitState<S> &s, double (&xn)[S][3], long long step,
+                                           const IntParams &p) {
+  const double h = 0.5 * p.dt;
+  if constexpr (S == 3)
+    settle_velocities(s.x[0], s.x[1], s.x[2], s.v[0], s.v[1], s.v[2], p);
+  double xh[S][3];
+  drift_settle<S>(s.x, s.v, xh, h, p);
+#pragma unroll
+  for (int m = 0; m < S; ++m) {
+    if (!(s.im[m] > 0.0)) continue;  // a particle without mass keeps its velocity
+    double r[3];
+    langevin_normals(p.seed, step, (uint32_t)(s.first + m), r);
+    const double width = p.spread * sqrt(s.im[m]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s.v[m][k] = p.decay * s.v[m][k] + width * r[k];
+  }
+  if constexpr (S == 3)
+    settle_velocities(xh[0], xh[1], xh[2], s.v[0], s.v[1], s.v[2], p);
+  drift_settle<S>(xh, s.v, xn, h, p);
+}
+
+// Stores x, v and the f32 positions of the pair loop after a drift to xn;
+// returns bit 0 when an atom moved beyond half the list buffer since the
+// build, bit 1 beyond half the pruned list's buffer since the last prune.
+template <int S>
+__device__ __forceinline__ int store_drifted(UnitState<S> &s, const double (&xn)[S][3],
+                                             double *__restrict__ x, double *__restrict__ v,
+                                             const double *__restrict__ shift,
+                                             const double *__restrict__ xref,
+                                             const double *__restrict__ q,
+                                             float4 *__restrict__ xq,
+                                             const float4 *__restrict__ gorig,
+                                             double4 *__restrict__ xpq,
+                                             double4 *__restrict__ xpa, const IntParams &p) {
   int moved = 0;  // bit 0: beyond the list buffer, bit 1: beyond the pruned one
 #pragma unroll
   for (int m = 0; m < S; ++m) {
@@ -261,7 +305,7 @@ __device__ __forceinline__ int drift_store(UnitState<S> &s, double *__restrict__
 enum StepPart { kFirstHalf = 1, kSecondHalf = 2, kBetween = 3 };
 
 template <int PART, int S>
-__device__ __forceinline__ void step_unit(int first, double *__restrict__ x,
+__device__ __forceinline__ void step_unit(int first, long long step, double *__restrict__ x,
                                           double *__restrict__ v,
                                           const double *__restrict__ inv_mass,
                                           long long *__restrict__ F,
@@ -284,7 +328,10 @@ __device__ __forceinline__ void step_unit(int first, double *__restrict__ x,
   }
   if (PART & kFirstHalf) {   // start of a step: half kick, drift, SETTLE
     half_kick<S>(s, p.dt);
-    const int moved = drift_store<S>(s, x, v, shift, xref, q, xq, gorig, xpq, xpa, p);
+    double xn[S][3];
+    if (p.langevin) baoab_rest<S>(s, xn, step, p);
+    else drift_settle<S>(s.x, s.v, xn, p.dt, p);
+    const int moved = store_drifted<S>(s, xn, x, v, shift, xref, q, xq, gorig, xpq, xpa, p);
     if (moved & 1) atomicOr(&flags[kFlagRebuild], 1);
     if (moved & 2) atomicOr(&flags[kFlagPrune], 1);
   } else {
@@ -311,14 +358,57 @@ __global__ void __launch_bounds__(64) integrate_part(
     double4 *__restrict__ xpq, double4 *__restrict__ xpa, IntParams p,
     int *__restrict__ flags) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool noise = (PART & kFirstHalf) && p.langevin;
+  const long long step = noise ? p.step[0] : 0;
+  if (u < nunits) {
+    const int first = unit_first[u];
+    if (unit_size[u] == 3)
+      step_unit<PART, 3>(first, step, x, v, inv_mass, F, Fr, place_of, shift, xref, q, xq,
+                         gorig, xpq, xpa, p, flags);
+    else
+      step_unit<PART, 1>(first, step, x, v, inv_mass, F, Fr, place_of, shift, xref, q, xq,
+                         gorig, xpq, xpa, p, flags);
+  }
+  if (noise && threadIdx.x == 0) {
+    // every block has read the step number before it counts itself done
+    // (the fence completes this block's loads); the last one advances it
+    __threadfence();
+    unsigned long long *count = reinterpret_cast<unsigned long long *>(p.step + 1);
+    if (atomicAdd(count, 1ull) + 1ull == (unsigned long long)gridDim.x) {
+      p.step[0] = step + 1;
+      p.step[1] = 0;
+    }
+  }
+}
+
+// The isotropic or axis-wise scaling of a Monte Carlo volume move: each
+// unit (a rigid water or a free atom) moves with its centre of mass,
+// X -> s X, and keeps its shape: x_i += (s - 1) X (per axis).  Masses come
+// from the inverse masses (a massless atom has weight 0; a unit without
+// mass moves with the mean of its positions).
+__global__ void scale_units(int nunits, const int *__restrict__ unit_first,
+                            const int *__restrict__ unit_size, const double *__restrict__ inv_mass,
+                            double *__restrict__ x, double s0, double s1, double s2) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= nunits) return;
-  const int first = unit_first[u];
-  if (unit_size[u] == 3)
-    step_unit<PART, 3>(first, x, v, inv_mass, F, Fr, place_of, shift, xref, q, xq, gorig, xpq,
-                       xpa, p, flags);
-  else
-    step_unit<PART, 1>(first, x, v, inv_mass, F, Fr, place_of, shift, xref, q, xq, gorig, xpq,
-                       xpa, p, flags);
+  const int first = unit_first[u], size = unit_size[u];
+  double com[3] = {0.0, 0.0, 0.0}, total = 0.0;
+  for (int m = 0; m < size; ++m) {
+    const double im = inv_mass[first + m];
+    const double w = im > 0.0 ? 1.0 / im : 0.0;
+    total += w;
+    for (int k = 0; k < 3; ++k) com[k] += w * x[3 * (first + m) + k];
+  }
+  if (total > 0.0) {
+    for (int k = 0; k < 3; ++k) com[k] /= total;
+  } else {
+    for (int k = 0; k < 3; ++k) com[k] = 0.0;
+    for (int m = 0; m < size; ++m)
+      for (int k = 0; k < 3; ++k) com[k] += x[3 * (first + m) + k] / size;
+  }
+  const double s[3] = {s0, s1, s2};
+  for (int m = 0; m < size; ++m)
+    for (int k = 0; k < 3; ++k) x[3 * (first + m) + k] += (s[k] - 1.0) * com[k];
 }
 
 // Positions and charges by atom for PME in atom order (evaluations outside

@@ -122,4 +122,50 @@ __device__ __forceinline__ float group_offset(uint32_t code, int d, float ogj, f
   return ((ogj - ogi) + k * box) + k * box_lo;
 }
 
+// Philox 4x32 with 10 rounds (Salmon, Moraes, Dror and Shaw, "Parallel
+// random numbers: as easy as 1, 2, 3", SC11): a counter-based generator,
+// so a particle's numbers at a step are a function of (seed, step,
+// particle) alone. One round maps the counter (c0, c1, c2, c3) under the
+// key (k0, k1) to (hi(B c2) ^ c1 ^ k0, lo(B c2), hi(A c0) ^ c3 ^ k1,
+// lo(A c0)) with the multipliers A = 0xD2511F53, B = 0xCD9E8D57; the key
+// is bumped by the Weyl constants (0x9E3779B9, 0xBB67AE85) between rounds.
+__host__ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0,
+                                                       uint32_t k1) {
+#pragma unroll
+  for (int round = 0; round < 10; ++round) {
+    const uint64_t pa = (uint64_t)0xD2511F53u * c[0], pb = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t hi_a = (uint32_t)(pa >> 32), lo_a = (uint32_t)pa;
+    const uint32_t hi_b = (uint32_t)(pb >> 32), lo_b = (uint32_t)pb;
+    c[0] = hi_b ^ c[1] ^ k0;
+    c[1] = lo_b;
+    c[2] = hi_a ^ c[3] ^ k1;
+    c[3] = lo_a;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+// The three standard normal numbers of particle `id` at `step` for the
+// Langevin O step (the reference's dyn.noise, docs/dialects/dyn.md:441-460,
+// lib/Driver/NoiseIR.cpp): one Philox block under the seed's two words,
+// with the counter (step low word, step high word, id, 2 << 24), stream 2
+// in the high byte of the last word; each word w gives the uniform
+// (w + 1/2) 2^-32 in (0, 1); Box-Muller on (u0, u1) gives the first two
+// numbers (cosine, sine), on (u2, u3) the third (cosine).
+__device__ __forceinline__ void langevin_normals(unsigned long long seed, long long step,
+                                                 uint32_t id, double r[3]) {
+  uint32_t c[4] = {(uint32_t)(unsigned long long)step,
+                   (uint32_t)((unsigned long long)step >> 32), id, 2u << 24};
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  double u[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) u[k] = ((double)c[k] + 0.5) * 0x1.0p-32;
+  const double two_pi = 6.283185307179586;
+  const double rad0 = sqrt(-2.0 * log(u[0])), th0 = two_pi * u[1];
+  const double rad1 = sqrt(-2.0 * log(u[2])), th1 = two_pi * u[3];
+  r[0] = rad0 * cos(th0);
+  r[1] = rad0 * sin(th0);
+  r[2] = rad1 * cos(th1);
+}
+
 }  // namespace mdhip

@@ -124,6 +124,14 @@ SIGNATURES = [
     ("mdrtHipSetLaunchShape", C.c_int, [C.c_void_p, I32, I32, I32]),
     ("mdrtHipGetLaunchShape", C.c_int, [C.c_void_p, PI32]),
     ("mdrtHipFramesSetPmeTransform", C.c_int, [C.c_void_p, I32]),
+    ("mdrtHipSetLangevin", C.c_int, [C.c_void_p, D, D, C.c_uint64]),
+    ("mdrtHipSetStepCount", C.c_int, [C.c_void_p, I64]),
+    ("mdrtHipGetBox", C.c_int, [C.c_void_p, PD]),
+    ("mdrtHipScaleBox", C.c_int, [C.c_void_p, PD]),
+    ("mdrtHipBarostatMove", C.c_int, [C.c_void_p, D, D, C.c_uint64, PI32, PD]),
+    ("mdrtHipRunNPT", C.c_int, [C.c_void_p, I64, D, I32, I32, D, D, C.c_uint64, PD]),
+    ("mdrtHipBarostatStats", C.c_int, [C.c_void_p, PI64, PD]),
+    ("mdrtHipGetStepCount", C.c_int, [C.c_void_p, PI64]),
     ("mdrtHipRun", C.c_int, [C.c_void_p, I64, D, I32, PD]),
     ("mdrtHipPrepareRun", C.c_int, [C.c_void_p, I64, D, I32]),
     ("mdrtHipCheckList", C.c_int, [C.c_void_p, PD, PD, PI64]),

@@ -125,8 +125,62 @@ class Simulation:
         _, e = self.forces()
         return dict(zip(rt.ENERGY_NAMES, e))
 
+    def set_langevin(self, temperature: float, friction: float, seed: int = 0):
+        """Langevin dynamics (g-BAOAB) for the following run() calls:
+        friction in 1/ps (0: back to velocity Verlet), temperature in K; the
+        noise of each atom and step comes from Philox under `seed`."""
+        rt.check(rt.lib().mdrtHipSetLangevin(self._h, float(temperature), float(friction),
+                                             int(seed) & 0xFFFFFFFFFFFFFFFF))
+
+    @property
+    def box(self) -> np.ndarray:
+        b = np.zeros(3)
+        rt.check(rt.lib().mdrtHipGetBox(self._h, rt.ptr(b, C.c_double)))
+        return b
+
+    def scale_box(self, scale):
+        """Scales the units' centres of mass and the box by scale (3,)."""
+        s = np.ascontiguousarray(np.broadcast_to(np.asarray(scale, np.float64), (3,)))
+        rt.check(rt.lib().mdrtHipScaleBox(self._h, rt.ptr(s, C.c_double)))
+
+    def barostat_move(self, pressure: float, temperature: float, seed: int = 0):
+        """One Monte Carlo volume move; returns (kept, {U0, U1, dV, W})."""
+        acc = C.c_int32(0)
+        info = np.zeros(4)
+        rt.check(rt.lib().mdrtHipBarostatMove(self._h, float(pressure), float(temperature),
+                                              int(seed) & 0xFFFFFFFFFFFFFFFF, C.byref(acc),
+                                              rt.ptr(info, C.c_double)))
+        return bool(acc.value), dict(zip(("U0", "U1", "dV", "W"), info))
+
+    def run_npt(self, steps: int, dt: float = 0.002, graph: int = 0, every: int = 100,
+                pressure: float = 1.0, temperature: float = 300.0, seed: int = 0) -> float:
+        """run() with a Monte Carlo volume move after every `every` steps."""
+        ke = C.c_double(0.0)
+        rt.check(rt.lib().mdrtHipRunNPT(self._h, int(steps), float(dt), int(graph), int(every),
+                                        float(pressure), float(temperature),
+                                        int(seed) & 0xFFFFFFFFFFFFFFFF, C.byref(ke)))
+        return ke.value
+
+    def barostat_stats(self) -> dict:
+        c = np.zeros(2, np.int64)
+        dv = C.c_double(0.0)
+        rt.check(rt.lib().mdrtHipBarostatStats(self._h, rt.ptr(c, C.c_int64), C.byref(dv)))
+        return {"tried": int(c[0]), "kept": int(c[1]), "max_dv": dv.value}
+
+    @property
+    def step_count(self) -> int:
+        """The step number of the next step (the noise's step counter)."""
+        v = C.c_int64(0)
+        rt.check(rt.lib().mdrtHipGetStepCount(self._h, C.byref(v)))
+        return v.value
+
+    @step_count.setter
+    def step_count(self, step: int):
+        rt.check(rt.lib().mdrtHipSetStepCount(self._h, int(step)))
+
     def run(self, steps: int, dt: float = 0.002, graph: int = 0) -> float:
-        """`steps` NVE steps; returns the kinetic energy at the end."""
+        """`steps` steps (velocity Verlet, or Langevin after set_langevin);
+        returns the kinetic energy at the end."""
         ke = C.c_double(0.0)
         rt.check(rt.lib().mdrtHipRun(self._h, int(steps), float(dt), int(graph),
                                      C.byref(ke)))

@@ -1220,6 +1220,108 @@ int orc_nve(const orc_system *s, double *x, double *v, double dt,
 }
 
 /* ======================================================================
+ * Langevin dynamics — the g-BAOAB step of the reference's Langevin run with
+ * constraints (lib/Driver/DynamicsIR.cpp, geodesic branch; Leimkuhler and
+ * Matthews, Proc. R. Soc. A 472, 20160138, 2016) and its O step, the
+ * reference's dyn.noise (docs/dialects/dyn.md:441-460,
+ * lib/Driver/NoiseIR.cpp): v <- c v + (sigma / sqrt m) R with
+ * c = exp(-gamma dt), sigma = sqrt((1 - c^2) kT), R three normals of
+ * Philox 4x32-10 (runtime/mdrt.c:289-312) at counter (step lo, step hi,
+ * particle, 2 << 24) under the seed, uniforms (w + 1/2) 2^-32, Box-Muller
+ * (cos, sin) of the first pair and cos of the second.
+ * Philox is pinned against the reference's own mdrtPhilox4x32 compiled from
+ * runtime/mdrt.c into oracle/_ref and against the published known-answer
+ * vectors of Random123.
+ * ====================================================================== */
+
+void orc_philox4x32_10(const uint32_t counter[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t x0 = counter[0], x1 = counter[1], x2 = counter[2], x3 = counter[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; ++r) {
+    uint64_t pa = (uint64_t)0xD2511F53u * x0;
+    uint64_t pb = (uint64_t)0xCD9E8D57u * x2;
+    uint32_t y0 = (uint32_t)(pb >> 32) ^ x1 ^ k0;
+    uint32_t y2 = (uint32_t)(pa >> 32) ^ x3 ^ k1;
+    x0 = y0;
+    x1 = (uint32_t)pb;
+    x2 = y2;
+    x3 = (uint32_t)pa;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = x0;
+  out[1] = x1;
+  out[2] = x2;
+  out[3] = x3;
+}
+
+void orc_langevin_normals(uint64_t seed, int64_t step, uint32_t id, double r[3]) {
+  uint32_t ctr[4] = {(uint32_t)(uint64_t)step, (uint32_t)((uint64_t)step >> 32), id, 2u << 24};
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)}, w[4];
+  orc_philox4x32_10(ctr, key, w);
+  double u[4];
+  for (int k = 0; k < 4; ++k) u[k] = ((double)w[k] + 0.5) * 0x1.0p-32;
+  double a = sqrt(-2.0 * log(u[0])), b = sqrt(-2.0 * log(u[2]));
+  r[0] = a * cos(6.283185307179586 * u[1]);
+  r[1] = a * sin(6.283185307179586 * u[1]);
+  r[2] = b * cos(6.283185307179586 * u[3]);
+}
+
+/* x += h v from x0, SETTLE the waters against x0, v += (x_c - x_u) / h. */
+static void half_drift(const orc_system *s, double *x, double *v, double h, double *x0,
+                       double *xu) {
+  int64_t n = s->n;
+  memcpy(x0, x, sizeof(double) * 3 * (size_t)n);
+  for (int64_t i = 0; i < 3 * n; ++i) x[i] += h * v[i];
+  if (!s->nwaters) return;
+  memcpy(xu, x, sizeof(double) * 3 * (size_t)n);
+  orc_settle_positions(s->nwaters, s->water_first, s->m_o, s->m_h, s->d_oh, s->d_hh, x0, x);
+  for (int64_t i = 0; i < 3 * n; ++i) v[i] += (x[i] - xu[i]) / h;
+}
+
+int orc_langevin(const orc_system *s, double *x, double *v, double dt, int64_t steps,
+                 int64_t first_step, uint64_t seed, double gamma, double kT,
+                 double *trace, int32_t mixed) {
+  int64_t n = s->n;
+  double *F = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+  double *x0 = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+  double *xu = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+  double e[ORC_NTERMS];
+  const double c = exp(-gamma * dt), sigma = sqrt((1.0 - c * c) * kT);
+  orc_total(s, x, F, e, mixed);
+  for (int64_t step = 0; step < steps; ++step) {
+    for (int64_t i = 0; i < n; ++i)
+      for (int k = 0; k < 3; ++k) v[3 * i + k] += 0.5 * dt * F[3 * i + k] / s->mass[i];
+    if (s->nwaters) orc_settle_velocities(s->nwaters, s->water_first, s->m_o, s->m_h, x, v);
+    half_drift(s, x, v, 0.5 * dt, x0, xu);
+    for (int64_t i = 0; i < n; ++i) {
+      if (!(s->mass[i] > 0.0)) continue;
+      double r[3];
+      orc_langevin_normals(seed, first_step + step, (uint32_t)i, r);
+      double width = sigma / sqrt(s->mass[i]);
+      for (int k = 0; k < 3; ++k) v[3 * i + k] = c * v[3 * i + k] + width * r[k];
+    }
+    if (s->nwaters) orc_settle_velocities(s->nwaters, s->water_first, s->m_o, s->m_h, x, v);
+    half_drift(s, x, v, 0.5 * dt, x0, xu);
+    orc_total(s, x, F, e, mixed);
+    for (int64_t i = 0; i < n; ++i)
+      for (int k = 0; k < 3; ++k) v[3 * i + k] += 0.5 * dt * F[3 * i + k] / s->mass[i];
+    if (s->nwaters) orc_settle_velocities(s->nwaters, s->water_first, s->m_o, s->m_h, x, v);
+    if (trace) {
+      double pot = 0.0, kin = 0.0;
+      for (int k = 0; k < ORC_NTERMS; ++k) pot += e[k];
+      for (int64_t i = 0; i < n; ++i) kin += 0.5 * s->mass[i] * dot3(v + 3 * i, v + 3 * i);
+      trace[2 * step] = pot;
+      trace[2 * step + 1] = kin;
+    }
+  }
+  free(F);
+  free(x0);
+  free(xu);
+  return 0;
+}
+
+/* ======================================================================
  * Initial velocities — System.cpp:1298-1350 and InitialVelocityDraws.h:13-98:
  * splitmix64 seeds xoshiro256**, six uniforms in (0,1] per particle at
  * counter 6 id, Box-Muller normals, constrained, centre of mass at rest,

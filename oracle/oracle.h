@@ -154,6 +154,12 @@ typedef struct {
 
 int orc_total(const orc_system *s, const double *x, double *F,
               double *energies, int32_t mixed);
+/* Langevin dynamics (g-BAOAB) and its Philox 4x32-10 noise. */
+void orc_philox4x32_10(const uint32_t counter[4], const uint32_t key[2], uint32_t out[4]);
+void orc_langevin_normals(uint64_t seed, int64_t step, uint32_t id, double r[3]);
+int orc_langevin(const orc_system *s, double *x, double *v, double dt, int64_t steps,
+                 int64_t first_step, uint64_t seed, double gamma, double kT,
+                 double *trace, int32_t mixed);
 /* mixed != 0 evaluates the pair arithmetic in f32 (timing only). */
 int orc_nve(const orc_system *s, double *x, double *v, double dt,
             int64_t steps, double *trace, int32_t mixed);

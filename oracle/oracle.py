@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "liboracle.so")
 REF_FFT = os.path.join(HERE, "_ref", "libmdrt_fft_ref.so")
 REF_DRAWS = os.path.join(HERE, "_ref", "libref_draws.so")
+REF_HOST = os.path.join(HERE, "_ref", "libmdrt_host_ref.so")
 
 E_LJ, E_COUL_DIRECT, E_COUL_EXCL, E_COUL_RECIP, E_COUL_SELF, E_DISP_CORR, \
     E_LJ14, E_COUL14 = range(8)
@@ -103,6 +104,12 @@ def _declare(L):
     L.orc_total.argtypes = [C.POINTER(OrcSystem), PD, PD, PD, I32]
     L.orc_nve.restype = C.c_int
     L.orc_nve.argtypes = [C.POINTER(OrcSystem), PD, PD, D, I64, PD, I32]
+    L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                    C.POINTER(C.c_uint32)]
+    L.orc_langevin_normals.argtypes = [C.c_uint64, I64, C.c_uint32, PD]
+    L.orc_langevin.restype = C.c_int
+    L.orc_langevin.argtypes = [C.POINTER(OrcSystem), PD, PD, D, I64, I64, C.c_uint64, D, D,
+                               PD, I32]
     L.orc_threads.restype = C.c_int
     L.orc_threads.argtypes = []
     L.orc_splitmix64.restype = C.c_uint64
@@ -281,6 +288,18 @@ class Oracle:
                       _p(trace, D), int(mixed))
         return x, v, trace
 
+    def langevin(self, x, v, dt, steps, seed, friction, temperature, first_step=0,
+                 mixed=False):
+        """g-BAOAB Langevin steps (friction in 1/ps, temperature in K) with
+        the Philox noise of (seed, step, particle)."""
+        from mdir_amd.systems import BOLTZMANN
+        x = f64(x).copy()
+        v = f64(v).copy()
+        trace = np.zeros((steps, 2))
+        lib().orc_langevin(C.byref(self.st), _p(x, D), _p(v, D), dt, steps, first_step,
+                           seed, friction, BOLTZMANN * temperature, _p(trace, D), int(mixed))
+        return x, v, trace
+
     def initial_velocities(self, seed, temperature):
         s = self.s
         v = np.zeros((s.n, 3))
@@ -289,6 +308,36 @@ class Oracle:
                                      _p(self.waters, I32),
                                      s.degrees_of_freedom(), _p(v, D))
         return v
+
+
+def philox4x32_10(counter, key):
+    """Philox 4x32-10 of four counter words under two key words."""
+    c = (C.c_uint32 * 4)(*[int(w) & 0xFFFFFFFF for w in counter])
+    k = (C.c_uint32 * 2)(*[int(w) & 0xFFFFFFFF for w in key])
+    out = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, out)
+    return list(out)
+
+
+def langevin_normals(seed, step, particle):
+    r = np.zeros(3)
+    lib().orc_langevin_normals(seed, step, particle, _p(r, D))
+    return r
+
+
+def ref_host_available():
+    return os.path.exists(REF_HOST)
+
+
+def ref_philox4x32(counter, key):
+    """The reference's own mdrtPhilox4x32 (runtime/mdrt.c, compiled into
+    oracle/_ref)."""
+    L = C.CDLL(REF_HOST)
+    c = (C.c_uint32 * 4)(*[int(w) & 0xFFFFFFFF for w in counter])
+    k = (C.c_uint32 * 2)(*[int(w) & 0xFFFFFFFF for w in key])
+    out = (C.c_uint32 * 4)()
+    L.mdrtPhilox4x32(c, k, out)
+    return list(out)
 
 
 def initial_normals(seed, n):

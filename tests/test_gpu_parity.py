@@ -294,3 +294,84 @@ def test_force_matching_workload_at_full_size():
         assert rel < 5e-4
     ev.close()
     ev4.close()
+
+
+@pytest.mark.parametrize("case", ["water", "argon"])
+def test_langevin_against_oracle_steps(case):
+    """g-BAOAB Langevin steps on the GPU against the oracle's from the same
+    state, seed and step numbers: the noise is the same Philox draw on both
+    sides, so the difference comes from the force error alone, bounded as
+    for velocity Verlet (eight times a_err dt^2 k(k+1)/2 for positions,
+    a_err dt k for velocities).  A noise of another step, particle or
+    stream would move velocities by about spread/sqrt(m), which the test
+    checks is far above the bar."""
+    from mdir_amd.simulation import Simulation
+    dt, gamma, T, seed = 0.002, 1.0, 300.0, 2718
+    if case == "water":
+        say("case: Langevin against the oracle, TIP3P 512 waters, 2.5 nm")
+        w = S.tip3p_box(512, 2.5, seed=21, grid=(24, 24, 24))
+        base = Simulation(w)
+        base.minimize(50, 0.01)
+        w = w.copy()
+        w.positions = base.positions
+        base.close()
+    else:
+        say("case: Langevin against the oracle, argon 864")
+        w = S.argon_config1()
+        w.positions = w.positions + np.random.default_rng(4).normal(0, 0.01, w.positions.shape)
+        T = 120.0
+    o = O.Oracle(w)
+    v0 = o.initial_velocities(7, T)
+    x0 = w.positions.copy()
+    sim = Simulation(w, skin=0.1)
+    sim.velocities = v0
+    sim.step_count = 40
+    sim.set_langevin(T, gamma, seed)
+    F, _ = sim.forces()
+    Fo, _ = o.total(x0)
+    a_err = (np.linalg.norm(F - Fo, axis=1) / w.masses).max()
+    c = np.exp(-gamma * dt)
+    noise = np.sqrt((1 - c * c) * S.BOLTZMANN * T / w.masses.max())
+    done = 0
+    for k, graph in ((1, 0), (10, 0), (30, 10)):
+        sim.run(k - done, dt, graph=graph)
+        done = k
+        assert sim.step_count == 40 + k
+        x, v = sim.positions, sim.velocities
+        xo, vo, _ = o.langevin(x0, v0, dt, k, seed=seed, friction=gamma, temperature=T,
+                               first_step=40)
+        dx, dv = np.abs(x - xo).max(), np.abs(v - vo).max()
+        bx = 8.0 * a_err * dt * dt * k * (k + 1) / 2 + 1e-12
+        bv = 8.0 * a_err * dt * k + 1e-10
+        say(f"  {k} steps: |dx| {dx:.3e} (bar {bx:.3e}), |dv| {dv:.3e} (bar {bv:.3e}); "
+            f"noise per step {noise:.3e} nm/ps")
+        assert noise > 100 * bv
+        assert dx < bx and dv < bv
+        if case == "water":
+            assert water_bonds_ok(w, x)
+            assert bond_velocity(w, x, v) < 1e-9
+    sim.close()
+
+
+def test_langevin_thermostat_holds_the_temperature():
+    """512 TIP3P waters started at 200 K under a 300 K bath (friction 5/ps)
+    reach and hold the bath temperature: the mean over the last 6 ps of the
+    instantaneous temperature is within 3 % of 300 K."""
+    from mdir_amd.simulation import Simulation
+    say("case: Langevin thermostat, 512 waters, 200 K -> 300 K bath")
+    w = S.tip3p_box(512, 2.5, seed=21, grid=(24, 24, 24))
+    sim = Simulation(w, skin=0.1)
+    sim.minimize(50, 0.01)
+    relaxed = w.copy()
+    relaxed.positions = sim.positions
+    sim.velocities = S.initial_velocities(relaxed, 3, 200.0)
+    sim.set_langevin(300.0, 5.0, 99)
+    dof = w.degrees_of_freedom()
+    temps = []
+    for k in range(40):
+        ke = sim.run(100, 0.002, graph=50)
+        temps.append(2.0 * ke / (dof * S.BOLTZMANN))
+    late = np.mean(temps[10:])
+    say(f"  temperature after 2 ps {temps[9]:.1f} K, mean of the last 6 ps {late:.1f} K")
+    assert abs(late / 300.0 - 1.0) < 0.03
+    assert water_bonds_ok(w, sim.positions)
