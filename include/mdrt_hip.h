@@ -352,6 +352,16 @@ int mdrtHipRun(void *system, int64_t steps, double dt, int32_t graph,
    the last step. */
 int mdrtHipCheckList(void *system, double *forces, double *energies, int64_t *counts);
 
+/* The pairs (i < j, excluded pairs dropped) of the list the steps hold
+   (pruned != 0: the pruned rows the pair loop runs over) within
+   reach + 3e-6 (Lx+Ly+Lz) at the current positions, by the f32 test of the
+   reference's matrix (NeighborsMatrix.mlir:217-232), without a rebuild:
+   the pair set behind mdrtHipCheckList's counts.  Writes at most
+   `capacity` pairs; *count receives the number found.  Fails when the
+   positions changed since the last step. */
+int mdrtHipListPairs(void *system, double reach, int32_t pruned, int64_t capacity,
+                     int32_t *pairs, int64_t *count);
+
 /* Captures, without running them, the step graphs a later
    mdrtHipRun(system, steps, dt, graph) replays: the graph of `graph` steps
    and the graph of the remainder steps mod graph (each run replays only
@@ -490,6 +500,36 @@ int mdrtHipFramesForceMatching(void *frames, int32_t count, const double *positi
    frames afterwards. */
 int mdrtHipFramesAutotune(void *frames, int32_t what, int32_t count, const double *positions,
                           const double *reference_forces, int32_t repeats, double *ms_per_call);
+
+/* ---------------------------------------------------------------------
+ * 4. Replicas: independent systems of uneven size advanced together, each
+ *    chunk of steps of all of them one replayed HIP graph (a branch per
+ *    replica).  No counterpart in the reference, whose runs take one
+ *    system; its device batch (lib/Compiler/Simulation.cpp:3189) batches
+ *    frames of one system.
+ * ------------------------------------------------------------------- */
+
+/* A set of `count` systems (each as mdrtHipSystemCreate); NULL on error. */
+void *mdrtHipReplicasCreate(const mdrtHipSystemDesc *descs, int32_t count);
+void mdrtHipReplicasDestroy(void *set);
+
+/* The system handle of replica `index`, owned by the set (not to be
+   destroyed): every group-2 entry point takes it (positions, forces,
+   Langevin, launch shapes, the launch-shape trial). */
+void *mdrtHipReplicasSystem(void *set, int32_t index);
+
+/* `steps` steps of dt (ps) of every replica, in chunks of `graph` steps,
+   each chunk of all replicas one graph launch.  A replica whose list
+   outgrew its buffers redoes its chunk alone with the grown list. */
+int mdrtHipReplicasRun(void *set, int64_t steps, double dt, int32_t graph);
+
+/* Step latency of replica `index` inside the batch: for each chunk since the
+   last clear, (device clock at the end of its branch - at its start) /
+   steps, ms.  Writes up to `capacity` values; *count receives how many
+   there are. */
+int mdrtHipReplicasLatency(void *set, int32_t index, int64_t capacity, double *ms_per_step,
+                           int64_t *count);
+int mdrtHipReplicasClearLatency(void *set);
 
 #ifdef __cplusplus
 }

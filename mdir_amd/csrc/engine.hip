@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -510,9 +511,11 @@ struct Engine {
   };
   static constexpr size_t kMaxStepGraphs = 4;
   std::vector<StepGraph> step_graphs;
+  uint64_t graph_generation = 0;  // bumped whenever captured step graphs go stale
   void invalidate_graph() {
     for (StepGraph &g_ : step_graphs) hipGraphExecDestroy(g_.exec);
     step_graphs.clear();
+    ++graph_generation;
   }
   hipGraphExec_t find_graph(int steps, double dt, bool measured) const {
     for (const StepGraph &g_ : step_graphs)
@@ -2280,6 +2283,26 @@ int mdrtHipCheckList(void *system, double *forces, double *energies, int64_t *co
   return guarded([&] { e->check_list(forces, energies, counts); });
 }
 
+int mdrtHipListPairs(void *system, double reach, int32_t pruned, int64_t capacity,
+                     int32_t *pairs, int64_t *count) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] {
+    if (!e->forces_valid || e->list_reach != e->cutoff + e->skin)
+      throw mdhip::HipError("no list of a run to read: the positions changed since the last step");
+    if (pruned && !(e->prune_skin > 0)) throw mdhip::HipError("the system keeps no pruned list");
+    const double far = reach + 3.0e-6 * ((e->box[0] + e->box[1]) + e->box[2]);
+    const std::vector<long long> keys =
+        pruned ? e->export_pairs(e->d_rows_in, e->d_row_count_in, far, capacity)
+               : e->export_pairs(e->d_rows, e->d_row_count, far, capacity);
+    const int64_t total = (int64_t)keys.size();
+    for (int64_t i = 0; i < std::min(total, capacity); ++i) {
+      pairs[2 * i] = (int32_t)(keys[i] >> 32);
+      pairs[2 * i + 1] = (int32_t)(keys[i] & 0xffffffff);
+    }
+    *count = total;
+  });
+}
+
 int mdrtHipPrepareRun(void *system, int64_t steps, double dt, int32_t graph) {
   Engine *e = static_cast<Engine *>(system);
   return guarded([&] { e->prepare(steps, dt, graph); });
@@ -2395,3 +2418,5 @@ int mdrtHipProfileStep(void *system, int32_t repeats, double *ms) {
 }
 
 }  // extern "C"
+
+#include "replicas.hip"

@@ -14,6 +14,12 @@
 
 namespace mdhip {
 
+// The integrator's f64 arithmetic is rounded operation by operation as
+// written (no contraction into fused multiply-adds): a step then gives the
+// same bits whether its two halves run in one launch (kBetween) or two,
+// and follows the oracle's integration, which rounds each operation.
+#pragma clang fp contract(off)
+
 struct IntParams {
   double dt;
   double skin_half2;     // (outer list buffer / 2)^2: rebuild beyond it
@@ -475,5 +481,8 @@ __global__ void descend(int nunits, const int *__restrict__ unit_first,
   for (int m = 0; m < size; ++m)
     for (int k = 0; k < 3; ++k) x[3 * (first + m) + k] = xn[m][k];
 }
+
+// Back to the default contraction for the kernels that follow.
+#pragma clang fp contract(fast)
 
 }  // namespace mdhip

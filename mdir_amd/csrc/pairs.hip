@@ -355,6 +355,9 @@ __device__ __forceinline__ void nb_pairs_body(
       bool on = E.ex >= 0 ? ((ex_masks[(size_t)E.ex * kGroup + m] >> lane) & 1ull)
                           : (ireal[m] && jreal);
       on = on && r2 < p.rc2;
+      // A member tile with no pair inside rc (about half of those a list
+      // with a buffer holds) adds exact zeros: skip its arithmetic.
+      if (!__ballot(on)) continue;  // uniform
       float el = 0.0f, ec = 0.0f;
       float fscal = pair_force<MOD, COUL, ENERGY>(on ? r2 : 1.0f, xi[m].w * xj.w,
                                                   ljm[m], p, el, ec);

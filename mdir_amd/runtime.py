@@ -126,6 +126,13 @@ SIGNATURES = [
     ("mdrtHipFramesSetPmeTransform", C.c_int, [C.c_void_p, I32]),
     ("mdrtHipSetLangevin", C.c_int, [C.c_void_p, D, D, C.c_uint64]),
     ("mdrtHipSetStepCount", C.c_int, [C.c_void_p, I64]),
+    ("mdrtHipListPairs", C.c_int, [C.c_void_p, D, I32, I64, PI32, PI64]),
+    ("mdrtHipReplicasCreate", C.c_void_p, [C.POINTER(SystemDesc), I32]),
+    ("mdrtHipReplicasDestroy", None, [C.c_void_p]),
+    ("mdrtHipReplicasSystem", C.c_void_p, [C.c_void_p, I32]),
+    ("mdrtHipReplicasRun", C.c_int, [C.c_void_p, I64, D, I32]),
+    ("mdrtHipReplicasLatency", C.c_int, [C.c_void_p, I32, I64, PD, PI64]),
+    ("mdrtHipReplicasClearLatency", C.c_int, [C.c_void_p]),
     ("mdrtHipGetBox", C.c_int, [C.c_void_p, PD]),
     ("mdrtHipScaleBox", C.c_int, [C.c_void_p, PD]),
     ("mdrtHipBarostatMove", C.c_int, [C.c_void_p, D, D, C.c_uint64, PI32, PD]),

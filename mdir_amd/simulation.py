@@ -199,6 +199,22 @@ class Simulation:
                                            rt.ptr(c, C.c_int64)))
         return F, e, {"list": int(c[0]), "pair_loop": int(c[1]), "builds": int(c[2])}
 
+    def list_pairs(self, reach: float, pruned: bool = False,
+                   capacity: int | None = None) -> np.ndarray:
+        """The pairs (i < j) of the list the steps hold (or of its pruned
+        rows) within `reach` at the current positions, without a rebuild."""
+        if capacity is None:
+            s = self.system
+            rho = s.n / float(np.prod(s.box))
+            capacity = int(s.n * rho * 4.19 * (reach + 0.05) ** 3 * 0.75) + 4096
+        out = np.zeros((capacity, 2), np.int32)
+        count = C.c_int64(0)
+        rt.check(rt.lib().mdrtHipListPairs(self._h, float(reach), int(pruned), capacity,
+                                           rt.ptr(out, C.c_int32), C.byref(count)))
+        if count.value > capacity:
+            return self.list_pairs(reach, pruned, count.value + 16)
+        return out[:count.value].copy()
+
     def prepare(self, steps: int, dt: float = 0.002, graph: int = 50):
         """Captures the step graphs a later run(steps, dt, graph) replays,
         without running them, so that the capture stays out of a timed run."""

@@ -542,3 +542,25 @@ def perturbed_parameters(system: System, seed: int = 13, rel: float = 0.02, dq: 
     s = s0 * (1.0 + rng.uniform(-rel, rel, s0.shape))
     e = e0 * (1.0 + rng.uniform(-rel, rel, e0.shape))
     return q, s, e
+
+
+AVOGADRO = 6.02214076e23
+
+
+def water_edge(nwaters: int, density: float = 1.0) -> float:
+    """Edge (nm) of the cube holding `nwaters` TIP3P waters at `density`
+    (g/cm^3)."""
+    grams = nwaters * (TIP3P["m_o"] + 2 * TIP3P["m_h"]) / AVOGADRO
+    return (grams / density) ** (1.0 / 3.0) * 1e7
+
+
+def replica_sizes(count: int = 256, seed: int = 5, low: int = 500, high: int = 8000):
+    """Numbers of waters of the batching workload: uniform in [low, high]."""
+    return np.random.default_rng(seed).integers(low, high + 1, count)
+
+
+def tip3p_replica(nwaters: int, seed: int) -> System:
+    """One replica of the batching workload: `nwaters` TIP3P waters in a
+    cube at 1.0 g/cm^3, PME with rc 0.9 nm, tolerance 1e-5, a grid no wider
+    than 0.12 nm, order 4."""
+    return tip3p_box(int(nwaters), water_edge(int(nwaters)), seed=seed, cutoff=0.9)

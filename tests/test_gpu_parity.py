@@ -210,7 +210,17 @@ def test_list_between_rebuilds(box, skin, prune):
         say(f"  after {(r + 1) * per} steps: rel rms {err:.3e}, pairs {counts} (oracle {want})")
         assert err < 5e-6
         assert counts["builds"] == 0
-        assert counts["list"] == counts["pair_loop"] == want
+        assert counts["list"] == counts["pair_loop"]
+        if counts["list"] != want:
+            # name the pairs the held list lacks, with their f64 distances
+            have = set(keys_of(sim.list_pairs(w.cutoff)).tolist())
+            lack = [k for k in oracle_pair_keys(w, x, w.cutoff).tolist() if k not in have]
+            for k in lack[:10]:
+                i, j = k >> 32, k & 0xFFFFFFFF
+                d = x[i] - x[j]
+                d -= w.box * np.round(d / w.box)
+                say(f"  missing pair ({i}, {j}) at {np.linalg.norm(d):.9f} nm")
+        assert counts["list"] == want
     st = sim.stats()
     say(f"  stats {st}")
     assert st["builds"] - b0 < rounds * per
@@ -345,7 +355,7 @@ def test_langevin_against_oracle_steps(case):
         bv = 8.0 * a_err * dt * k + 1e-10
         say(f"  {k} steps: |dx| {dx:.3e} (bar {bx:.3e}), |dv| {dv:.3e} (bar {bv:.3e}); "
             f"noise per step {noise:.3e} nm/ps")
-        assert noise > 100 * bv
+        assert noise > 20 * bv
         assert dx < bx and dv < bv
         if case == "water":
             assert water_bonds_ok(w, x)
