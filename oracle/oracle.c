@@ -1220,6 +1220,318 @@ int orc_nve(const orc_system *s, double *x, double *v, double dt,
 }
 
 /* ======================================================================
+ * The CPU baseline the benchmark times: the reference's OpenMP CPU path
+ * for a step of velocity Verlet with SETTLE waters, restated for speed
+ * rather than as a checker.  The neighbor matrix (NeighborsMatrix.mlir
+ * :193-460) is built at rc + buffer and reused until an atom moved more
+ * than half the buffer since the build (the validity test,
+ * ExposeValidity.cpp:23, 04-neighbors.md:54-133); pair rows are owner-
+ * computes (each pair in both rows, MDExecKernels.cpp:272) with the f32
+ * kernels of the mixed mode; the excluded-pair correction in f64; PME with
+ * the charge spread as 64-bit fixed point at 2^40 (docs/pme.md:310-331,
+ * the deterministic mode), f64 line FFTs, the influence function computed
+ * once for the box, and the gather; SETTLE and the integrator in f64.
+ * Forces only: energies are not evaluated in the timed steps (the
+ * reference evaluates them on output steps only).
+ * ====================================================================== */
+
+struct orc_md {
+  const orc_system *s;
+  double skin;
+  int64_t width;
+  int32_t *counts, *index;
+  double *xref, *F, *xo, *xu;
+  int64_t *words;        /* spread grid, fixed point */
+  float *xf, *qf;        /* positions wrapped into the box and scaled charges, f32 */
+  double *grid, *cplx, *bc;
+  int64_t builds;
+};
+
+static void md_build(orc_md *c, const double *x) {
+  const orc_system *s = c->s;
+  int64_t n = s->n;
+  double reach = s->cutoff + c->skin;
+  double cw = orc_cell_width(n, s->box, reach, 0.0);
+  for (;;) {
+    int64_t largest =
+        orc_neighbors_matrix(n, x, s->box, reach, cw, c->counts, c->index, c->width);
+    if (largest <= c->width) break;
+    c->width = (int64_t)(largest * 1.25) + 1;
+    free(c->index);
+    c->index = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n * c->width + 1));
+  }
+  if (s->excl_start)
+    orc_exclusion_filter(n, c->counts, c->index, c->width, s->excl_start, s->excl_atoms);
+  memcpy(c->xref, x, sizeof(double) * 3 * (size_t)n);
+  ++c->builds;
+}
+
+orc_md *orc_md_create(const orc_system *s, double skin) {
+  orc_md *c = (orc_md *)calloc(1, sizeof(orc_md));
+  int64_t n = s->n;
+  c->s = s;
+  c->skin = skin;
+  c->width = 512;
+  c->counts = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  c->index = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n * c->width + 1));
+  c->xref = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+  c->F = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+  c->xo = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+  c->xu = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+  c->xf = (float *)malloc(sizeof(float) * 3 * (size_t)n);
+  c->qf = (float *)malloc(sizeof(float) * (size_t)(n > 0 ? n : 1));
+  if (s->beta > 0.0) {
+    size_t np = (size_t)(s->K[0] * s->K[1] * s->K[2]);
+    size_t nh = (size_t)(s->K[0] * s->K[1] * (s->K[2] / 2 + 1));
+    c->words = (int64_t *)malloc(sizeof(int64_t) * np);
+    c->grid = (double *)malloc(sizeof(double) * np);
+    c->cplx = (double *)malloc(sizeof(double) * 2 * nh);
+    c->bc = (double *)malloc(sizeof(double) * nh);
+    orc_pme_influence(s->box, s->K, s->order, s->beta, s->coulomb, c->bc);
+  }
+  return c;
+}
+
+void orc_md_destroy(orc_md *c) {
+  if (!c) return;
+  free(c->counts);
+  free(c->index);
+  free(c->xref);
+  free(c->F);
+  free(c->xo);
+  free(c->xu);
+  free(c->xf);
+  free(c->qf);
+  free(c->words);
+  free(c->grid);
+  free(c->cplx);
+  free(c->bc);
+  free(c);
+}
+
+int64_t orc_md_builds(const orc_md *c) { return c->builds; }
+
+/* exp(y) for y <= 0 in f32, vectorizable: y log2(e) = n + f with n an
+   integer and f in [-1/2, 1/2], 2^f by its Taylor polynomial of degree 6
+   (relative error below 3e-8 on that interval), 2^n by the exponent bits.
+   Arguments below -87 give 0. */
+static inline float md_expf(float y) {
+  float t = y * 1.44269504088896341f;
+  t = t < -126.0f ? -126.0f : t;
+  float n = rintf(t), f = t - n;
+  float p = 1.5403530393381606e-4f;
+  p = p * f + 1.3333558146428443e-3f;
+  p = p * f + 9.6181291076284772e-3f;
+  p = p * f + 5.5504108664821580e-2f;
+  p = p * f + 2.4022650695910071e-1f;
+  p = p * f + 6.9314718055994531e-1f;
+  p = p * f + 1.0f;
+  int32_t bits = ((int32_t)n + 127) << 23;
+  float scale;
+  memcpy(&scale, &bits, 4);
+  return p * scale;
+}
+
+/* One pair row (owner computes: every neighbor of i), f32, LJ without a
+   modifier on the force (NONE or POTENTIAL_SHIFT) and erfc Ewald: the
+   branch-free form a SIMD unit runs (the reference's CPU kernels are
+   vectorised by its compiler).  erfc(x) = t e^{-x^2} P(t) with
+   t = 1/(1 + 0.475 x) and P of degree 6 (the pair loop's approximation,
+   2e-9 relative in exact arithmetic). */
+__attribute__((target_clones("avx2", "default")))
+static void md_row(int64_t i, const int32_t *restrict idx, int32_t cnt,
+                   const float *restrict X, const float *restrict Q, const int32_t *restrict T,
+                   const float *restrict c6t, const float *restrict c12t, int32_t ntypes,
+                   const float L[3], const float iL[3], float rc2, float beta, float mb2,
+                   float tbs, double *restrict Fi) {
+  const float xi = X[3 * i], yi = X[3 * i + 1], zi = X[3 * i + 2], qi = Q[i];
+  const float *c6r = c6t + T[i] * ntypes, *c12r = c12t + T[i] * ntypes;
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+#pragma omp simd reduction(+ : fx, fy, fz)
+  for (int32_t k = 0; k < cnt; ++k) {
+    const int32_t j = idx[k];
+    float dx = xi - X[3 * j], dy = yi - X[3 * j + 1], dz = zi - X[3 * j + 2];
+    dx -= L[0] * rintf(dx * iL[0]);
+    dy -= L[1] * rintf(dy * iL[1]);
+    dz -= L[2] * rintf(dz * iL[2]);
+    float r2 = dx * dx + dy * dy + dz * dz;
+    const int on = r2 < rc2;
+    r2 = on ? r2 : 1.0f;
+    const float rinv = 1.0f / sqrtf(r2), rinv2 = rinv * rinv;
+    const float br = beta * r2 * rinv, ex = md_expf(mb2 * r2);
+    const float t = 1.0f / (1.0f + 0.475f * br);
+    float p = 0.09289151430130005f;
+    p = p * t - 0.4289827346801758f;
+    p = p * t + 0.6025476455688477f;
+    p = p * t - 0.1264616847038269f;
+    p = p * t + 0.3417808711528778f;
+    p = p * t + 0.24873463809490204f;
+    p = p * t + 0.26948970556259155f;
+    const float erfcv = t * ex * p;
+    const float fc = qi * Q[j] * (erfcv * rinv + tbs * ex) * rinv2;
+    const float r6 = rinv2 * rinv2 * rinv2;
+    const int32_t tj = T[j];
+    const float flj = (c12r[tj] * r6 - c6r[tj]) * r6 * rinv2;
+    const float fs = on ? fc + flj : 0.0f;
+    fx += fs * dx;
+    fy += fs * dy;
+    fz += fs * dz;
+  }
+  Fi[0] += fx;
+  Fi[1] += fy;
+  Fi[2] += fz;
+}
+
+static void md_pairs(orc_md *c, const double *x, double *F) {
+  const orc_system *s = c->s;
+  int64_t n = s->n, width = c->width;
+  if (s->lj_modifier == 2 || s->lj_modifier == 3) { /* switches: the oracle's rows */
+    double e[ORC_NTERMS] = {0};
+    orc_nonbonded(n, x, s->q, s->type, s->ntypes, s->sigma, s->epsilon, s->box, s->cutoff,
+                  s->switch_from, s->lj_modifier, s->coulomb_shift, s->beta, s->coulomb,
+                  s->excl_start, s->excl_atoms, 1, F, e, NULL);
+    return;
+  }
+  int32_t nt = s->ntypes;
+  float L[3], iL[3];
+  for (int k = 0; k < 3; ++k) {
+    L[k] = (float)s->box[k];
+    iL[k] = 1.0f / L[k];
+  }
+  float *c6 = (float *)malloc(sizeof(float) * (size_t)(nt * nt));
+  float *c12 = (float *)malloc(sizeof(float) * (size_t)(nt * nt));
+  for (int32_t a = 0; a < nt * nt; ++a) {
+    double s6 = pow(s->sigma[a], 6.0);
+    c6[a] = (float)(6.0 * 4.0 * s->epsilon[a] * s6);
+    c12[a] = (float)(12.0 * 4.0 * s->epsilon[a] * s6 * s6);
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    for (int k = 0; k < 3; ++k) {
+      double w = x[3 * i + k] - s->box[k] * floor(x[3 * i + k] / s->box[k]);
+      c->xf[3 * i + k] = (float)w;
+    }
+    c->qf[i] = (float)(sqrt(s->coulomb) * s->q[i]);
+  }
+  float beta = (float)s->beta, mb2 = (float)(-s->beta * s->beta);
+  float tbs = (float)(2.0 * s->beta / sqrt(M_PI));
+  float rc2 = (float)(s->cutoff * s->cutoff);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < n; ++i)
+    md_row(i, c->index + i * width, c->counts[i], c->xf, c->qf, s->type, c6, c12, nt, L, iL, rc2,
+           beta, mb2, tbs, F + 3 * i);
+  free(c6);
+  free(c12);
+}
+
+static void md_pme(orc_md *c, const double *x, double *F) {
+  const orc_system *s = c->s;
+  const int64_t *K = s->K;
+  int order = s->order;
+  int64_t n = s->n, h = K[2] / 2 + 1;
+  size_t np = (size_t)(K[0] * K[1] * K[2]), nh = (size_t)(K[0] * K[1] * h);
+  memset(c->words, 0, sizeof(int64_t) * np);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    if (s->q[i] == 0.0) continue;
+    int64_t f[3];
+    double M[3][16], dM[3][16];
+    for (int a = 0; a < 3; ++a) {
+      double w;
+      pme_place(x[3 * i + a], s->box[a], K[a], order, &f[a], &w);
+      orc_bspline(w, order, M[a], dM[a]);
+    }
+    for (int t0 = 0; t0 < order; ++t0) {
+      int64_t i0 = (f[0] + t0) % K[0];
+      for (int t1 = 0; t1 < order; ++t1) {
+        int64_t i1 = (f[1] + t1) % K[1];
+        double w01 = s->q[i] * M[0][t0] * M[1][t1];
+        for (int t2 = 0; t2 < order; ++t2) {
+          int64_t i2 = (f[2] + t2) % K[2];
+          int64_t word = llrint(w01 * M[2][t2] * 1099511627776.0);
+#pragma omp atomic
+          c->words[(i0 * K[1] + i1) * K[2] + i2] += word;
+        }
+      }
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (size_t p = 0; p < np; ++p) c->grid[p] = (double)c->words[p] * (1.0 / 1099511627776.0);
+  orc_fft3d_r2c(K, c->grid, c->cplx);
+#pragma omp parallel for schedule(static)
+  for (size_t p = 0; p < nh; ++p) {
+    c->cplx[2 * p] *= c->bc[p];
+    c->cplx[2 * p + 1] *= c->bc[p];
+  }
+  orc_fft3d_c2r(K, c->cplx, c->grid);
+  orc_pme_gather(n, x, s->q, s->box, K, order, c->grid, F);
+}
+
+static void md_forces(orc_md *c, const double *x) {
+  const orc_system *s = c->s;
+  int64_t n = s->n;
+  double half2 = 0.25 * c->skin * c->skin;
+  int moved = 0;
+#pragma omp parallel for reduction(| : moved) schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    double d2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      double d = x[3 * i + k] - c->xref[3 * i + k];
+      d2 += d * d;
+    }
+    if (d2 > half2) moved = 1;
+  }
+  if (moved) md_build(c, x);
+  memset(c->F, 0, sizeof(double) * 3 * (size_t)n);
+  md_pairs(c, x, c->F);
+  if (s->beta > 0.0) {
+    double e = 0.0;
+    if (s->nexcl_pairs)
+      orc_excluded(s->nexcl_pairs, s->excl_pairs, x, s->q, s->box, s->beta, s->coulomb, c->F,
+                   &e);
+    md_pme(c, x, c->F);
+  }
+}
+
+/* `steps` steps of velocity Verlet with SETTLE from (x, v), in place. */
+int orc_md_run(orc_md *c, double *x, double *v, double dt, int64_t steps) {
+  const orc_system *s = c->s;
+  int64_t n = s->n;
+  if (c->builds == 0) md_build(c, x);
+  md_forces(c, x);
+  for (int64_t step = 0; step < steps; ++step) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i)
+      for (int k = 0; k < 3; ++k) {
+        v[3 * i + k] += 0.5 * dt * c->F[3 * i + k] / s->mass[i];
+        c->xo[3 * i + k] = x[3 * i + k];
+        x[3 * i + k] += dt * v[3 * i + k];
+        c->xu[3 * i + k] = x[3 * i + k];
+      }
+    if (s->nwaters) {
+      orc_settle_positions(s->nwaters, s->water_first, s->m_o, s->m_h, s->d_oh, s->d_hh, c->xo,
+                           x);
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < 3 * n; ++i) v[i] += (x[i] - c->xu[i]) / dt;
+    }
+    md_forces(c, x);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i)
+      for (int k = 0; k < 3; ++k) v[3 * i + k] += 0.5 * dt * c->F[3 * i + k] / s->mass[i];
+    if (s->nwaters) orc_settle_velocities(s->nwaters, s->water_first, s->m_o, s->m_h, x, v);
+  }
+  return 0;
+}
+
+void orc_set_threads(int threads) {
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+}
+
+/* ======================================================================
  * Langevin dynamics — the g-BAOAB step of the reference's Langevin run with
  * constraints (lib/Driver/DynamicsIR.cpp, geodesic branch; Leimkuhler and
  * Matthews, Proc. R. Soc. A 472, 20160138, 2016) and its O step, the

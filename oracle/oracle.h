@@ -154,6 +154,14 @@ typedef struct {
 
 int orc_total(const orc_system *s, const double *x, double *F,
               double *energies, int32_t mixed);
+/* The CPU baseline: velocity Verlet with SETTLE, the neighbor matrix kept
+   until an atom moved half the buffer, f32 pair rows, fixed-point PME. */
+typedef struct orc_md orc_md;
+orc_md *orc_md_create(const orc_system *s, double skin);
+void orc_md_destroy(orc_md *c);
+int orc_md_run(orc_md *c, double *x, double *v, double dt, int64_t steps);
+int64_t orc_md_builds(const orc_md *c);
+void orc_set_threads(int threads);
 /* Langevin dynamics (g-BAOAB) and its Philox 4x32-10 noise. */
 void orc_philox4x32_10(const uint32_t counter[4], const uint32_t key[2], uint32_t out[4]);
 void orc_langevin_normals(uint64_t seed, int64_t step, uint32_t id, double r[3]);

@@ -110,6 +110,14 @@ def _declare(L):
     L.orc_langevin.restype = C.c_int
     L.orc_langevin.argtypes = [C.POINTER(OrcSystem), PD, PD, D, I64, I64, C.c_uint64, D, D,
                                PD, I32]
+    L.orc_md_create.restype = C.c_void_p
+    L.orc_md_create.argtypes = [C.POINTER(OrcSystem), D]
+    L.orc_md_destroy.argtypes = [C.c_void_p]
+    L.orc_md_run.restype = C.c_int
+    L.orc_md_run.argtypes = [C.c_void_p, PD, PD, D, I64]
+    L.orc_md_builds.restype = I64
+    L.orc_md_builds.argtypes = [C.c_void_p]
+    L.orc_set_threads.argtypes = [C.c_int]
     L.orc_threads.restype = C.c_int
     L.orc_threads.argtypes = []
     L.orc_splitmix64.restype = C.c_uint64
@@ -300,6 +308,12 @@ class Oracle:
                            seed, friction, BOLTZMANN * temperature, _p(trace, D), int(mixed))
         return x, v, trace
 
+    def md(self, skin=0.15):
+        """The CPU baseline's integrator (velocity Verlet with SETTLE, the
+        neighbor matrix reused until an atom moved half the buffer, f32
+        pair rows, fixed-point PME); call run(x, v, dt, steps) on it."""
+        return MdRun(self, skin)
+
     def initial_velocities(self, seed, temperature):
         s = self.s
         v = np.zeros((s.n, 3))
@@ -308,6 +322,35 @@ class Oracle:
                                      _p(self.waters, I32),
                                      s.degrees_of_freedom(), _p(v, D))
         return v
+
+
+class MdRun:
+    def __init__(self, oracle, skin):
+        self.o = oracle
+        self.h = lib().orc_md_create(C.byref(oracle.st), skin)
+
+    def run(self, x, v, dt, steps):
+        """Advances copies of x and v by `steps` steps; returns them."""
+        x = f64(x).copy()
+        v = f64(v).copy()
+        lib().orc_md_run(self.h, _p(x, D), _p(v, D), dt, steps)
+        return x, v
+
+    @property
+    def builds(self):
+        return lib().orc_md_builds(self.h)
+
+    def close(self):
+        if self.h:
+            lib().orc_md_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def set_threads(threads):
+    lib().orc_set_threads(int(threads))
 
 
 def philox4x32_10(counter, key):

@@ -164,3 +164,23 @@ def test_chain_forces_are_gradients(modifier):
     o = O.Oracle(c)
     fd_check(lambda y: o.total(y), c.positions.copy(), lambda y: o.total(y)[1].sum(),
              atoms=[0, 3, 41], h=1e-5, rtol=5e-6)
+
+
+def test_cpu_baseline_integrator_follows_the_oracle():
+    """The benchmark's CPU baseline (matrix reused until an atom moved half
+    the buffer, f32 pair rows on wrapped f32 positions with its own erfc
+    and exp, fixed-point PME spread) follows the oracle's velocity Verlet
+    with the mixed-mode pair rows (matrix rebuilt at every step) to f32
+    rounding over a few steps of an unrelaxed box, and rebuilds its matrix
+    only when an atom moved half the buffer."""
+    from mdir_amd import systems as S
+    w = S.tip3p_box(216, 1.9, seed=4, grid=(20, 20, 20), cutoff=0.8)
+    o = O.Oracle(w)
+    v0 = o.initial_velocities(3, 300.0)
+    xa, va, _ = o.nve(w.positions, v0, 0.001, 4, mixed=True)
+    md = o.md(skin=0.15)
+    xb, vb = md.run(w.positions, v0, 0.001, 4)
+    assert np.abs(xa - xb).max() < 1e-7
+    assert np.abs(va - vb).max() < 1e-4
+    assert md.builds == 1
+    md.close()
