@@ -531,6 +531,24 @@ int mdrtHipReplicasLatency(void *set, int32_t index, int64_t capacity, double *m
                            int64_t *count);
 int mdrtHipReplicasClearLatency(void *set);
 
+/* ---------------------------------------------------------------------
+ * 5. No counterpart in the reference: for callers that cannot allocate
+ *    device memory (code under HIP graph capture, tight serving loops,
+ *    harnesses that forbid hipMalloc).
+ * ------------------------------------------------------------------- */
+
+/* _mlir_ciface_mdrtCudaFFTForward3DF32 without a plan: the real grid
+   k1 x k2 x k3 (f32, device) to its half spectrum k1 x k2 x (k3/2+1)
+   (interleaved complex f32, device), forward exp(-2 pi i), unnormalised,
+   on `stream` (NULL: the default stream).  The three axes' twiddle tables
+   are its only workspace and live in a fixed device array of the library
+   (3 x 1024 complex f32 = 24 KiB per GPU, held until the process ends);
+   it calls no allocator.  Edges in [2, 1024] with factors 2, 3, 5, 7 only,
+   k3 even; anything else returns -1 (mdrtHipLastError).  Calls that
+   overlap on one GPU (different streams) race on the tables. */
+int mdrtHipFFTForward3DF32NoAlloc(const float *real, void *complex, int64_t k1, int64_t k2,
+                                  int64_t k3, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2420,3 +2420,4 @@ int mdrtHipProfileStep(void *system, int32_t repeats, double *ms) {
 }  // extern "C"
 
 #include "replicas.hip"
+#include "noalloc.hip"

@@ -127,6 +127,8 @@ SIGNATURES = [
     ("mdrtHipSetLangevin", C.c_int, [C.c_void_p, D, D, C.c_uint64]),
     ("mdrtHipSetStepCount", C.c_int, [C.c_void_p, I64]),
     ("mdrtHipListPairs", C.c_int, [C.c_void_p, D, I32, I64, PI32, PI64]),
+    ("mdrtHipFFTForward3DF32NoAlloc", C.c_int, [C.c_void_p, C.c_void_p, I64, I64, I64,
+                                                C.c_void_p]),
     ("mdrtHipReplicasCreate", C.c_void_p, [C.POINTER(SystemDesc), I32]),
     ("mdrtHipReplicasDestroy", None, [C.c_void_p]),
     ("mdrtHipReplicasSystem", C.c_void_p, [C.c_void_p, I32]),
