@@ -79,6 +79,14 @@ def parse():
                    help="frames of the force-matching gradient measurement (0: skip)")
     p.add_argument("--frame-repeats", type=int, default=20)
     p.add_argument("--frame-loop-passes", type=int, default=3)
+    p.add_argument("--replicas", type=int, default=0,
+                   help="also measure the batching workload: this many TIP3P replicas of "
+                        "500-8,000 waters, Langevin 300 K, one graph per chunk (0: skip)")
+    p.add_argument("--replica-steps", type=int, default=500)
+    p.add_argument("--replica-chunk", type=int, default=10)
+    p.add_argument("--replica-launch", type=int, default=1, choices=(0, 1),
+                   help="0: one graph of all replicas per chunk; 1: a graph per replica on "
+                        "its own stream")
     p.add_argument("--calibrate-copy-mib", type=int, default=0,
                    help="after the bench, run a plain device copy of this many MiB "
                         "5 times (the reference kernel for counter-based traffic)")
@@ -291,6 +299,15 @@ def main():
         out["frame_gradients"] = force_matching_benchmark(args.frames, args.frame_repeats,
                                                           args.frame_loop_passes,
                                                           autotune=args.autotune)
+    if args.replicas > 0:
+        from mdir_amd.replicas import replica_benchmark
+        out["replicas"] = replica_benchmark(args.replicas, steps=args.replica_steps,
+                                            chunk=args.replica_chunk,
+                                            autotune=bool(args.autotune),
+                                            launch=args.replica_launch)
+        print(f"replicas: {args.replicas}, {out['replicas']['value']} ns/day aggregate, "
+              f"{out['replicas']['ms_per_step']} ms per step of all", file=sys.stderr,
+              flush=True)
     print(json.dumps(out), flush=True)
     if args.calibrate_copy_mib:
         calibrate_copy(args.calibrate_copy_mib)

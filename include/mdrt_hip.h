@@ -523,6 +523,12 @@ void *mdrtHipReplicasSystem(void *set, int32_t index);
    outgrew its buffers redoes its chunk alone with the grown list. */
 int mdrtHipReplicasRun(void *set, int64_t steps, double dt, int32_t graph);
 
+/* How a chunk is launched: 0 (the default) one graph holding every
+   replica's branch, on one stream; 1 one graph per replica (the same
+   branch), each on the replica's own stream, so that the runtime's
+   hardware queues (GPU_MAX_HW_QUEUES) run replicas side by side. */
+int mdrtHipReplicasSetLaunch(void *set, int32_t mode);
+
 /* Step latency of replica `index` inside the batch: for each chunk since the
    last clear, (device clock at the end of its branch - at its start) /
    steps, ms.  Writes up to `capacity` values; *count receives how many

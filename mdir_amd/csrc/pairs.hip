@@ -45,7 +45,8 @@ __device__ __forceinline__ void add_fixed(long long *F, int place, float fx,
 
 // erfc(x) = t exp(-x^2) P(t), t = 1/(1 + 0.475 x): a degree-6 polynomial
 // fitted (weighted least squares on the relative error, this project's own
-// fit) on [0, 3.6]; 2e-9 relative in exact arithmetic, ~1e-6 in f32, where
+// fit, tests/golden/erfc_fit.py) on [0, 3.6]; below 1e-7 relative in exact
+// arithmetic (9.8e-8 at worst), ~1e-6 in f32, where
 // the rounding of exp(-x^2) itself dominates as it does for any f32 erfc.
 // It reuses the exp(-x^2) that the force needs anyway.  t comes from the
 // hardware reciprocal (1 ulp).

@@ -133,6 +133,7 @@ SIGNATURES = [
     ("mdrtHipReplicasDestroy", None, [C.c_void_p]),
     ("mdrtHipReplicasSystem", C.c_void_p, [C.c_void_p, I32]),
     ("mdrtHipReplicasRun", C.c_int, [C.c_void_p, I64, D, I32]),
+    ("mdrtHipReplicasSetLaunch", C.c_int, [C.c_void_p, I32]),
     ("mdrtHipReplicasLatency", C.c_int, [C.c_void_p, I32, I64, PD, PI64]),
     ("mdrtHipReplicasClearLatency", C.c_int, [C.c_void_p]),
     ("mdrtHipGetBox", C.c_int, [C.c_void_p, PD]),

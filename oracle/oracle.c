@@ -1337,7 +1337,7 @@ static inline float md_expf(float y) {
    branch-free form a SIMD unit runs (the reference's CPU kernels are
    vectorised by its compiler).  erfc(x) = t e^{-x^2} P(t) with
    t = 1/(1 + 0.475 x) and P of degree 6 (the pair loop's approximation,
-   2e-9 relative in exact arithmetic). */
+   below 1e-7 relative in exact arithmetic). */
 __attribute__((target_clones("avx2", "default")))
 static void md_row(int64_t i, const int32_t *restrict idx, int32_t cnt,
                    const float *restrict X, const float *restrict Q, const int32_t *restrict T,

@@ -17,11 +17,13 @@ def say(msg):
     print(msg, flush=True)
 
 
-def test_replicas_match_single_runs_bit_for_bit():
+@pytest.mark.parametrize("launch", [0, 1])
+def test_replicas_match_single_runs_bit_for_bit(launch):
     from mdir_amd.replicas import ReplicaSet
     from mdir_amd.simulation import Simulation
     sizes = [500, 1333, 2200, 777]
-    say(f"case: {len(sizes)} replicas of {sizes} waters, 40 Langevin steps in chunks of 10")
+    say(f"case: {len(sizes)} replicas of {sizes} waters, 40 Langevin steps in chunks of 10, "
+        f"launch mode {launch}")
     systems = []
     for k, nw in enumerate(sizes):
         s = S.tip3p_replica(nw, seed=100 + k)
@@ -33,6 +35,7 @@ def test_replicas_match_single_runs_bit_for_bit():
         sim.close()
         systems.append(s)
     rs = ReplicaSet(systems)
+    rs.set_launch(launch)
     for k, r in enumerate(rs):
         r.set_langevin(300.0, 1.0, 1000 + k)
     rs.run(40, 0.002, graph=10)

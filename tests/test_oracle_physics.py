@@ -184,3 +184,21 @@ def test_cpu_baseline_integrator_follows_the_oracle():
     assert np.abs(va - vb).max() < 1e-4
     assert md.builds == 1
     md.close()
+
+
+def test_erfc_fit_of_the_pair_kernels():
+    """The pair kernels' erfc, t exp(-x^2) P(t) with t = 1/(1 +
+    0.475 x), meets erfc to 1.2e-7 relative over [0, 3.6], and the fit of
+    tests/golden/erfc_fit.py reproduces coefficients at least as good."""
+    import importlib.util
+    import os
+    from scipy.special import erfc
+    path = os.path.join(os.path.dirname(__file__), "golden", "erfc_fit.py")
+    spec = importlib.util.spec_from_file_location("erfc_fit", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    x = np.linspace(0.0, 3.6, 50001)
+    assert np.abs(mod.relative_error(mod.KERNEL, x)).max() < 1.2e-7
+    assert np.abs(mod.relative_error(mod.fit(), x)).max() < 1.0e-7
+    # the fit is of erfc, not of something close to it
+    assert abs(mod.approx(mod.KERNEL, np.array([1.0]))[0] - erfc(1.0)) < 2e-8
