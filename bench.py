@@ -79,9 +79,9 @@ def parse():
                    help="frames of the force-matching gradient measurement (0: skip)")
     p.add_argument("--frame-repeats", type=int, default=20)
     p.add_argument("--frame-loop-passes", type=int, default=3)
-    p.add_argument("--replicas", type=int, default=0,
+    p.add_argument("--replicas", type=int, default=256,
                    help="also measure the batching workload: this many TIP3P replicas of "
-                        "500-8,000 waters, Langevin 300 K, one graph per chunk (0: skip)")
+                        "500-8,000 waters, Langevin 300 K, graphs of one chunk (0: skip)")
     p.add_argument("--replica-steps", type=int, default=500)
     p.add_argument("--replica-chunk", type=int, default=10)
     p.add_argument("--replica-launch", type=int, default=1, choices=(0, 1),

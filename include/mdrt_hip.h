@@ -1,7 +1,7 @@
 /* mdrt_hip.h — C ABI of libmdrt_hip.so, the MI355X (gfx950) runtime and
  * per-step force path.
  *
- * Three groups of entry points:
+ * Groups of entry points:
  *
  *  1. The runtime ABI of the reference's runtime/mdrt_cuda.c, same names and
  *     signatures, on HIP (modules are gfx950 code objects, streams are HIP
@@ -16,6 +16,14 @@
  *  3. The frame evaluator: energies, forces and reverse products (charges,
  *     per-type sigma and epsilon; position Hessian products) of many frames
  *     of one system per launch, and force matching in one call.
+ *
+ *  4. Replica sets: many systems of uneven size stepped together.
+ *
+ *  5. For callers that cannot allocate device memory.
+ *
+ *  6. Outputs: trajectories in DCD, written from staged copies on a thread
+ *     of their own (the reference's driver, lib/Driver/Trajectory.cpp and
+ *     lib/Driver/Output.cpp, over group 1's staging entry points).
  *
  * Units are the reference's: nm, ps, amu, kJ/mol, e.  All pointers in
  * group 2 are host pointers unless the name says device.
@@ -130,6 +138,36 @@ void mdrtSetFailureHook(void (*hook)(void));
 /* mdrt_cuda.c:2824, :2836 */
 uint64_t mdrtDeviceTrim(void);
 void mdrtDeviceMemoryCounts(int64_t counts[8]);
+
+/* mdrt_cuda.c:3530-3576 — kept memory: device memory outside the pool that
+   an embedding program holds across runs, and copies to and from it on the
+   current stream (CopyWithin does not wait; CopyToHost returns when the
+   bytes are on the host). */
+void *mdrtDeviceAllocateKept(uint64_t size);
+void mdrtDeviceFreeKept(void *pointer);
+void mdrtDeviceCopyWithin(void *destination, const void *source, uint64_t size);
+void mdrtDeviceCopyToHost(void *destination, const void *source, uint64_t size);
+
+/* mdrt_cuda.c:3611-3845 — staged copies of a frame while the steps go on.
+   A slot holds up to 4 parts.  Copy: a device-to-device copy of `size`
+   bytes at `source` into the slot's device buffer of that part, on the
+   current stream; returns the pinned host memory the bytes will be in.
+   End: a stream of copies of the runtime's own waits for the current
+   stream, copies the parts copied since the last end to the host, and then
+   has the device store the end's serial number into the slot's mapped
+   word.  Wait (any thread): 0 once the copies of the last end are on the
+   host, else the HIP error of the wait, with the call that returned it in
+   *call, which the thread of the steps hands to Fail (it reports the error
+   as any failed device call is reported, and aborts).  The environment
+   variable MDRT_STAGING_MARK_DELAY=<ms> holds the stored word back by that
+   long (a host function on the stream of copies), for tests of a word that
+   lands after its wait has returned. */
+void *mdrtStagingCreate(void);
+void *mdrtStagingCopy(void *slot, int32_t part, const void *source, uint64_t size);
+void mdrtStagingEnd(void *slot);
+int32_t mdrtStagingWait(void *slot, const char **call);
+void mdrtStagingFail(int32_t result, const char *call);
+void mdrtStagingDestroy(void *slot);
 
 /* mdrt_cuda.c:2901-2991 — the 3D transforms of the PME grid, with this
    library's FFT kernels: real k1 x k2 x k3 <-> half spectrum
@@ -331,6 +369,33 @@ int mdrtHipBarostatStats(void *system, int64_t *counts, double *max_dv);
    to the checkpoint's count). */
 int mdrtHipSetStepCount(void *system, int64_t step);
 int mdrtHipGetStepCount(void *system, int64_t *step);
+
+/* The state of a run beyond its positions and velocities, as a checkpoint
+   keeps it (the reference's include/mdir/Driver/Checkpoint.h:36-136 holds
+   the same quantities in H5MD, lib/Driver/Checkpoint.cpp). */
+typedef struct {
+  int64_t step_count;          /* the step number of the next step        */
+  double box[3];               /* nm                                      */
+  double barostat_dv;          /* the volume moves' largest change, nm^3  */
+  int64_t barostat_counts[4];  /* moves tried, kept; in the current
+                                  window of 10: tried, kept              */
+  int32_t list_kept;           /* 1: the origins are those of the run's
+                                  list; 0: the next run builds one at x  */
+  int32_t reserved;
+} mdrtHipRunState;
+
+/* The run's state: positions, velocities, the forces the next step begins
+   with (f64, atom order; NULL to skip), the positions the current list was
+   built at and those it was last pruned at (n x 3 each, or NULL), and
+   *state.  Restore puts such a state back: it builds the list where the
+   saved run built it (and prunes where it pruned), then evaluates the
+   forces at `x` through that list, so the steps that follow are those the
+   saved run takes, bit for bit. */
+int mdrtHipSaveState(void *system, double *x, double *v, double *forces, double *list_origin,
+                     double *prune_origin, mdrtHipRunState *state);
+int mdrtHipRestoreState(void *system, const double *x, const double *v,
+                        const double *list_origin, const double *prune_origin,
+                        const mdrtHipRunState *state);
 
 /* `steps` velocity-Verlet steps of dt (ps) with SETTLE waters.  graph != 0
    replays the step sequence as captured HIP graphs of `graph` steps (and
@@ -554,6 +619,44 @@ int mdrtHipReplicasClearLatency(void *set);
    overlap on one GPU (different streams) race on the tables. */
 int mdrtHipFFTForward3DF32NoAlloc(const float *real, void *complex, int64_t k1, int64_t k2,
                                   int64_t k3, void *stream);
+
+/* ---------------------------------------------------------------------
+ * 6. Outputs (lib/Driver/Trajectory.cpp:48-206, lib/Driver/Output.cpp
+ *    frame thread): DCD files as the reference writes them — the CORD
+ *    record (frames, first step, period, frames x period, the time step in
+ *    AKMA units as f32, the cell flag, version 24), one title line, the
+ *    particle count, then per frame the cell (a, cos gamma, b, cos beta,
+ *    cos alpha, c in Angstrom, f64) and x, y, z records of f32 Angstrom.
+ * ------------------------------------------------------------------- */
+
+/* A trajectory of `system` at `path`: a frame every `period` steps from
+   step `first` (the step of the first frame), time step in ps.  keep < 0
+   starts a new file; keep >= 0 continues a file this writer wrote with the
+   same period and cell flag, keeping its first `keep` frames (a restart
+   from a checkpoint that counted them).  NULL on failure
+   (mdrtHipLastError). */
+void *mdrtHipTrajectoryOpen(void *system, const char *path, int64_t first, int64_t period,
+                            double timestep_ps, int64_t keep);
+/* Stages the system's positions now (after the steps issued so far) as the
+   frame of step `step`; a thread of the writer converts and writes it
+   while the next steps run.  At most 3 frames are in flight; the call of a
+   fourth waits for the oldest.  A copy that failed on the writer's thread
+   is reported here, through mdrtStagingFail. */
+int mdrtHipTrajectoryFrame(void *trajectory, int64_t step);
+/* Waits until every staged frame is in the file. */
+int mdrtHipTrajectoryFlush(void *trajectory);
+/* Frames the file holds once the staged ones are written. */
+int64_t mdrtHipTrajectoryFrames(void *trajectory);
+/* Flushes, closes the file and frees the slots. */
+int mdrtHipTrajectoryClose(void *trajectory);
+
+/* A whole DCD file at once from host arrays: `frames` frames of `n`
+   particles, positions in nm (frames x n x 3, f64), the orthorhombic cell
+   of each frame in nm (frames x 3; NULL: no cell records), converted as the
+   trajectory writer converts them (f32 Angstrom). */
+int mdrtHipDcdWrite(const char *path, int64_t n, int64_t frames, int64_t first,
+                    int64_t period, double timestep_ps, const double *positions,
+                    const double *cells);
 
 #ifdef __cplusplus
 }

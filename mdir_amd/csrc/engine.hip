@@ -411,6 +411,95 @@ struct Engine {
     to_device(d_lstep, h, sizeof(h));
   }
 
+  // ---- checkpoints ----
+  // A run's state, and the positions its list was built and pruned at:
+  // with those, restore_state() rebuilds the same list, so a restored run
+  // takes the steps the saved one would have taken, bit for bit.  When the
+  // next run would build a list anyway (positions set, a volume move), the
+  // origins are the positions themselves.
+  void save_state(double *x, double *v, double *forces_out, double *origin,
+                  double *prune_origin, mdrtHipRunState *st) {
+    if (nz != 1) throw HipError("checkpoint: not for frame batches");
+    const bool kept = forces_valid && list_reach == cutoff + skin;
+    if (x) to_host(x, d_x, 3 * (size_t)n * sizeof(double));
+    if (v) to_host(v, d_v, 3 * (size_t)n * sizeof(double));
+    if (origin) to_host(origin, kept ? d_xref : d_x, 3 * (size_t)n * sizeof(double));
+    if (prune_origin)
+      to_host(prune_origin, kept && d_xprune ? d_xprune : d_x, 3 * (size_t)n * sizeof(double));
+    if (forces_out) {
+      if (!forces_valid) evaluate(false);
+      fold_pme_atoms();  // F + Fr is what the next half kick reads; folding keeps the sum
+      double *d_out = dalloc<double>(3 * (size_t)n);
+      forces_to_atoms<<<blocks_for(n), 256, 0, stream>>>(n, d_place_of, d_F, d_out, 0);
+      to_host(forces_out, d_out, 3 * (size_t)n * sizeof(double));
+      dfree(d_out);
+    }
+    if (st) {
+      st->step_count = step_count;
+      for (int k = 0; k < 3; ++k) st->box[k] = box[k];
+      st->barostat_dv = mc_dv;
+      st->barostat_counts[0] = mc_tried;
+      st->barostat_counts[1] = mc_kept;
+      st->barostat_counts[2] = mc_window_tried;
+      st->barostat_counts[3] = mc_window_kept;
+      st->list_kept = kept ? 1 : 0;
+      st->reserved = 0;
+    }
+  }
+
+  // The forces at the current positions through the list the engine holds,
+  // with no build (the validity flags are clear) unless `prune` asks the
+  // build chain for a prune at these positions.
+  void forces_in_list(bool prune) {
+    refresh_places<<<blocks_for(n), 256, 0, stream>>>(
+        n, d_x, d_shift, d_q, d_place_of, use_origins() ? d_gorig : nullptr, ip.half_box[0],
+        ip.half_box[1], ip.half_box[2], d_xq, d_xpa ? nullptr : d_xpq);
+    HIPCHK(hipGetLastError());
+    if (prune) {
+      const int one = 1;
+      to_device(d_flags + kFlagPrune, &one, sizeof(int));
+    }
+    clear_forces();
+    pack_pme_atoms();
+    forces(false, -1, true);
+    fold_pme_atoms();
+    const std::vector<int> f = read_flags();
+    if (f[kFlagOverflowColumn] || f[kFlagOverflowRow] || f[kFlagBrickOverflow])
+      throw HipError("restore: the list overflowed without a build");
+  }
+
+  void restore_state(const double *x, const double *v, const double *origin,
+                     const double *prune_origin, const mdrtHipRunState *st) {
+    if (nz != 1) throw HipError("restore: not for frame batches");
+    if (!x || !v || !st) throw HipError("restore: positions, velocities and state are needed");
+    if (st->box[0] != box[0] || st->box[1] != box[1] || st->box[2] != box[2])
+      set_box(st->box);
+    step_count = st->step_count;
+    mc_dv = st->barostat_dv;
+    mc_tried = st->barostat_counts[0];
+    mc_kept = st->barostat_counts[1];
+    mc_window_tried = st->barostat_counts[2];
+    mc_window_kept = st->barostat_counts[3];
+    invalidate_graph();
+    const size_t bytes = 3 * (size_t)n * sizeof(double);
+    to_device(d_v, v, bytes);
+    if (!st->list_kept || !origin) {
+      to_device(d_x, x, bytes);
+      forces_valid = false;  // the next run builds at x, as the saved one would have
+      return;
+    }
+    if (list_reach != cutoff + skin) configure_list(cutoff + skin);
+    to_device(d_x, origin, bytes);
+    evaluate(false);  // the list (and its prune) at the positions the run built it at
+    if (d_xprune && prune_origin) {
+      to_device(d_x, prune_origin, bytes);
+      forces_in_list(true);  // the pruned rows where the run last pruned
+    }
+    to_device(d_x, x, bytes);
+    forces_in_list(false);
+    forces_valid = true;
+  }
+
   // ---- force terms ----
   // Which terms a force evaluation computes (bits of MDRT_TERM_*): the
   // nonbonded pairs (LJ + direct Coulomb), the excluded and 1-4 pairs, the
@@ -2062,6 +2151,19 @@ AbiPlan *find_plan(int64_t k1, int64_t k2, int64_t k3, int single) {
 }
 }  // namespace
 
+// The serial number of a staging end, stored by the device into the slot's
+// mapped host word once the copies before it on the stream are done: a
+// vector store with a system-scope release, so that the host sees the
+// copies' bytes before it sees the number.
+__global__ void stage_mark(unsigned int *word, unsigned int serial) {
+  __hip_atomic_store(word, serial, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+extern "C" int mdhip_stage_mark(void *word, uint32_t serial, void *stream) {
+  stage_mark<<<1, 1, 0, (hipStream_t)stream>>>(static_cast<unsigned int *>(word), serial);
+  return (int)hipGetLastError();
+}
+
 extern "C" int mdhip_fft_execute(int direction, int single, void *in, void *out, int64_t k1,
                                  int64_t k2, int64_t k3, const float *tables, void *stream) {
   return guarded([&] {
@@ -2273,6 +2375,19 @@ int mdrtHipGetStepCount(void *system, int64_t *step) {
   return guarded([&] { *step = e->step_count; });
 }
 
+int mdrtHipSaveState(void *system, double *x, double *v, double *forces, double *list_origin,
+                     double *prune_origin, mdrtHipRunState *state) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] { e->save_state(x, v, forces, list_origin, prune_origin, state); });
+}
+
+int mdrtHipRestoreState(void *system, const double *x, const double *v,
+                        const double *list_origin, const double *prune_origin,
+                        const mdrtHipRunState *state) {
+  Engine *e = static_cast<Engine *>(system);
+  return guarded([&] { e->restore_state(x, v, list_origin, prune_origin, state); });
+}
+
 int mdrtHipRun(void *system, int64_t steps, double dt, int32_t graph, double *kinetic) {
   Engine *e = static_cast<Engine *>(system);
   return guarded([&] { e->run(steps, dt, graph, kinetic); });
@@ -2421,3 +2536,4 @@ int mdrtHipProfileStep(void *system, int32_t repeats, double *ms) {
 
 #include "replicas.hip"
 #include "noalloc.hip"
+#include "trajectory.hip"

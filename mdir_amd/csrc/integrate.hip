@@ -425,6 +425,27 @@ __global__ void pme_pack_atoms(int n, const double *__restrict__ x, const double
   if (i < n) xpa[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], q[i]);
 }
 
+// The pair loop's f32 positions (and the f64 ones of PME in place order) of
+// every atom at its place in the current list, from x: the values a drift
+// stores (store_drifted, the same expressions without contraction), for
+// positions set outside the steps while the list stays.
+__global__ void refresh_places(int n, const double *__restrict__ x,
+                               const double *__restrict__ shift, const double *__restrict__ q,
+                               const int *__restrict__ place_of,
+                               const float4 *__restrict__ gorig, double hx, double hy,
+                               double hz, float4 *__restrict__ xq,
+                               double4 *__restrict__ xpq) {
+  const int at = blockIdx.x * blockDim.x + threadIdx.x;
+  if (at >= n) return;
+  const int pl = place_of[at];
+  const float4 o = gorig ? gorig[pl / (kCluster * kGroup)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const double x0 = x[3 * at], x1 = x[3 * at + 1], x2 = x[3 * at + 2];
+  xq[pl] = make_float4((float)(x0 - shift[3 * at + 0] - hx - (double)o.x),
+                       (float)(x1 - shift[3 * at + 1] - hy - (double)o.y),
+                       (float)(x2 - shift[3 * at + 2] - hz - (double)o.z), (float)q[at]);
+  if (xpq) xpq[pl] = make_double4(x0, x1, x2, q[at]);
+}
+
 // Reciprocal force words by atom added into the place-ordered words (and
 // cleared): integer sums, so the result is the word a single buffer holds.
 __global__ void pme_fold_atoms(int n, const int *__restrict__ place_of,

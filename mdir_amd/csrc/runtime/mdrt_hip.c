@@ -23,12 +23,14 @@
  * recover from goes to the stop handler when one is registered, else the
  * process exits with status 1.
  */
+#define _POSIX_C_SOURCE 200809L /* nanosleep */
 #include <hip/hip_runtime_api.h>
 #include <stdarg.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "../../../include/mdrt_hip.h"
 
@@ -41,6 +43,9 @@ void mdhip_fft_release(int handle);
 int64_t mdhip_fft_plan_count(void);
 int64_t mdhip_fft_plan_bytes(void);
 void mdhip_fft_trim(void);
+/* A one-thread kernel that stores `serial` into the mapped host word at
+   device address `word`, with a system-scope release (a vector store). */
+int mdhip_stage_mark(void *word, uint32_t serial, void *stream);
 
 /* ======================================================================
    Failures
@@ -738,6 +743,200 @@ int32_t mdrtFlagTake(void *device_flag) {
   int32_t value = flags.host[place];
   flags.host[place] = 0;
   return value;
+}
+
+/* ======================================================================
+   Kept memory and staged frames
+   ======================================================================
+   Kept memory is device memory outside the pool, taken from HIP and given
+   back to it directly: an embedding program holds it across runs (the
+   state a failed part of a run returns to, the slots of staged frames).
+
+   A staging slot carries up to STAGE_PARTS device buffers of one frame to
+   pinned host memory while the steps go on:
+
+     mdrtStagingCopy  copies a buffer into the slot's own device buffer on
+                      the current stream, after the work that produced it
+                      and before any later work overwrites it;
+     mdrtStagingEnd   the runtime's stream of copies waits for the current
+                      stream, copies the slot's device buffers to its host
+                      buffers, records an event, and runs a one-thread
+                      kernel that stores the end's serial number into the
+                      slot's mapped word;
+     mdrtStagingWait  returns at once when the word holds the serial of the
+                      last end, else waits for the event.
+
+   The word is stored by the device as stream work behind the copies, so it
+   never claims copies that did not run; a serial number rather than a flag,
+   so that a word that lands late (after its wait returned on the event and
+   the slot was staged again) is not taken for the current end. */
+
+enum { STAGE_PARTS = 4 };
+
+typedef struct {
+  hipEvent_t produced;        /* the current stream at the end */
+  hipEvent_t copied;          /* the stream of copies after the end's copies */
+  uint32_t ends;              /* serial number of the last end */
+  volatile uint32_t *mark;    /* the mapped word the device stores into */
+  void *mark_device;
+  struct {
+    void *device, *host;
+    uint64_t room, size;
+    int pending;              /* copied since the last end */
+  } part[STAGE_PARTS];
+} StagingSlot;
+
+static hipStream_t copy_lane = NULL; /* the stream of copies */
+
+void *mdrtDeviceAllocateKept(uint64_t size) {
+  use_device();
+  if (size == 0) return NULL;
+  void *pointer = NULL;
+  hipError_t status = hipMalloc(&pointer, size);
+  if (status == hipErrorOutOfMemory) { /* the pool's spare blocks go first */
+    (void)hipGetLastError();
+    drain();
+    release_spare();
+    mdhip_fft_trim();
+    status = hipMalloc(&pointer, size);
+  }
+  hip_ok(status, "hipMalloc");
+  return pointer;
+}
+
+void mdrtDeviceFreeKept(void *pointer) {
+  if (!pointer) return;
+  drain(); /* work issued before may still use it */
+  if (copy_lane) hip_ok(hipStreamSynchronize(copy_lane), "hipStreamSynchronize");
+  hip_ok(hipFree(pointer), "hipFree");
+}
+
+void mdrtDeviceCopyWithin(void *destination, const void *source, uint64_t size) {
+  if (size == 0) return;
+  hipStream_t stream = (hipStream_t)mgpuStreamCreate();
+  hip_ok(hipMemcpyAsync(destination, source, (size_t)size, hipMemcpyDeviceToDevice, stream),
+         "hipMemcpyAsync");
+}
+
+void mdrtDeviceCopyToHost(void *destination, const void *source, uint64_t size) {
+  if (size == 0) return;
+  hipStream_t stream = (hipStream_t)mgpuStreamCreate();
+  mdrtCountHostRead((size_t)size, 0, 0);
+  hip_ok(hipMemcpyAsync(destination, source, (size_t)size, hipMemcpyDeviceToHost, stream),
+         "hipMemcpyAsync");
+  mdrtCountHostRead(0, 1, 0);
+  drain();
+  hip_ok(hipStreamSynchronize(stream), "hipStreamSynchronize");
+}
+
+void *mdrtStagingCreate(void) {
+  ensure_trunk();
+  StagingSlot *slot = calloc(1, sizeof *slot);
+  if (!slot) {
+    fputs("mdrt: out of host memory for a staging slot\n", stderr);
+    exit(1);
+  }
+  if (!copy_lane)
+    hip_ok(hipStreamCreateWithFlags(&copy_lane, hipStreamNonBlocking),
+           "hipStreamCreateWithFlags");
+  slot->produced = new_marker();
+  slot->copied = new_marker();
+  void *word = NULL;
+  hip_ok(hipHostMalloc(&word, sizeof(uint32_t), hipHostMallocMapped | hipHostMallocCoherent),
+         "hipHostMalloc");
+  slot->mark = (volatile uint32_t *)word;
+  *slot->mark = 0;
+  hip_ok(hipHostGetDevicePointer(&slot->mark_device, word, 0), "hipHostGetDevicePointer");
+  return slot;
+}
+
+void *mdrtStagingCopy(void *opaque, int32_t part, const void *source, uint64_t size) {
+  StagingSlot *slot = opaque;
+  if (!slot || part < 0 || part >= STAGE_PARTS) return NULL;
+  if (slot->part[part].room < size) {
+    /* the stream of copies may still read the old buffers */
+    hip_ok(hipStreamSynchronize(copy_lane), "hipStreamSynchronize");
+    mdrtDeviceFreeKept(slot->part[part].device);
+    if (slot->part[part].host) hip_ok(hipHostFree(slot->part[part].host), "hipHostFree");
+    slot->part[part].device = mdrtDeviceAllocateKept(size);
+    hip_ok(hipHostMalloc(&slot->part[part].host, (size_t)size, hipHostMallocDefault),
+           "hipHostMalloc");
+    slot->part[part].room = size;
+  }
+  if (size)
+    hip_ok(hipMemcpyAsync(slot->part[part].device, source, (size_t)size,
+                          hipMemcpyDeviceToDevice, (hipStream_t)mgpuStreamCreate()),
+           "hipMemcpyAsync");
+  slot->part[part].size = size;
+  slot->part[part].pending = 1;
+  return slot->part[part].host;
+}
+
+/* MDRT_STAGING_MARK_DELAY: the stream of copies sleeps this long on the
+   host between an end's copies and its word. */
+static void hold_mark(void *opaque) {
+  const long ms = (long)(intptr_t)opaque;
+  struct timespec pause = {ms / 1000, (ms % 1000) * 1000000L};
+  nanosleep(&pause, NULL);
+}
+
+void mdrtStagingEnd(void *opaque) {
+  StagingSlot *slot = opaque;
+  if (!slot) misuse("mdrtStagingEnd of no slot");
+  hipStream_t stream = (hipStream_t)mgpuStreamCreate();
+  ++slot->ends;
+  hip_ok(hipEventRecord(slot->produced, stream), "hipEventRecord");
+  hip_ok(hipStreamWaitEvent(copy_lane, slot->produced, 0), "hipStreamWaitEvent");
+  for (int i = 0; i < STAGE_PARTS; ++i) {
+    if (!slot->part[i].pending) continue;
+    slot->part[i].pending = 0;
+    if (!slot->part[i].size) continue;
+    hip_ok(hipMemcpyAsync(slot->part[i].host, slot->part[i].device, (size_t)slot->part[i].size,
+                          hipMemcpyDeviceToHost, copy_lane),
+           "hipMemcpyAsync");
+  }
+  hip_ok(hipEventRecord(slot->copied, copy_lane), "hipEventRecord");
+  const char *delay = getenv("MDRT_STAGING_MARK_DELAY");
+  const long ms = delay && *delay ? strtol(delay, NULL, 10) : 0;
+  if (ms > 0)
+    hip_ok(hipLaunchHostFunc(copy_lane, hold_mark, (void *)(intptr_t)ms), "hipLaunchHostFunc");
+  hip_ok((hipError_t)mdhip_stage_mark(slot->mark_device, slot->ends, copy_lane),
+         "hipLaunchKernel (staging word)");
+}
+
+int32_t mdrtStagingWait(void *opaque, const char **call) {
+  StagingSlot *slot = opaque;
+  if (__atomic_load_n(slot->mark, __ATOMIC_ACQUIRE) == slot->ends) return 0;
+  *call = "hipSetDevice";
+  hipError_t result = hipSetDevice(device.ordinal); /* a writer's own thread */
+  if (result == hipSuccess) {
+    *call = "hipEventSynchronize";
+    result = hipEventSynchronize(slot->copied);
+  }
+  /* the word may have landed while the event was waited for */
+  if (result != hipSuccess && __atomic_load_n(slot->mark, __ATOMIC_ACQUIRE) == slot->ends)
+    return 0;
+  return (int32_t)result;
+}
+
+void mdrtStagingFail(int32_t result, const char *call) {
+  char what[96];
+  snprintf(what, sizeof what, "%s of a staged frame", call ? call : "the wait");
+  hip_ok((hipError_t)result, what);
+}
+
+void mdrtStagingDestroy(void *opaque) {
+  StagingSlot *slot = opaque;
+  if (!slot) return;
+  if (copy_lane) hip_ok(hipStreamSynchronize(copy_lane), "hipStreamSynchronize");
+  for (int i = 0; i < STAGE_PARTS; ++i) {
+    mdrtDeviceFreeKept(slot->part[i].device);
+    if (slot->part[i].host) hip_ok(hipHostFree(slot->part[i].host), "hipHostFree");
+  }
+  hip_ok(hipHostFree((void *)slot->mark), "hipHostFree");
+  hip_ok(hipEventDestroy(slot->produced), "hipEventDestroy");
+  hip_ok(hipEventDestroy(slot->copied), "hipEventDestroy");
+  free(slot);
 }
 
 /* ======================================================================

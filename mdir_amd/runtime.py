@@ -25,6 +25,12 @@ D, I32, I64 = C.c_double, C.c_int32, C.c_int64
 PD, PI32, PI64 = C.POINTER(D), C.POINTER(I32), C.POINTER(I64)
 
 
+class RunState(C.Structure):
+    """mdrtHipRunState: a run's state beyond positions and velocities."""
+    _fields_ = [("step_count", I64), ("box", D * 3), ("barostat_dv", D),
+                ("barostat_counts", I64 * 4), ("list_kept", I32), ("reserved", I32)]
+
+
 class SystemDesc(C.Structure):
     """mdrtHipSystemDesc."""
     _fields_ = [("n", I64), ("positions", PD), ("velocities", PD),
@@ -98,6 +104,16 @@ SIGNATURES = [
     ("mdrtSetFailureHook", None, [C.c_void_p]),
     ("mdrtDeviceTrim", C.c_uint64, []),
     ("mdrtDeviceMemoryCounts", None, [PI64]),
+    ("mdrtDeviceAllocateKept", C.c_void_p, [C.c_uint64]),
+    ("mdrtDeviceFreeKept", None, [C.c_void_p]),
+    ("mdrtDeviceCopyWithin", None, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    ("mdrtDeviceCopyToHost", None, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    ("mdrtStagingCreate", C.c_void_p, []),
+    ("mdrtStagingCopy", C.c_void_p, [C.c_void_p, I32, C.c_void_p, C.c_uint64]),
+    ("mdrtStagingEnd", None, [C.c_void_p]),
+    ("mdrtStagingWait", I32, [C.c_void_p, C.POINTER(C.c_char_p)]),
+    ("mdrtStagingFail", None, [I32, C.c_char_p]),
+    ("mdrtStagingDestroy", None, [C.c_void_p]),
     ("_mlir_ciface_mdrtCudaFFTForward3D", None,
      [C.POINTER(Buffer1), C.POINTER(Buffer1), I64, I64, I64]),
     ("_mlir_ciface_mdrtCudaFFTBackward3D", None,
@@ -142,6 +158,14 @@ SIGNATURES = [
     ("mdrtHipRunNPT", C.c_int, [C.c_void_p, I64, D, I32, I32, D, D, C.c_uint64, PD]),
     ("mdrtHipBarostatStats", C.c_int, [C.c_void_p, PI64, PD]),
     ("mdrtHipGetStepCount", C.c_int, [C.c_void_p, PI64]),
+    ("mdrtHipSaveState", C.c_int, [C.c_void_p, PD, PD, PD, PD, PD, C.POINTER(RunState)]),
+    ("mdrtHipRestoreState", C.c_int, [C.c_void_p, PD, PD, PD, PD, C.POINTER(RunState)]),
+    ("mdrtHipTrajectoryOpen", C.c_void_p, [C.c_void_p, C.c_char_p, I64, I64, D, I64]),
+    ("mdrtHipTrajectoryFrame", C.c_int, [C.c_void_p, I64]),
+    ("mdrtHipTrajectoryFlush", C.c_int, [C.c_void_p]),
+    ("mdrtHipTrajectoryFrames", I64, [C.c_void_p]),
+    ("mdrtHipTrajectoryClose", C.c_int, [C.c_void_p]),
+    ("mdrtHipDcdWrite", C.c_int, [C.c_char_p, I64, I64, I64, I64, D, PD, PD]),
     ("mdrtHipRun", C.c_int, [C.c_void_p, I64, D, I32, PD]),
     ("mdrtHipPrepareRun", C.c_int, [C.c_void_p, I64, D, I32]),
     ("mdrtHipCheckList", C.c_int, [C.c_void_p, PD, PD, PI64]),

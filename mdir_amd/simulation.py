@@ -8,6 +8,7 @@ read the state and the energies.  The arithmetic runs in libmdrt_hip.so.
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -35,11 +36,35 @@ class Simulation:
         self._h = h
         self._arrays = []
         self.n = s.n
+        # run settings a checkpoint records
+        self.timestep = 0.002
+        self.temperature, self.friction, self.seed = 0.0, 0.0, 0
+        self.first_step, self.part = 0, 1
+        self._reporters: list = []
+        self._continued_trajectory = ("", 0)
+
+    @classmethod
+    def from_checkpoint(cls, system: System, checkpoint, skin: float = 0.1,
+                        prune_skin: float = 0.0) -> "Simulation":
+        """A simulation of `system` that continues the run of `checkpoint`
+        (a path or a Checkpoint): the reference's
+        mdir.Simulation(program, checkpoint=path).  A TrajectoryReporter on
+        the file the checkpoint names continues it after the frames the
+        checkpoint counts."""
+        from .output import read_checkpoint, restore
+        ck = read_checkpoint(checkpoint) if isinstance(checkpoint, (str, os.PathLike)) \
+            else checkpoint
+        sim = cls(system, skin, prune_skin)
+        restore(sim, ck)
+        return sim
 
     def close(self):
         if getattr(self, "_h", None):
-            rt.lib().mdrtHipSystemDestroy(self._h)
-            self._h = None
+            try:
+                self.close_reporters()
+            finally:
+                rt.lib().mdrtHipSystemDestroy(self._h)
+                self._h = None
 
     def __del__(self):
         try:
@@ -131,6 +156,8 @@ class Simulation:
         noise of each atom and step comes from Philox under `seed`."""
         rt.check(rt.lib().mdrtHipSetLangevin(self._h, float(temperature), float(friction),
                                              int(seed) & 0xFFFFFFFFFFFFFFFF))
+        self.temperature, self.friction = float(temperature), float(friction)
+        self.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
 
     @property
     def box(self) -> np.ndarray:
@@ -180,11 +207,73 @@ class Simulation:
 
     def run(self, steps: int, dt: float = 0.002, graph: int = 0) -> float:
         """`steps` steps (velocity Verlet, or Langevin after set_langevin);
-        returns the kinetic energy at the end."""
+        returns the kinetic energy at the end.  The reporters report at the
+        steps that are multiples of their periods."""
+        self.timestep = float(dt)
+        if not self._reporters:
+            return self._run(steps, dt, graph)
+        from .output import TrajectoryReporter
+        self._open_reporters()
+        step = self.step_count
+        end = step + int(steps)
+        # frames before checkpoints, which count them
+        order = sorted(self._reporters, key=lambda r: not isinstance(r, TrajectoryReporter))
+        ke = 0.0
+        while step < end:
+            nxt = min([end] + [(step // r.period + 1) * r.period for r in order])
+            ke = self._run(nxt - step, dt, graph)
+            step = nxt
+            for r in order:
+                if step % r.period == 0:
+                    r._report(self, step)
+        return ke
+
+    def _run(self, steps: int, dt: float, graph: int) -> float:
         ke = C.c_double(0.0)
         rt.check(rt.lib().mdrtHipRun(self._h, int(steps), float(dt), int(graph),
                                      C.byref(ke)))
         return ke.value
+
+    # ---- reporters and checkpoints ----
+    @property
+    def reporters(self) -> list:
+        return list(self._reporters)
+
+    @reporters.setter
+    def reporters(self, items):
+        from .output import CheckpointReporter, TrajectoryReporter
+        items = list(items)
+        kinds = [type(r) for r in items]
+        for r in items:
+            if not isinstance(r, (TrajectoryReporter, CheckpointReporter)):
+                raise TypeError(f"not a reporter: {r!r}")
+        if kinds.count(TrajectoryReporter) > 1 or kinds.count(CheckpointReporter) > 1:
+            raise ValueError("a simulation takes one reporter of each kind")
+        self.close_reporters()
+        self._reporters = items
+
+    def _open_reporters(self):
+        from .output import TrajectoryReporter
+        for r in self._reporters:
+            if isinstance(r, TrajectoryReporter) and r._h is None:
+                name, frames = self._continued_trajectory
+                keep = frames if name and os.path.basename(r.file) == name else -1
+                r._open(self, keep)
+
+    def close_reporters(self):
+        """Writes what the reporters hold and closes their files."""
+        for r in getattr(self, "_reporters", []):
+            r.close()
+
+    def state(self):
+        """The Checkpoint of the run now."""
+        from .output import state_of
+        return state_of(self)
+
+    def save_checkpoint(self, path: str):
+        """Writes the checkpoint of the run now (python/Bindings.cpp:1703)."""
+        from .output import state_of, write_checkpoint
+        write_checkpoint(str(path), state_of(self))
 
     def check_list(self):
         """The check of the list between rebuilds (the reference's
