@@ -175,8 +175,15 @@ __global__ void __launch_bounds__(256) nb_sort(
   for (int t = tid; t < count; t += blockDim.x) {
     double za = s_z[t];
     int ia = s_atom[t];
-    int rank = 0;
-    for (int u = 0; u < count; ++u) {
+    int rank = 0, u = 0;
+    // four independent LDS reads per pass, so that their latencies overlap
+    for (; u + 4 <= count; u += 4) {
+      const double z0 = s_z[u], z1 = s_z[u + 1], z2 = s_z[u + 2], z3 = s_z[u + 3];
+      const int i0 = s_atom[u], i1 = s_atom[u + 1], i2 = s_atom[u + 2], i3 = s_atom[u + 3];
+      rank += ((z0 < za) || (z0 == za && i0 < ia)) + ((z1 < za) || (z1 == za && i1 < ia)) +
+              ((z2 < za) || (z2 == za && i2 < ia)) + ((z3 < za) || (z3 == za && i3 < ia));
+    }
+    for (; u < count; ++u) {
       double zb = s_z[u];
       int ib = s_atom[u];
       rank += (zb < za) || (zb == za && ib < ia);
@@ -337,6 +344,12 @@ __global__ void __launch_bounds__(256) nb_search(
   __shared__ int s_cstart[64], s_cbase[64], s_ccode[64];
   __shared__ int s_k0[3][64], s_n[2][64];  // per z shift: first cluster, count
   __shared__ int s_total;                  // candidates of the column chunk
+  // the z bounds of the chunk's columns' clusters, staged so that the
+  // binary searches read LDS instead of chains of dependent global loads
+  constexpr int kZStage = 2048;
+  __shared__ float s_zlo[kZStage], s_zhi[kZStage];
+  __shared__ int s_zoff[64], s_zn[64], s_kmin[64];
+  __shared__ int s_ztotal;
   __shared__ int s_wcount[kWaves];         // entries per wave in a round
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int gi = blockIdx.x;
@@ -408,11 +421,11 @@ __global__ void __launch_bounds__(256) nb_search(
   int count = 0;  // entries of the row so far (the same in every thread)
   PairEntry *row = rows + (size_t)gi * g.row_cap;
   for (int cbase = 0; cbase < ncols; cbase += 64) {
-    // column slots of the chunk (wave 1; wave 0 with a single wave)
+    // column slots of the chunk (wave 1): the column within reach of the
+    // group's box, its periodic code and its clusters
     if (wave == 1) {
       const int c = cbase + lane;
-      int ncand = 0, cl0 = 0, code = 0;
-      int k0[3] = {0, 0, 0}, nz[3] = {0, 0, 0};
+      int cl0 = 0, code = 0, nreal = 0, kmin = 0;
       if (c < ncols) {
         const int dy = c / ncolx - g.ry, dx = c % ncolx - g.rx;
         int jy = cy + dy, sy = 0, jx = cx + dx, sx = 0;
@@ -427,32 +440,72 @@ __global__ void __launch_bounds__(256) nb_search(
         if (gx * gx + gy * gy < g.rlist2 && sx >= -1 && sx <= 1 && sy >= -1 && sy <= 1) {
           const int jcol = jy * g.ncx + jx;
           cl0 = col_cl_start[jcol];
-          const int nreal = col_cl_count[jcol];
+          nreal = col_cl_count[jcol];
           // a tile is ordered only when cj >= ci: clusters before the group
           // never pair with it
-          const int kmin = g0 - cl0 > 0 ? g0 - cl0 : 0;
+          kmin = g0 - cl0 > 0 ? g0 - cl0 : 0;
           code = (sx + 1) | ((sy + 1) << 2);
-#pragma unroll
-          for (int s = 0; s < 3; ++s) {
-            // the z window in j's own frame: bb_hi.z >= zlo and bb_lo.z <= zhi
-            const float oz = (float)(s - 1) * Lz;
-            const float zlo = glo.z - zreach - oz, zhi = ghi.z + zreach - oz;
-            int lo = kmin < nreal ? kmin : nreal, hi = nreal;
-            while (lo < hi) {
-              const int mid = (lo + hi) >> 1;
-              if (bb_hi[cl0 + mid].z < zlo) lo = mid + 1; else hi = mid;
-            }
-            const int first = lo;
-            hi = nreal;
-            while (lo < hi) {
-              const int mid = (lo + hi) >> 1;
-              if (bb_lo[cl0 + mid].z <= zhi) lo = mid + 1; else hi = mid;
-            }
-            k0[s] = first;
-            nz[s] = lo - first;
-          }
-          ncand = nz[0] + nz[1] + nz[2];
         }
+      }
+      int zscan = nreal;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int up = __shfl_up(zscan, off);
+        if (lane >= off) zscan += up;
+      }
+      s_cstart[lane] = cl0;
+      s_ccode[lane] = code;
+      s_zn[lane] = nreal;
+      s_zoff[lane] = zscan - nreal;
+      s_kmin[lane] = kmin;
+      if (lane == 63) s_ztotal = zscan;
+    }
+    __syncthreads();
+    const int ztotal = s_ztotal;
+    const bool staged = ztotal <= kZStage;
+    if (staged) {
+      for (int i = threadIdx.x; i < ztotal; i += blockDim.x) {
+        // the slot holding i: the last slot whose offset is <= i (slots
+        // without clusters share their offset with the next one)
+        int lo = 0, hi = 63;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (s_zoff[mid] <= i) lo = mid; else hi = mid - 1;
+        }
+        const int cl = s_cstart[lo] + (i - s_zoff[lo]);
+        s_zlo[i] = bb_lo[cl].z;
+        s_zhi[i] = bb_hi[cl].z;
+      }
+    }
+    __syncthreads();
+    if (wave == 1) {
+      int ncand = 0;
+      int k0[3] = {0, 0, 0}, nz[3] = {0, 0, 0};
+      const int nreal = s_zn[lane], kmin = s_kmin[lane];
+      if (nreal > 0) {
+        const int cl0 = s_cstart[lane], zo = s_zoff[lane];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          // the z window in j's own frame: bb_hi.z >= zlo and bb_lo.z <= zhi
+          const float oz = (float)(s - 1) * Lz;
+          const float zlo = glo.z - zreach - oz, zhi = ghi.z + zreach - oz;
+          int lo = kmin < nreal ? kmin : nreal, hi = nreal;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            const float zh = staged ? s_zhi[zo + mid] : bb_hi[cl0 + mid].z;
+            if (zh < zlo) lo = mid + 1; else hi = mid;
+          }
+          const int first = lo;
+          hi = nreal;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            const float zl = staged ? s_zlo[zo + mid] : bb_lo[cl0 + mid].z;
+            if (zl <= zhi) lo = mid + 1; else hi = mid;
+          }
+          k0[s] = first;
+          nz[s] = lo - first;
+        }
+        ncand = nz[0] + nz[1] + nz[2];
       }
       // inclusive wave scan of ncand
       int scan = ncand;
@@ -461,9 +514,7 @@ __global__ void __launch_bounds__(256) nb_search(
         const int up = __shfl_up(scan, off);
         if (lane >= off) scan += up;
       }
-      s_cstart[lane] = cl0;
       s_cbase[lane] = scan - ncand;  // exclusive
-      s_ccode[lane] = code;
 #pragma unroll
       for (int s = 0; s < 3; ++s) s_k0[s][lane] = k0[s];
       s_n[0][lane] = nz[0];
