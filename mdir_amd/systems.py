@@ -96,6 +96,35 @@ class System:
         default_factory=lambda: np.zeros(0, np.int32))
     water_geometry: tuple = (TIP3P["d_oh"], TIP3P["d_hh"])
     velocities: Optional[np.ndarray] = None
+    # bonded tuples (energies in kJ/mol, lengths nm, angles rad):
+    # bonds and Urey-Bradley 1-3 pairs (k/2)(r - r0)^2, angles (apex the
+    # middle atom) (k/2)(theta - theta0)^2, dihedrals k (1 + cos(n phi -
+    # phase)) (Amber's impropers among them), harmonic impropers
+    # (k/2) wrap(phi - phi0)^2
+    bonds: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros((0, 2), np.int32))
+    bond_k: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
+    bond_r0: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
+    urey_bradley: np.ndarray = dataclasses.field(
+        default_factory=lambda: np.zeros((0, 2), np.int32))
+    ub_k: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
+    ub_r0: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
+    angles: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros((0, 3), np.int32))
+    angle_k: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
+    angle_theta0: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
+    dihedrals: np.ndarray = dataclasses.field(
+        default_factory=lambda: np.zeros((0, 4), np.int32))
+    dihedral_k: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
+    dihedral_n: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0, np.int32))
+    dihedral_phase: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
+    impropers: np.ndarray = dataclasses.field(
+        default_factory=lambda: np.zeros((0, 4), np.int32))
+    improper_k: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
+    improper_phi0: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
+    # bond constraints (SHAKE clusters: one centre atom and the atoms bonded
+    # to it, contiguous in the atom order)
+    constraints: np.ndarray = dataclasses.field(
+        default_factory=lambda: np.zeros((0, 2), np.int32))
+    constraint_length: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
 
     @property
     def n(self) -> int:
@@ -106,8 +135,78 @@ class System:
         return int(self.sigma.shape[0])
 
     def degrees_of_freedom(self) -> int:
-        """3N - 3 - 3 per rigid water (System.cpp getDegreesOfFreedom)."""
-        return 3 * self.n - 3 - 3 * len(self.waters)
+        """3N - 3 - 3 per rigid water - 1 per bond constraint
+        (System.cpp getDegreesOfFreedom)."""
+        return 3 * self.n - 3 - 3 * len(self.waters) - len(self.constraints)
+
+    # tuple kinds, in the order the kernels sort them
+    TUPLE_BOND, TUPLE_UREY_BRADLEY, TUPLE_ANGLE, TUPLE_DIHEDRAL, TUPLE_IMPROPER = range(5)
+
+    def tuples(self):
+        """All bonded tuples as (atoms (T, 4) i32, kind (T,) i32, params
+        (T, 4) f64), sorted by kind: bond (k, r0), Urey-Bradley (k, r0),
+        angle (k, theta0), dihedral (k, n, phase), improper (k, phi0)."""
+        rows, kinds, params = [], [], []
+
+        def add(atoms, kind, cols):
+            atoms = np.asarray(atoms, np.int32).reshape(-1, atoms.shape[1] if np.ndim(atoms) == 2
+                                                        else 1)
+            m = atoms.shape[0]
+            if m == 0:
+                return
+            a = np.zeros((m, 4), np.int32)
+            a[:, :atoms.shape[1]] = atoms
+            pr = np.zeros((m, 4))
+            for c, v in enumerate(cols):
+                pr[:, c] = np.asarray(v, np.float64)
+            rows.append(a)
+            kinds.append(np.full(m, kind, np.int32))
+            params.append(pr)
+
+        add(self.bonds, self.TUPLE_BOND, (self.bond_k, self.bond_r0))
+        add(self.urey_bradley, self.TUPLE_UREY_BRADLEY, (self.ub_k, self.ub_r0))
+        add(self.angles, self.TUPLE_ANGLE, (self.angle_k, self.angle_theta0))
+        add(self.dihedrals, self.TUPLE_DIHEDRAL,
+            (self.dihedral_k, self.dihedral_n, self.dihedral_phase))
+        add(self.impropers, self.TUPLE_IMPROPER, (self.improper_k, self.improper_phi0))
+        if not rows:
+            return np.zeros((0, 4), np.int32), np.zeros(0, np.int32), np.zeros((0, 4))
+        return np.vstack(rows), np.concatenate(kinds), np.vstack(params)
+
+    def clusters(self):
+        """The constraint clusters: (first, size, centre offset) (m, 3) i32
+        and each atom's constraint length to its centre (n,; 0 for a centre
+        and for atoms in no cluster).  A cluster is an atom and the atoms
+        constrained to it, each of which has no other constraint; they must
+        be consecutive in the atom order (as in Amber topologies, where
+        hydrogens follow their heavy atom), at most 5."""
+        n = self.n
+        length = np.zeros(n)
+        if len(self.constraints) == 0:
+            return np.zeros((0, 3), np.int32), length
+        partners = {}
+        for (i, j), d in zip(np.asarray(self.constraints), np.asarray(self.constraint_length)):
+            partners.setdefault(int(i), []).append((int(j), float(d)))
+            partners.setdefault(int(j), []).append((int(i), float(d)))
+        centres = sorted(a for a, ps in partners.items() if len(ps) > 1 or
+                         (len(ps) == 1 and len(partners[ps[0][0]]) == 1 and a < ps[0][0]))
+        out, seen = [], set()
+        for c in centres:
+            if c in seen:
+                continue
+            members = [c] + [j for j, _ in partners[c]]
+            for j, d in partners[c]:
+                if len(partners[j]) != 1:
+                    raise ValueError(f"constraint of atoms {c} and {j}: clusters are an atom "
+                                     "and atoms bonded to it alone")
+                length[j] = d
+            lo, hi = min(members), max(members)
+            if hi - lo + 1 != len(members) or len(members) > 5:
+                raise ValueError(f"the constraint cluster of atom {c} is not {len(members)} "
+                                 "consecutive atoms (at most 5)")
+            seen.update(members)
+            out.append((lo, len(members), c - lo))
+        return np.array(out, np.int32).reshape(-1, 3), length
 
     def exclusion_csr(self):
         """Symmetric CSR of excluded partners per atom."""

@@ -474,6 +474,207 @@ void orc_pairs14(int64_t npairs, const int32_t *pairs, const double *sigma,
   *e_coul += ec;
 }
 
+static void cross3(const double *a, const double *b, double *c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+static double dot3(const double *a, const double *b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+/* ---- bonded tuples ------------------------------------------------------
+ * The energies of lib/Driver/TermBond.cpp:11-37 (bond), TermUreyBradley.cpp
+ * (the 1-3 distance), TermAngle.cpp:10-37, TermDihedral.cpp:14-58 and
+ * TermImproper.cpp:10-45, over the coordinates of
+ * lib/Dialect/MD/MDCoordinates.cpp:305-330 (distance; angle = acos of the
+ * clamped cosine of the arms from the apex; dihedral = atan2 of the
+ * Blondel-Karplus sine and cosine) and their gradients (:336-400). */
+static void bonded_sub(const double *a, const double *b, const double box[3], double *d) {
+  for (int k = 0; k < 3; ++k) d[k] = min_image(a[k] - b[k], box[k]);
+}
+
+void orc_bonded(int64_t ntuples, const int32_t *atoms, const int32_t *kind,
+                const double *param, const double *x, const double box[3], double *F,
+                double *energies) {
+  for (int64_t t = 0; t < ntuples; ++t) {
+    const int32_t *at = atoms + 4 * t;
+    const double *p = param + 4 * t;
+    const double *xa = x + 3 * at[0], *xb = x + 3 * at[1], *xc = x + 3 * at[2],
+                 *xd = x + 3 * at[3];
+    switch (kind[t]) {
+      case 0:
+      case 1: { /* distance of (a, b): dr/dx_a = (x_a - x_b) / r */
+        double d[3];
+        bonded_sub(xa, xb, box, d);
+        double r = sqrt(dot3(d, d)), dr = r - p[1];
+        energies[kind[t] == 0 ? ORC_E_BOND : ORC_E_UREY_BRADLEY] += 0.5 * p[0] * dr * dr;
+        double dudr = p[0] * dr;
+        for (int k = 0; k < 3; ++k) {
+          double g = dudr * d[k] / r;
+          F[3 * at[0] + k] -= g;
+          F[3 * at[1] + k] += g;
+        }
+        break;
+      }
+      case 2: { /* angle at b between u = a - b and v = c - b */
+        double u[3], v[3];
+        bonded_sub(xa, xb, box, u);
+        bonded_sub(xc, xb, box, v);
+        double iu = 1.0 / sqrt(dot3(u, u)), iv = 1.0 / sqrt(dot3(v, v));
+        double c = dot3(u, v) * iu * iv;
+        double cc = c > 1.0 ? 1.0 : (c < -1.0 ? -1.0 : c);
+        double theta = acos(cc), dt = theta - p[1];
+        energies[ORC_E_ANGLE] += 0.5 * p[0] * dt * dt;
+        double dudt = p[0] * dt;
+        double sine = sqrt(1.0 - c * c);
+        double slope = -1.0 / sine; /* dtheta/dc */
+        for (int k = 0; k < 3; ++k) {
+          double eu = u[k] * iu, ev = v[k] * iv;
+          double ga = slope * iu * (ev - c * eu);
+          double gc = slope * iv * (eu - c * ev);
+          F[3 * at[0] + k] -= dudt * ga;
+          F[3 * at[2] + k] -= dudt * gc;
+          F[3 * at[1] + k] += dudt * (ga + gc);
+        }
+        break;
+      }
+      case 3:
+      case 4: { /* f = a - b, g = b - c, h = d - c; p = f x g, q = h x g */
+        double f[3], g[3], h[3], pv[3], qv[3], qp[3];
+        bonded_sub(xa, xb, box, f);
+        bonded_sub(xb, xc, box, g);
+        bonded_sub(xd, xc, box, h);
+        cross3(f, g, pv);
+        cross3(h, g, qv);
+        cross3(qv, pv, qp);
+        double ng = sqrt(dot3(g, g)), pp = dot3(pv, pv), qq = dot3(qv, qv);
+        double phi = atan2(dot3(qp, g) / ng, dot3(pv, qv));
+        double dudp;
+        if (kind[t] == 3) {
+          double arg = p[1] * phi - p[2];
+          energies[ORC_E_DIHEDRAL] += p[0] * (1.0 + cos(arg));
+          dudp = -p[0] * p[1] * sin(arg);
+        } else {
+          double d = phi - p[1];
+          d -= 2.0 * M_PI * floor((d + M_PI) / (2.0 * M_PI));
+          energies[ORC_E_IMPROPER] += 0.5 * p[0] * d * d;
+          dudp = p[0] * d;
+        }
+        double fg = dot3(f, g), hg = dot3(h, g);
+        for (int k = 0; k < 3; ++k) {
+          double ga = -ng * pv[k] / pp;
+          double gd = ng * qv[k] / qq;
+          double gb = ng * pv[k] / pp + fg * pv[k] / (pp * ng) - hg * qv[k] / (qq * ng);
+          double gc = -ng * qv[k] / qq + hg * qv[k] / (qq * ng) - fg * pv[k] / (pp * ng);
+          F[3 * at[0] + k] -= dudp * ga;
+          F[3 * at[1] + k] -= dudp * gb;
+          F[3 * at[2] + k] -= dudp * gc;
+          F[3 * at[3] + k] -= dudp * gd;
+        }
+        break;
+      }
+      default:
+        break;
+    }
+  }
+}
+
+/* ---- constraint clusters: M-SHAKE --------------------------------------
+ * ConstraintIR.cpp:595-747 (Krautler, van Gunsteren and Hunenberger,
+ * J. Comput. Chem. 22, 501 (2001)): a cluster's bonds k, from its centre c
+ * to member h_k, of lengths d_k.  The members move along the old bonds
+ * s_l by +lambda_l s_l / m_h and the centre by -sum_l lambda_l s_l / m_c;
+ * each sweep solves the linearized equations
+ *   sum_l 2 (r_k . s_l) w_kl lambda_l = d_k^2 - r_k . r_k,
+ * w_kk = 1/m_c + 1/m_hk, w_kl = 1/m_c (k != l), by elimination; six sweeps,
+ * as the reference takes. */
+enum { ORC_CLUSTER_MAX = 5 };
+
+static void solve_small(int m, double A[ORC_CLUSTER_MAX][ORC_CLUSTER_MAX], double *b) {
+  for (int i = 0; i < m; ++i) {
+    for (int r = i + 1; r < m; ++r) {
+      double f = A[r][i] / A[i][i];
+      for (int c = i; c < m; ++c) A[r][c] -= f * A[i][c];
+      b[r] -= f * b[i];
+    }
+  }
+  for (int i = m - 1; i >= 0; --i) {
+    double sum = b[i];
+    for (int c = i + 1; c < m; ++c) sum -= A[i][c] * b[c];
+    b[i] = sum / A[i][i];
+  }
+}
+
+void orc_shake_positions(const orc_system *s, const double *x_old, double *x_new) {
+  for (int64_t u = 0; u < s->nclusters; ++u) {
+    const int32_t first = s->cluster_first[u], size = s->cluster_size[u];
+    const int32_t c = first + s->cluster_center[u];
+    int32_t h[ORC_CLUSTER_MAX];
+    int m = 0;
+    for (int32_t a = first; a < first + size; ++a)
+      if (a != c) h[m++] = a;
+    const double ic = 1.0 / s->mass[c];
+    double sv[ORC_CLUSTER_MAX][3], ih[ORC_CLUSTER_MAX], d2[ORC_CLUSTER_MAX];
+    for (int k = 0; k < m; ++k) {
+      ih[k] = 1.0 / s->mass[h[k]];
+      d2[k] = s->cluster_length[h[k]] * s->cluster_length[h[k]];
+      for (int j = 0; j < 3; ++j) sv[k][j] = x_old[3 * h[k] + j] - x_old[3 * c + j];
+    }
+    for (int sweep = 0; sweep < 6; ++sweep) {
+      double r[ORC_CLUSTER_MAX][3], A[ORC_CLUSTER_MAX][ORC_CLUSTER_MAX], lam[ORC_CLUSTER_MAX];
+      for (int k = 0; k < m; ++k) {
+        for (int j = 0; j < 3; ++j) r[k][j] = x_new[3 * h[k] + j] - x_new[3 * c + j];
+        lam[k] = d2[k] - dot3(r[k], r[k]);
+      }
+      for (int k = 0; k < m; ++k)
+        for (int l = 0; l < m; ++l) {
+          double w = (k == l) ? ic + ih[k] : ic;
+          A[k][l] = 2.0 * dot3(r[k], sv[l]) * w;
+        }
+      solve_small(m, A, lam);
+      for (int l = 0; l < m; ++l)
+        for (int j = 0; j < 3; ++j) {
+          x_new[3 * c + j] -= lam[l] * sv[l][j] * ic;
+          x_new[3 * h[l] + j] += lam[l] * sv[l][j] * ih[l];
+        }
+    }
+  }
+}
+
+/* The velocity constraint: impulses mu_l along the bonds r_l with
+   sum_l (r_k . r_l) w_kl mu_l = -r_k . (v_hk - v_c). */
+void orc_rattle_velocities(const orc_system *s, const double *x, double *v) {
+  for (int64_t u = 0; u < s->nclusters; ++u) {
+    const int32_t first = s->cluster_first[u], size = s->cluster_size[u];
+    const int32_t c = first + s->cluster_center[u];
+    int32_t h[ORC_CLUSTER_MAX];
+    int m = 0;
+    for (int32_t a = first; a < first + size; ++a)
+      if (a != c) h[m++] = a;
+    const double ic = 1.0 / s->mass[c];
+    double r[ORC_CLUSTER_MAX][3], ih[ORC_CLUSTER_MAX], A[ORC_CLUSTER_MAX][ORC_CLUSTER_MAX],
+        mu[ORC_CLUSTER_MAX];
+    for (int k = 0; k < m; ++k) {
+      ih[k] = 1.0 / s->mass[h[k]];
+      double dv[3];
+      for (int j = 0; j < 3; ++j) {
+        r[k][j] = x[3 * h[k] + j] - x[3 * c + j];
+        dv[j] = v[3 * h[k] + j] - v[3 * c + j];
+      }
+      mu[k] = -dot3(r[k], dv);
+    }
+    for (int k = 0; k < m; ++k)
+      for (int l = 0; l < m; ++l) A[k][l] = dot3(r[k], r[l]) * ((k == l) ? ic + ih[k] : ic);
+    solve_small(m, A, mu);
+    for (int l = 0; l < m; ++l)
+      for (int j = 0; j < 3; ++j) {
+        v[3 * c + j] -= mu[l] * r[l][j] * ic;
+        v[3 * h[l] + j] += mu[l] * r[l][j] * ih[l];
+      }
+  }
+}
+
 /* TermNonbonded.cpp:778-794: beta by bisection of erfc(beta rc) = tol.
    Pinned by test/Driver/pme.test:8 (3.47045919371208278 nm^-1 at 0.9 nm). */
 double orc_ewald_beta(double cutoff, double tolerance) {
@@ -1005,14 +1206,6 @@ double orc_ewald_reciprocal(int64_t n, const double *x, const double *q,
  * tests/test_oracle_constraints.py.
  * ====================================================================== */
 
-static void cross3(const double *a, const double *b, double *c) {
-  c[0] = a[1] * b[2] - a[2] * b[1];
-  c[1] = a[2] * b[0] - a[0] * b[2];
-  c[2] = a[0] * b[1] - a[1] * b[0];
-}
-static double dot3(const double *a, const double *b) {
-  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
-}
 static void unit3(double *a) {
   double s = 1.0 / sqrt(dot3(a, a));
   a[0] *= s;
@@ -1169,6 +1362,13 @@ int orc_total(const orc_system *s, const double *x, double *F,
     energies[ORC_E_DISP_CORR] += orc_dispersion_energy(
         n, s->type, s->ntypes, s->sigma, s->epsilon, s->nexcl_pairs,
         s->excl_pairs, s->box, s->cutoff);
+  if (s->npairs14)
+    orc_pairs14(s->npairs14, s->pairs14, s->p14_sigma, s->p14_eps, s->p14_qq, x, s->box,
+                s->cutoff, s->switch_from, s->lj_modifier, F, &energies[ORC_E_LJ14],
+                &energies[ORC_E_COUL14]);
+  if (s->ntuples)
+    orc_bonded(s->ntuples, s->tuple_atoms, s->tuple_kind, s->tuple_param, x, s->box, F,
+               energies);
   return 0;
 }
 
@@ -1192,11 +1392,13 @@ int orc_nve(const orc_system *s, double *x, double *v, double dt,
       for (int k = 0; k < 3; ++k) v[3 * i + k] += 0.5 * dt * F[3 * i + k] / s->mass[i];
     memcpy(xo, x, sizeof(double) * 3 * (size_t)n);
     for (int64_t i = 0; i < 3 * n; ++i) x[i] += dt * v[i];
-    if (s->nwaters) {
+    if (s->nwaters || s->nclusters) {
       double *xu = (double *)malloc(sizeof(double) * 3 * (size_t)n);
       memcpy(xu, x, sizeof(double) * 3 * (size_t)n);
-      orc_settle_positions(s->nwaters, s->water_first, s->m_o, s->m_h, s->d_oh,
-                           s->d_hh, xo, x);
+      if (s->nwaters)
+        orc_settle_positions(s->nwaters, s->water_first, s->m_o, s->m_h, s->d_oh,
+                             s->d_hh, xo, x);
+      if (s->nclusters) orc_shake_positions(s, xo, x);
       for (int64_t i = 0; i < 3 * n; ++i) v[i] += (x[i] - xu[i]) / dt;
       free(xu);
     }
@@ -1205,6 +1407,7 @@ int orc_nve(const orc_system *s, double *x, double *v, double dt,
       for (int k = 0; k < 3; ++k) v[3 * i + k] += 0.5 * dt * F[3 * i + k] / s->mass[i];
     if (s->nwaters)
       orc_settle_velocities(s->nwaters, s->water_first, s->m_o, s->m_h, x, v);
+    if (s->nclusters) orc_rattle_velocities(s, x, v);
     if (trace) {
       double pot = 0.0, kin = 0.0;
       for (int k = 0; k < ORC_NTERMS; ++k) pot += e[k];
@@ -1585,10 +1788,18 @@ static void half_drift(const orc_system *s, double *x, double *v, double h, doub
   int64_t n = s->n;
   memcpy(x0, x, sizeof(double) * 3 * (size_t)n);
   for (int64_t i = 0; i < 3 * n; ++i) x[i] += h * v[i];
-  if (!s->nwaters) return;
+  if (!s->nwaters && !s->nclusters) return;
   memcpy(xu, x, sizeof(double) * 3 * (size_t)n);
-  orc_settle_positions(s->nwaters, s->water_first, s->m_o, s->m_h, s->d_oh, s->d_hh, x0, x);
+  if (s->nwaters)
+    orc_settle_positions(s->nwaters, s->water_first, s->m_o, s->m_h, s->d_oh, s->d_hh, x0, x);
+  if (s->nclusters) orc_shake_positions(s, x0, x);
   for (int64_t i = 0; i < 3 * n; ++i) v[i] += (x[i] - xu[i]) / h;
+}
+
+/* The velocity constraints of waters and clusters. */
+static void project_velocities(const orc_system *s, const double *x, double *v) {
+  if (s->nwaters) orc_settle_velocities(s->nwaters, s->water_first, s->m_o, s->m_h, x, v);
+  if (s->nclusters) orc_rattle_velocities(s, x, v);
 }
 
 int orc_langevin(const orc_system *s, double *x, double *v, double dt, int64_t steps,
@@ -1604,7 +1815,7 @@ int orc_langevin(const orc_system *s, double *x, double *v, double dt, int64_t s
   for (int64_t step = 0; step < steps; ++step) {
     for (int64_t i = 0; i < n; ++i)
       for (int k = 0; k < 3; ++k) v[3 * i + k] += 0.5 * dt * F[3 * i + k] / s->mass[i];
-    if (s->nwaters) orc_settle_velocities(s->nwaters, s->water_first, s->m_o, s->m_h, x, v);
+    project_velocities(s, x, v);
     half_drift(s, x, v, 0.5 * dt, x0, xu);
     for (int64_t i = 0; i < n; ++i) {
       if (!(s->mass[i] > 0.0)) continue;
@@ -1613,12 +1824,12 @@ int orc_langevin(const orc_system *s, double *x, double *v, double dt, int64_t s
       double width = sigma / sqrt(s->mass[i]);
       for (int k = 0; k < 3; ++k) v[3 * i + k] = c * v[3 * i + k] + width * r[k];
     }
-    if (s->nwaters) orc_settle_velocities(s->nwaters, s->water_first, s->m_o, s->m_h, x, v);
+    project_velocities(s, x, v);
     half_drift(s, x, v, 0.5 * dt, x0, xu);
     orc_total(s, x, F, e, mixed);
     for (int64_t i = 0; i < n; ++i)
       for (int k = 0; k < 3; ++k) v[3 * i + k] += 0.5 * dt * F[3 * i + k] / s->mass[i];
-    if (s->nwaters) orc_settle_velocities(s->nwaters, s->water_first, s->m_o, s->m_h, x, v);
+    project_velocities(s, x, v);
     if (trace) {
       double pot = 0.0, kin = 0.0;
       for (int k = 0; k < ORC_NTERMS; ++k) pot += e[k];

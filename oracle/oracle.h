@@ -35,7 +35,12 @@ enum {
   ORC_E_DISP_CORR = 5,   /* dispersion correction                            */
   ORC_E_LJ14 = 6,        /* scaled 1-4 Lennard-Jones                         */
   ORC_E_COUL14 = 7,      /* scaled 1-4 Coulomb                               */
-  ORC_NTERMS = 8
+  ORC_E_BOND = 8,        /* (k/2)(r - r0)^2                                  */
+  ORC_E_UREY_BRADLEY = 9,/* (k/2)(r13 - r0)^2                                */
+  ORC_E_ANGLE = 10,      /* (k/2)(theta - theta0)^2                          */
+  ORC_E_DIHEDRAL = 11,   /* k (1 + cos(n phi - phase)), Amber impropers too  */
+  ORC_E_IMPROPER = 12,   /* (k/2) wrap(phi - phi0)^2                         */
+  ORC_NTERMS = 13
 };
 
 /* ---- neighbor matrix: lib/Runtime/Templates/NeighborsMatrix.mlir ---- */
@@ -150,7 +155,36 @@ typedef struct {
   int64_t nwaters;
   const int32_t *water_first;
   double m_o, m_h, d_oh, d_hh;
+  /* 1-4 pairs (orc_pairs14) */
+  int64_t npairs14;
+  const int32_t *pairs14;
+  const double *p14_sigma, *p14_eps, *p14_qq;
+  /* bonded tuples (orc_bonded): 4 atoms, a kind, 4 parameters each */
+  int64_t ntuples;
+  const int32_t *tuple_atoms, *tuple_kind;
+  const double *tuple_param;
+  /* constraint clusters (orc_shake_positions): a centre atom and the atoms
+     bonded to it at fixed lengths, atoms first .. first + size - 1 */
+  int64_t nclusters;
+  const int32_t *cluster_first, *cluster_size, *cluster_center;
+  const double *cluster_length; /* per atom: its length to the centre */
 } orc_system;
+
+/* Bonded tuples (lib/Driver/TermBond.cpp, TermUreyBradley.cpp,
+   TermAngle.cpp, TermDihedral.cpp, TermImproper.cpp; the coordinates and
+   their gradients of lib/Dialect/MD/MDCoordinates.cpp:279-400).  kind: 0
+   bond (i, j; k, r0), 1 Urey-Bradley (i, k; k, r0), 2 angle (i, j, k apex
+   j; k, theta0), 3 dihedral (i, j, k, l; k, n, phase), 4 harmonic improper
+   (i, j, k, l; k, phi0).  Energies are ADDED to e[ORC_E_BOND ..], forces to
+   F. */
+void orc_bonded(int64_t ntuples, const int32_t *atoms, const int32_t *kind,
+                const double *param, const double *x, const double box[3], double *F,
+                double *energies);
+/* M-SHAKE of the constraint clusters (ConstraintIR.cpp:595-747): six
+   Newton sweeps of the linearized bond equations, the members moving along
+   the old bonds; then the velocity constraint, one linear solve. */
+void orc_shake_positions(const orc_system *s, const double *x_old, double *x_new);
+void orc_rattle_velocities(const orc_system *s, const double *x, double *v);
 
 int orc_total(const orc_system *s, const double *x, double *F,
               double *energies, int32_t mixed);

@@ -18,10 +18,11 @@ REF_DRAWS = os.path.join(HERE, "_ref", "libref_draws.so")
 REF_HOST = os.path.join(HERE, "_ref", "libmdrt_host_ref.so")
 
 E_LJ, E_COUL_DIRECT, E_COUL_EXCL, E_COUL_RECIP, E_COUL_SELF, E_DISP_CORR, \
-    E_LJ14, E_COUL14 = range(8)
-NTERMS = 8
+    E_LJ14, E_COUL14, E_BOND, E_UREY_BRADLEY, E_ANGLE, E_DIHEDRAL, E_IMPROPER = range(13)
+NTERMS = 13
 TERM_NAMES = ["lj", "coul_direct", "coul_excl", "coul_recip", "coul_self",
-              "disp_corr", "lj14", "coul14"]
+              "disp_corr", "lj14", "coul14", "bonds", "urey_bradley", "angles",
+              "dihedrals", "impropers"]
 
 _lib = None
 
@@ -58,7 +59,12 @@ class OrcSystem(C.Structure):
                 ("coulomb", D), ("excl_start", PI32), ("excl_atoms", PI32),
                 ("nexcl_pairs", I64), ("excl_pairs", PI32), ("K", I64 * 3),
                 ("order", I32), ("nwaters", I64), ("water_first", PI32),
-                ("m_o", D), ("m_h", D), ("d_oh", D), ("d_hh", D)]
+                ("m_o", D), ("m_h", D), ("d_oh", D), ("d_hh", D),
+                ("npairs14", I64), ("pairs14", PI32), ("p14_sigma", PD), ("p14_eps", PD),
+                ("p14_qq", PD), ("ntuples", I64), ("tuple_atoms", PI32),
+                ("tuple_kind", PI32), ("tuple_param", PD), ("nclusters", I64),
+                ("cluster_first", PI32), ("cluster_size", PI32), ("cluster_center", PI32),
+                ("cluster_length", PD)]
 
 
 def _declare(L):
@@ -79,6 +85,12 @@ def _declare(L):
                               PD, PD]
     L.orc_ewald_beta.restype = D
     L.orc_ewald_beta.argtypes = [D, D]
+    L.orc_bonded.restype = None
+    L.orc_bonded.argtypes = [I64, PI32, PI32, PD, PD, PD, PD, PD]
+    L.orc_shake_positions.restype = None
+    L.orc_shake_positions.argtypes = [C.POINTER(OrcSystem), PD, PD]
+    L.orc_rattle_velocities.restype = None
+    L.orc_rattle_velocities.argtypes = [C.POINTER(OrcSystem), PD, PD]
     L.orc_smooth_size.restype = I64
     L.orc_smooth_size.argtypes = [I64]
     L.orc_self_energy.restype = D
@@ -212,6 +224,30 @@ class Oracle:
         else:
             st.m_o, st.m_h = TIP3P["m_o"], TIP3P["m_h"]
         st.d_oh, st.d_hh = s.water_geometry
+        # 1-4 pairs, bonded tuples and constraint clusters
+        self.p14 = i32(s.pairs14.reshape(-1, 2))
+        self.p14_s, self.p14_e, self.p14_q = (f64(s.pairs14_sigma), f64(s.pairs14_eps),
+                                              f64(s.pairs14_qq))
+        st.npairs14 = self.p14.shape[0]
+        st.pairs14 = _p(self.p14, I32)
+        st.p14_sigma, st.p14_eps, st.p14_qq = (_p(self.p14_s, D), _p(self.p14_e, D),
+                                               _p(self.p14_q, D))
+        ta, tk, tp = s.tuples()
+        self.t_atoms, self.t_kind, self.t_param = i32(ta), i32(tk), f64(tp)
+        st.ntuples = self.t_kind.shape[0]
+        st.tuple_atoms = _p(self.t_atoms, I32)
+        st.tuple_kind = _p(self.t_kind, I32)
+        st.tuple_param = _p(self.t_param, D)
+        cl, length = s.clusters()
+        self.cl_first = i32(cl[:, 0])
+        self.cl_size = i32(cl[:, 1])
+        self.cl_center = i32(cl[:, 2])
+        self.cl_length = f64(length)
+        st.nclusters = cl.shape[0]
+        st.cluster_first = _p(self.cl_first, I32)
+        st.cluster_size = _p(self.cl_size, I32)
+        st.cluster_center = _p(self.cl_center, I32)
+        st.cluster_length = _p(self.cl_length, D)
         self.st = st
 
     def total(self, x=None, mixed=False):
@@ -221,18 +257,28 @@ class Oracle:
         e = np.zeros(NTERMS)
         lib().orc_total(C.byref(self.st), _p(x, D), _p(F, D), _p(e, D),
                         int(mixed))
-        if self.s.pairs14.shape[0]:
-            el, ec = D(0.0), D(0.0)
-            lib().orc_pairs14(self.s.pairs14.shape[0], _p(i32(self.s.pairs14), I32),
-                              _p(f64(self.s.pairs14_sigma), D),
-                              _p(f64(self.s.pairs14_eps), D),
-                              _p(f64(self.s.pairs14_qq), D), _p(x, D),
-                              _p(f64(self.s.box), D), self.s.cutoff,
-                              self.s.switch_from, self.s.lj_modifier,
-                              _p(F, D), C.byref(el), C.byref(ec))
-            e[E_LJ14] += el.value
-            e[E_COUL14] += ec.value
         return F, e
+
+    def bonded(self, x=None):
+        """Forces (n, 3) and energy terms (NTERMS,) of the bonded tuples
+        alone."""
+        x = f64(self.s.positions if x is None else x)
+        F = np.zeros_like(x)
+        e = np.zeros(NTERMS)
+        lib().orc_bonded(self.t_kind.shape[0], _p(self.t_atoms, I32), _p(self.t_kind, I32),
+                         _p(self.t_param, D), _p(x, D), _p(f64(self.s.box), D), _p(F, D),
+                         _p(e, D))
+        return F, e
+
+    def shake_positions(self, x_old, x_new):
+        out = f64(x_new).copy()
+        lib().orc_shake_positions(C.byref(self.st), _p(f64(x_old), D), _p(out, D))
+        return out
+
+    def rattle_velocities(self, x, v):
+        out = f64(v).copy()
+        lib().orc_rattle_velocities(C.byref(self.st), _p(f64(x), D), _p(out, D))
+        return out
 
     def nonbonded(self, x=None, virial=False):
         s = self.s
