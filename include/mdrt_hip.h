@@ -220,7 +220,21 @@ enum {
   MDRT_E_LJ14 = 6,
   MDRT_E_COUL14 = 7,
   MDRT_E_KINETIC = 8,
-  MDRT_NUM_ENERGIES = 9
+  MDRT_E_BONDS = 9,           /* TermBond.cpp          (k/2)(r - r0)^2          */
+  MDRT_E_UREY_BRADLEY = 10,   /* TermUreyBradley.cpp   (k/2)(r13 - r0)^2        */
+  MDRT_E_ANGLES = 11,         /* TermAngle.cpp         (k/2)(theta - theta0)^2  */
+  MDRT_E_DIHEDRALS = 12,      /* TermDihedral.cpp      k (1 + cos(n phi - d))   */
+  MDRT_E_IMPROPERS = 13,      /* TermImproper.cpp      (k/2) wrap(phi - phi0)^2 */
+  MDRT_NUM_ENERGIES = 14
+};
+
+/* Kinds of bonded tuples (the atoms they take; their four parameters). */
+enum {
+  MDRT_TUPLE_BOND = 0,          /* i, j;       k, r0          */
+  MDRT_TUPLE_UREY_BRADLEY = 1,  /* i, k;       k, r0          */
+  MDRT_TUPLE_ANGLE = 2,         /* i, j, k (apex j); k, theta0 */
+  MDRT_TUPLE_DIHEDRAL = 3,      /* i, j, k, l; k, n, phase    */
+  MDRT_TUPLE_IMPROPER = 4       /* i, j, k, l; k, phi0        */
 };
 
 /* A system: the topology and settings of lib/Driver (Topology, Control).
@@ -257,6 +271,20 @@ typedef struct {
   double skin;                /* list buffer beyond rc (nm)             */
   double prune_skin;          /* buffer of the pruned list the pair loop
                                  runs over (nm, < skin); 0: no pruning   */
+  /* bonded tuples (lib/Driver/TermBond.cpp, TermAngle.cpp, ...): atoms 4
+     each (unused 0), a kind MDRT_TUPLE_*, parameters 4 each */
+  int64_t ntuples;
+  const int32_t *tuple_atoms;
+  const int32_t *tuple_kind;
+  const double *tuple_param;
+  /* bond constraints in clusters (ConstraintIR.cpp:595-747, M-SHAKE): the
+     atoms first .. first + size - 1 (size <= 5), the centre at
+     first + centre, the others bonded to it at cluster_length[atom] */
+  int64_t nclusters;
+  const int32_t *cluster_first;
+  const int32_t *cluster_size;
+  const int32_t *cluster_center;
+  const double *cluster_length; /* n (0 for atoms in no cluster)         */
 } mdrtHipSystemDesc;
 
 /* Creates a system on the current device; NULL on error (see
@@ -278,7 +306,8 @@ int mdrtHipComputeForces(void *system, double *forces, double *energies);
 enum {
   MDRT_TERM_PAIRS = 1,    /* nonbonded pairs within rc: LJ + direct Coulomb */
   MDRT_TERM_SPECIAL = 2,  /* excluded-pair Ewald correction and 1-4 pairs */
-  MDRT_TERM_RECIP = 4     /* PME reciprocal sum */
+  MDRT_TERM_RECIP = 4,    /* PME reciprocal sum */
+  MDRT_TERM_BONDED = 8    /* bonded tuples: bonds, angles, dihedrals, ... */
 };
 
 /* As mdrtHipComputeForces, for the terms of the mask `terms` only (the

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <chrono>
 #include <mutex>
+#include <unordered_map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -30,6 +31,7 @@
 #include "../../include/mdrt_hip.h"
 #include "integrate.hip"
 #include "nbnxm.hip"
+#include "bonded.hip"
 #include "pairs.hip"
 #include "pme.hip"
 #include "pme_mfma.hip"
@@ -161,6 +163,14 @@ struct Fft3D {
 
 static inline int blocks_for(size_t n, int b = 256) { return (int)((n + b - 1) / b); }
 
+// The potential energy: every term but the kinetic energy.
+inline double potential_of(const double *E) {
+  double u = 0.0;
+  for (int k = 0; k < MDRT_NUM_ENERGIES; ++k)
+    if (k != MDRT_E_KINETIC) u += E[k];
+  return u;
+}
+
 struct Engine {
   // ---- topology ----
   int n = 0, ntypes = 0, nunits = 0;
@@ -240,6 +250,22 @@ struct Engine {
   NbParams nbp{};
   float2 *d_lj = nullptr;
   double *d_epair = nullptr;
+
+  // ---- bonded tuples (sorted by kind) ----
+  int ntuples = 0;
+  int4 *d_tuple_atoms = nullptr;
+  int *d_tuple_kind = nullptr;
+  double4 *d_tuple_param = nullptr;
+  double *d_etup = nullptr;                // kTupleKinds per tuple block
+  static constexpr int kTupleBlocksMax = 64;
+  int tuple_blocks() const {
+    return ntuples && (terms & 8) ? std::min(blocks_for(ntuples), kTupleBlocksMax) : 0;
+  }
+
+  // ---- constraint clusters ----
+  int nclusters = 0;
+  double *d_clen = nullptr;                // each member's length to its centre
+  int *d_cl_first = nullptr, *d_cl_code = nullptr;  // cluster units
 
   // ---- special pairs ----
   int nspecial = 0;
@@ -325,6 +351,9 @@ struct Engine {
   void scale_box(const double *s) {
     scale_units<<<blocks_for(nunits, 64), 64, 0, stream>>>(nunits, d_unit_first, d_unit_size,
                                                            d_inv_mass, d_x, s[0], s[1], s[2]);
+    if (nclusters)
+      scale_units<<<blocks_for(nclusters, 64), 64, 0, stream>>>(
+          nclusters, d_cl_first, d_cl_code, d_inv_mass, d_x, s[0], s[1], s[2]);
     HIPCHK(hipGetLastError());
     const double nb[3] = {box[0] * s[0], box[1] * s[1], box[2] * s[2]};
     set_box(nb);
@@ -333,9 +362,7 @@ struct Engine {
   double potential_energy() {
     double E[MDRT_NUM_ENERGIES];
     compute(nullptr, E);
-    double u = 0.0;
-    for (int k = 0; k < MDRT_E_KINETIC; ++k) u += E[k];
-    return u;
+    return potential_of(E);
   }
 
   // One volume move; info (or null): U0, U1, V1 - V0, W (kJ/mol, nm^3).
@@ -362,7 +389,7 @@ struct Engine {
     const double s3[3] = {sc, sc, sc};
     scale_box(s3);
     const double u1 = potential_energy();
-    const double work = (u1 - u0) + P * dv - (double)nunits * kT * log(v1 / v0);
+    const double work = (u1 - u0) + P * dv - (double)(nunits + nclusters) * kT * log(v1 / v0);
     const bool keep = work <= 0.0 || uniform(w[2], w[3]) < exp(-work / kT);
     if (!keep) {
       on_device(d_x, d_save_x, 3 * (size_t)n * sizeof(double));
@@ -504,7 +531,7 @@ struct Engine {
   // Which terms a force evaluation computes (bits of MDRT_TERM_*): the
   // nonbonded pairs (LJ + direct Coulomb), the excluded and 1-4 pairs, the
   // PME reciprocal sum.  Steps always take all of them.
-  int terms = 7;
+  int terms = 15;
 
   // ---- frame batches ----
   // The frame-path launches run nz frames (blockIdx.z); the per-frame
@@ -541,6 +568,8 @@ struct Engine {
   // starting as a copy of the single system's buffers.
   void batch(int frames) {
     if (nz != 1 || d_arena) throw HipError("already batched");
+    if (ntuples || nclusters)
+      throw HipError("frame batches take systems without bonded tuples or constraint clusters");
     if (frames < 1) throw HipError("a batch needs at least one frame");
     if (use_pme && !plane_fft())
       throw HipError("frame batches need a PME grid whose xy plane the plane transform holds");
@@ -645,6 +674,8 @@ struct Engine {
     dfree(d_xprune); dfree(d_xpa); dfree(d_Fr); dfree(d_pme_n);
     release_list();
     dfree(d_excl_start); dfree(d_excl_atoms); dfree(d_flags); dfree(d_lj);
+    dfree(d_tuple_atoms); dfree(d_tuple_kind); dfree(d_tuple_param); dfree(d_etup);
+    dfree(d_clen); dfree(d_cl_first); dfree(d_cl_code);
     dfree(d_special); dfree(d_is14); dfree(d_isexcl); dfree(d_p14_sigma);
     dfree(d_p14_eps); dfree(d_p14_qq); dfree(d_espec);
     fft.release();
@@ -883,18 +914,19 @@ struct Engine {
         is14.push_back(-1);
         isex.push_back(1);
       }
+      std::unordered_map<long long, size_t> where;
+      where.reserve(sp.size() * 2 + 1);
+      for (size_t e = 0; e < sp.size(); ++e) {
+        const int a = std::min(sp[e].x, sp[e].y), b = std::max(sp[e].x, sp[e].y);
+        where.emplace(((long long)a << 32) | (unsigned)b, e);
+      }
       for (int64_t p = 0; p < d->npairs14; ++p) {
         int i = d->pairs14[2 * p], j = d->pairs14[2 * p + 1];
         if (i > j) std::swap(i, j);
-        bool found = false;
-        for (size_t e = 0; e < sp.size(); ++e)
-          if (sp[e].x == i && sp[e].y == j && isex[e]) {
-            is14[e] = (int)p;
-            found = true;
-            break;
-          }
-        if (!found)
+        const auto it = where.find(((long long)i << 32) | (unsigned)j);
+        if (it == where.end())
           throw HipError("a 1-4 pair must also be an excluded pair");
+        is14[it->second] = (int)p;
       }
       nspecial = (int)sp.size();
       d_special = dupload(sp);
@@ -916,6 +948,43 @@ struct Engine {
       spp.rs = switch_from;
       spp.lj_modifier = lj_modifier;
       d_espec = dalloc<double>(3 * (size_t)std::max(1, blocks_for(nspecial)));
+    }
+
+    // bonded tuples, sorted by kind so that a wave mostly runs one kind
+    if (d->ntuples > 0) {
+      if (!d->tuple_atoms || !d->tuple_kind || !d->tuple_param)
+        throw HipError("tuples without their atoms, kinds or parameters");
+      std::vector<int> order((size_t)d->ntuples);
+      for (int64_t t = 0; t < d->ntuples; ++t) {
+        const int k = d->tuple_kind[t];
+        if (k < 0 || k >= kTupleKinds) throw HipError("a tuple kind outside MDRT_TUPLE_*");
+        const int members = k <= kTupleUreyBradley ? 2 : (k == kTupleAngle ? 3 : 4);
+        for (int m = 0; m < members; ++m) {
+          const int a = d->tuple_atoms[4 * t + m];
+          if (a < 0 || a >= n) throw HipError("a tuple atom out of range");
+        }
+        order[t] = (int)t;
+      }
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return d->tuple_kind[a] < d->tuple_kind[b];
+      });
+      std::vector<int4> ta;
+      std::vector<int> tk;
+      std::vector<double4> tp;
+      for (int t : order) {
+        const int32_t *a = d->tuple_atoms + 4 * (size_t)t;
+        const double *q = d->tuple_param + 4 * (size_t)t;
+        const int k = d->tuple_kind[t];
+        const int members = k <= kTupleUreyBradley ? 2 : (k == kTupleAngle ? 3 : 4);
+        ta.push_back(make_int4(a[0], a[1], members > 2 ? a[2] : 0, members > 3 ? a[3] : 0));
+        tk.push_back(k);
+        tp.push_back(make_double4(q[0], q[1], q[2], q[3]));
+      }
+      ntuples = (int)d->ntuples;
+      d_tuple_atoms = dupload(ta);
+      d_tuple_kind = dupload(tk);
+      d_tuple_param = dupload(tp);
+      d_etup = dalloc<double>((size_t)kTupleKinds * kTupleBlocksMax);
     }
 
     host_constants();
@@ -955,10 +1024,41 @@ struct Engine {
       for (int64_t w = 0; w < d->nwaters; ++w) {
         const int o = d->water_first[w];
         if (o < 0 || o + 2 >= n) throw HipError("bad water index");
+        if (in_water[o] || in_water[o + 1] || in_water[o + 2])
+          throw HipError("two waters share an atom");
         uf.push_back(o);
         us.push_back(3);
         in_water[o] = in_water[o + 1] = in_water[o + 2] = 1;
       }
+      // constraint clusters: the atoms first .. first + size - 1
+      std::vector<double> clen(n, 0.0);
+      std::vector<int> cf, cc;
+      for (int64_t c = 0; c < d->nclusters; ++c) {
+        const int first = d->cluster_first[c], size = d->cluster_size[c];
+        const int centre = d->cluster_center[c];
+        if (size < 2 || size > 5) throw HipError("a constraint cluster of other than 2-5 atoms");
+        if (first < 0 || first + size > n || centre < 0 || centre >= size)
+          throw HipError("a constraint cluster out of range");
+        for (int a = first; a < first + size; ++a) {
+          if (in_water[a]) throw HipError("a constraint cluster overlaps a water or a cluster");
+          in_water[a] = 1;
+          if (a != first + centre) {
+            clen[a] = d->cluster_length ? d->cluster_length[a] : 0.0;
+            if (!(clen[a] > 0.0)) throw HipError("a constraint length must be positive");
+          }
+          if (!(h_mass[a] > 0.0)) throw HipError("a constrained atom needs a mass");
+        }
+        cf.push_back(first);
+        cc.push_back(size | kUnitCluster | (centre << kUnitCentreShift));
+      }
+      nclusters = (int)d->nclusters;
+      if (nclusters) {
+        d_clen = dupload(clen);
+        d_cl_first = dupload(cf);
+        d_cl_code = dupload(cc);
+      }
+      ip.clen = d_clen;
+      ip.inv_mass = d_inv_mass;
       for (int i = 0; i < n; ++i)
         if (!in_water[i]) {
           uf.push_back(i);
@@ -1135,6 +1235,15 @@ struct Engine {
     sa.place_of = d_place_of;
     sa.energy_partial = ENERGY ? d_espec : nullptr;
     sa.p = spp;
+    if (const int tb = tuple_blocks()) {
+      TupleArgs ta{};
+      ta.ntuples = ntuples;
+      ta.atoms = d_tuple_atoms;
+      ta.kind = d_tuple_kind;
+      ta.param = d_tuple_param;
+      ta.energy_partial = ENERGY ? d_etup : nullptr;
+      bonded_tuples<<<tb, 256, 0, lstream>>>(ta, d_x, d_place_of, box[0], box[1], box[2], d_F);
+    }
     const dim3 grid(std::max(blocks_for(max_groups, 4), blocks_for(nspecial)),
                     (terms & 1) ? nseg + 1 : 1, nz);
     const bool pruned = prune_skin > 0;
@@ -1463,7 +1572,13 @@ struct Engine {
     } else {
       ip.step = nullptr;
     }
-    integrate_part<PART><<<blocks_for(nunits, 64), 64, 0, stream>>>(
+    if (nclusters)
+      integrate_clusters<PART><<<blocks_for(nclusters, 64), 64, 0, stream>>>(
+          nclusters, d_cl_first, d_cl_code, d_x, d_v, d_inv_mass, d_F, d_Fr, d_place_of,
+          d_shift, d_xref, d_q, d_xq, use_origins() ? d_gorig : nullptr,
+          d_xpa ? nullptr : d_xpq, d_xpa, ip, d_flags);
+    // at least one block: its last block advances the Langevin step number
+    integrate_part<PART><<<std::max(blocks_for(nunits, 64), 1), 64, 0, stream>>>(
         nunits, d_unit_first, d_unit_size, d_x, d_v, d_inv_mass, d_F, d_Fr, d_place_of,
         d_shift, d_xref, d_q, d_xq, use_origins() ? d_gorig : nullptr,
         d_xpa ? nullptr : d_xpq, d_xpa, ip, d_flags);
@@ -1576,15 +1691,15 @@ struct Engine {
 
   // Forces and energies of the terms `mask` names (the others are 0).
   void compute_terms(int mask, double *F_out, double *E_out) {
-    if (mask < 1 || mask > 7) throw HipError("terms: a nonzero mask of MDRT_TERM_* bits");
+    if (mask < 1 || mask > 15) throw HipError("terms: a nonzero mask of MDRT_TERM_* bits");
     terms = mask;
     try {
       compute(F_out, E_out);
     } catch (...) {
-      terms = 7;
+      terms = 15;
       throw;
     }
-    terms = 7;
+    terms = 15;
     forces_valid = false;
   }
 
@@ -1621,6 +1736,14 @@ struct Engine {
           E_out[MDRT_E_LJ14] += es[3 * b + 1];
           E_out[MDRT_E_COUL14] += es[3 * b + 2];
         }
+      }
+      if (const int tb = tuple_blocks()) {
+        std::vector<double> et((size_t)kTupleKinds * tb);
+        to_host(et.data(), d_etup, et.size() * sizeof(double));
+        const int slot[kTupleKinds] = {MDRT_E_BONDS, MDRT_E_UREY_BRADLEY, MDRT_E_ANGLES,
+                                       MDRT_E_DIHEDRALS, MDRT_E_IMPROPERS};
+        for (int b = 0; b < tb; ++b)
+          for (int k = 0; k < kTupleKinds; ++k) E_out[slot[k]] += et[(size_t)kTupleKinds * b + k];
       }
       if (use_pme && (terms & 4)) {
         std::vector<double> er(recip_blocks());
@@ -1929,11 +2052,7 @@ struct Engine {
     double E[MDRT_NUM_ENERGIES];
     std::vector<double> F(3 * (size_t)n);
     compute(F.data(), E);
-    auto potential = [](const double *e) {
-      double s = 0;
-      for (int k = 0; k < MDRT_E_KINETIC; ++k) s += e[k];
-      return s;
-    };
+    auto potential = [](const double *e) { return potential_of(e); };
     double e0 = potential(E);
     double h = max_step;
     double *d_x0 = dalloc<double>(3 * (size_t)n);
@@ -1945,6 +2064,10 @@ struct Engine {
       // the force words of the current positions are still in d_F
       descend<<<blocks_for(nunits), 256, 0, stream>>>(nunits, d_unit_first, d_unit_size, d_x0,
                                                       d_x, d_F, d_place_of, h / fmax, ip);
+      if (nclusters)
+        descend<<<blocks_for(nclusters), 256, 0, stream>>>(nclusters, d_cl_first, d_cl_code,
+                                                           d_x0, d_x, d_F, d_place_of, h / fmax,
+                                                           ip);
       std::vector<double> F1(3 * (size_t)n);
       double E1[MDRT_NUM_ENERGIES];
       double e1;

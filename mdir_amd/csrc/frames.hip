@@ -84,6 +84,8 @@ struct FrameBatch {
 
   void create(const mdrtHipSystemDesc *d, int frames) {
     if (frames < 1) throw HipError("max_frames must be at least 1");
+    if (d->ntuples > 0 || d->nclusters > 0)
+      throw HipError("frame batches take systems without bonded tuples or constraint clusters");
     mdrtHipSystemDesc dd = *d;
     dd.skin = 0.02;       // the list is built at every evaluation
     dd.prune_skin = 0.0;

@@ -7,7 +7,9 @@
 //   the validity test |x - x_ref|^2 > (skin/2)^2 (ExposeValidity.cpp:23);
 //   then after the forces: half kick and the velocity constraint
 //   (ConstraintIR.cpp:392-471).
-// A unit is a SETTLE water (3 consecutive atoms) or a free atom; one thread
+// A unit is a SETTLE water (3 consecutive atoms), a constraint cluster (an
+// atom and up to four atoms bonded to it at fixed lengths, consecutive,
+// M-SHAKE as ConstraintIR.cpp:595-747 takes it) or a free atom; one thread
 // owns a unit, reads its force words and clears them for the next step.
 // Blocks of 64 threads: a few thousand units still spread over every CU.
 #include "md_common.h"
@@ -38,7 +40,17 @@ struct IntParams {
   double decay, spread;
   unsigned long long seed;
   long long *step;
+  // constraint clusters: each member's bond length to its centre, by atom,
+  // and the inverse masses (the descent step of a minimisation)
+  const double *clen;
+  const double *inv_mass;
 };
+
+// A unit's code (unit_size[u]): its atom count in bits 0-7, kUnitCluster
+// for a constraint cluster, the cluster centre's offset from the unit's
+// first atom in bits 12-15.
+constexpr int kUnitCluster = 0x100;
+constexpr int kUnitCentreShift = 12;
 
 __device__ __forceinline__ void cross(const double *a, const double *b, double *c) {
   c[0] = a[1] * b[2] - a[2] * b[1];
@@ -154,32 +166,130 @@ __device__ void settle_velocities(const double *xa, const double *xb, const doub
   }
 }
 
-// The state of one unit of S atoms (a SETTLE water, S = 3, or a free
-// atom, S = 1) in registers: S is a compile-time constant so that the
-// arrays stay in registers.
+// The state of one unit of S atoms (a SETTLE water, S = 3, a constraint
+// cluster, S = 2..5, with its centre at index 0, or a free atom, S = 1) in
+// registers: S is a compile-time constant so that the arrays stay in
+// registers.  at[m] is the atom of index m.
 template <int S>
 struct UnitState {
-  int first;
+  int at[S];
   int pl[S];
   double im[S], x[S][3], v[S][3], f[S][3];
+  double len[S];  // clusters: index m's bond length to the centre
 };
+
+// Linear equations A y = b of at most 4 unknowns by elimination (the
+// cluster equations are diagonally dominant: no pivoting).
+template <int M>
+__device__ __forceinline__ void solve_small(double (&A)[M][M], double (&b)[M]) {
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+#pragma unroll
+    for (int r = i + 1; r < M; ++r) {
+      const double f = A[r][i] / A[i][i];
+#pragma unroll
+      for (int c = i; c < M; ++c) A[r][c] -= f * A[i][c];
+      b[r] -= f * b[i];
+    }
+#pragma unroll
+  for (int i = M - 1; i >= 0; --i) {
+    double sum = b[i];
+#pragma unroll
+    for (int c = i + 1; c < M; ++c) sum -= A[i][c] * b[c];
+    b[i] = sum / A[i][i];
+  }
+}
+
+// M-SHAKE of a cluster (centre at index 0): six Newton sweeps of the
+// linearized bond equations sum_l 2 (r_k . s_l) w_kl lambda_l = d_k^2 -
+// r_k . r_k, the members moving by +lambda_l s_l / m_l along the old bonds
+// s_l and the centre by -sum_l lambda_l s_l / m_0 (w_kk = 1/m_0 + 1/m_k,
+// w_kl = 1/m_0), the sweeps of the reference (ConstraintIR.cpp:595-747).
+template <int S>
+__device__ __forceinline__ void mshake_positions(const double (&x0)[S][3], double (&xn)[S][3],
+                                                 const double (&im)[S],
+                                                 const double (&len)[S]) {
+  constexpr int M = S - 1;
+  double sv[M][3], d2[M];
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    d2[k] = len[k + 1] * len[k + 1];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) sv[k][j] = x0[k + 1][j] - x0[0][j];
+  }
+  for (int sweep = 0; sweep < 6; ++sweep) {
+    double r[M][3], A[M][M], lam[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) r[k][j] = xn[k + 1][j] - xn[0][j];
+      lam[k] = d2[k] - dot(r[k], r[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+#pragma unroll
+      for (int l = 0; l < M; ++l) A[k][l] = 2.0 * dot(r[k], sv[l]) * ((k == l) ? im[0] + im[k + 1] : im[0]);
+    solve_small<M>(A, lam);
+#pragma unroll
+    for (int l = 0; l < M; ++l)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        xn[0][j] -= lam[l] * sv[l][j] * im[0];
+        xn[l + 1][j] += lam[l] * sv[l][j] * im[l + 1];
+      }
+  }
+}
+
+// The velocity constraint of a cluster: impulses mu_l along the bonds r_l
+// with sum_l (r_k . r_l) w_kl mu_l = -r_k . (v_k - v_0), one solve.
+template <int S>
+__device__ __forceinline__ void rattle_velocities(const double (&x)[S][3], double (&v)[S][3],
+                                                  const double (&im)[S]) {
+  constexpr int M = S - 1;
+  double r[M][3], A[M][M], mu[M];
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    double dv[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      r[k][j] = x[k + 1][j] - x[0][j];
+      dv[j] = v[k + 1][j] - v[0][j];
+    }
+    mu[k] = -dot(r[k], dv);
+  }
+#pragma unroll
+  for (int k = 0; k < M; ++k)
+#pragma unroll
+    for (int l = 0; l < M; ++l) A[k][l] = dot(r[k], r[l]) * ((k == l) ? im[0] + im[k + 1] : im[0]);
+  solve_small<M>(A, mu);
+#pragma unroll
+  for (int l = 0; l < M; ++l)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      v[0][j] -= mu[l] * r[l][j] * im[0];
+      v[l + 1][j] += mu[l] * r[l][j] * im[l + 1];
+    }
+}
 
 // Loads the unit at `first`; with clear, the force words are zeroed for the
 // next evaluation once read.  Fr (or null): reciprocal force words by atom,
 // added to the place-ordered words as integers before the conversion, so
 // the force is the one a single word would hold.
-template <int S>
-__device__ __forceinline__ void load_unit(int first, const double *__restrict__ x,
+template <int S, bool CL>
+__device__ __forceinline__ void load_unit(int first, int code, const double *__restrict__ x,
                                           const double *__restrict__ v,
                                           const double *__restrict__ inv_mass,
                                           long long *__restrict__ F,
                                           long long *__restrict__ Fr,
                                           const int *__restrict__ place_of, bool clear,
-                                          UnitState<S> &s) {
-  s.first = first;
+                                          const IntParams &p, UnitState<S> &s) {
+  const int centre = CL ? (code >> kUnitCentreShift) & 15 : 0;
 #pragma unroll
   for (int m = 0; m < S; ++m) {
-    const int at = first + m;
+    // clusters: the centre first, then the others in their order
+    const int at = CL ? first + (m == 0 ? centre : (m - 1 < centre ? m - 1 : m)) : first + m;
+    s.at[m] = at;
+    s.len[m] = CL && m > 0 ? p.clen[at] : 0.0;
     s.pl[m] = place_of[at];
     s.im[m] = inv_mass[at];
 #pragma unroll
@@ -205,30 +315,34 @@ __device__ __forceinline__ void half_kick(UnitState<S> &s, double dt) {
     for (int k = 0; k < 3; ++k) s.v[m][k] = s.v[m][k] + 0.5 * dt * s.f[m][k] * s.im[m];
 }
 
-template <int S>
-__device__ __forceinline__ void constrain_velocities(UnitState<S> &s, const IntParams &p) {
-  if constexpr (S == 3) settle_velocities(s.x[0], s.x[1], s.x[2], s.v[0], s.v[1], s.v[2], p);
+template <int S, bool CL>
+__device__ __forceinline__ void constrain_velocities(UnitState<S> &s, const double (&x)[S][3],
+                                                     const IntParams &p) {
+  if constexpr (CL) rattle_velocities<S>(x, s.v, s.im);
+  else if constexpr (S == 3) settle_velocities(x[0], x[1], x[2], s.v[0], s.v[1], s.v[2], p);
 }
 
 // A drift of length h from positions x0 (constrained): xn = x0 + h v,
-// SETTLE from x0, and v += (xn_constrained - xn) / h.
-template <int S>
-__device__ __forceinline__ void drift_settle(const double (&x0)[S][3], double (&v)[S][3],
-                                             double (&xn)[S][3], double h, const IntParams &p) {
+// SETTLE or M-SHAKE from x0, and v += (xn_constrained - xn) / h.
+template <int S, bool CL>
+__device__ __forceinline__ void drift_settle(const UnitState<S> &s, const double (&x0)[S][3],
+                                             double (&v)[S][3], double (&xn)[S][3], double h,
+                                             const IntParams &p) {
 #pragma unroll
   for (int m = 0; m < S; ++m)
 #pragma unroll
     for (int k = 0; k < 3; ++k) xn[m][k] = x0[m][k] + h * v[m][k];
-  if constexpr (S == 3) {
-    double xu[3][3];
+  if constexpr (CL || S == 3) {
+    double xu[S][3];
 #pragma unroll
-    for (int m = 0; m < 3; ++m)
+    for (int m = 0; m < S; ++m)
 #pragma unroll
       for (int k = 0; k < 3; ++k) xu[m][k] = xn[m][k];
-    settle_positions(x0[0], x0[1], x0[2], xn[0], xn[1], xn[2], p);
+    if constexpr (CL) mshake_positions<S>(x0, xn, s.im, s.len);
+    else settle_positions(x0[0], x0[1], x0[2], xn[0], xn[1], xn[2], p);
     const double ih = 1.0 / h;
 #pragma unroll
-    for (int m = 0; m < 3; ++m)
+    for (int m = 0; m < S; ++m)
 #pragma unroll
       for (int k = 0; k < 3; ++k) v[m][k] += (xn[m][k] - xu[m][k]) * ih;
   }
@@ -242,26 +356,24 @@ __device__ __forceinline__ void drift_settle(const double (&x0)[S][3], double (&
 // spread / sqrt(m) R, projected again at the new positions; A: half drift
 // from there, SETTLE, v += dx_c / (dt/2).  The closing B (kick and
 // projection) is the second half of the step, as in velocity Verlet.
-template <int S>
+template <int S, bool CL>
 __device__ __forceinline__ void baoab_rest(UnitState<S> &s, double (&xn)[S][3], long long step,
                                            const IntParams &p) {
   const double h = 0.5 * p.dt;
-  if constexpr (S == 3)
-    settle_velocities(s.x[0], s.x[1], s.x[2], s.v[0], s.v[1], s.v[2], p);
+  constrain_velocities<S, CL>(s, s.x, p);
   double xh[S][3];
-  drift_settle<S>(s.x, s.v, xh, h, p);
+  drift_settle<S, CL>(s, s.x, s.v, xh, h, p);
 #pragma unroll
   for (int m = 0; m < S; ++m) {
     if (!(s.im[m] > 0.0)) continue;  // a particle without mass keeps its velocity
     double r[3];
-    langevin_normals(p.seed, step, (uint32_t)(s.first + m), r);
+    langevin_normals(p.seed, step, (uint32_t)s.at[m], r);
     const double width = p.spread * sqrt(s.im[m]);
 #pragma unroll
     for (int k = 0; k < 3; ++k) s.v[m][k] = p.decay * s.v[m][k] + width * r[k];
   }
-  if constexpr (S == 3)
-    settle_velocities(xh[0], xh[1], xh[2], s.v[0], s.v[1], s.v[2], p);
-  drift_settle<S>(xh, s.v, xn, h, p);
+  constrain_velocities<S, CL>(s, xh, p);
+  drift_settle<S, CL>(s, xh, s.v, xn, h, p);
 }
 
 // Stores x, v and the f32 positions of the pair loop after a drift to xn;
@@ -280,7 +392,7 @@ __device__ __forceinline__ int store_drifted(UnitState<S> &s, const double (&xn)
   int moved = 0;  // bit 0: beyond the list buffer, bit 1: beyond the pruned one
 #pragma unroll
   for (int m = 0; m < S; ++m) {
-    const int at = s.first + m;
+    const int at = s.at[m];
     double d2 = 0.0, d2p = 0.0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -310,8 +422,9 @@ __device__ __forceinline__ int store_drifted(UnitState<S> &s, const double (&xn)
 // Which halves of velocity Verlet a launch runs on each unit.
 enum StepPart { kFirstHalf = 1, kSecondHalf = 2, kBetween = 3 };
 
-template <int PART, int S>
-__device__ __forceinline__ void step_unit(int first, long long step, double *__restrict__ x,
+template <int PART, int S, bool CL>
+__device__ __forceinline__ void step_unit(int first, int code, long long step,
+                                          double *__restrict__ x,
                                           double *__restrict__ v,
                                           const double *__restrict__ inv_mass,
                                           long long *__restrict__ F,
@@ -327,16 +440,16 @@ __device__ __forceinline__ void step_unit(int first, long long step, double *__r
                                           int *__restrict__ flags) {
   UnitState<S> s;
   // the force words are cleared by the launch that reads them last
-  load_unit<S>(first, x, v, inv_mass, F, Fr, place_of, PART != kSecondHalf, s);
+  load_unit<S, CL>(first, code, x, v, inv_mass, F, Fr, place_of, PART != kSecondHalf, p, s);
   if (PART & kSecondHalf) {  // end of a step: half kick, velocity constraint
     half_kick<S>(s, p.dt);
-    constrain_velocities<S>(s, p);
+    constrain_velocities<S, CL>(s, s.x, p);
   }
-  if (PART & kFirstHalf) {   // start of a step: half kick, drift, SETTLE
+  if (PART & kFirstHalf) {   // start of a step: half kick, drift, constraints
     half_kick<S>(s, p.dt);
     double xn[S][3];
-    if (p.langevin) baoab_rest<S>(s, xn, step, p);
-    else drift_settle<S>(s.x, s.v, xn, p.dt, p);
+    if (p.langevin) baoab_rest<S, CL>(s, xn, step, p);
+    else drift_settle<S, CL>(s, s.x, s.v, xn, p.dt, p);
     const int moved = store_drifted<S>(s, xn, x, v, shift, xref, q, xq, gorig, xpq, xpa, p);
     if (moved & 1) atomicOr(&flags[kFlagRebuild], 1);
     if (moved & 2) atomicOr(&flags[kFlagPrune], 1);
@@ -344,7 +457,7 @@ __device__ __forceinline__ void step_unit(int first, long long step, double *__r
 #pragma unroll
     for (int m = 0; m < S; ++m)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) v[3 * (first + m) + k] = s.v[m][k];
+      for (int k = 0; k < 3; ++k) v[3 * s.at[m] + k] = s.v[m][k];
   }
 }
 
@@ -367,13 +480,13 @@ __global__ void __launch_bounds__(64) integrate_part(
   const bool noise = (PART & kFirstHalf) && p.langevin;
   const long long step = noise ? p.step[0] : 0;
   if (u < nunits) {
-    const int first = unit_first[u];
-    if (unit_size[u] == 3)
-      step_unit<PART, 3>(first, step, x, v, inv_mass, F, Fr, place_of, shift, xref, q, xq,
-                         gorig, xpq, xpa, p, flags);
+    const int first = unit_first[u], code = unit_size[u];
+    if (code == 3)
+      step_unit<PART, 3, false>(first, code, step, x, v, inv_mass, F, Fr, place_of, shift,
+                                xref, q, xq, gorig, xpq, xpa, p, flags);
     else
-      step_unit<PART, 1>(first, step, x, v, inv_mass, F, Fr, place_of, shift, xref, q, xq,
-                         gorig, xpq, xpa, p, flags);
+      step_unit<PART, 1, false>(first, code, step, x, v, inv_mass, F, Fr, place_of, shift,
+                                xref, q, xq, gorig, xpq, xpa, p, flags);
   }
   if (noise && threadIdx.x == 0) {
     // every block has read the step number before it counts itself done
@@ -387,6 +500,35 @@ __global__ void __launch_bounds__(64) integrate_part(
   }
 }
 
+// The same parts for the constraint clusters, a launch of their own (their
+// solves would raise the water kernel's registers): launched before
+// integrate_part on the same stream, it reads the step number and leaves
+// its advance to integrate_part.
+template <int PART>
+__global__ void __launch_bounds__(64) integrate_clusters(
+    int nclusters, const int *__restrict__ cluster_first, const int *__restrict__ cluster_code,
+    double *__restrict__ x, double *__restrict__ v, const double *__restrict__ inv_mass,
+    long long *__restrict__ F, long long *__restrict__ Fr, const int *__restrict__ place_of,
+    const double *__restrict__ shift, const double *__restrict__ xref,
+    const double *__restrict__ q, float4 *__restrict__ xq, const float4 *__restrict__ gorig,
+    double4 *__restrict__ xpq, double4 *__restrict__ xpa, IntParams p,
+    int *__restrict__ flags) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= nclusters) return;
+  const long long step = (PART & kFirstHalf) && p.langevin ? p.step[0] : 0;
+  const int first = cluster_first[u], code = cluster_code[u];
+#define STEP_CLUSTER(S)                                                                     \
+  step_unit<PART, S, true>(first, code, step, x, v, inv_mass, F, Fr, place_of, shift, xref, q, \
+                           xq, gorig, xpq, xpa, p, flags)
+  switch (code & 0xff) {
+    case 2: STEP_CLUSTER(2); break;
+    case 3: STEP_CLUSTER(3); break;
+    case 4: STEP_CLUSTER(4); break;
+    default: STEP_CLUSTER(5); break;
+  }
+#undef STEP_CLUSTER
+}
+
 // The isotropic or axis-wise scaling of a Monte Carlo volume move: each
 // unit (a rigid water or a free atom) moves with its centre of mass,
 // X -> s X, and keeps its shape: x_i += (s - 1) X (per axis).  Masses come
@@ -397,7 +539,7 @@ __global__ void scale_units(int nunits, const int *__restrict__ unit_first,
                             double *__restrict__ x, double s0, double s1, double s2) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= nunits) return;
-  const int first = unit_first[u], size = unit_size[u];
+  const int first = unit_first[u], size = unit_size[u] & 0xff;
   double com[3] = {0.0, 0.0, 0.0}, total = 0.0;
   for (int m = 0; m < size; ++m) {
     const double im = inv_mass[first + m];
@@ -482,13 +624,52 @@ __global__ void clear_words(long long *__restrict__ w, size_t count, size_t fs) 
 // Steepest descent of rigid waters (setup only): x_new = x + h F / |F|max,
 // constrained from the old positions; the step is kept by the host when the
 // energy falls.
+// A descent step of a constraint cluster: each atom along its force, then
+// M-SHAKE against the cluster's positions before the step.
+template <int S>
+__device__ void descend_cluster(int first, int code, const double *__restrict__ x0,
+                                double *__restrict__ x, const long long *__restrict__ F,
+                                const int *__restrict__ place_of, double step,
+                                const IntParams &p) {
+  const int centre = (code >> kUnitCentreShift) & 15;
+  double xo[S][3], xn[S][3], im[S], len[S];
+  int at[S];
+#pragma unroll
+  for (int m = 0; m < S; ++m) {
+    at[m] = first + (m == 0 ? centre : (m - 1 < centre ? m - 1 : m));
+    im[m] = p.inv_mass[at[m]];
+    len[m] = m > 0 ? p.clen[at[m]] : 0.0;
+    const int pl = place_of[at[m]];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      xo[m][k] = x0[3 * at[m] + k];
+      xn[m][k] = xo[m][k] + step * (double)F[3 * pl + k] * kForceUnscale;
+    }
+  }
+  mshake_positions<S>(xo, xn, im, len);
+#pragma unroll
+  for (int m = 0; m < S; ++m)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) x[3 * at[m] + k] = xn[m][k];
+}
+
 __global__ void descend(int nunits, const int *__restrict__ unit_first,
                         const int *__restrict__ unit_size, const double *__restrict__ x0,
                         double *__restrict__ x, const long long *__restrict__ F,
                         const int *__restrict__ place_of, double step, IntParams p) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= nunits) return;
-  const int first = unit_first[u], size = unit_size[u];
+  const int first = unit_first[u], code = unit_size[u];
+  if (code & kUnitCluster) {
+    switch (code & 0xff) {
+      case 2: descend_cluster<2>(first, code, x0, x, F, place_of, step, p); break;
+      case 3: descend_cluster<3>(first, code, x0, x, F, place_of, step, p); break;
+      case 4: descend_cluster<4>(first, code, x0, x, F, place_of, step, p); break;
+      default: descend_cluster<5>(first, code, x0, x, F, place_of, step, p); break;
+    }
+    return;
+  }
+  const int size = code & 0xff;
   double xo[3][3], xn[3][3];
   for (int m = 0; m < size; ++m) {
     const int at = first + m;

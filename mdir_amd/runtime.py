@@ -16,10 +16,12 @@ LIB_PATH = os.path.join(HERE, "lib", "libmdrt_hip.so")
 
 E_LJ, E_COUL_DIRECT, E_COUL_EXCL, E_COUL_RECIP, E_COUL_SELF, E_DISP_CORR, \
     E_LJ14, E_COUL14, E_KINETIC = range(9)
-NUM_ENERGIES = 9
-TERM_PAIRS, TERM_SPECIAL, TERM_RECIP = 1, 2, 4
+NUM_ENERGIES = 14
+TERM_PAIRS, TERM_SPECIAL, TERM_RECIP, TERM_BONDED = 1, 2, 4, 8
 ENERGY_NAMES = ["lj", "coul_direct", "coul_excl", "coul_recip", "coul_self",
-                "disp_corr", "lj14", "coul14", "kinetic"]
+                "disp_corr", "lj14", "coul14", "kinetic", "bonds", "urey_bradley",
+                "angles", "dihedrals", "impropers"]
+KINETIC = 8
 
 D, I32, I64 = C.c_double, C.c_int32, C.c_int64
 PD, PI32, PI64 = C.POINTER(D), C.POINTER(I32), C.POINTER(I64)
@@ -43,7 +45,10 @@ class SystemDesc(C.Structure):
                 ("npairs14", I64), ("pairs14", PI32), ("pairs14_sigma", PD),
                 ("pairs14_eps", PD), ("pairs14_qq", PD), ("nwaters", I64),
                 ("water_first", PI32), ("d_oh", D), ("d_hh", D), ("skin", D),
-                ("prune_skin", D)]
+                ("prune_skin", D), ("ntuples", I64), ("tuple_atoms", PI32),
+                ("tuple_kind", PI32), ("tuple_param", PD), ("nclusters", I64),
+                ("cluster_first", PI32), ("cluster_size", PI32), ("cluster_center", PI32),
+                ("cluster_length", PD)]
 
 
 class Buffer1(C.Structure):
@@ -262,4 +267,15 @@ def system_desc(s, skin: float = 0.1, prune_skin: float = 0.0):
     d.d_oh, d.d_hh = s.water_geometry
     d.skin = skin
     d.prune_skin = prune_skin
+    ta, tk, tp = s.tuples()
+    d.ntuples = tk.shape[0]
+    d.tuple_atoms = ptr(k(ta, np.int32), I32)
+    d.tuple_kind = ptr(k(tk, np.int32), I32)
+    d.tuple_param = ptr(k(tp, np.float64), D)
+    cl, length = s.clusters()
+    d.nclusters = cl.shape[0]
+    d.cluster_first = ptr(k(cl[:, 0], np.int32), I32)
+    d.cluster_size = ptr(k(cl[:, 1], np.int32), I32)
+    d.cluster_center = ptr(k(cl[:, 2], np.int32), I32)
+    d.cluster_length = ptr(k(length, np.float64), D)
     return d, keep

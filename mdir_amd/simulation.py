@@ -106,7 +106,7 @@ class Simulation:
 
     def force_terms(self, terms: int):
         """Forces and energies of the terms `terms` names (bits
-        runtime.TERM_PAIRS, TERM_SPECIAL, TERM_RECIP) only."""
+        runtime.TERM_PAIRS, TERM_SPECIAL, TERM_RECIP, TERM_BONDED) only."""
         F = np.zeros((self.n, 3))
         e = np.zeros(rt.NUM_ENERGIES)
         rt.check(rt.lib().mdrtHipComputeForceTerms(self._h, int(terms), rt.ptr(F, C.c_double),
