@@ -4,7 +4,6 @@ from __future__ import annotations
 
 import math
 import os
-import re
 
 import numpy as np
 
@@ -14,51 +13,10 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 KCAL = 4.184
 
 
-# Amber stores charges in units that make q_i q_j / r come out in kcal/mol
-# with r in Angstrom: e times sqrt(332.0522...) = 18.2223 (the Amber
-# topology format's fixed factor).
-AMBER_CHARGE_UNIT = 18.2223
-
-_FORTRAN_FIELD = re.compile(r"(\d*)([AaIiEeFf])(\d+)(?:\.\d+)?")
-
-
-def read_prmtop(path):
-    """An Amber topology (prmtop) as {FLAG: list of typed values}.
-
-    The file is a series of sections, each "%FLAG NAME", then
-    "%FORMAT(<count><kind><width>[.<decimals>])" (a Fortran edit
-    descriptor: kind a = text, I = integer, E/F = real), then lines holding
-    up to <count> fields of <width> columns each."""
-    sections = {}
-    name, kind, width = None, None, None
-    with open(path) as fh:
-        for raw in fh:
-            line = raw.rstrip("\n")
-            if line.startswith("%FLAG"):
-                name = line[5:].strip()
-                sections[name] = []
-            elif line.startswith("%FORMAT"):
-                m = _FORTRAN_FIELD.search(line)
-                kind, width = m.group(2).lower(), int(m.group(3))
-            elif line.startswith("%") or name is None:
-                continue
-            else:
-                cast = {"a": str.strip, "i": int}.get(kind, float)
-                fields = (line[k: k + width] for k in range(0, len(line), width))
-                sections[name].extend(cast(f) for f in fields if f.strip())
-    return sections
-
-
-def read_inpcrd(path, n):
-    """Coordinates (nm) and box edges (nm) of an Amber restart file: a title,
-    a line with the atom count, then reals in fields of 12 columns: 3n
-    coordinates in Angstrom followed by the box lengths and angles."""
-    with open(path) as fh:
-        body = fh.read().splitlines()[2:]
-    vals = [float(line[k: k + 12]) for line in body for k in range(0, len(line), 12)
-            if line[k: k + 12].strip()]
-    xyz = 0.1 * np.asarray(vals[: 3 * n]).reshape(n, 3)
-    return xyz, 0.1 * np.asarray(vals[3 * n: 3 * n + 3])
+# Amber's topology and coordinate readers are the package's own
+# (mdir_amd/amber.py); the charge unit is e x sqrt(332.0522...) = 18.2223.
+from mdir_amd.amber import CHARGE_UNIT as AMBER_CHARGE_UNIT  # noqa: E402
+from mdir_amd.amber import read_inpcrd, read_prmtop  # noqa: E402,F401
 
 
 def load_opc3(cutoff=0.9, beta=3.47045919371208278, grid=(64, 64, 64), order=8):
