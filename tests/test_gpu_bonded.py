@@ -100,17 +100,19 @@ def test_constrained_steps_against_the_oracle(langevin):
 
 
 def test_constrained_nve_conserves_energy():
-    """2,000 steps of 1 fs (graphs of 50) of the dipeptide with SETTLE
+    """20,000 steps of 1 fs (graphs of 50) of the dipeptide with SETTLE
     waters and hydrogen-bond clusters, the Lennard-Jones and direct Coulomb
     energies shifted to zero at the cutoff (a plain cutoff adds a random
-    walk of about 1 kJ/mol here from pairs crossing it, the integrator's
-    error hidden under it): the total energy, sampled every 100 steps,
-    drifts by less than 2e-5 kJ/mol/ps per atom (twice the reference's NVE
-    drift on JAC, 10-performance.md:549; 0.009 kJ/mol/ps measured, 7.7e-6
-    per atom) and fluctuates within 1 kJ/mol (3e-4 of the kinetic energy),
-    and the bonds stay at their lengths."""
+    walk of about 1 kJ/mol per 2 ps here from pairs crossing it, the
+    integrator's error hidden under it).  The total energy, sampled every
+    1,000 steps, fluctuates by about 0.3 kJ/mol, so a slope fitted over
+    20 ps is known to about 0.015 kJ/mol/ps; the bar is 5e-5 kJ/mol/ps per
+    atom (0.058 kJ/mol/ps, five times the reference's NVE drift on JAC,
+    10-performance.md:549), far below what a wrong kick, constraint or
+    velocity correction gives (several kJ/mol/ps); the bonds stay at their
+    lengths."""
     from mdir_amd.simulation import Simulation
-    say("case: dipeptide NVE, 2000 steps")
+    say("case: dipeptide NVE, 20000 steps")
     s = dipeptide(rigid_water=True, hydrogen_bonds=True, pme=True)
     s.lj_modifier = S.LJ_POTENTIAL_SHIFT
     s.coulomb_shift = True
@@ -126,13 +128,13 @@ def test_constrained_nve_conserves_energy():
     _, e0 = sim.forces()
     totals = [e0.sum()]
     for _ in range(20):
-        sim.run(100, 0.001, graph=50)
+        sim.run(1000, 0.001, graph=50)
         totals.append(sim.forces()[1].sum())
     dev = np.abs(np.array(totals) - totals[0]).max()
-    slope = np.polyfit(np.arange(21) * 0.1, totals, 1)[0]
+    slope = np.polyfit(np.arange(21) * 1.0, totals, 1)[0]
     say(f"  total {totals[0]:.4f} -> {totals[-1]:.4f} kJ/mol (largest change {dev:.4f}, "
         f"slope {slope:.4f} kJ/mol/ps), kinetic {e0[rt.KINETIC]:.2f}")
-    assert abs(slope) < 2e-5 * s.n and dev < 1.0
+    assert abs(slope) < 5e-5 * s.n and dev < 1.5
     x = sim.positions
     cl, length = s.clusters()
     for first, size, off in cl:

@@ -103,7 +103,26 @@ def test_trajectory_matches_the_oracle_file(tmp_path):
     assert numbers[0] == 4 and pos.shape == (4, w.n, 3)
 
 
-@pytest.mark.parametrize("case", ["nve", "langevin", "pruned"])
+def dipeptide_started(langevin=False, **kw):
+    """The reference's alanine dipeptide in water (tests/golden/dipeptide)
+    with bonded tuples and hydrogen-bond clusters."""
+    from mdir_amd.amber import load_amber
+    from mdir_amd.simulation import Simulation
+    from oracle import oracle as O
+    g = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "dipeptide")
+    s = load_amber(os.path.join(g, "dipeptide.prmtop"), os.path.join(g, "dipeptide.inpcrd"),
+                   hydrogen_bonds=True)
+    o = O.Oracle(s)
+    s.positions = o.shake_positions(s.positions, s.positions)
+    sim = Simulation(s, **kw)
+    v = S.initial_velocities(s, 4, 300.0)
+    sim.velocities = o.rattle_velocities(s.positions, o.settle_velocities(s.positions, v))
+    if langevin:
+        sim.set_langevin(300.0, 1.0, seed=5)
+    return sim, s
+
+
+@pytest.mark.parametrize("case", ["nve", "langevin", "pruned", "dipeptide"])
 def test_checkpoint_continues_bit_for_bit(tmp_path, case):
     """Whole: 60 steps, frames every 10, checkpoints every 20.  Part: 30
     steps of the same, stopped; a new simulation from the checkpoint of
@@ -120,13 +139,16 @@ def test_checkpoint_continues_bit_for_bit(tmp_path, case):
     whole_dir.mkdir()
     part_dir.mkdir()
 
-    whole = started(w, langevin=lang, **kw)
+    if case == "dipeptide":
+        whole, w = dipeptide_started(**kw)
+    else:
+        whole = started(w, langevin=lang, **kw)
     whole.reporters = [TrajectoryReporter(str(whole_dir / "run.dcd"), 10),
                        CheckpointReporter(str(whole_dir / "run.ckpt"), 20)]
     whole.run(60, 0.002, graph=50)
     whole.close_reporters()
 
-    part = started(w, langevin=lang, **kw)
+    part = dipeptide_started(**kw)[0] if case == "dipeptide" else started(w, langevin=lang, **kw)
     part.reporters = [TrajectoryReporter(str(part_dir / "run.dcd"), 10),
                       CheckpointReporter(str(part_dir / "run.ckpt"), 20)]
     part.run(30, 0.002, graph=50)
