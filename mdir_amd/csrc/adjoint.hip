@@ -106,7 +106,7 @@ __device__ __forceinline__ Dual f_pos(Dual x) { return x.v > 0.0f ? x : Dual{0.0
 // kernel.  erfc(x) = t exp(-x^2) P(t), t = 1/(1 + 0.475 x), P of degree 6
 // fitted by weighted least squares on the relative error over [0, 3.6]
 // (this project's own fit, below 1e-7 relative in exact arithmetic; the
-// fit is tests/golden/erfc_fit.py), the erfc_from_exp of pairs.hip.
+// fit is tests/golden/erfc_fit.py), the erfc_poly of pairs.hip.
 template <typename T>
 struct PairParts {
   T uc, hc, u12, f12, u6, f6;

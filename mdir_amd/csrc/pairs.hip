@@ -13,16 +13,21 @@
 // pair per lane.  The force on j is accumulated over the members an entry
 // pairs with, summed over the 8 lanes of a column by three DPP row operations
 // and added once as 64-bit fixed point; the forces on the members stay in
-// registers across the segment and are summed once at the end.
+// registers across the segment and are summed once at the end, over the
+// wave's 8 columns through LDS.
 #include "md_common.h"
 
 namespace mdhip {
 
 constexpr int kMaxTypes = 32;
 
+// A DPP read of a lane of the same row.  Every control used here names a
+// lane inside the row, so bound_ctrl never applies; with it set and the old
+// value zero the compiler folds the move into the add that consumes it
+// (v_add_f32_dpp).
 template <int CTRL>
 __device__ __forceinline__ float dpp(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 
 // Sum over the 8 consecutive lanes of a half row (result in all 8 lanes):
@@ -34,24 +39,14 @@ __device__ __forceinline__ float sum8(float v) {
   return v;
 }
 
-__device__ __forceinline__ void add_fixed(long long *F, int place, float fx,
-                                          float fy, float fz) {
-  unsigned long long *u = reinterpret_cast<unsigned long long *>(F + 3 * place);
-  long long wx = to_fixed(fx), wy = to_fixed(fy), wz = to_fixed(fz);
-  if (wx) atomicAdd(u + 0, (unsigned long long)wx);
-  if (wy) atomicAdd(u + 1, (unsigned long long)wy);
-  if (wz) atomicAdd(u + 2, (unsigned long long)wz);
-}
-
 // erfc(x) = t exp(-x^2) P(t), t = 1/(1 + 0.475 x): a degree-6 polynomial
 // fitted (weighted least squares on the relative error, this project's own
 // fit, tests/golden/erfc_fit.py) on [0, 3.6]; below 1e-7 relative in exact
 // arithmetic (9.8e-8 at worst), ~1e-6 in f32, where
 // the rounding of exp(-x^2) itself dominates as it does for any f32 erfc.
 // It reuses the exp(-x^2) that the force needs anyway.  t comes from the
-// hardware reciprocal (1 ulp).
-__device__ __forceinline__ float erfc_from_exp(float x, float expmx2) {
-  const float t = __builtin_amdgcn_rcpf(fmaf(0.475f, x, 1.0f));
+// hardware reciprocal (1 ulp).  erfc_poly(t) is P(t).
+__device__ __forceinline__ float erfc_poly(float t) {
   float pol = 0.09289151430130005f;
   pol = fmaf(pol, t, -0.4289827346801758f);
   pol = fmaf(pol, t, 0.6025476455688477f);
@@ -59,64 +54,73 @@ __device__ __forceinline__ float erfc_from_exp(float x, float expmx2) {
   pol = fmaf(pol, t, 0.3417808711528778f);
   pol = fmaf(pol, t, 0.24873463809490204f);
   pol = fmaf(pol, t, 0.26948970556259155f);
-  return t * expmx2 * pol;
+  return pol;
 }
 
 // Scalar force -dU/dr / r and energy of one pair in f32.  lj holds
 // (6 c6, 12 c12), the force's own scaling; the energy divides it back out
-// and the modifier constants carry the 1/6 and 1/12.  r2 is never zero or
-// denormal (padding is masked, overlapping atoms are not a state), so the
-// hardware rsq and exp2 are used without range fix-ups.
+// and the modifier constants carry the 1/6 and 1/12.  b475 = 0.475 beta
+// (the t of erfc_poly is 1 / (1 + b475 r)).  Both parts are formed as
+// (-dU/dr) r first and share the final 1/r^2:
+//   Coulomb  qq exp(-beta^2 r^2) (t P(t) / r + 2 beta / sqrt(pi))
+//   LJ       (12 c12 / r^6 - 6 c6) / r^6
+// The hardware rsq, rcp and exp2 are used without range fix-ups: the
+// caller runs this only on the lanes whose pair is on (r2 within rc and
+// never zero or denormal: padding, excluded and i == j pairs are off, and
+// overlapping atoms are not a state).
 template <int MOD, bool COUL, bool ENERGY>
 __device__ __forceinline__ float pair_force(float r2, float qq, float2 lj,
-                                            const NbParams &p, float &e_lj,
+                                            const NbParams &p, float b475, float &e_lj,
                                             float &e_c) {
   const float rinv = __builtin_amdgcn_rsqf(r2);
   const float rinv2 = rinv * rinv;
-  float fscal = 0.0f;
+  float g = 0.0f;  // Coulomb (-dU/dr) r
   if (COUL) {
-    const float br = p.beta * (r2 * rinv);
+    const float r = r2 * rinv;
     const float expt = __builtin_amdgcn_exp2f(p.mb2l2e * r2);  // exp(-beta^2 r^2)
-    const float erfcv = erfc_from_exp(br, expt);
-    fscal = qq * fmaf(erfcv, rinv, p.tbs * expt) * rinv2;
-    if (ENERGY) e_c += qq * (erfcv * rinv - p.shift_c);
+    const float t = __builtin_amdgcn_rcpf(fmaf(b475, r, 1.0f));
+    const float tp = t * erfc_poly(t);                          // erfc(beta r) exp(beta^2 r^2)
+    g = qq * expt * fmaf(tp, rinv, p.tbs);
+    if (ENERGY) e_c += qq * (tp * expt * rinv - p.shift_c);
   }
   const float c6x = lj.x, c12x = lj.y;
-  {  // zero parameters give zero: no branch
-    const float rinv6 = rinv2 * rinv2 * rinv2;
-    float flj = fmaf(c12x, rinv6, -c6x) * rinv6 * rinv2;
-    float ulj = 0.0f;
-    if (ENERGY || MOD == kLJPotentialSwitch)
-      ulj = fmaf(c12x * (1.0f / 12.0f), rinv6, -c6x * (1.0f / 6.0f)) * rinv6;
-    if (MOD == kLJPotentialShift) {
-      if (ENERGY) ulj -= c12x * p.shift12 - c6x * p.shift6;
-    } else if (MOD == kLJForceSwitch) {
-      const float r = r2 * rinv;
-      const float u = fmaxf(r - p.rs, 0.0f);
-      const float u2 = u * u, u3 = u2 * u;
-      const float dp12 = p.sw_a12 * u2 + p.sw_b12 * u3;
-      const float dp6 = p.sw_a6 * u2 + p.sw_b6 * u3;
-      flj += (c12x * dp12 - c6x * dp6) * rinv;
-      if (ENERGY) {
-        const float p12 = p.sw_c12 + u3 * (p.sw_a12 * (1.0f / 3.0f) + p.sw_b12 * 0.25f * u);
-        const float p6 = p.sw_c6 + u3 * (p.sw_a6 * (1.0f / 3.0f) + p.sw_b6 * 0.25f * u);
-        ulj -= c12x * p12 - c6x * p6;
-      }
-    } else if (MOD == kLJPotentialSwitch) {
-      const float r = r2 * rinv;
-      if (r > p.rs) {
-        const float width = p.rc - p.rs;
-        const float t = (r - p.rs) / width;
-        const float t2 = t * t, t3 = t2 * t;
-        const float S = 1.0f + t3 * (-10.0f + t * (15.0f - 6.0f * t));
-        const float dS = (-30.0f * t2 + 60.0f * t3 - 30.0f * t3 * t) / width;
-        flj = flj * S - ulj * dS * rinv;
-        ulj *= S;
-      }
+  // zero parameters give zero: no branch
+  const float rinv6 = rinv2 * rinv2 * rinv2;
+  const float l = fmaf(c12x, rinv6, -c6x);
+  float ulj = 0.0f;
+  if (ENERGY || MOD == kLJPotentialSwitch)
+    ulj = fmaf(c12x * (1.0f / 12.0f), rinv6, -c6x * (1.0f / 6.0f)) * rinv6;
+  float fscal;
+  if (MOD == kLJForceSwitch) {
+    const float r = r2 * rinv;
+    const float u = fmaxf(r - p.rs, 0.0f);
+    const float u2 = u * u, u3 = u2 * u;
+    const float dp12 = p.sw_a12 * u2 + p.sw_b12 * u3;
+    const float dp6 = p.sw_a6 * u2 + p.sw_b6 * u3;
+    fscal = fmaf(l, rinv6, g) * rinv2 + (c12x * dp12 - c6x * dp6) * rinv;
+    if (ENERGY) {
+      const float p12 = p.sw_c12 + u3 * (p.sw_a12 * (1.0f / 3.0f) + p.sw_b12 * 0.25f * u);
+      const float p6 = p.sw_c6 + u3 * (p.sw_a6 * (1.0f / 3.0f) + p.sw_b6 * 0.25f * u);
+      ulj -= c12x * p12 - c6x * p6;
     }
-    fscal += flj;
-    if (ENERGY) e_lj += ulj;
+  } else if (MOD == kLJPotentialSwitch) {
+    float flj = l * rinv6 * rinv2;
+    const float r = r2 * rinv;
+    if (r > p.rs) {
+      const float width = p.rc - p.rs;
+      const float t = (r - p.rs) / width;
+      const float t2 = t * t, t3 = t2 * t;
+      const float S = 1.0f + t3 * (-10.0f + t * (15.0f - 6.0f * t));
+      const float dS = (-30.0f * t2 + 60.0f * t3 - 30.0f * t3 * t) / width;
+      flj = flj * S - ulj * dS * rinv;
+      ulj *= S;
+    }
+    fscal = fmaf(g, rinv2, flj);
+  } else {  // none, potential shift: the force is the plain one
+    if (MOD == kLJPotentialShift && ENERGY) ulj -= c12x * p.shift12 - c6x * p.shift6;
+    fscal = fmaf(l, rinv6, g) * rinv2;
   }
+  if (ENERGY) e_lj += ulj;
   return fscal;
 }
 
@@ -261,7 +265,7 @@ __device__ __forceinline__ void nb_pairs_body(
     const float2 *__restrict__ ljtab, const NbParams &p, long long *__restrict__ F,
     double *__restrict__ energy_partial, int *__restrict__ flags, const SpecialArgs &sa) {
   extern __shared__ float2 s_lj[];                   // ntypes^2 (launch size)
-  __shared__ union {
+  __shared__ union __align__(16) {
     struct {
       float fj[4][kSegmentMax * kCluster * 3];       // j forces of the entries
       int cj[4][kSegmentMax];
@@ -313,23 +317,28 @@ __device__ __forceinline__ void nb_pairs_body(
   // ORG: positions relative to group origins (large boxes), else relative
   // to the box centre with zero origins
   const float4 oi = ORG ? gorig[gi] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const float b475 = 0.475f * p.beta;
   double e_lj = 0.0, e_c = 0.0;  // energy steps only: f64 partial sums
   const PairEntry *row = rows + (size_t)gi * row_cap + e0;
-  // Software pipeline: the entry two ahead and the j atoms one ahead are
-  // loaded before the current entry is computed (indices clamped to the
-  // segment, so every load is in bounds).
-  PairEntry E = row[0];
-  PairEntry E1 = row[n > 1 ? 1 : 0];
-  float4 xj = xq[E.cj * kCluster + b];
-  float4 oj = ORG ? gorig[E.cj / kGroup] : oi;
-  int tj = type_at[E.cj * kCluster + b];
+  // The segment's entries (n <= 48), one per lane in one vector load, read
+  // out per entry with readlane (no scalar loads in the loop, whose waits
+  // would also hold the LDS reads of the LJ parameters).  Software
+  // pipeline: the j atoms one entry ahead are loaded before the current
+  // entry is computed (indices clamped to the segment, so every load is in
+  // bounds).
+  const PairEntry mine = row[lane < n ? lane : n - 1];
+  int cj = __builtin_amdgcn_readlane(mine.cj, 0);
+  float4 xj = xq[cj * kCluster + b];
+  float4 oj = ORG ? gorig[cj / kGroup] : oi;
+  int tj = type_at[cj * kCluster + b];
   for (int e = 0; e < n; ++e) {
-    const PairEntry E2 = row[e + 2 < n ? e + 2 : n - 1];
-    const float4 xj1 = xq[E1.cj * kCluster + b];
-    const float4 oj1 = ORG ? gorig[E1.cj / kGroup] : oi;
-    const int tj1 = type_at[E1.cj * kCluster + b];
+    const int cj1 = __builtin_amdgcn_readlane(mine.cj, e + 1 < n ? e + 1 : n - 1);
+    const float4 xj1 = xq[cj1 * kCluster + b];
+    const float4 oj1 = ORG ? gorig[cj1 / kGroup] : oi;
+    const int tj1 = type_at[cj1 * kCluster + b];
 
-    const uint32_t info = E.info;
+    const uint32_t info = __builtin_amdgcn_readlane(mine.info, e);
+    const int ex = __builtin_amdgcn_readlane(mine.ex, e);
     float2 ljm[kGroup];  // issued early: LDS latency overlaps the set-up
 #pragma unroll
     for (int m = 0; m < kGroup; ++m) ljm[m] = s_lj[ti[m] + tj];
@@ -347,33 +356,42 @@ __device__ __forceinline__ void nb_pairs_body(
       xjz = xj.z + sz;
     }
     const bool jreal = (info >> (16 + b)) & 1u;
+    // explicit masks (a subset of the real atoms' outer product), or all
+    // pairs: wave-uniform words, taken as lane masks
+    uint64_t exm[kGroup];
+#pragma unroll
+    for (int m = 0; m < kGroup; ++m) exm[m] = ~0ull;
+    if (ex >= 0) {
+#pragma unroll
+      for (int m = 0; m < kGroup; ++m) exm[m] = ex_masks[(size_t)ex * kGroup + m];
+    }
     float fjx = 0.0f, fjy = 0.0f, fjz = 0.0f;
 #pragma unroll
     for (int m = 0; m < kGroup; ++m) {
       if (!((info >> (8 + m)) & 1u)) continue;  // uniform
       const float dx = xi[m].x - xjx, dy = xi[m].y - xjy, dz = xi[m].z - xjz;
       const float r2 = dx * dx + dy * dy + dz * dz;
-      bool on = E.ex >= 0 ? ((ex_masks[(size_t)E.ex * kGroup + m] >> lane) & 1ull)
-                          : (ireal[m] && jreal);
-      on = on && r2 < p.rc2;
-      // A member tile with no pair inside rc (about half of those a list
-      // with a buffer holds) adds exact zeros: skip its arithmetic.
-      if (!__ballot(on)) continue;  // uniform
-      float el = 0.0f, ec = 0.0f;
-      float fscal = pair_force<MOD, COUL, ENERGY>(on ? r2 : 1.0f, xi[m].w * xj.w,
-                                                  ljm[m], p, el, ec);
-      fscal = on ? fscal : 0.0f;
-      if (ENERGY && on) {
-        e_lj += (double)el;
-        e_c += (double)ec;
+      const bool on = __builtin_amdgcn_inverse_ballot_w64(exm[m]) && ireal[m] && jreal &&
+                      r2 < p.rc2;
+      // The pair's arithmetic runs under the exec mask of the lanes that are
+      // on; a member tile with no pair inside rc (about half of those a
+      // list with a buffer holds) branches over it.
+      if (on) {
+        float el = 0.0f, ec = 0.0f;
+        const float fscal = pair_force<MOD, COUL, ENERGY>(r2, xi[m].w * xj.w, ljm[m], p, b475,
+                                                          el, ec);
+        if (ENERGY) {
+          e_lj += (double)el;
+          e_c += (double)ec;
+        }
+        const float fx = fscal * dx, fy = fscal * dy, fz = fscal * dz;
+        fi[m][0] += fx;
+        fi[m][1] += fy;
+        fi[m][2] += fz;
+        fjx -= fx;
+        fjy -= fy;
+        fjz -= fz;
       }
-      const float fx = fscal * dx, fy = fscal * dy, fz = fscal * dz;
-      fi[m][0] += fx;
-      fi[m][1] += fy;
-      fi[m][2] += fz;
-      fjx -= fx;
-      fjy -= fy;
-      fjz -= fz;
     }
     // j force: the 8 i lanes of column b, summed across the members, kept
     // in LDS; the atomics are issued after the loop, so that the loads of
@@ -385,9 +403,8 @@ __device__ __forceinline__ void nb_pairs_body(
       dst[1] = gy;
       dst[2] = gz;
     }
-    if (lane == 0) sm.pr.cj[wave][e] = E.cj;
-    E = E1;
-    E1 = E2;
+    if (lane == 0) sm.pr.cj[wave][e] = cj;
+    cj = cj1;
     xj = xj1;
     oj = oj1;
     tj = tj1;
@@ -396,25 +413,46 @@ __device__ __forceinline__ void nb_pairs_body(
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   // j forces of the segment: 24 words per entry, zero words skipped
-  // (padding atoms and pairs beyond the cutoff give exact zeros)
+  // (padding atoms and pairs beyond the cutoff give exact zeros).  A lane
+  // per word, so one atomic instruction covers 64 consecutive words of a
+  // few entries: a handful of cache lines.  (A lane per atom with its three
+  // words takes three instructions over 64 atoms each, three times the
+  // lines per word; that ran the kernel 8 µs slower.)
+  unsigned long long *Fw = reinterpret_cast<unsigned long long *>(F);
   for (int k = lane; k < n * kCluster * 3; k += 64) {
     const int e = k / (kCluster * 3), rem = k - e * (kCluster * 3);
     const long long w = to_fixed(sm.pr.fj[wave][k]);
-    if (w) {
-      unsigned long long *u = reinterpret_cast<unsigned long long *>(F);
-      atomicAdd(u + 3 * (sm.pr.cj[wave][e] * kCluster) + rem, (unsigned long long)w);
-    }
+    if (w) atomicAdd(Fw + 3 * (sm.pr.cj[wave][e] * kCluster) + rem, (unsigned long long)w);
   }
-  // i forces: sum over the 8 j columns of the wave
+  // i forces: the 8 j columns of the wave summed through LDS.  Lane (a, b)
+  // stores its 12 sums (member m, coordinate c; v = 3 m + c) at
+  // 64 v + 8 a + b, so the 8 columns of output o = 8 v + a are the 8 words
+  // at 8 o; 96 outputs over two passes of the wave, summed in the tree
+  // order ((b0 + b1) + (b2 + b3)) + ((b4 + b5) + (b6 + b7)).  Output o is
+  // word 24 m + 3 a + c of the group's 96 consecutive force words, so each
+  // pass's atomics cover consecutive words.
+  float *red = sm.pr.fj[wave];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  uint32_t rbits = 0;
 #pragma unroll
   for (int m = 0; m < kGroup; ++m) {
+    rbits |= (uint32_t)ireal[m] << m;
 #pragma unroll
-    for (int s = 8; s < 64; s <<= 1) {
-      fi[m][0] += __shfl_xor(fi[m][0], s);
-      fi[m][1] += __shfl_xor(fi[m][1], s);
-      fi[m][2] += __shfl_xor(fi[m][2], s);
-    }
-    if (b == 0 && ireal[m]) add_fixed(F, (g0 + m) * kCluster + a, fi[m][0], fi[m][1], fi[m][2]);
+    for (int c = 0; c < 3; ++c) red[64 * (3 * m + c) + 8 * a + b] = fi[m][c];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int o = lane; o < kGroup * 3 * kCluster; o += 64) {
+    const float4 u = *reinterpret_cast<const float4 *>(red + 8 * o);
+    const float4 v = *reinterpret_cast<const float4 *>(red + 8 * o + 4);
+    const float s = ((u.x + u.y) + (u.z + u.w)) + ((v.x + v.y) + (v.z + v.w));
+    const int vi = o >> 3, m = vi / 3, c = vi - 3 * m;  // o & 7 == a
+    const long long w = to_fixed(s);
+    if (((rbits >> m) & 1u) && w)
+      atomicAdd(Fw + 3 * ((size_t)(g0 + m) * kCluster + a) + c, (unsigned long long)w);
   }
   if (ENERGY) {
 #pragma unroll
