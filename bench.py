@@ -4,13 +4,18 @@ Workload: 7,900 TIP3P waters (23,700 atoms) in a 6.2 nm cube, seeded
 jittered lattice relaxed by 200 steepest-descent steps, PME with rc 0.9 nm,
 Ewald tolerance 1e-5, 64^3 grid, order 4, SETTLE waters, NVE velocity
 Verlet at 2 fs, mixed precision (f32 pair/PME arithmetic, 64-bit fixed-point
-force words, f64 integration and constraints).
+force words, f64 integration and constraints); the pair list is built at
+rc + 0.20 nm and rebuilt when an atom moved 0.10 nm.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
 A step is one velocity-Verlet step of the whole box (neighbor validity
 test and conditional rebuild, pair loop, excluded pairs, PME, integration),
 replayed from captured HIP graphs.  Prints one JSON line.
+
+`--workload tip3p_333000_waters_pme216` runs the same recipe at 999,000
+atoms in a 21.5 nm cube with a 216^3 grid, NPT: Langevin at 300 K and a
+Monte Carlo volume move at 1 bar every 100 steps.
 """
 from __future__ import annotations
 
@@ -37,16 +42,19 @@ PEAK_HBM_GBS = 8000.0            # MI355X HBM3E
 NB_PAIRS_TRAFFIC_BYTES = 28.54e6
 
 # The measured workloads: the 23,700-atom box (the default bench line) and
-# the 999,000-atom box at full size on one GPU (same recipe, NVE).
+# the 999,000-atom box at full size on one GPU (same recipe, NPT).  The list
+# buffer (skin) is each workload's fastest of a sweep: 0.20 nm for the small
+# box, where 0.20-0.32 nm ran within noise of each other and 0.15 nm 3 %
+# slower (fewer rebuilds outweigh the longer rows).
 WORKLOADS = {
     "tip3p_7900_waters_pme64": {
         "waters": 7900, "edge": 6.2, "grid": 64, "traffic": NB_PAIRS_TRAFFIC_BYTES,
-        "metric": "ns/day on ~24k-atom PME water box (1 GPU)"},
+        "ensemble": "nve", "skin": 0.20, "metric": "ns/day on ~24k-atom PME water box (1 GPU)"},
     "tip3p_333000_waters_pme216": {
         "waters": 333000, "edge": 21.5, "grid": 216,
         # the same two --pmc runs with --workload tip3p_333000_waters_pme216:
         # profiles/md_step_tip3p_333000_pme216.md
-        "traffic": 1.5954e9,
+        "traffic": 1.3888e9, "ensemble": "npt", "skin": 0.15,
         "metric": "ns/day on 999k-atom PME water box (1 GPU)"},
 }
 
@@ -56,12 +64,19 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--workload", choices=sorted(WORKLOADS), default="tip3p_7900_waters_pme64",
                    help="TIP3P box: the 23,700-atom bench line or the 999,000-atom box")
+    p.add_argument("--ensemble", choices=("nve", "npt"), default=None,
+                   help="nve: velocity Verlet; npt: Langevin 300 K (friction 1/ps) with a "
+                        "Monte Carlo volume move at 1 bar every --barostat-every steps. "
+                        "Default: the workload's own (nve for the 23,700-atom box, npt for "
+                        "the 999,000-atom box)")
+    p.add_argument("--barostat-every", type=int, default=100)
     p.add_argument("--steps", type=int, default=5000)
     p.add_argument("--warmup", type=int, default=500)
     p.add_argument("--graph", type=int, default=50, help="steps per captured graph")
-    p.add_argument("--skin", type=float, default=0.15,
+    p.add_argument("--skin", type=float, default=None,
                    help="list buffer beyond the cutoff (nm): the list is rebuilt when an "
-                        "atom moved half of it")
+                        "atom moved half of it.  Default: the workload's (0.20 nm for the "
+                        "23,700-atom box, 0.15 nm for the 999,000-atom box)")
     p.add_argument("--prune-skin", type=float, default=0.0,
                    help="buffer of a pruned inner list (nm, below --skin); 0: none")
     p.add_argument("--autotune", type=int, default=7,
@@ -179,6 +194,8 @@ def main():
     else:
         system = S.tip3p_box(wl["waters"], wl["edge"], seed=args.seed, cutoff=0.9,
                              grid=(wl["grid"],) * 3)
+    if args.skin is None:
+        args.skin = wl["skin"]
     sim = Simulation(system, skin=args.skin, prune_skin=args.prune_skin)
     sim.minimize(200, 0.01)
     x = sim.positions
@@ -188,6 +205,17 @@ def main():
     sim.velocities = v
     print(f"setup: {system.n} atoms minimised, velocities at 300 K", file=sys.stderr,
           flush=True)
+    ensemble = args.ensemble or wl["ensemble"]
+    npt = ensemble == "npt"
+    if npt:
+        sim.set_langevin(300.0, 1.0, seed=args.seed)
+
+    def advance(steps):
+        if npt:
+            sim.run_npt(steps, dt, graph=args.graph, every=args.barostat_every, pressure=1.0,
+                        temperature=300.0, seed=args.seed)
+        else:
+            sim.run(steps, dt, graph=args.graph)
 
     tuned = {}
     if args.autotune:
@@ -196,13 +224,16 @@ def main():
         print(f"autotune: best bricks of {best[0]} z lines, transform {best[1]}, segments "
               f"of {best[2]} ({tuned[best] * 1e3:.1f} us/step)", file=sys.stderr, flush=True)
 
-    sim.run(args.warmup, dt, graph=args.graph)
+    advance(args.warmup)
     # the graphs the timed run replays (its chunks of --graph steps and the
-    # remainder), captured before the clock starts
-    sim.prepare(args.steps, dt, graph=args.graph)
+    # remainder), captured before the clock starts; under NPT a kept volume
+    # move changes the box the graphs hold, so the run captures again after
+    # it, inside the timed region
+    sim.prepare(args.barostat_every if npt else args.steps, dt, graph=args.graph)
     builds0 = sim.stats()["builds"]
+    mc0 = sim.barostat_stats() if npt else None
     t0 = time.perf_counter()
-    sim.run(args.steps, dt, graph=args.graph)   # returns after a device sync
+    advance(args.steps)   # returns after a device sync
     t1 = time.perf_counter()
     builds = sim.stats()["builds"] - builds0
     elapsed = t1 - t0
@@ -211,21 +242,38 @@ def main():
     print(f"timed: {args.steps} steps in {elapsed:.3f} s, {ns_day:.1f} ns/day",
           file=sys.stderr, flush=True)
 
+    barostat = None
+    if npt:
+        mc = sim.barostat_stats()
+        box = sim.box
+        mass_g = float(system.masses.sum()) / 6.02214076e23
+        barostat = {"moves": mc["tried"] - mc0["tried"], "kept": mc["kept"] - mc0["kept"],
+                    "every_steps": args.barostat_every, "pressure_bar": 1.0,
+                    "box_nm": [round(float(b), 5) for b in box],
+                    "density_g_cm3": round(mass_g / (float(np.prod(box)) * 1e-21), 4)}
+        print(f"barostat: {barostat['kept']} of {barostat['moves']} volume moves kept, box "
+              f"{box[0]:.4f} nm, {barostat['density_g_cm3']} g/cm3", file=sys.stderr,
+              flush=True)
+
     # conserved-energy drift of an NVE run (untimed): end minus start over
-    # the run, and the least-squares slope of one sample per picosecond
+    # the run, and the least-squares slope of one sample per picosecond;
+    # under NPT there is no conserved energy, and no drift run
     per_ps = int(round(1.0 / dt))
-    samples = max(1, int(round(args.drift_ps)))
-    times, energies = [0.0], [total_energy(sim)[0]]
-    for k in range(samples):
-        sim.run(per_ps, dt, graph=args.graph)
-        times.append((k + 1) * per_ps * dt)
-        energies.append(total_energy(sim)[0])
+    drift = slope = None
+    times = np.array([0.0])
+    if not npt:
+        samples = max(1, int(round(args.drift_ps)))
+        times, energies = [0.0], [total_energy(sim)[0]]
+        for k in range(samples):
+            sim.run(per_ps, dt, graph=args.graph)
+            times.append((k + 1) * per_ps * dt)
+            energies.append(total_energy(sim)[0])
+        times, energies = np.array(times), np.array(energies)
+        drift = (energies[-1] - energies[0]) / times[-1] / system.n  # kJ/mol/ps per atom
+        slope = float(np.polyfit(times, energies, 1)[0]) / system.n
+        print(f"drift: {samples * per_ps} steps, {drift:.3e} kJ/mol/ps/atom "
+              f"(fit {slope:.3e})", file=sys.stderr, flush=True)
     terms = total_energy(sim)[1]
-    times, energies = np.array(times), np.array(energies)
-    drift = (energies[-1] - energies[0]) / times[-1] / system.n  # kJ/mol/ps per atom
-    slope = float(np.polyfit(times, energies, 1)[0]) / system.n
-    print(f"drift: {samples * per_ps} steps, {drift:.3e} kJ/mol/ps/atom "
-          f"(fit {slope:.3e})", file=sys.stderr, flush=True)
 
     # The pair kernel's duration inside the replayed steps, on its stream:
     # device-clock readings bracket it in every step of a run of the same
@@ -275,7 +323,10 @@ def main():
         "config": {"workload": args.workload, "atoms": system.n,
                    "box_nm": wl["edge"], "cutoff_nm": system.cutoff, "pme_grid": list(system.pme_grid),
                    "pme_order": system.pme_order, "ewald_beta": system.beta,
-                   "dt_fs": 2.0, "ensemble": "NVE", "constraints": "SETTLE",
+                   "dt_fs": 2.0,
+                   "ensemble": ("NPT: Langevin 300 K (1/ps), Monte Carlo volume moves at "
+                                f"1 bar every {args.barostat_every} steps") if npt else "NVE",
+                   "constraints": "SETTLE",
                    "list_skin_nm": args.skin, "prune_skin_nm": args.prune_skin,
                    "graph_steps": args.graph},
         "roofline": roofline,
@@ -286,6 +337,7 @@ def main():
         "nve_drift_kj_mol_ps_atom": drift,
         "nve_drift_fit_kj_mol_ps_atom": slope,
         "nve_drift_ps": float(times[-1]),
+        "barostat": barostat,
         "energy_terms_kj_mol": [round(float(t), 4) for t in terms],
     }
     if tuned:

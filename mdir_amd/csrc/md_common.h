@@ -151,7 +151,10 @@ __host__ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k
 // with the counter (step low word, step high word, id, 2 << 24), stream 2
 // in the high byte of the last word; each word w gives the uniform
 // (w + 1/2) 2^-32 in (0, 1); Box-Muller on (u0, u1) gives the first two
-// numbers (cosine, sine), on (u2, u3) the third (cosine).
+// numbers (cosine, sine), on (u2, u3) the third (cosine).  The angle 2 pi u
+// is taken as sincospi / cospi of 2u (2u exact): one argument reduction,
+// exact for multiples of pi, shared by the cosine and sine of the first
+// pair.  cos(2 pi u) of a rounded 2 pi u differs from it by a few ulp.
 __device__ __forceinline__ void langevin_normals(unsigned long long seed, long long step,
                                                  uint32_t id, double r[3]) {
   uint32_t c[4] = {(uint32_t)(unsigned long long)step,
@@ -160,12 +163,13 @@ __device__ __forceinline__ void langevin_normals(unsigned long long seed, long l
   double u[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) u[k] = ((double)c[k] + 0.5) * 0x1.0p-32;
-  const double two_pi = 6.283185307179586;
-  const double rad0 = sqrt(-2.0 * log(u[0])), th0 = two_pi * u[1];
-  const double rad1 = sqrt(-2.0 * log(u[2])), th1 = two_pi * u[3];
-  r[0] = rad0 * cos(th0);
-  r[1] = rad0 * sin(th0);
-  r[2] = rad1 * cos(th1);
+  const double rad0 = sqrt(-2.0 * log(u[0]));
+  const double rad1 = sqrt(-2.0 * log(u[2]));
+  double s0, c0;
+  sincospi(2.0 * u[1], &s0, &c0);
+  r[0] = rad0 * c0;
+  r[1] = rad0 * s0;
+  r[2] = rad1 * cospi(2.0 * u[3]);
 }
 
 }  // namespace mdhip
