@@ -45,7 +45,8 @@ NB_PAIRS_TRAFFIC_BYTES = 28.54e6
 # the 999,000-atom box at full size on one GPU (same recipe, NPT).  The list
 # buffer (skin) is each workload's fastest of a sweep: 0.20 nm for the small
 # box, where 0.20-0.32 nm ran within noise of each other and 0.15 nm 3 %
-# slower (fewer rebuilds outweigh the longer rows).
+# slower (fewer rebuilds outweigh the longer rows); 0.15 nm for the large
+# one, where 0.20 and 0.25 nm ran 1 % and 3 % slower (its rows cost more).
 WORKLOADS = {
     "tip3p_7900_waters_pme64": {
         "waters": 7900, "edge": 6.2, "grid": 64, "traffic": NB_PAIRS_TRAFFIC_BYTES,
