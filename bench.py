@@ -39,7 +39,7 @@ PEAK_HBM_GBS = 8000.0            # MI355X HBM3E
 # rocprofv3 --pmc runs of this script (FETCH_SIZE x 2.000 and WRITE_SIZE x
 # 1.000, the factors that make a 256 MiB device copy read and write 256 MiB):
 # profiles/nb_pairs_tip3p_7900_pme64.md
-NB_PAIRS_TRAFFIC_BYTES = 28.54e6
+NB_PAIRS_TRAFFIC_BYTES = 28.90e6
 
 # The measured workloads: the 23,700-atom box (the default bench line) and
 # the 999,000-atom box at full size on one GPU (same recipe, NPT).  The list
