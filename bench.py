@@ -55,7 +55,7 @@ WORKLOADS = {
         "waters": 333000, "edge": 21.5, "grid": 216,
         # the same two --pmc runs with --workload tip3p_333000_waters_pme216:
         # profiles/md_step_tip3p_333000_pme216.md
-        "traffic": 1.3888e9, "ensemble": "npt", "skin": 0.15,
+        "traffic": 1.3870e9, "ensemble": "npt", "skin": 0.15,
         "metric": "ns/day on 999k-atom PME water box (1 GPU)"},
 }
 

@@ -1247,11 +1247,17 @@ struct Engine {
     const dim3 grid(std::max(blocks_for(max_groups, 4), blocks_for(nspecial)),
                     (terms & 1) ? nseg + 1 : 1, nz);
     const bool pruned = prune_skin > 0;
-    nb_pairs<MOD, COUL, ENERGY, ORG><<<grid, 256, ntypes * ntypes * sizeof(float2), lstream>>>(
-        d_xq, d_gorig, d_type_at, d_real_mask, pruned ? d_rows_in : d_rows,
-        pruned ? d_row_count_in : d_row_count, d_ex, g.row_cap, seg_len, max_groups, d_counts,
-        d_lj,
-        nbp, d_F, d_epair, d_flags, sa, fs);
+    const size_t lds = ntypes * ntypes * sizeof(float2);
+    if constexpr (ORG)
+      nb_pairs_org<MOD, COUL, ENERGY><<<grid, 256, lds, lstream>>>(
+          d_xq, d_gorig, d_type_at, d_real_mask, pruned ? d_rows_in : d_rows,
+          pruned ? d_row_count_in : d_row_count, d_ex, g.row_cap, seg_len, max_groups, d_counts,
+          d_lj, nbp, d_F, d_epair, d_flags, sa, fs);
+    else
+      nb_pairs<MOD, COUL, ENERGY, false><<<grid, 256, lds, lstream>>>(
+          d_xq, d_gorig, d_type_at, d_real_mask, pruned ? d_rows_in : d_rows,
+          pruned ? d_row_count_in : d_row_count, d_ex, g.row_cap, seg_len, max_groups, d_counts,
+          d_lj, nbp, d_F, d_epair, d_flags, sa, fs);
   }
 
   template <int MOD, bool ORG>

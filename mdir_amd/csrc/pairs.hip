@@ -467,16 +467,16 @@ __device__ __forceinline__ void nb_pairs_body(
   }
 }
 
-// The pair kernel.
+// The kernels' entry: frame offsets, then the body.
 template <int MOD, bool COUL, bool ENERGY, bool ORG>
-__global__ void __launch_bounds__(256) nb_pairs(
+__device__ __forceinline__ void nb_pairs_entry(
     const float4 *__restrict__ xq, const float4 *__restrict__ gorig,
     const int *__restrict__ type_at,
     const uint32_t *__restrict__ real_mask, const PairEntry *__restrict__ rows,
     const int *__restrict__ row_count, const uint64_t *__restrict__ ex_masks,
     int row_cap, int seg_len, int max_groups, const int *__restrict__ counts_in,
-    const float2 *__restrict__ ljtab, NbParams p, long long *__restrict__ F,
-    double *__restrict__ energy_partial, int *__restrict__ flags, SpecialArgs sa,
+    const float2 *__restrict__ ljtab, const NbParams &p, long long *__restrict__ F,
+    double *__restrict__ energy_partial, int *__restrict__ flags, SpecialArgs &sa,
     size_t fs) {
   xq = frame_at(xq, fs);
   gorig = frame_at(gorig, fs);
@@ -497,6 +497,39 @@ __global__ void __launch_bounds__(256) nb_pairs(
                                    sa);
 }
 
+#define MDHIP_PAIRS_PARAMS                                                                  \
+  const float4 *__restrict__ xq, const float4 *__restrict__ gorig,                          \
+      const int *__restrict__ type_at, const uint32_t *__restrict__ real_mask,              \
+      const PairEntry *__restrict__ rows, const int *__restrict__ row_count,                \
+      const uint64_t *__restrict__ ex_masks, int row_cap, int seg_len, int max_groups,      \
+      const int *__restrict__ counts_in, const float2 *__restrict__ ljtab, NbParams p,      \
+      long long *__restrict__ F, double *__restrict__ energy_partial,                       \
+      int *__restrict__ flags, SpecialArgs sa, size_t fs
+#define MDHIP_PAIRS_ARGS                                                                    \
+  xq, gorig, type_at, real_mask, rows, row_count, ex_masks, row_cap, seg_len, max_groups,   \
+      counts_in, ljtab, p, F, energy_partial, flags, sa, fs
+
+// The pair kernel, positions about the box centre (ORG false; boxes below
+// 4 nm of half edge, and frame batches).
+template <int MOD, bool COUL, bool ENERGY, bool ORG>
+__global__ void __launch_bounds__(256) nb_pairs(MDHIP_PAIRS_PARAMS) {
+  static_assert(!ORG, "boxes with group origins take nb_pairs_org");
+  nb_pairs_entry<MOD, COUL, ENERGY, false>(MDHIP_PAIRS_ARGS);
+}
+
+// The pair kernel with positions relative to group origins: boxes of 4 nm
+// of half edge or more.  There the pair loop is the step's critical path by
+// far, so it runs with a 96-VGPR limit (five waves per SIMD instead of
+// four) to hide more of its latency: 1.51 against 1.57 ms in place at
+// 999,000 atoms.  In the smaller boxes the same limit speeds the kernel up
+// too, but the PME chain beside it loses the CUs it takes and the step gets
+// slower.
+template <int MOD, bool COUL, bool ENERGY>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
+nb_pairs_org(MDHIP_PAIRS_PARAMS) {
+  nb_pairs_entry<MOD, COUL, ENERGY, true>(MDHIP_PAIRS_ARGS);
+}
+
 // One device-clock reading (wall_clock64), stored by lane 0 with a vector
 // store: launched on a stream before and after a kernel, the two readings
 // bracket it inside captured graphs, where events give no elapsed times.
@@ -507,13 +540,14 @@ __global__ void stamp_clock(unsigned long long *__restrict__ slot) {
 
 // Instantiations used by the engine.
 #define MDHIP_PAIRS_ONE(MOD, C, E)                                                       \
-  MDHIP_PAIRS_ORG(MOD, C, E, false)                                                      \
-  MDHIP_PAIRS_ORG(MOD, C, E, true)
-#define MDHIP_PAIRS_ORG(MOD, C, E, O)                                                    \
-  template __global__ void nb_pairs<MOD, C, E, O>(const float4 *, const float4 *, const int *, \
-      const uint32_t *, const PairEntry *, const int *, const uint64_t *, int, int, int, \
-      const int *, const float2 *, NbParams, long long *, double *, int *, SpecialArgs,     \
-      size_t);
+  template __global__ void nb_pairs<MOD, C, E, false>(const float4 *, const float4 *,    \
+      const int *, const uint32_t *, const PairEntry *, const int *, const uint64_t *, int, \
+      int, int, const int *, const float2 *, NbParams, long long *, double *, int *,        \
+      SpecialArgs, size_t);                                                              \
+  template __global__ void nb_pairs_org<MOD, C, E>(const float4 *, const float4 *,       \
+      const int *, const uint32_t *, const PairEntry *, const int *, const uint64_t *, int, \
+      int, int, const int *, const float2 *, NbParams, long long *, double *, int *,        \
+      SpecialArgs, size_t);
 #define MDHIP_PAIRS(MOD)                                                                 \
   MDHIP_PAIRS_ONE(MOD, true, false)                                                      \
   MDHIP_PAIRS_ONE(MOD, true, true)                                                       \
