@@ -1399,11 +1399,21 @@ struct Engine {
            pme_grid[0] <= kPlaneAxisMax && pme_grid[1] <= kPlaneAxisMax;
   }
   static constexpr int kPlaneMax = 4096;
-  int recip_blocks() const { return plane_fft() ? fft.H() : fft.x_blocks(); }
-  static bool mfma_edge(int k) { return k == 32 || k == 64; }
-  bool mfma_dft() const {
-    return pme_transform == 1 && plane_fft() && mfma_edge(pme_grid[0]) && mfma_edge(pme_grid[1]);
+  int recip_blocks() const {
+    if (plane_fft()) return fft.H();
+    return mfma_axis() ? pme_grid[1] * ((fft.H() + 31) / 32) : fft.x_blocks();
   }
+  static bool mfma_edge(int k) { return k == 32 || k == 64; }
+  // The MFMA products take planes of edges 32 or 64 held in LDS, and, on
+  // grids whose planes do not fit, x and y edges up to kMfmaAxisMax.
+  bool mfma_possible() const {
+    if (!use_pme) return false;
+    if (plane_fft()) return mfma_edge(pme_grid[0]) && mfma_edge(pme_grid[1]);
+    return pme_grid[0] <= kMfmaAxisMax && pme_grid[1] <= kMfmaAxisMax;
+  }
+  bool mfma_dft() const { return pme_transform == 1 && plane_fft() && mfma_possible(); }
+  bool mfma_axis() const { return pme_transform == 1 && !plane_fft() && mfma_possible(); }
+  static int pad32(int k) { return (k + 31) / 32 * 32; }
 
   // The DFT matrix W_K[n][k] = exp(-2 pi i n k / K) in three bf16 pieces
   // per part (h + m + l = W to 2^-24), laid out as fft_plane_conv_mfma reads
@@ -1420,8 +1430,10 @@ struct Engine {
     memcpy(&v, &u, 4);
     return v;
   }
-  static __bf16 *dft_table(int K) {
-    std::vector<uint16_t> t((size_t)6 * K * K);
+  // Kp >= K: the table padded to Kp x Kp with zeros beyond K (the layout
+  // of dft_axis_mfma; Kp = K for the plane kernel).
+  static __bf16 *dft_table(int K, int Kp) {
+    std::vector<uint16_t> t((size_t)6 * Kp * Kp, 0);
     for (int n = 0; n < K; ++n)
       for (int k = 0; k < K; ++k) {
         const long nk = ((long)n * k) % K;
@@ -1431,7 +1443,7 @@ struct Engine {
           float r = part[c];
           for (int p = 0; p < 3; ++p) {
             const uint16_t b = bf16_bits(r);
-            t[((size_t)(p * 2 + c) * K + n) * K + k] = b;
+            t[((size_t)(p * 2 + c) * Kp + n) * Kp + k] = b;
             r -= bf16_value(b);
           }
         }
@@ -1443,11 +1455,14 @@ struct Engine {
 
   void set_pme_transform(int mode) {
     if (mode != 0 && mode != 1) throw HipError("PME transform: 0 (FFT) or 1 (MFMA DFT)");
-    if (mode == 1 && !(use_pme && plane_fft() && mfma_edge(pme_grid[0]) && mfma_edge(pme_grid[1])))
-      throw HipError("the MFMA DFT takes PME grids with x and y edges of 32 or 64");
+    if (mode == 1 && !mfma_possible())
+      throw HipError("the MFMA DFT takes PME grids with x and y edges of 32 or 64 when a plane "
+                     "fits in LDS (at most 4096 points), else edges of at most 224");
     if (mode == 1 && !d_dft1) {
-      d_dft1 = dft_table(pme_grid[0]);
-      d_dft2 = pme_grid[1] == pme_grid[0] ? nullptr : dft_table(pme_grid[1]);
+      const bool pad = !plane_fft();
+      const int k0 = pme_grid[0], k1 = pme_grid[1];
+      d_dft1 = dft_table(k0, pad ? pad32(k0) : k0);
+      d_dft2 = k1 == k0 ? nullptr : dft_table(k1, pad ? pad32(k1) : k1);
     }
     pme_transform = mode;
     invalidate_graph();
@@ -1484,10 +1499,42 @@ struct Engine {
         data, fft.K[0], fft.K[1], fft.ax[0].axis, fft.ax[1].axis, ct, ep, vp, fs);
   }
 
+  // The y transforms (dft sign +1 forward, -1 backward) and the x pass of
+  // the convolution as MFMA DFT products, on grids whose planes do not fit.
+  void axis_mfma(int axis, float sign, double *ep) {
+    const int K = pme_grid[axis == 1 ? 1 : 0], Kp = pad32(K);
+    const size_t lds = axis_mfma_lds(Kp);
+    static bool attr = false;
+    if (!attr) {
+      HIPCHK(hipFuncSetAttribute((const void *)dft_axis_mfma<1>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)axis_mfma_lds(kMfmaAxisMax)));
+      HIPCHK(hipFuncSetAttribute((const void *)dft_axis_mfma<0>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)axis_mfma_lds(kMfmaAxisMax)));
+      attr = true;
+    }
+    const __bf16 *tab = axis == 1 && d_dft2 ? d_dft2 : d_dft1;
+    const int tiles = (fft.H() + 31) / 32;
+    const int lines = axis == 1 ? pme_grid[0] : pme_grid[1];
+    if (axis == 1)
+      dft_axis_mfma<1><<<dim3(lines * tiles, 1, nz), 512, lds, pstream>>>(
+          d_cplx, pme_grid[0], pme_grid[1], fft.H(), Kp, tab, ct, sign, nullptr, fs);
+    else
+      dft_axis_mfma<0><<<dim3(lines * tiles, 1, nz), 512, lds, pstream>>>(
+          d_cplx, pme_grid[0], pme_grid[1], fft.H(), Kp, tab, ct, 1.0f, ep, fs);
+  }
+
   void pme_yx(bool energy) {
     double *ep = energy ? d_erecip : nullptr;
     if (plane_fft()) {
       plane_conv(d_cplx, ep, energy ? d_vrecip : nullptr);
+      return;
+    }
+    if (mfma_axis()) {
+      axis_mfma(1, 1.0f, nullptr);
+      axis_mfma(0, 1.0f, ep);
+      axis_mfma(1, -1.0f, nullptr);
       return;
     }
     const int TL = Fft3D<float>::TL;
@@ -2025,8 +2072,7 @@ struct Engine {
       forces_valid = false;
     };
     if (!use_pme) what &= 4;  // no PME shapes to try
-    const bool mfma_ok =
-        use_pme && plane_fft() && mfma_edge(pme_grid[0]) && mfma_edge(pme_grid[1]);
+    const bool mfma_ok = mfma_possible();
     const int tl0 = use_pme ? brick_tl : 8, tr0 = pme_transform;
     const Shape cur{seg_index(seg_len), tr0, tl0 == 8 ? 0 : (tl0 == 4 ? 1 : (tl0 == 2 ? 2 : 3))};
     auto apply = [&](const Shape &c) {

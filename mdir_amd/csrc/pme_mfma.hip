@@ -1,7 +1,8 @@
 // pme_mfma.hip — the y and x transforms of a PME plane as DFT matrix
 // products on the matrix cores (gfx950 MFMA), in split BF16x3.
 //
-// For grid edges K1, K2 in {32, 64}: per kz plane of the half spectrum,
+// For grid edges K1, K2 in {32, 64} (planes in LDS): per kz plane of the
+// half spectrum,
 //   y pass    Y = X W2        (lines x, transform along y)
 //   x pass    Z = W1 Y        (computed as Z^T = Y^T W1, W symmetric)
 //   influence Z *= BC, energy and virial as fft_plane_conv
@@ -269,6 +270,150 @@ __global__ void __launch_bounds__(512) fft_plane_conv_mfma(
       }
       if (threadIdx.x == 0) virial_partial[6 * blockIdx.x + c] = s_e[0];
     }
+  }
+}
+
+// ---- general axis lengths, grids whose planes do not fit LDS ------------
+//
+// The half spectrum of these grids is [x][y][kz] (kz fastest, as the
+// Stockham column passes take it).  A block takes 32 consecutive kz of one
+// set of lines: all y of one x for the y pass, all x of one y for the x
+// pass.  Its 32 lines, padded with zeros to Kp (K rounded up to 32), are
+// the rows of an f32 plane in LDS; their bf16 pieces are the A operand, and
+// the DFT table, Kp x Kp in the same piece layout with zeros beyond K, the B
+// operand read from global memory (it is shared by every block).
+constexpr int kMfmaAxisMax = 224;
+
+// out[m][n] = sum_k in[m][k] W^(s)[k][n] for the 32 rows m and n < Kp: the
+// same 32 x 32 tiles as dft_pass, with Kp known at run time.
+__device__ void dft_rows32(const __bf16 *pieces, int ldp, float *out_re, float *out_im, int ldo,
+                           int Kp, const __bf16 *tab, float sign) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  const int ntasks = (Kp / 32) * 2;
+  const size_t ps = (size_t)32 * ldp;
+  for (int task = wave; task < ntasks; task += nwaves) {
+    const int part = task & 1, tn = (task >> 1) * 32;
+    const int row = lane & 31;
+    floatx16 acc = {};
+    for (int k0 = 0; k0 < Kp; k0 += 16) {
+      const int k = k0 + 8 * (lane >> 5);
+      bf16x8 ah = load8(pieces + 0 * ps, ldp, row, k);
+      bf16x8 am = load8(pieces + 2 * ps, ldp, row, k);
+      bf16x8 al = load8(pieces + 4 * ps, ldp, row, k);
+      if (part == 1 && sign < 0.0f) {
+        ah = negate8(ah);
+        am = negate8(am);
+        al = negate8(al);
+      }
+      acc = mfma6(ah, am, al, b_fragment(tab, Kp, 0, part, tn, k0, lane),
+                  b_fragment(tab, Kp, 1, part, tn, k0, lane),
+                  b_fragment(tab, Kp, 2, part, tn, k0, lane), acc);
+      ah = load8(pieces + 1 * ps, ldp, row, k);
+      am = load8(pieces + 3 * ps, ldp, row, k);
+      al = load8(pieces + 5 * ps, ldp, row, k);
+      if (part == 0 && sign > 0.0f) {
+        ah = negate8(ah);
+        am = negate8(am);
+        al = negate8(al);
+      }
+      acc = mfma6(ah, am, al, b_fragment(tab, Kp, 0, 1 - part, tn, k0, lane),
+                  b_fragment(tab, Kp, 1, 1 - part, tn, k0, lane),
+                  b_fragment(tab, Kp, 2, 1 - part, tn, k0, lane), acc);
+    }
+    float *out = part == 0 ? out_re : out_im;
+    const int col = tn + (lane & 31);
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int r0 = 8 * g + 4 * (lane >> 5);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) out[(size_t)(r0 + i) * ldo + col] = acc[4 * g + i];
+    }
+  }
+}
+
+// LDS bytes of dft_axis_mfma for a padded length Kp: the f32 plane of 32
+// rows (re, im; stride Kp + 4) and the six bf16 piece planes (stride Kp + 8).
+inline size_t axis_mfma_lds(int Kp) {
+  return (size_t)2 * 32 * (Kp + 4) * sizeof(float) + (size_t)6 * 32 * (Kp + 8) * 2;
+}
+
+// AXIS 1: the y transform of every line (dft sign +1 forward, -1
+// backward).  AXIS 0: the x pass of the convolution: forward x transform,
+// the influence product (and its energy, one partial per block), backward
+// x transform.  Grid: (lines of the other axis) x ceil(H / 32), frames in z.
+template <int AXIS>
+__global__ void __launch_bounds__(512) dft_axis_mfma(cpx<float> *__restrict__ data, int K1,
+                                                     int K2, int H, int Kp,
+                                                     const __bf16 *__restrict__ tab,
+                                                     ConvTables ct, float sign,
+                                                     double *__restrict__ energy_partial,
+                                                     size_t fs) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  __shared__ double s_e[512];
+  data = frame_at(data, fs);
+  energy_partial = frame_at(energy_partial, fs);
+  const int K = AXIS == 1 ? K2 : K1;
+  const int P = Kp + 4, Q = Kp + 8;
+  float *f_re = reinterpret_cast<float *>(smem);
+  float *f_im = f_re + 32 * P;
+  __bf16 *pieces = reinterpret_cast<__bf16 *>(f_im + 32 * P);
+  const int tiles = (H + 31) / 32;
+  const int line = blockIdx.x / tiles, z0 = (blockIdx.x % tiles) * 32;
+  const size_t stride = AXIS == 1 ? (size_t)H : (size_t)K2 * H;
+  cpx<float> *base = AXIS == 1 ? data + (size_t)line * K2 * H + z0 : data + (size_t)line * H + z0;
+  // row t (kz = z0 + t), column k (the transformed axis), zero beyond
+  for (int w = threadIdx.x; w < 32 * Kp; w += blockDim.x) {
+    const int t = w & 31, k = w >> 5;
+    cpx<float> v = {0.0f, 0.0f};
+    if (k < K && z0 + t < H) v = base[(size_t)k * stride + t];
+    f_re[t * P + k] = v.x;
+    f_im[t * P + k] = v.y;
+  }
+  __syncthreads();
+  split_plane(f_re, f_im, P, 32, Kp, pieces, Q);
+  __syncthreads();
+  dft_rows32(pieces, Q, f_re, f_im, P, Kp, tab, AXIS == 1 ? sign : 1.0f);
+  __syncthreads();
+  double e = 0.0;
+  if (AXIS == 0) {
+    const float *t0 = (const float *)ct.t[0], *t1 = (const float *)ct.t[1],
+                *t2 = (const float *)ct.t[2];
+    const float *m0 = (const float *)ct.m[0], *m1 = (const float *)ct.m[1],
+                *m2 = (const float *)ct.m[2];
+    for (int w = threadIdx.x; w < 32 * Kp; w += blockDim.x) {
+      const int t = w & 31, kx = w >> 5, kz = z0 + t;
+      const int i = t * P + kx;
+      const float vr = f_re[i], vi = f_im[i];
+      float bc = 0.0f;
+      if (kx < K1 && kz < H && (kx | line | kz) != 0) {
+        const float msq = m0[kx] * m0[kx] + m1[line] * m1[line] + m2[kz] * m2[kz];
+        bc = t0[kx] * t1[line] * t2[kz] / msq;
+        if (energy_partial) {
+          const double h = (kz == 0 || (ct.K3 % 2 == 0 && kz == ct.K3 / 2)) ? 0.5 : 1.0;
+          e += h * (double)bc * ((double)vr * vr + (double)vi * vi);
+        }
+      }
+      f_re[i] = vr * bc;
+      f_im[i] = vi * bc;
+    }
+    __syncthreads();
+    split_plane(f_re, f_im, P, 32, Kp, pieces, Q);
+    __syncthreads();
+    dft_rows32(pieces, Q, f_re, f_im, P, Kp, tab, -1.0f);
+    __syncthreads();
+  }
+  for (int w = threadIdx.x; w < 32 * K; w += blockDim.x) {
+    const int t = w & 31, k = w >> 5;
+    if (z0 + t < H) base[(size_t)k * stride + t] = {f_re[t * P + k], f_im[t * P + k]};
+  }
+  if (AXIS == 0 && energy_partial) {
+    s_e[threadIdx.x] = e;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) s_e[threadIdx.x] += s_e[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) energy_partial[blockIdx.x] = s_e[0];
   }
 }
 

@@ -1,6 +1,7 @@
-"""The MFMA plane transform (DFT matrix products in split BF16x3 on the
-matrix cores) against the FP32 Stockham FFT of the same library and the
-FP64 oracle.
+"""The MFMA transforms (DFT matrix products in split BF16x3 on the matrix
+cores: whole planes in LDS for edges of 32 and 64, tiles of lines for the
+other grids) against the FP32 Stockham FFT of the same library and the FP64
+oracle.
 
 Bars: the reciprocal forces of the two transforms within 3e-5 of the
 largest (the reference's mixed-precision bar for its axis transforms,
@@ -43,6 +44,46 @@ def test_mfma_transform_forces(case):
     say(f"  total rel rms force error {err:.2e}; reciprocal term {errr:.2e}")
     assert err < 5e-6
     assert abs(e[O.E_COUL_RECIP] - er) <= 2e-6 * abs(er)
+
+
+@pytest.mark.parametrize("case", [("tip3p 2000, 72^3", (72, 72, 72), True),
+                                  ("tip3p 2000, 80 x 96 x 72", (80, 96, 72), True),
+                                  ("tip3p 2000, 216^3", (216, 216, 216), False)])
+def test_mfma_axis_transform_forces(case):
+    """Grids whose planes do not fit in LDS: the y and x DFTs as MFMA
+    products over tiles of 32 kz lines, the lines padded to a multiple of 32
+    (72 -> 96, 80 -> 96, 216 -> 224: 216 = 2^3 3^3, the 999,000-atom box's
+    edge), against the Stockham FFT; with the FP64 oracle where it runs in
+    seconds."""
+    from mdir_amd.simulation import Simulation
+    name, grid, oracle = case
+    say(f"case: MFMA axis transform, {name}")
+    w = S.tip3p_box(2000, 3.9, seed=31, grid=grid)
+    sim = Simulation(w)
+    Ff, ef = sim.force_terms(rt.TERM_RECIP)
+    sim.set_pme_transform(1)
+    Fm, em = sim.force_terms(rt.TERM_RECIP)
+    d = np.abs(Fm - Ff).max() / np.abs(Ff).max()
+    say(f"  reciprocal forces, MFMA vs FFT: max difference / largest {d:.2e}; "
+        f"energy {em[O.E_COUL_RECIP]:.6f} vs {ef[O.E_COUL_RECIP]:.6f}")
+    assert d < 3e-5
+    assert abs(em[O.E_COUL_RECIP] - ef[O.E_COUL_RECIP]) <= 2e-6 * abs(ef[O.E_COUL_RECIP])
+    if oracle:
+        F, e = sim.forces()
+        Fo, eo = O.Oracle(w).total()
+        err = O.rel_rms(F, Fo)
+        Fr, er = O.Oracle(w).pme()
+        say(f"  total rel rms force error {err:.2e}; reciprocal term {O.rel_rms(Fm, Fr):.2e}")
+        assert err < 5e-6
+        assert abs(e[O.E_COUL_RECIP] - er) <= 2e-6 * abs(er)
+
+
+def test_mfma_transform_refuses_wide_axes():
+    from mdir_amd.simulation import Simulation
+    w = S.tip3p_box(512, 2.5, seed=3, grid=(48, 240, 48))
+    sim = Simulation(w)
+    with pytest.raises(RuntimeError, match="MFMA DFT"):
+        sim.set_pme_transform(1)
 
 
 def test_mfma_transform_frames():

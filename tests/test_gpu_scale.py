@@ -91,6 +91,18 @@ def test_million_atom_box():
     for k in (O.E_LJ, O.E_COUL_DIRECT):
         say(f"  energy term {k}: {e[k]:.4f} vs {eo[k]:.4f}")
         assert abs(e[k] - eo[k]) <= 2e-6 * abs(eo[k]) + 5e-7 * w.n
+    # the split-BF16x3 MFMA DFT variant of the y and x transforms (216 padded
+    # to 224) against the FP32 FFT and the FP64 oracle
+    Ff, ef = sim.force_terms(rt.TERM_RECIP)
+    sim.set_pme_transform(1)
+    Fm, em = sim.force_terms(rt.TERM_RECIP)
+    sim.set_pme_transform(0)
+    d = np.abs(Fm - Ff).max() / np.abs(Ff).max()
+    err = O.rel_rms(Fm, Fr)
+    say(f"  MFMA DFT: reciprocal forces vs FFT, max difference / largest {d:.2e}; "
+        f"rel rms error vs FP64 {err:.2e}; energy {em[O.E_COUL_RECIP]:.4f} vs {er:.4f}")
+    assert d < 3e-5 and err < 5e-6
+    assert abs(em[O.E_COUL_RECIP] - er) <= 2e-6 * abs(er)
     net = np.abs(F.sum(axis=0)).max() / np.sqrt((F * F).sum(axis=1).mean())
     say(f"  net force / rms atom force {net:.3e}")
     assert net < 1e-3 * np.sqrt(w.n)
