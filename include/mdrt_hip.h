@@ -320,9 +320,10 @@ int mdrtHipComputeForceTerms(void *system, int32_t terms, double *forces, double
    default), 1 = DFT matrix products on the matrix cores in split BF16x3
    (f32 operands as three bf16 pieces, six MFMA products per real product,
    f32 accumulation): whole planes in LDS for plane edges of 32 or 64, and
-   on grids whose planes do not fit in LDS (more than 4096 points) tiles of
-   32 lines padded to a multiple of 32, x and y edges up to 224; other
-   grids are refused.  Replaces the reference's cuFFT plan choice
+   for any other x and y edges up to 224 tiles of 32 lines padded to a
+   multiple of 32 (the spread and gather then write and read the half
+   spectrum as [x][y][kz]); wider grids are refused, and frame batches take
+   edges of 32 or 64 only.  Replaces the reference's cuFFT plan choice
    (mdrt_cuda.c:2646-2991) and axis transform (AxisFFTEmitter.cpp:305) with
    this library's own. */
 int mdrtHipSetPmeTransform(void *system, int32_t mode);

@@ -1394,25 +1394,30 @@ struct Engine {
   // The y and x passes with the influence product: one kernel per kz plane
   // when a plane fits in LDS, else the y pass, the fused x pass and the y
   // pass again over tiles of lines.
-  bool plane_fft() const {
+  bool plane_fits() const {
     return use_pme && pme_grid[0] * pme_grid[1] <= kPlaneMax &&
            pme_grid[0] <= kPlaneAxisMax && pme_grid[1] <= kPlaneAxisMax;
   }
+  // The plane kernels run when a plane fits, except the MFMA products of
+  // edges other than 32 and 64, which take the tiles of lines (and with
+  // them the [x][y][kz] half spectrum the spread and gather then write).
+  bool plane_fft() const { return plane_fits() && !(pme_transform == 1 && !mfma_plane_edges()); }
   static constexpr int kPlaneMax = 4096;
   int recip_blocks() const {
     if (plane_fft()) return fft.H();
     return mfma_axis() ? pme_grid[1] * ((fft.H() + 31) / 32) : fft.x_blocks();
   }
   static bool mfma_edge(int k) { return k == 32 || k == 64; }
-  // The MFMA products take planes of edges 32 or 64 held in LDS, and, on
-  // grids whose planes do not fit, x and y edges up to kMfmaAxisMax.
+  bool mfma_plane_edges() const { return mfma_edge(pme_grid[0]) && mfma_edge(pme_grid[1]); }
+  // The MFMA products take planes of edges 32 or 64 held in LDS, and any
+  // other x and y edges up to kMfmaAxisMax over tiles of lines.
   bool mfma_possible() const {
     if (!use_pme) return false;
-    if (plane_fft()) return mfma_edge(pme_grid[0]) && mfma_edge(pme_grid[1]);
+    if (plane_fits() && mfma_plane_edges()) return true;
     return pme_grid[0] <= kMfmaAxisMax && pme_grid[1] <= kMfmaAxisMax;
   }
-  bool mfma_dft() const { return pme_transform == 1 && plane_fft() && mfma_possible(); }
-  bool mfma_axis() const { return pme_transform == 1 && !plane_fft() && mfma_possible(); }
+  bool mfma_dft() const { return pme_transform == 1 && plane_fft(); }
+  bool mfma_axis() const { return pme_transform == 1 && !plane_fft(); }
   static int pad32(int k) { return (k + 31) / 32 * 32; }
 
   // The DFT matrix W_K[n][k] = exp(-2 pi i n k / K) in three bf16 pieces
@@ -1456,15 +1461,18 @@ struct Engine {
   void set_pme_transform(int mode) {
     if (mode != 0 && mode != 1) throw HipError("PME transform: 0 (FFT) or 1 (MFMA DFT)");
     if (mode == 1 && !mfma_possible())
-      throw HipError("the MFMA DFT takes PME grids with x and y edges of 32 or 64 when a plane "
-                     "fits in LDS (at most 4096 points), else edges of at most 224");
+      throw HipError("the MFMA DFT takes PME grids with x and y edges of at most 224");
+    if (mode == 1 && nz > 1 && !mfma_plane_edges())
+      throw HipError("the MFMA DFT of frame batches takes x and y edges of 32 or 64");
     if (mode == 1 && !d_dft1) {
-      const bool pad = !plane_fft();
+      const bool pad = !(plane_fits() && mfma_plane_edges());
       const int k0 = pme_grid[0], k1 = pme_grid[1];
       d_dft1 = dft_table(k0, pad ? pad32(k0) : k0);
       d_dft2 = k1 == k0 ? nullptr : dft_table(k1, pad ? pad32(k1) : k1);
     }
+    const bool layout = plane_fft();
     pme_transform = mode;
+    if (use_pme && plane_fft() != layout) make_bricks();  // the spread's half-spectrum layout
     invalidate_graph();
   }
 

@@ -115,8 +115,8 @@ class Simulation:
 
     def set_pme_transform(self, mode: int):
         """0: the LDS Stockham FFT; 1: DFT matrix products on MFMA in split
-        BF16x3 (planes of edges 32 or 64 in LDS; on grids whose planes do not
-        fit, x and y edges up to 224)."""
+        BF16x3 (planes of edges 32 or 64 in LDS; other x and y edges up to
+        224 over tiles of lines)."""
         rt.check(rt.lib().mdrtHipSetPmeTransform(self._h, int(mode)))
 
     TUNE_BRICKS = 1

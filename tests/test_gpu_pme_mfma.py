@@ -48,13 +48,16 @@ def test_mfma_transform_forces(case):
 
 @pytest.mark.parametrize("case", [("tip3p 2000, 72^3", (72, 72, 72), True),
                                   ("tip3p 2000, 80 x 96 x 72", (80, 96, 72), True),
-                                  ("tip3p 2000, 216^3", (216, 216, 216), False)])
+                                  ("tip3p 2000, 216^3", (216, 216, 216), False),
+                                  ("tip3p 2000, 48^3", (48, 48, 48), True),
+                                  ("tip3p 2000, 36 x 56 x 40", (36, 56, 40), True)])
 def test_mfma_axis_transform_forces(case):
-    """Grids whose planes do not fit in LDS: the y and x DFTs as MFMA
-    products over tiles of 32 kz lines, the lines padded to a multiple of 32
-    (72 -> 96, 80 -> 96, 216 -> 224: 216 = 2^3 3^3, the 999,000-atom box's
-    edge), against the Stockham FFT; with the FP64 oracle where it runs in
-    seconds."""
+    """Edges other than 32 and 64: the y and x DFTs as MFMA products over
+    tiles of 32 kz lines, the lines padded to a multiple of 32 (72 -> 96,
+    80 -> 96, 216 -> 224: 216 = 2^3 3^3, the 999,000-atom box's edge; 48,
+    36, 56, 40 on grids whose planes would fit in LDS, which the variant
+    switches to the tiles' layout), against the Stockham FFT; with the FP64
+    oracle where it runs in seconds."""
     from mdir_amd.simulation import Simulation
     name, grid, oracle = case
     say(f"case: MFMA axis transform, {name}")
@@ -84,6 +87,22 @@ def test_mfma_transform_refuses_wide_axes():
     sim = Simulation(w)
     with pytest.raises(RuntimeError, match="MFMA DFT"):
         sim.set_pme_transform(1)
+
+
+def test_mfma_layout_switch_and_back():
+    """A 48^3 grid: the plane kernel (FFT), the MFMA tiles (another
+    half-spectrum layout for the spread and gather), and the FFT again give
+    the same reciprocal forces as before the switches, to the bit."""
+    from mdir_amd.simulation import Simulation
+    w = S.tip3p_box(512, 2.5, seed=4, grid=(48, 48, 48))
+    sim = Simulation(w)
+    F0, e0 = sim.force_terms(rt.TERM_RECIP)
+    sim.set_pme_transform(1)
+    F1, _ = sim.force_terms(rt.TERM_RECIP)
+    sim.set_pme_transform(0)
+    F2, e2 = sim.force_terms(rt.TERM_RECIP)
+    assert np.abs(F1 - F0).max() < 3e-5 * np.abs(F0).max()
+    assert np.array_equal(F2, F0) and e2[O.E_COUL_RECIP] == e0[O.E_COUL_RECIP]
 
 
 def test_mfma_transform_frames():
