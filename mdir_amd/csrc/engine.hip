@@ -1503,8 +1503,16 @@ struct Engine {
       return;
     }
     const size_t lds = (size_t)2 * pme_grid[0] * pme_grid[1] * sizeof(cpx<float>);
-    fft_plane_conv<<<dim3(fft.H(), 1, nz), 512, lds, pstream>>>(
-        data, fft.K[0], fft.K[1], fft.ax[0].axis, fft.ax[1].axis, ct, ep, vp, fs);
+    const int k0 = pme_grid[0], k1 = pme_grid[1];
+    if (k0 == 64 && k1 == 64)
+      fft_plane_conv<64, 64><<<dim3(fft.H(), 1, nz), 512, lds, pstream>>>(
+          data, k0, k1, fft.ax[0].axis, fft.ax[1].axis, ct, ep, vp, fs);
+    else if (k0 == 32 && k1 == 32)
+      fft_plane_conv<32, 32><<<dim3(fft.H(), 1, nz), 512, lds, pstream>>>(
+          data, k0, k1, fft.ax[0].axis, fft.ax[1].axis, ct, ep, vp, fs);
+    else
+      fft_plane_conv<0, 0><<<dim3(fft.H(), 1, nz), 512, lds, pstream>>>(
+          data, k0, k1, fft.ax[0].axis, fft.ax[1].axis, ct, ep, vp, fs);
   }
 
   // The y transforms (dft sign +1 forward, -1 backward) and the x pass of

@@ -238,4 +238,67 @@ __device__ cpx<T> *lds_fft_plane(cpx<T> *a, cpx<T> *b, const FftAxis &ax, int si
   return in;
 }
 
+// The same passes with the plane's edges known at compile time (the grids
+// the bench and its configs use): every index division is by a constant,
+// the radix order is the host's (8s, then 4, 2, 3s, 5s, 7s), and the pass
+// loop unrolls.
+constexpr int next_radix(int rest) {
+  return rest % 8 == 0 ? 8 : rest % 4 == 0 ? 4 : rest % 2 == 0 ? 2 : rest % 3 == 0 ? 3
+                                                                 : rest % 5 == 0 ? 5 : 7;
+}
+
+template <typename T, int R, int AXIS, int K1, int K2, int NS>
+__device__ __forceinline__ void stockham_pass_plane_k(const cpx<T> *in, cpx<T> *out, int swz,
+                                                      const cpx<T> *tw, int sign) {
+  constexpr int K = AXIS == 1 ? K2 : K1, nlines = AXIS == 1 ? K1 : K2;
+  constexpr int Kr = K / R;
+  constexpr int tstep = K / (NS * R);
+  for (int w = threadIdx.x; w < nlines * Kr; w += blockDim.x) {
+    int line, j;
+    if (AXIS == 1) {
+      j = w % Kr;
+      line = w / Kr;
+    } else {
+      line = w % nlines;
+      j = w / nlines;
+    }
+    cpx<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int e = j + r * Kr;
+      v[r] = in[AXIS == 1 ? plane_index(line, e, K2, swz) : plane_index(e, line, K2, swz)];
+    }
+    const int k = j % NS;
+    if (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        cpx<T> t = tw[k * r * tstep];
+        if (sign > 0) t.y = -t.y;
+        v[r] = cmul(v[r], t);
+      }
+    }
+    butterfly<T, R>(v, sign);
+    const int base = (j / NS) * NS * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int e = base + r * NS;
+      out[AXIS == 1 ? plane_index(line, e, K2, swz) : plane_index(e, line, K2, swz)] = v[r];
+    }
+  }
+}
+
+template <typename T, int AXIS, int K1, int K2, int NS = 1>
+__device__ __forceinline__ cpx<T> *lds_fft_plane_k(cpx<T> *in, cpx<T> *out, int sign, int swz,
+                                                   const cpx<T> *tw) {
+  constexpr int K = AXIS == 1 ? K2 : K1;
+  if constexpr (NS >= K) {
+    return in;
+  } else {
+    constexpr int R = next_radix(K / NS);
+    stockham_pass_plane_k<T, R, AXIS, K1, K2, NS>(in, out, swz, tw, sign);
+    __syncthreads();
+    return lds_fft_plane_k<T, AXIS, K1, K2, NS * R>(out, in, sign, swz, tw);
+  }
+}
+
 }  // namespace mdhip

@@ -659,12 +659,19 @@ constexpr int kPlaneAxisMax = 256;
 // influence product (and its energy), backward x, backward y.  The half
 // spectrum is kz-plane major ([kz][x][y]), so a plane is one contiguous read
 // and write.
+// CK1, CK2: the plane's edges at compile time (0: the run-time K1, K2 and
+// the axes' radix lists).
+template <int CK1, int CK2>
 __global__ void __launch_bounds__(512) fft_plane_conv(cpx<float> *__restrict__ data, int K1,
                                                       int K2, FftAxis ax, FftAxis ay,
                                                       ConvTables ct,
                                                       double *__restrict__ energy_partial,
                                                       double *__restrict__ virial_partial,
                                                       size_t fs) {
+  if (CK1 > 0) {
+    K1 = CK1;
+    K2 = CK2;
+  }
   extern __shared__ unsigned char smem[];
   __shared__ double s_e[512];
   data = frame_at(data, fs);
@@ -695,8 +702,14 @@ __global__ void __launch_bounds__(512) fft_plane_conv(cpx<float> *__restrict__ d
     s_m1[w] = static_cast<const float *>(ct.m[1])[w];
   }
   __syncthreads();
-  cpx<float> *r = lds_fft_plane<float, 1>(a, b, ay, -1, K1, K2, swz, s_twy);
-  r = lds_fft_plane<float, 0>(r, r == a ? b : a, ax, -1, K1, K2, swz, s_twx);
+  cpx<float> *r;
+  if constexpr (CK1 > 0) {
+    r = lds_fft_plane_k<float, 1, CK1, CK2>(a, b, -1, swz, s_twy);
+    r = lds_fft_plane_k<float, 0, CK1, CK2>(r, r == a ? b : a, -1, swz, s_twx);
+  } else {
+    r = lds_fft_plane<float, 1>(a, b, ay, -1, K1, K2, swz, s_twy);
+    r = lds_fft_plane<float, 0>(r, r == a ? b : a, ax, -1, K1, K2, swz, s_twx);
+  }
   const float t2 = static_cast<const float *>(ct.t[2])[kz];
   const float m2 = static_cast<const float *>(ct.m[2])[kz];
   const double h = (kz == 0 || (ct.K3 % 2 == 0 && kz == ct.K3 / 2)) ? 0.5 : 1.0;
@@ -730,8 +743,13 @@ __global__ void __launch_bounds__(512) fft_plane_conv(cpx<float> *__restrict__ d
     r[i] = {v.x * bc, v.y * bc};
   }
   __syncthreads();
-  r = lds_fft_plane<float, 0>(r, r == a ? b : a, ax, +1, K1, K2, swz, s_twx);
-  r = lds_fft_plane<float, 1>(r, r == a ? b : a, ay, +1, K1, K2, swz, s_twy);
+  if constexpr (CK1 > 0) {
+    r = lds_fft_plane_k<float, 0, CK1, CK2>(r, r == a ? b : a, +1, swz, s_twx);
+    r = lds_fft_plane_k<float, 1, CK1, CK2>(r, r == a ? b : a, +1, swz, s_twy);
+  } else {
+    r = lds_fft_plane<float, 0>(r, r == a ? b : a, ax, +1, K1, K2, swz, s_twx);
+    r = lds_fft_plane<float, 1>(r, r == a ? b : a, ay, +1, K1, K2, swz, s_twy);
+  }
   for (int w = threadIdx.x; w < np; w += blockDim.x)
     plane[w] = r[plane_index(w / K2, w % K2, K2, swz)];
   if (energy_partial) {
