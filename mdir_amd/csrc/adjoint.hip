@@ -544,18 +544,22 @@ __global__ void __launch_bounds__(256) nb_adjoint(
       bool on = E.ex >= 0 ? ((ex_masks[(size_t)E.ex * kGroup + m] >> lane) & 1ull)
                           : (ireal[m] && jreal);
       on = on && r2 < p.rc2;
+      // The pair's arithmetic runs under the exec mask of the lanes that
+      // are on (r2 inside rc, never zero); a member tile with no such pair
+      // (about half of those of a list with a buffer) branches over it.
+      if (!on) continue;
       const float4 T4 = s_lj4[ti[m] + tj];
       const float dvx = ci[m].x - cj.x, dvy = ci[m].y - cj.y, dvz = ci[m].z - cj.z;
       const float s = dx * dvx + dy * dvy + dz * dvz;
       if (KIND == kAdjPairTable) {
         const int sel = s_sel[ti[m] + tj];
         if (sel >= 0) {  // the pairs of a selected entry only (divergent)
-          const PairParts<float> P = pair_parts<MOD, COUL, float>(on ? r2 : 1.0f, p);
+          const PairParts<float> P = pair_parts<MOD, COUL, float>(r2, p);
           const float P12 = gs * P.u12 - P.f12 * s;
           const float P6 = P.f6 * s - gs * P.u6;
           const float2 U = s_unit[ti[m] + tj];
-          const float Ss = on ? T4.z * P12 + T4.w * P6 : 0.0f;
-          const float Eu = on ? U.x * P12 + U.y * P6 : 0.0f;
+          const float Ss = T4.z * P12 + T4.w * P6;
+          const float Eu = U.x * P12 + U.y * P6;
 #pragma unroll
           for (int t = 0; t < kMaxTablePairs; ++t)
             if (sel == t) {
@@ -564,10 +568,10 @@ __global__ void __launch_bounds__(256) nb_adjoint(
             }
         }
       } else if (KIND == kAdjVirial) {
-        const PairParts<float> P = pair_parts<MOD, COUL, float>(on ? r2 : 1.0f, p);
+        const PairParts<float> P = pair_parts<MOD, COUL, float>(r2, p);
         float fsv = T4.x * P.f12 - T4.y * P.f6;
         if (COUL) fsv += xi[m].w * xj.w * P.hc;
-        const double f = on ? (double)fsv : 0.0;
+        const double f = (double)fsv;
         vir[0] += f * (double)(dx * dx);
         vir[1] += f * (double)(dy * dy);
         vir[2] += f * (double)(dz * dz);
@@ -575,13 +579,13 @@ __global__ void __launch_bounds__(256) nb_adjoint(
         vir[4] += f * (double)(dx * dz);
         vir[5] += f * (double)(dy * dz);
       } else if (KIND == kAdjTheta) {
-        const PairParts<float> P = pair_parts<MOD, COUL, float>(on ? r2 : 1.0f, p);
+        const PairParts<float> P = pair_parts<MOD, COUL, float>(r2, p);
         const float Bc = COUL ? gs * P.uc - P.hc * s : 0.0f;
         const float P12 = gs * P.u12 - P.f12 * s;
         const float P6 = P.f6 * s - gs * P.u6;
-        const float Ss = on ? T4.z * P12 + T4.w * P6 : 0.0f;
-        const float Ee = on ? T4.x * P12 + T4.y * P6 : 0.0f;
-        const float B = on ? Bc : 0.0f;
+        const float Ss = T4.z * P12 + T4.w * P6;
+        const float Ee = T4.x * P12 + T4.y * P6;
+        const float B = Bc;
         ai[m][0] += qfj * B;
         ai[m][1] += Ss;
         ai[m][2] += Ee;
@@ -589,7 +593,7 @@ __global__ void __launch_bounds__(256) nb_adjoint(
         aj1 += Ss;
         aj2 += Ee;
       } else {
-        const PairParts<Dual> P = pair_parts<MOD, COUL, Dual>(Dual{on ? r2 : 1.0f, 1.0f}, p);
+        const PairParts<Dual> P = pair_parts<MOD, COUL, Dual>(Dual{r2, 1.0f}, p);
         const float qq = xi[m].w * xj.w;
         float fsv = T4.x * P.f12.v - T4.y * P.f6.v;
         float D = T4.x * P.f12.d - T4.y * P.f6.d;
@@ -598,9 +602,9 @@ __global__ void __launch_bounds__(256) nb_adjoint(
           D += qq * P.hc.d;
         }
         D *= 2.0f;
-        const float hx = on ? -fsv * dvx - D * s * dx : 0.0f;
-        const float hy = on ? -fsv * dvy - D * s * dy : 0.0f;
-        const float hz = on ? -fsv * dvz - D * s * dz : 0.0f;
+        const float hx = -fsv * dvx - D * s * dx;
+        const float hy = -fsv * dvy - D * s * dy;
+        const float hz = -fsv * dvz - D * s * dz;
         ai[m][0] += hx;
         ai[m][1] += hy;
         ai[m][2] += hz;
