@@ -302,7 +302,7 @@ __device__ __forceinline__ float axis_gap(float lo_i, float hi_i, float lo_j,
 // pair occurs once.  Tiles of a cluster with itself, and tiles holding
 // excluded pairs, get four explicit 64-bit masks (MDExecKernels.cpp:1748-1823
 // removes the same partners from a matrix row).
-__global__ void __launch_bounds__(256) nb_search(
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) nb_search(
     const float4 *__restrict__ bb_lo, const float4 *__restrict__ bb_hi,
     const int *__restrict__ cluster_col, const uint32_t *__restrict__ real_mask,
     const int *__restrict__ col_cl_start, const int *__restrict__ col_cl_count,
@@ -346,7 +346,10 @@ __global__ void __launch_bounds__(256) nb_search(
   __shared__ int s_total;                  // candidates of the column chunk
   // the z bounds of the chunk's columns' clusters, staged so that the
   // binary searches read LDS instead of chains of dependent global loads
-  constexpr int kZStage = 2048;
+  // (chunks of more clusters read them from global memory).  1,024 keeps
+  // the block at 19 KB of LDS and, with 63 VGPRs, eight blocks fit a CU:
+  // a batch of frames launches thousands of these blocks.
+  constexpr int kZStage = 1024;
   __shared__ float s_zlo[kZStage], s_zhi[kZStage];
   __shared__ int s_zoff[64], s_zn[64], s_kmin[64];
   __shared__ int s_ztotal;
