@@ -1253,6 +1253,11 @@ struct Engine {
           d_xq, d_gorig, d_type_at, d_real_mask, pruned ? d_rows_in : d_rows,
           pruned ? d_row_count_in : d_row_count, d_ex, g.row_cap, seg_len, max_groups, d_counts,
           d_lj, nbp, d_F, d_epair, d_flags, sa, fs);
+    else if (nz > 1)  // a batch of frames
+      nb_pairs_batch<MOD, COUL, ENERGY><<<grid, 256, lds, lstream>>>(
+          d_xq, d_gorig, d_type_at, d_real_mask, pruned ? d_rows_in : d_rows,
+          pruned ? d_row_count_in : d_row_count, d_ex, g.row_cap, seg_len, max_groups, d_counts,
+          d_lj, nbp, d_F, d_epair, d_flags, sa, fs);
     else
       nb_pairs<MOD, COUL, ENERGY, false><<<grid, 256, lds, lstream>>>(
           d_xq, d_gorig, d_type_at, d_real_mask, pruned ? d_rows_in : d_rows,
